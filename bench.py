@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the MPS time step (Computer::ForwardTime) on one MI355X.
+
+Workload (default): the 1M-particle dam break, a 2.0 x 1.2 m tank with a
+0.5 x 1.0 m water column at l0 = 0.7 mm (1,020,306 fluid + walls), positions
+jittered +/-0.02 l0 from a fixed seed, r_e = 2.1 l0, Laplacian radius 3.1 l0,
+dt = 1e-4 s, pressure solved to |b - A p| <= 1e-8 |b| (FP64 residual).
+
+A step is one full ForwardTime: neighbour search, two density passes, HS/ECS,
+viscosity + gravity, PPE assembly, the pressure solve, pressure gradient and
+DS.  The headline solver runs the timed steps; the plain FP64 CG (the
+reference algorithm) runs the same steps in a second context for the solve
+time comparison.  Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M; solve ms/step learned+mixed vs plain FP64 @res 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_computer(cfg, solver: str, tol: float):
+    from openmps_amd import Computer, make_environment
+    from openmps_amd import _lib as L
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    kind = L.SOLVER_MIXED_IR if solver.startswith("mixed") else L.SOLVER_CG_FP64
+    c.set_solver(kind=kind, stop_rule=L.STOP_RHS_NORM, warm_start=L.WARM_PREVIOUS,
+                 jacobi=solver.endswith("jacobi"), tol=tol, inner_tol=1e-4)
+    c.add_particles(cfg["particles"])
+    return c
+
+
+def run(c, steps: int, label: str):
+    """Runs `steps` ForwardTime calls; returns wall seconds and per-step stats.
+    Every call is synchronous at return (stream synchronised inside)."""
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        c.forward_time(c._bench_dt)
+        s = c.solve_stats()
+        stats.append(s)
+        log(f"  [{label}] step {k + 1}/{steps}: CG {s['iterations']} it "
+            f"({s['outer_iterations']} sweeps), solve {s['solve_ms']:.1f} ms, "
+            f"|b-Ax|/|b| {s['rel_residual']:.2e} (from {s['initial_rel_residual']:.2e})")
+    return time.perf_counter() - t0, stats
+
+
+def kernel_table(c):
+    out = {}
+    for name, s in c.kernel_stats().items():
+        if s["launches"] == 0 or s["ms"] <= 0:
+            continue
+        out[name] = {"launches": s["launches"], "ms_per_launch": s["ms"] / s["launches"],
+                     "GBps": s["bytes"] / s["ms"] / 1e6,
+                     "bytes_per_launch": s["bytes"] / s["launches"]}
+    return out
+
+
+def cpu_baseline(cfg, tol: float, budget_s: float):
+    """The CPU oracle (C++/OpenMP FP64 restatement of the reference path) on
+    one step of the same scene with the same stop rule."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = oracle.Oracle(cfg["env"], cfg["eps"])
+    o.set_stop_rule(1, tol)
+    o.add_particles(cfg["particles"])
+    n = o.size()
+    t0 = time.perf_counter()
+    o.forward_time(cfg["dt"])
+    el = time.perf_counter() - t0
+    it = o.cg_stats()["iterations"]
+    log(f"  [cpu oracle] 1 step in {el:.2f} s ({it} CG iterations, {threads} threads)")
+    return {"value": n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"1 ForwardTime step of the same {n}-particle scene, same stop rule "
+                      f"({it} CG iterations), {el:.1f} s", "ms_per_step": el * 1e3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="dambreak_1m")
+    ap.add_argument("--solver", default="mixed_jacobi",
+                    choices=["mixed_jacobi", "mixed", "fp64"])
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=60.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        return 2
+
+    from openmps_amd import scenes
+    log(f"building scene {args.workload} ...")
+    cfg = scenes.config(args.workload)
+    N = len(cfg["particles"]["x"])
+    nfluid = int((cfg["particles"]["type"] == 0).sum())
+    log(f"  {N} particles ({nfluid} fluid)")
+
+    main_c = make_computer(cfg, args.solver, args.tol)
+    main_c._bench_dt = cfg["dt"]
+    if args.warmup:
+        run(main_c, args.warmup, f"{args.solver} warmup")
+    main_c.reset_kernel_stats()
+    main_c.set_profiling(True)
+    elapsed, st = run(main_c, args.steps, f"{args.solver} timed")
+    main_c.set_profiling(False)
+    kt = kernel_table(main_c)
+    solve_ms = sum(s["solve_ms"] for s in st) / len(st)
+    iters = sum(s["iterations"] for s in st) / len(st)
+
+    solve = {args.solver: {"ms_per_step": solve_ms, "cg_iterations": iters,
+                           "rel_residual": max(s["rel_residual"] for s in st)}}
+    if not args.no_compare and args.solver != "fp64":
+        ref_c = make_computer(cfg, "fp64", args.tol)
+        ref_c._bench_dt = cfg["dt"]
+        if args.warmup:
+            run(ref_c, args.warmup, "fp64 warmup")
+        _, st64 = run(ref_c, args.steps, "fp64 timed")
+        solve["fp64"] = {"ms_per_step": sum(s["solve_ms"] for s in st64) / len(st64),
+                         "cg_iterations": sum(s["iterations"] for s in st64) / len(st64),
+                         "rel_residual": max(s["rel_residual"] for s in st64)}
+        del ref_c
+
+    # dominant kernel: the SpMV of the solver that ran the timed steps
+    dom = "cg_spmv32" if args.solver.startswith("mixed") else "cg_spmv"
+    d = kt.get(dom)
+    roofline = None
+    if d:
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GBps"], 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": None, "bytes_per_launch": d["bytes_per_launch"],
+                    "ms_per_launch": d["ms_per_launch"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(cfg, args.tol, args.cpu_budget)
+        except Exception as e:  # the oracle is optional for the measurement itself
+            log(f"  cpu baseline failed: {e}")
+
+    value = N * args.steps / elapsed
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.solver == "fp64" else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": N, "fluid_particles": nfluid,
+                   "l0": cfg["env"]["l0"], "dt": cfg["dt"], "r_e_by_l0": cfg["env"]["r_e_by_l0"],
+                   "r_lap_by_l0": cfg["env"]["r_lap_by_l0"], "solver": args.solver,
+                   "stop_rule": f"|b-Ax|/|b| <= {args.tol:g}", "parallelism": "single GPU"},
+        "fluid_particle_steps_per_s": nfluid * args.steps / elapsed,
+        "solve": solve,
+        "solve_speedup_vs_fp64": (solve["fp64"]["ms_per_step"] / solve_ms
+                                  if "fp64" in solve and args.solver != "fp64" else None),
+        "roofline": roofline,
+        "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
+                        "launches": v["launches"]} for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,218 @@
+/*
+ * mps.h - C ABI of the MI355X (gfx950) implementation of the OpenMps
+ * per-timestep MPS path (Computer::ForwardTime).
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.
+ * Every call is synchronous at return.  One host thread per context; a context
+ * owns one HIP stream and every device buffer it uses.  Host pointers passed
+ * in are only borrowed for the duration of the call.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the OpenMps source tree, src/OpenMps/).
+ */
+#ifndef MPS_H_
+#define MPS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPS_ABI_VERSION 1
+
+/* Particle::Type, Particle.hpp:16-29 */
+enum mps_particle_type { MPS_FLUID = 0, MPS_WALL = 1, MPS_DUMMY = 2, MPS_DISABLED = 3 };
+
+/* Status codes.  The reference throws instead:
+ *   MPS_ERR_CG_NOT_CONVERGED <- Computer::Exception, Computer.hpp:1424-1428
+ *   MPS_ERR_CELL_OVERFLOW    <- Grid::Exception,     Grid.hpp:314-318      */
+enum mps_status {
+  MPS_OK = 0,
+  MPS_ERR_CG_NOT_CONVERGED = 1,
+  MPS_ERR_CELL_OVERFLOW = 2,
+  MPS_ERR_HIP = 3,
+  MPS_ERR_BAD_ARG = 4,
+  MPS_ERR_NO_DEVICE = 5,
+  MPS_ERR_STATE = 6
+};
+
+/* Environment (Environment.hpp:9-256).  The first block holds the constructor
+ * arguments, the second the derived constants the constructor computes. */
+typedef struct mps_env {
+  double max_dt_arg, courant, g, rho, nu, r_e_by_l0, l0, min_x, min_z, max_x, max_z;
+  double r_lap_by_l0; /* extension: Laplacian radius / l0 (== r_e_by_l0 in the reference) */
+  double max_dt;          /* MaxDt,  Environment.hpp:138 */
+  double max_dx;          /* MaxDx,  Environment.hpp:143 */
+  double r_e;             /* R_e,    Environment.hpp:145 */
+  double r_lap;           /* extension: Laplacian radius (== r_e in the reference) */
+  double gx, gz;          /* G,      Environment.hpp:149 */
+  double neighbor_length; /* NeighborLength, Environment.hpp:161 */
+  double n0;              /* N0(),   Environment.hpp:164-209 */
+  double n0_lap;          /* extension: lattice sum at r_lap (== n0 in the reference) */
+} mps_env;
+
+/* Environment::Environment(maxDt, courant, g, rho, nu, r_eByl_0, l_0, minX,
+ * minZ, maxX, maxZ), Environment.hpp:101-216 (default macro set, DIM = 2). */
+int mps_env_init(mps_env* env, double max_dt, double courant, double g, double rho, double nu,
+                 double r_e_by_l0, double l0, double min_x, double min_z, double max_x,
+                 double max_z);
+/* Extension: separate Laplacian radius for the viscosity term and the PPE
+ * matrix; recomputes r_lap, n0_lap and neighbor_length. */
+int mps_env_set_laplacian_radius(mps_env* env, double r_lap_by_l0);
+
+typedef struct mps_ctx mps_ctx;
+
+/* Computer(allowableResidual, env, posWall, posWallPre), Computer.hpp:1674-1691
+ * and CreateComputer, Computer.hpp:1800-1815.  Selects HIP device `device`. */
+int mps_create(const mps_env* env, double allowable_residual, int device, mps_ctx** out);
+/* ~Computer */
+void mps_destroy(mps_ctx* ctx);
+/* what() of the last error raised on this context ("" if none). */
+const char* mps_last_error(const mps_ctx* ctx);
+/* Computer::GetEnvironment, Computer.hpp:1786-1789 */
+int mps_get_env(const mps_ctx* ctx, mps_env* out);
+/* Grid shape, Grid.hpp:137-150: out = {nx, nz, cell capacity, max neighbours} */
+int mps_grid_shape(const mps_ctx* ctx, int64_t* out4);
+
+/* Computer::AddParticles, Computer.hpp:1754-1777 (appends; the appended
+ * positions also become the static wall positions, Main.cpp:305-318). */
+int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double* z,
+                      const double* u, const double* w, const double* p, const double* n,
+                      const uint8_t* type);
+/* Particles().size() */
+size_t mps_size(const mps_ctx* ctx);
+/* Computer::Particles, Computer.hpp:1780-1783.  NULL pointers are skipped. */
+int mps_get_particles(mps_ctx* ctx, double* x, double* z, double* u, double* w, double* p,
+                      double* n, uint8_t* type);
+/* Overwrites particle state in place (test hook; the reference tests write
+ * Particles() directly through friend access).  NULL pointers are skipped. */
+int mps_set_particles(mps_ctx* ctx, const double* x, const double* z, const double* u,
+                      const double* w, const double* p, const double* n, const uint8_t* type);
+/* Environment::T / Dt, Environment.hpp:225-242 */
+int mps_get_time(const mps_ctx* ctx, double* t, double* dt);
+/* Sets the clock (the reference tests do Dt() = dt; SetNextT()). */
+int mps_set_time(mps_ctx* ctx, double t, double dt);
+
+/* positionWall(i, t, dt) / positionWallPre(t, dt), Computer.hpp:993 and
+ * :1013-1019.  Without callbacks every non-fluid particle returns to its
+ * position at mps_add_particles, as Main.cpp:312-318 does. */
+typedef void (*mps_wall_pre_fn)(void* user, double t, double dt);
+typedef void (*mps_wall_fn)(void* user, double t, double dt, size_t count, const uint32_t* index,
+                            double* x_out, double* z_out);
+int mps_set_wall_callbacks(mps_ctx* ctx, mps_wall_fn wall, mps_wall_pre_fn pre, void* user);
+
+/* Computer::ForwardTime(dt), Computer.hpp:1700-1742 */
+int mps_forward_time(mps_ctx* ctx, double dt);
+/* Computer::ForwardTime() with Computer::DetermineDt, Computer.hpp:1745-1751, :759-777 */
+int mps_forward_time_cfl(mps_ctx* ctx, double* dt_out);
+/* Computer::DetermineDt alone */
+int mps_determine_dt(mps_ctx* ctx, double* dt_out);
+
+/* ---- pressure solver selection (extension; default = the reference CG) ---- */
+enum mps_solver_kind {
+  MPS_SOLVER_CG_FP64 = 0,  /* Computer.hpp:1359-1429, FP64 throughout */
+  MPS_SOLVER_MIXED_IR = 1  /* FP32 CG inside FP64 iterative refinement */
+};
+enum mps_stop_rule {
+  MPS_STOP_INITIAL_RESIDUAL = 0, /* r.r < tol^2 r0.r0 (Computer.hpp:1386, :1408) */
+  MPS_STOP_RHS_NORM = 1          /* |b - A x| <= tol |b|, residual in FP64 */
+};
+enum mps_warm_start {
+  MPS_WARM_PREVIOUS = 0, /* x0 = p of the previous step (Computer.hpp:1217-1219) */
+  MPS_WARM_ZERO = 1,
+  MPS_WARM_LEARNED = 2   /* x0 from the learned per-particle model */
+};
+typedef struct mps_solver_opts {
+  int kind;          /* mps_solver_kind */
+  int stop_rule;     /* mps_stop_rule */
+  int warm_start;    /* mps_warm_start */
+  int jacobi;        /* 1: diagonal preconditioner in the mixed solver's inner CG */
+  double tol;        /* <= 0: the allowable residual given to mps_create */
+  int64_t max_iter;  /* <= 0: number of unknowns, as the reference */
+  double inner_tol;  /* mixed: relative residual reduction per refinement sweep */
+  int64_t inner_max_iter;
+} mps_solver_opts;
+int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts);
+int mps_get_solver(const mps_ctx* ctx, mps_solver_opts* out);
+
+typedef struct mps_solve_stats {
+  int64_t iterations;       /* CG iterations (inner ones summed for the mixed solver) */
+  int64_t outer_iterations; /* refinement sweeps (mixed), 1 otherwise */
+  double rr0;               /* r0.r0 of the warm start */
+  double rr;                /* final r.r of the recursion */
+  double rel_residual;      /* |b - A x| / |b| recomputed in FP64 after the solve */
+  double initial_rel_residual; /* |b - A x0| / |b| */
+  double solve_ms;          /* device time of the solve (HIP events) */
+} mps_solve_stats;
+int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out);
+
+/* ---- stages (the friend-class test hooks of Computer.hpp:437-460) ---- */
+int mps_stage_search(mps_ctx* ctx);            /* SearchNeighbor, Computer.hpp:698-756 */
+int mps_stage_density(mps_ctx* ctx);           /* ComputeNeighborDensities, :780-833 */
+int mps_stage_error_correction(mps_ctx* ctx);  /* ComputeErrorCorrection, :877-910 */
+int mps_stage_explicit(mps_ctx* ctx);          /* ComputeExplicitForces, :914-1021 */
+int mps_stage_save_x(mps_ctx* ctx);            /* SaveX, :1025-1039 */
+int mps_stage_ppe_assemble(mps_ctx* ctx);      /* SetPressurePoissonEquation, :1145-1356 */
+int mps_stage_ppe_solve(mps_ctx* ctx);         /* SolvePressurePoissonEquation, :1359-1429 */
+int mps_stage_pressure_writeback(mps_ctx* ctx);/* ComputeImplicitForces p = max(x,0), :1076-1097 */
+int mps_stage_pressure_gradient(mps_ctx* ctx); /* ModifyByPressureGradient, :1433-1564 */
+int mps_stage_dynamic_stabilize(mps_ctx* ctx); /* DynamicStabilize, :1566-1657 */
+/* NeighborDensityVariationSpeed(i) for every i (HS), Computer.hpp:838-872 */
+int mps_density_variation(mps_ctx* ctx, double* out);
+
+/* ---- state getters for parity checks ---- */
+/* neighbor[i][0], Computer.hpp:580-588 */
+int mps_get_neighbor_counts(mps_ctx* ctx, uint32_t* counts);
+/* neighbor[i][1..count], concatenated in particle order (CSR by the counts). */
+int mps_get_neighbors(mps_ctx* ctx, uint32_t* ids);
+enum mps_aux {
+  MPS_AUX_N_WITHOUT_SPP = 0, /* nWithoutSpp, Computer.hpp:530-533 */
+  MPS_AUX_ECS = 1,           /* ecs, :520-523 */
+  MPS_AUX_SAVED_X = 2,       /* originalX[.][0], :525-528 */
+  MPS_AUX_SAVED_Z = 3,       /* originalX[.][1] */
+  MPS_AUX_DU_X = 4,          /* du[.][0], :515-518 */
+  MPS_AUX_DU_Z = 5,          /* du[.][1] */
+  MPS_AUX_PPE_B = 6,         /* ppe.b */
+  MPS_AUX_PPE_X = 7          /* ppe.x */
+};
+int mps_get_aux(mps_ctx* ctx, int which, double* out);
+int mps_set_aux(mps_ctx* ctx, int which, const double* in);
+/* ppe.A as CSR with ascending columns per row, the layout viennacl::copy
+ * produces (Computer.hpp:1352-1355). */
+int mps_ppe_shape(mps_ctx* ctx, size_t* n, size_t* nnz); /* unknowns, stored entries */
+int mps_get_ppe(mps_ctx* ctx, uint64_t* rowptr, uint32_t* col, double* val);
+/* Installs an arbitrary system for the solve stage (the CG tests write
+ * ppe.A, ppe.b and ppe.x directly, test_ComputerConjugateGradient.cpp:115-229). */
+int mps_set_ppe(mps_ctx* ctx, size_t n, const uint64_t* rowptr, const uint32_t* col,
+                const double* val, const double* b, const double* x0);
+
+/* ---- measurement (no reference counterpart) ---- */
+enum mps_kernel_id {
+  MPS_K_CELL_KEYS = 0, MPS_K_CELL_SCAN, MPS_K_CELL_SCATTER, MPS_K_CELL_SORT, MPS_K_NEIGHBORS,
+  MPS_K_DENSITY_ECS, MPS_K_EXPLICIT, MPS_K_UPDATE, MPS_K_DENSITY_SAVE, MPS_K_PPE_COUNT,
+  MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_SPMV32,
+  MPS_K_CG_UPDATE32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
+  MPS_K_COUNT
+};
+typedef struct mps_kernel_stat {
+  const char* name;
+  int64_t launches;
+  double total_ms;    /* summed HIP-event durations on the context stream */
+  double total_bytes; /* summed algorithmic HBM bytes of those launches */
+} mps_kernel_stat;
+/* 1: bracket every tracked launch with HIP events on the context stream. */
+int mps_set_profiling(mps_ctx* ctx, int on);
+int mps_get_kernel_stat(mps_ctx* ctx, int id, mps_kernel_stat* out);
+int mps_reset_kernel_stats(mps_ctx* ctx);
+
+/* Device arithmetic self-check: out_sqrt[i] = sqrt(a[i]), out_div[i] = a[i]/b[i]
+ * computed by the same code the kernels use (pins rounding against the host). */
+int mps_device_math_check(size_t n, const double* a, const double* b, double* out_sqrt,
+                          double* out_div);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS_H_ */

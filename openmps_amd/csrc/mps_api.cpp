@@ -1,0 +1,703 @@
+// C ABI of the MI355X MPS library (declarations and reference citations in
+// include/mps.h).  Owns the context: device buffers, the HIP stream, the
+// ForwardTime orchestration and the error-code mapping.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <exception>
+#include <new>
+
+#include "mps_internal.hpp"
+
+struct mps_ctx {
+  mps::Context c;
+};
+
+namespace mps {
+
+void check_launch() {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Failure{MPS_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e)};
+}
+
+void collect_events(Context& c, long long max_tag) {
+  for (size_t k = 0; k < c.events_used; k++) {
+    const EventPair& e = c.events[k];
+    if (e.tag > max_tag) continue;
+    float ms = 0;
+    MPS_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+    KStat& s = c.kstat[e.kid];
+    s.ms += ms;
+    s.launches++;
+    s.bytes += e.bytes;
+  }
+  c.events_used = 0;
+}
+
+void uncount(Context& c, int kid, long long count, double bytes) {
+  if (c.profiling || count <= 0) return;
+  c.kstat[kid].launches -= count;
+  c.kstat[kid].bytes -= count * bytes;
+}
+
+namespace {
+
+// Particle::W, Particle.hpp:71-75
+double weight_h(double r, double re) { return ((0 < r) && (r < re)) ? (re / r - 1) : 0; }
+
+// Environment.hpp:164-209: n0 = sum over the lattice points (i, j) l0,
+// i, j in [-R, R), R = ceil(r_e / l0), (0, 0) excluded, in that loop order.
+double lattice_sum(double r_by_l0, double l0, double r) {
+  const int range = static_cast<int>(std::ceil(r_by_l0));
+  double s = 0;
+  for (int i = -range; i < range; i++)
+    for (int j = -range; j < range; j++) {
+      if (i == 0 && j == 0) continue;
+      const double px = i * l0, pz = j * l0;
+      const double d = std::sqrt(std::fabs(px) * std::fabs(px) + std::fabs(pz) * std::fabs(pz));
+      if (d < r) s += weight_h(d, r);
+    }
+  return s;
+}
+
+void derive(mps_env& e) {
+  e.max_dt = std::min(e.max_dt_arg, std::sqrt(2 * (e.courant * e.l0) / e.g));  // :138
+  e.max_dx = e.courant * e.l0;                                                 // :143
+  e.r_e = e.r_e_by_l0 * e.l0;                                                  // :145
+  e.gx = 0;
+  e.gz = -e.g;                                                                 // :149
+  const double rmax = std::max(e.r_e_by_l0, e.r_lap_by_l0);
+  e.neighbor_length = rmax * e.l0 * (1 + e.courant * 2);                       // :161
+  e.n0 = lattice_sum(e.r_e_by_l0, e.l0, e.r_e);                                // :164-209
+  if (e.r_lap_by_l0 == e.r_e_by_l0) {
+    e.r_lap = e.r_e;
+    e.n0_lap = e.n0;
+  } else {
+    e.r_lap = e.r_lap_by_l0 * e.l0;
+    e.n0_lap = lattice_sum(e.r_lap_by_l0, e.l0, e.r_lap);
+  }
+}
+
+template <class T>
+void grow(T*& p, size_t old_n, size_t new_n, hipStream_t s) {
+  T* q = dalloc<T>(new_n);
+  MPS_HIP(hipMemsetAsync(q, 0, new_n * sizeof(T), s));
+  if (p && old_n) MPS_HIP(hipMemcpyAsync(q, p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  MPS_HIP(hipStreamSynchronize(s));
+  dfree(p);
+  p = q;
+}
+
+template <class F>
+int guard(mps_ctx* ctx, F&& f) {
+  if (!ctx) return MPS_ERR_BAD_ARG;
+  try {
+    f(ctx->c);
+    ctx->c.err.clear();
+    return MPS_OK;
+  } catch (Failure& e) {
+    ctx->c.err = e.msg;
+    if (e.code == MPS_ERR_HIP) (void)hipGetLastError();
+    return e.code;
+  } catch (std::bad_alloc&) {
+    ctx->c.err = "host allocation failed";
+    return MPS_ERR_BAD_ARG;
+  } catch (std::exception& e) {
+    ctx->c.err = e.what();
+    return MPS_ERR_BAD_ARG;
+  }
+}
+
+void sync(Context& c) {
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  collect_events(c);
+}
+
+void require_neighbors(const Context& c) {
+  if (!c.have_neighbors && c.n > 0) throw Failure{MPS_ERR_STATE, "neighbour search has not run"};
+}
+
+void free_all(Context& c) {
+  dfree(c.x); dfree(c.z); dfree(c.u); dfree(c.w); dfree(c.p); dfree(c.nd); dfree(c.type);
+  dfree(c.nws); dfree(c.ecs); dfree(c.ox); dfree(c.oz); dfree(c.dux); dfree(c.duz);
+  dfree(c.wall_x); dfree(c.wall_z); dfree(c.hs_out);
+  dfree(c.key); dfree(c.sorted); dfree(c.nbr); dfree(c.ncount);
+  dfree(c.cell_count); dfree(c.cell_start); dfree(c.cell_fill); dfree(c.scan_tmp); dfree(c.flags);
+  dfree(c.nbr_total_dev); dfree(c.ppe_counters);
+  dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.sval); dfree(c.sval32);
+  dfree(c.scol); dfree(c.diag); dfree(c.diag32); dfree(c.b); dfree(c.sol);
+  dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
+  dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
+  dfree(c.r32); dfree(c.pa32); dfree(c.pb32); dfree(c.q32); dfree(c.d32); dfree(c.dinv32);
+  if (c.flags_host) (void)hipHostFree(c.flags_host);
+  if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
+  if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
+  if (c.red_host) (void)hipHostFree(c.red_host);
+  if (c.st_host) (void)hipHostFree(c.st_host);
+  if (c.st32_host) (void)hipHostFree(c.st32_host);
+  if (c.ir_host) (void)hipHostFree(c.ir_host);
+  for (auto& e : c.events) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  c.events.clear();
+  if (c.solve_a) (void)hipEventDestroy(c.solve_a);
+  if (c.solve_b) (void)hipEventDestroy(c.solve_b);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+template <class T>
+void pinned(T*& p, size_t count) {
+  void* q = nullptr;
+  MPS_HIP(hipHostMalloc(&q, count * sizeof(T), hipHostMallocDefault));
+  std::memset(q, 0, count * sizeof(T));
+  p = static_cast<T*>(q);
+}
+
+double* aux_ptr(Context& c, int which, size_t& count) {
+  count = c.n;
+  switch (which) {
+    case MPS_AUX_N_WITHOUT_SPP: return c.nws;
+    case MPS_AUX_ECS: return c.ecs;
+    case MPS_AUX_SAVED_X: return c.ox;
+    case MPS_AUX_SAVED_Z: return c.oz;
+    case MPS_AUX_DU_X: return c.dux;
+    case MPS_AUX_DU_Z: return c.duz;
+    case MPS_AUX_PPE_B: count = c.ppe_n; return c.b;
+    case MPS_AUX_PPE_X: count = c.ppe_n; return c.sol;
+    default: throw Failure{MPS_ERR_BAD_ARG, "unknown auxiliary array"};
+  }
+}
+
+void forward(Context& c, double dt) {
+  // Computer::ForwardTime(dt), Computer.hpp:1700-1742
+  c.dt = dt;
+  c.t += dt;
+  search(c);
+  density(c, /*ecs=*/true, /*savex=*/false, /*dens=*/true);
+  explicit_forces(c);
+  density(c, /*ecs=*/false, /*savex=*/true, /*dens=*/true);
+  ppe_assemble(c);
+  ppe_solve(c);
+  pressure_writeback(c);
+  pressure_gradient(c);
+  dynamic_stabilize(c);
+  sync(c);
+}
+
+const char* kKernelNames[MPS_K_COUNT] = {
+    "cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs", "explicit",
+    "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv", "cg_update",
+    "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other"};
+
+}  // namespace
+
+void ensure_ppe_vectors(Context& c, size_t n) {
+  if (n <= c.ppe_alloc && c.r) return;
+  const size_t m = (n + 63) / 64 * 64 + 64;
+  const size_t ns = m / 64 + 1;
+  dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.diag); dfree(c.diag32);
+  dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q);
+  dfree(c.r32); dfree(c.pa32); dfree(c.pb32); dfree(c.q32); dfree(c.d32); dfree(c.dinv32);
+  c.row_len = dalloc<uint32_t>(m);
+  c.slice_width = dalloc<uint32_t>(ns);
+  c.slice_off = dalloc<uint64_t>(ns + 1);
+  c.diag = dalloc<double>(m);
+  c.diag32 = dalloc<float>(m);
+  c.b = dalloc<double>(m);
+  c.sol = dalloc<double>(m);
+  c.r = dalloc<double>(m);
+  c.pa = dalloc<double>(m);
+  c.pb = dalloc<double>(m);
+  c.q = dalloc<double>(m);
+  c.r32 = dalloc<float>(m);
+  c.pa32 = dalloc<float>(m);
+  c.pb32 = dalloc<float>(m);
+  c.q32 = dalloc<float>(m);
+  c.d32 = dalloc<float>(m);
+  c.dinv32 = dalloc<float>(m);
+  MPS_HIP(hipMemsetAsync(c.sol, 0, m * 8, c.stream));
+  MPS_HIP(hipMemsetAsync(c.pa, 0, m * 8, c.stream));
+  MPS_HIP(hipMemsetAsync(c.pb, 0, m * 8, c.stream));
+  MPS_HIP(hipMemsetAsync(c.pa32, 0, m * 4, c.stream));
+  MPS_HIP(hipMemsetAsync(c.pb32, 0, m * 4, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.ppe_alloc = n;
+  c.have_ppe = false;
+  c.have_ppe32 = false;
+}
+
+}  // namespace mps
+
+using mps::Context;
+using mps::Failure;
+
+extern "C" {
+
+int mps_env_init(mps_env* env, double max_dt, double courant, double g, double rho, double nu,
+                 double r_e_by_l0, double l0, double min_x, double min_z, double max_x, double max_z) {
+  if (!env || !(l0 > 0) || !(r_e_by_l0 > 0)) return MPS_ERR_BAD_ARG;
+  mps_env e{};
+  e.max_dt_arg = max_dt;
+  e.courant = courant;
+  e.g = g;
+  e.rho = rho;
+  e.nu = nu;
+  e.r_e_by_l0 = r_e_by_l0;
+  e.r_lap_by_l0 = r_e_by_l0;
+  e.l0 = l0;
+  e.min_x = min_x;
+  e.min_z = min_z;
+  e.max_x = max_x;
+  e.max_z = max_z;
+  mps::derive(e);
+  *env = e;
+  return MPS_OK;
+}
+
+int mps_env_set_laplacian_radius(mps_env* env, double r_lap_by_l0) {
+  if (!env || !(r_lap_by_l0 > 0)) return MPS_ERR_BAD_ARG;
+  env->r_lap_by_l0 = r_lap_by_l0;
+  mps::derive(*env);
+  return MPS_OK;
+}
+
+int mps_create(const mps_env* env, double allowable_residual, int device, mps_ctx** out) {
+  if (!env || !out) return MPS_ERR_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return MPS_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= ndev) return MPS_ERR_BAD_ARG;
+  mps_ctx* h = new (std::nothrow) mps_ctx;
+  if (!h) return MPS_ERR_BAD_ARG;
+  Context& c = h->c;
+  try {
+    MPS_HIP(hipSetDevice(device));
+    c.device = device;
+    c.env = *env;
+    c.eps = allowable_residual;
+    c.solver.kind = MPS_SOLVER_CG_FP64;
+    c.solver.stop_rule = MPS_STOP_INITIAL_RESIDUAL;
+    c.solver.warm_start = MPS_WARM_PREVIOUS;
+    c.solver.inner_tol = 1e-4;
+    MPS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    // Grid.hpp:137-150
+    const double NL = c.env.neighbor_length;
+    c.gnx = static_cast<long long>(std::ceil((c.env.max_x - c.env.min_x) / NL)) + 2;
+    c.gnz = static_cast<long long>(std::ceil((c.env.max_z - c.env.min_z) / NL)) + 2;
+    const long long c1 = static_cast<long long>(std::ceil(NL / c.env.l0)) + 1;
+    if (c.gnx <= 0 || c.gnz <= 0 || c1 <= 0 || static_cast<double>(c.gnx) * c.gnz > 4.0e9)
+      throw Failure{MPS_ERR_BAD_ARG, "grid shape out of range"};
+    c.cap = static_cast<uint32_t>(c1 * c1);
+    c.kmax = c.cap * 9;
+    c.ncells = static_cast<size_t>(c.gnx * c.gnz);
+    c.cell_count = mps::dalloc<uint32_t>(c.ncells);
+    c.cell_start = mps::dalloc<uint32_t>(c.ncells);
+    c.cell_fill = mps::dalloc<uint32_t>(c.ncells);
+    c.scan_tmp = mps::dalloc<uint64_t>(c.ncells / 2048 + 4096);
+    c.flags = mps::dalloc<int>(4);
+    c.nbr_total_dev = mps::dalloc<unsigned long long>(1);
+    c.ppe_counters = mps::dalloc<unsigned long long>(2);
+    c.partA = mps::dalloc<double>(mps::kRedBlocks);
+    c.partB = mps::dalloc<double>(mps::kRedBlocks);
+    c.partC = mps::dalloc<double>(mps::kRedBlocks);
+    c.red_out = mps::dalloc<double>(2);
+    c.st = mps::dalloc<mps::CgStatus>(1);
+    c.st32 = mps::dalloc<mps::CgStatus>(1);
+    c.ir = mps::dalloc<mps::IrStatus>(1);
+    mps::pinned(c.flags_host, 4);
+    mps::pinned(c.nbr_total_host, 1);
+    mps::pinned(c.ppe_counters_host, 2);
+    mps::pinned(c.red_host, 2);
+    mps::pinned(c.st_host, 1);
+    mps::pinned(c.st32_host, 1);
+    mps::pinned(c.ir_host, 1);
+    MPS_HIP(hipEventCreate(&c.solve_a));
+    MPS_HIP(hipEventCreate(&c.solve_b));
+  } catch (Failure& f) {
+    std::fprintf(stderr, "mps_create: %s\n", f.msg.c_str());
+    mps::free_all(c);
+    delete h;
+    return f.code;
+  }
+  *out = h;
+  return MPS_OK;
+}
+
+void mps_destroy(mps_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  mps::free_all(ctx->c);
+  delete ctx;
+}
+
+const char* mps_last_error(const mps_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int mps_get_env(const mps_ctx* ctx, mps_env* out) {
+  if (!ctx || !out) return MPS_ERR_BAD_ARG;
+  *out = ctx->c.env;
+  return MPS_OK;
+}
+
+int mps_grid_shape(const mps_ctx* ctx, int64_t* out4) {
+  if (!ctx || !out4) return MPS_ERR_BAD_ARG;
+  out4[0] = ctx->c.gnx;
+  out4[1] = ctx->c.gnz;
+  out4[2] = ctx->c.cap;
+  out4[3] = ctx->c.kmax;
+  return MPS_OK;
+}
+
+int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double* z, const double* u,
+                      const double* w, const double* p, const double* n, const uint8_t* type) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (count == 0) return;
+    if (!x || !z || !u || !w || !p || !n || !type) throw Failure{MPS_ERR_BAD_ARG, "null particle array"};
+    for (size_t k = 0; k < count; k++)
+      if (type[k] > MPS_DISABLED) throw Failure{MPS_ERR_BAD_ARG, "particle type out of range"};
+    const size_t old = c.n, nn = c.n + count;
+    if (nn > 0xffffffffull) throw Failure{MPS_ERR_BAD_ARG, "too many particles"};
+    MPS_HIP(hipSetDevice(c.device));
+    if (nn > c.alloc_n) {
+      const size_t m = (nn + 63) / 64 * 64;  // whole slices
+      const size_t mo = c.alloc_n;
+      mps::grow(c.x, old, m, c.stream);
+      mps::grow(c.z, old, m, c.stream);
+      mps::grow(c.u, old, m, c.stream);
+      mps::grow(c.w, old, m, c.stream);
+      mps::grow(c.p, old, m, c.stream);
+      mps::grow(c.nd, old, m, c.stream);
+      mps::grow(c.type, old, m, c.stream);
+      mps::grow(c.nws, old, m, c.stream);
+      mps::grow(c.ecs, old, m, c.stream);
+      mps::grow(c.ox, old, m, c.stream);
+      mps::grow(c.oz, old, m, c.stream);
+      mps::grow(c.dux, old, m, c.stream);
+      mps::grow(c.duz, old, m, c.stream);
+      mps::grow(c.wall_x, old, m, c.stream);
+      mps::grow(c.wall_z, old, m, c.stream);
+      mps::grow(c.hs_out, 0, m, c.stream);
+      mps::grow(c.key, 0, m, c.stream);
+      mps::grow(c.sorted, 0, m, c.stream);
+      mps::grow(c.ncount, old, m, c.stream);
+      // neighbour rows of existing particles are kept (boost::multi_array::resize)
+      mps::grow(c.nbr, (mo / 64) * 64 * static_cast<size_t>(c.kmax), m * static_cast<size_t>(c.kmax),
+                c.stream);
+      c.alloc_n = m;
+    }
+    mps::ensure_ppe_vectors(c, nn);
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    MPS_HIP(hipMemcpyAsync(c.x + old, x, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.z + old, z, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.u + old, u, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.w + old, w, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.p + old, p, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.nd + old, n, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.type + old, type, count, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.wall_x + old, x, count * 8, h2d, c.stream));
+    MPS_HIP(hipMemcpyAsync(c.wall_z + old, z, count * 8, h2d, c.stream));
+    MPS_HIP(hipStreamSynchronize(c.stream));
+    c.n = nn;
+    c.have_ppe = false;
+  });
+}
+
+size_t mps_size(const mps_ctx* ctx) { return ctx ? ctx->c.n : 0; }
+
+int mps_get_particles(mps_ctx* ctx, double* x, double* z, double* u, double* w, double* p, double* n,
+                      uint8_t* type) {
+  return mps::guard(ctx, [&](Context& c) {
+    const size_t N = c.n;
+    if (N == 0) return;
+    const hipMemcpyKind d2h = hipMemcpyDeviceToHost;
+    if (x) MPS_HIP(hipMemcpyAsync(x, c.x, N * 8, d2h, c.stream));
+    if (z) MPS_HIP(hipMemcpyAsync(z, c.z, N * 8, d2h, c.stream));
+    if (u) MPS_HIP(hipMemcpyAsync(u, c.u, N * 8, d2h, c.stream));
+    if (w) MPS_HIP(hipMemcpyAsync(w, c.w, N * 8, d2h, c.stream));
+    if (p) MPS_HIP(hipMemcpyAsync(p, c.p, N * 8, d2h, c.stream));
+    if (n) MPS_HIP(hipMemcpyAsync(n, c.nd, N * 8, d2h, c.stream));
+    if (type) MPS_HIP(hipMemcpyAsync(type, c.type, N, d2h, c.stream));
+    MPS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int mps_set_particles(mps_ctx* ctx, const double* x, const double* z, const double* u, const double* w,
+                      const double* p, const double* n, const uint8_t* type) {
+  return mps::guard(ctx, [&](Context& c) {
+    const size_t N = c.n;
+    if (N == 0) return;
+    if (type)
+      for (size_t k = 0; k < N; k++)
+        if (type[k] > MPS_DISABLED) throw Failure{MPS_ERR_BAD_ARG, "particle type out of range"};
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    if (x) MPS_HIP(hipMemcpyAsync(c.x, x, N * 8, h2d, c.stream));
+    if (z) MPS_HIP(hipMemcpyAsync(c.z, z, N * 8, h2d, c.stream));
+    if (u) MPS_HIP(hipMemcpyAsync(c.u, u, N * 8, h2d, c.stream));
+    if (w) MPS_HIP(hipMemcpyAsync(c.w, w, N * 8, h2d, c.stream));
+    if (p) MPS_HIP(hipMemcpyAsync(c.p, p, N * 8, h2d, c.stream));
+    if (n) MPS_HIP(hipMemcpyAsync(c.nd, n, N * 8, h2d, c.stream));
+    if (type) MPS_HIP(hipMemcpyAsync(c.type, type, N, h2d, c.stream));
+    MPS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int mps_get_time(const mps_ctx* ctx, double* t, double* dt) {
+  if (!ctx) return MPS_ERR_BAD_ARG;
+  if (t) *t = ctx->c.t;
+  if (dt) *dt = ctx->c.dt;
+  return MPS_OK;
+}
+
+int mps_set_time(mps_ctx* ctx, double t, double dt) {
+  if (!ctx) return MPS_ERR_BAD_ARG;
+  ctx->c.t = t;
+  ctx->c.dt = dt;
+  return MPS_OK;
+}
+
+int mps_set_wall_callbacks(mps_ctx* ctx, mps_wall_fn wall, mps_wall_pre_fn pre, void* user) {
+  if (!ctx) return MPS_ERR_BAD_ARG;
+  ctx->c.wall_fn = wall;
+  ctx->c.wall_pre = pre;
+  ctx->c.wall_user = user;
+  return MPS_OK;
+}
+
+int mps_forward_time(mps_ctx* ctx, double dt) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (!(dt > 0)) throw Failure{MPS_ERR_BAD_ARG, "dt must be positive"};
+    mps::forward(c, dt);
+  });
+}
+
+int mps_determine_dt(mps_ctx* ctx, double* dt_out) {
+  return mps::guard(ctx, [&](Context& c) {
+    const double dt = mps::max_speed_dt(c);
+    if (dt_out) *dt_out = dt;
+  });
+}
+
+int mps_forward_time_cfl(mps_ctx* ctx, double* dt_out) {
+  return mps::guard(ctx, [&](Context& c) {
+    const double dt = mps::max_speed_dt(c);  // Computer::DetermineDt
+    if (dt_out) *dt_out = dt;
+    mps::forward(c, dt);
+  });
+}
+
+int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts) {
+  if (!ctx || !opts) return MPS_ERR_BAD_ARG;
+  if (opts->kind < 0 || opts->kind > MPS_SOLVER_MIXED_IR) return MPS_ERR_BAD_ARG;
+  if (opts->stop_rule < 0 || opts->stop_rule > MPS_STOP_RHS_NORM) return MPS_ERR_BAD_ARG;
+  if (opts->warm_start < 0 || opts->warm_start > MPS_WARM_LEARNED) return MPS_ERR_BAD_ARG;
+  ctx->c.solver = *opts;
+  return MPS_OK;
+}
+
+int mps_get_solver(const mps_ctx* ctx, mps_solver_opts* out) {
+  if (!ctx || !out) return MPS_ERR_BAD_ARG;
+  *out = ctx->c.solver;
+  return MPS_OK;
+}
+
+int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out) {
+  if (!ctx || !out) return MPS_ERR_BAD_ARG;
+  *out = ctx->c.stats;
+  return MPS_OK;
+}
+
+int mps_stage_search(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::search(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_density(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::density(c, false, false, true);
+    mps::sync(c);
+  });
+}
+int mps_stage_error_correction(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::density(c, true, false, false);
+    mps::sync(c);
+  });
+}
+int mps_stage_explicit(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::explicit_forces(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_save_x(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::density(c, false, true, false);
+    mps::sync(c);
+  });
+}
+int mps_stage_ppe_assemble(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::ppe_assemble(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_ppe_solve(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::ppe_solve(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_pressure_writeback(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (c.ppe_n != c.n) throw Failure{MPS_ERR_STATE, "installed system does not match the particles"};
+    mps::pressure_writeback(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_pressure_gradient(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::pressure_gradient(c);
+    mps::sync(c);
+  });
+}
+int mps_stage_dynamic_stabilize(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    mps::dynamic_stabilize(c);
+    mps::sync(c);
+  });
+}
+
+int mps_density_variation(mps_ctx* ctx, double* out) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::require_neighbors(c);
+    if (c.n == 0) return;
+    if (!out) throw Failure{MPS_ERR_BAD_ARG, "null output"};
+    mps::density_variation(c, c.hs_out);
+    MPS_HIP(hipMemcpyAsync(out, c.hs_out, c.n * 8, hipMemcpyDeviceToHost, c.stream));
+    mps::sync(c);
+  });
+}
+
+int mps_get_neighbor_counts(mps_ctx* ctx, uint32_t* counts) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (c.n == 0) return;
+    MPS_HIP(hipMemcpyAsync(counts, c.ncount, c.n * 4, hipMemcpyDeviceToHost, c.stream));
+    mps::sync(c);
+  });
+}
+
+int mps_get_neighbors(mps_ctx* ctx, uint32_t* ids) {
+  return mps::guard(ctx, [&](Context& c) {
+    const size_t N = c.n;
+    if (N == 0) return;
+    std::vector<uint32_t> cnt(N);
+    const size_t slots = (N + 63) / 64 * 64 * static_cast<size_t>(c.kmax);
+    std::vector<uint32_t> all(slots);
+    MPS_HIP(hipMemcpyAsync(cnt.data(), c.ncount, N * 4, hipMemcpyDeviceToHost, c.stream));
+    MPS_HIP(hipMemcpyAsync(all.data(), c.nbr, slots * 4, hipMemcpyDeviceToHost, c.stream));
+    mps::sync(c);
+    size_t o = 0;
+    for (size_t i = 0; i < N; i++)
+      for (uint32_t k = 0; k < cnt[i]; k++)
+        ids[o++] = all[((i / 64) * c.kmax + k) * 64 + (i % 64)];
+  });
+}
+
+int mps_get_aux(mps_ctx* ctx, int which, double* out) {
+  return mps::guard(ctx, [&](Context& c) {
+    size_t count = 0;
+    double* p = mps::aux_ptr(c, which, count);
+    if (count == 0) return;
+    MPS_HIP(hipMemcpyAsync(out, p, count * 8, hipMemcpyDeviceToHost, c.stream));
+    mps::sync(c);
+  });
+}
+
+int mps_set_aux(mps_ctx* ctx, int which, const double* in) {
+  return mps::guard(ctx, [&](Context& c) {
+    size_t count = 0;
+    double* p = mps::aux_ptr(c, which, count);
+    if (count == 0) return;
+    MPS_HIP(hipMemcpyAsync(p, in, count * 8, hipMemcpyHostToDevice, c.stream));
+    mps::sync(c);
+  });
+}
+
+int mps_ppe_shape(mps_ctx* ctx, size_t* n, size_t* nnz) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (!c.have_ppe) throw Failure{MPS_ERR_STATE, "no pressure Poisson system assembled"};
+    if (n) *n = c.ppe_n;
+    if (nnz) *nnz = static_cast<size_t>(c.nnz_offdiag) + c.ppe_n;
+  });
+}
+
+int mps_get_ppe(mps_ctx* ctx, uint64_t* rowptr, uint32_t* col, double* val) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (!c.have_ppe) throw Failure{MPS_ERR_STATE, "no pressure Poisson system assembled"};
+    std::vector<uint64_t> rp;
+    std::vector<uint32_t> cl;
+    std::vector<double> vl;
+    mps::ppe_download_csr(c, rp, cl, vl);
+    std::memcpy(rowptr, rp.data(), rp.size() * 8);
+    std::memcpy(col, cl.data(), cl.size() * 4);
+    std::memcpy(val, vl.data(), vl.size() * 8);
+  });
+}
+
+int mps_set_ppe(mps_ctx* ctx, size_t n, const uint64_t* rowptr, const uint32_t* col, const double* val,
+                const double* b, const double* x0) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (!rowptr || !b || !x0 || (rowptr[n] > 0 && (!col || !val)))
+      throw Failure{MPS_ERR_BAD_ARG, "null system array"};
+    mps::ppe_upload_csr(c, n, rowptr, col, val, b, x0);
+  });
+}
+
+int mps_set_profiling(mps_ctx* ctx, int on) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::sync(c);
+    c.profiling = (on != 0);
+  });
+}
+
+int mps_get_kernel_stat(mps_ctx* ctx, int id, mps_kernel_stat* out) {
+  if (!ctx || !out || id < 0 || id >= MPS_K_COUNT) return MPS_ERR_BAD_ARG;
+  const mps::KStat& s = ctx->c.kstat[id];
+  out->name = mps::kKernelNames[id];
+  out->launches = s.launches;
+  out->total_ms = s.ms;
+  out->total_bytes = s.bytes;
+  return MPS_OK;
+}
+
+int mps_reset_kernel_stats(mps_ctx* ctx) {
+  return mps::guard(ctx, [&](Context& c) {
+    mps::sync(c);
+    for (auto& s : c.kstat) s = mps::KStat{};
+  });
+}
+
+int mps_device_math_check(size_t n, const double* a, const double* b, double* out_sqrt, double* out_div) {
+  try {
+    mps::device_math_check(n, a, b, out_sqrt, out_div);
+  } catch (Failure& f) {
+    std::fprintf(stderr, "mps_device_math_check: %s\n", f.msg.c_str());
+    return f.code;
+  }
+  return MPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,600 @@
+// Pressure solve: Computer::SolvePressurePoissonEquation (Computer.hpp:1359-1429)
+// on the GPU, plus the mixed-precision iterative-refinement solver.
+//
+// CG iteration k is two kernels:
+//   k_cg_spmv   reduces the r.r partials of the previous kernel, applies the
+//               stop rule, forms p_k = r + beta p_{k-1} on the fly (for its own
+//               rows and for every gathered column), q = A p_k, and writes
+//               partial sums of p_k.q;
+//   k_cg_update reduces the p.q partials, alpha = rr / pq, x += alpha p,
+//               r -= alpha q, and writes partial sums of r.r.
+// Partial sums live in fixed slots (one per block of a fixed-size grid); every
+// block of the consuming kernel re-reduces them in the same order, so all
+// blocks agree bitwise on alpha, beta and the stop decision without atomics or
+// grid-wide fences.  The host enqueues iterations in chunks and reads the
+// device status once per chunk; kernels after convergence exit immediately.
+#include <algorithm>
+#include <cmath>
+
+#include "mps_device.hpp"
+#include "mps_internal.hpp"
+
+namespace mps {
+namespace {
+
+template <class T>
+struct Sell {
+  const uint64_t* __restrict__ off;
+  const uint32_t* __restrict__ width;
+  const T* __restrict__ val;
+  const uint32_t* __restrict__ col;
+  const T* __restrict__ diag;
+  size_t n;
+  size_t rpb;  // rows per block (multiple of 256)
+};
+
+struct Rows {
+  size_t lo, hi;
+};
+__device__ __forceinline__ Rows block_rows(size_t n, size_t rpb) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t lo = blk * rpb;
+  const size_t hi = (lo + rpb < n) ? lo + rpb : n;
+  return Rows{lo, hi};
+}
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// Block-uniform read of the convergence flag (thread 0 reads, LDS broadcast),
+// so that a block either runs every barrier below or none.
+__device__ __forceinline__ bool already_converged(const CgStatus* st) {
+  __shared__ int s_conv;
+  if (threadIdx.x == 0) s_conv = st->converged;
+  __syncthreads();
+  return s_conv != 0;
+}
+
+// r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
+__global__ void __launch_bounds__(256) k_cg_init(Sell<double> A, const double* __restrict__ b,
+                                                 const double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ part_rr,
+                                                 double* __restrict__ part_bb) {
+  __shared__ double lds4[4];
+  const Rows rw = block_rows(A.n, A.rpb);
+  double arr = 0, abb = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    double s = A.diag[i] * x[i];
+    const size_t base = A.off[i >> 6] + (i & 63);
+    const uint32_t wdt = A.width[i >> 6];
+    for (uint32_t k = 0; k < wdt; k++) {
+      const size_t o = base + static_cast<size_t>(k) * 64;
+      s += A.val[o] * x[A.col[o]];
+    }
+    const double ri = b[i] - s;
+    r[i] = ri;
+    arr += ri * ri;
+    abb += b[i] * b[i];
+  }
+  const double srr = block_sum256(arr, lds4);
+  const double sbb = block_sum256(abb, lds4);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = srr;
+    part_bb[blockIdx.x] = sbb;
+  }
+}
+
+struct StopRule {
+  int rule;      // mps_stop_rule
+  double tol;    // eps (initial-residual rule) or relative tolerance on |b|
+};
+
+// One CG iteration, first half (see file comment).  `it` is the 1-based
+// iteration number; FIRST marks it == 1, where p_1 = r_0.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
+                                                 const double* __restrict__ pold,
+                                                 double* __restrict__ pnew, double* __restrict__ q,
+                                                 const double* __restrict__ part_in,
+                                                 const double* __restrict__ part_bb, int nparts,
+                                                 double* __restrict__ part_out, CgStatus* st,
+                                                 StopRule sr, long long it) {
+  __shared__ double lds4[4];
+  if (already_converged(st)) return;
+  const int par = static_cast<int>(it & 1);
+  const double rr = reduce_partials(part_in, nparts, lds4);
+  double beta = 0, tol;
+  bool conv;
+  double bb = 0;
+  if (FIRST) {
+    if (sr.rule == MPS_STOP_RHS_NORM) {
+      bb = reduce_partials(part_bb, nparts, lds4);
+      tol = (sr.tol * sr.tol) * bb;
+      conv = (rr <= tol);
+    } else {
+      tol = rr * sr.tol * sr.tol;  // residual0, Computer.hpp:1386
+      conv = (tol == 0);
+    }
+  } else {
+    tol = st->tol;
+    conv = (sr.rule == MPS_STOP_RHS_NORM) ? (rr <= tol) : (rr < tol);  // :1408
+  }
+  if (lead()) {
+    st->rr[par] = rr;
+    if (FIRST) {
+      st->tol = tol;
+      st->rr0 = rr;
+      st->b2 = bb;
+    }
+    if (conv) {
+      st->converged = 1;
+      st->iterations = it - 1;
+    }
+  }
+  if (conv) return;
+  if (!FIRST) beta = rr / st->rr[par ^ 1];
+  const Rows rw = block_rows(A.n, A.rpb);
+  double acc = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    const double pi = FIRST ? r[i] : r[i] + beta * pold[i];
+    pnew[i] = pi;
+    double s = A.diag[i] * pi;
+    const size_t base = A.off[i >> 6] + (i & 63);
+    const uint32_t wdt = A.width[i >> 6];
+    for (uint32_t k = 0; k < wdt; k++) {
+      const size_t o = base + static_cast<size_t>(k) * 64;
+      const uint32_t j = A.col[o];
+      const double pj = FIRST ? r[j] : r[j] + beta * pold[j];
+      s += A.val[o] * pj;
+    }
+    q[i] = s;
+    acc += pi * s;
+  }
+  const double sp = block_sum256(acc, lds4);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
+}
+
+// One CG iteration, second half (Computer.hpp:1398-1404).
+__global__ void __launch_bounds__(256) k_cg_update(size_t n, size_t rpb, double* __restrict__ x,
+                                                   double* __restrict__ r, const double* __restrict__ p,
+                                                   const double* __restrict__ q,
+                                                   const double* __restrict__ part_in, int nparts,
+                                                   double* __restrict__ part_out, CgStatus* st,
+                                                   long long it) {
+  __shared__ double lds4[4];
+  if (already_converged(st)) return;
+  const int par = static_cast<int>(it & 1);
+  const double pq = reduce_partials(part_in, nparts, lds4);
+  const double alpha = st->rr[par] / pq;
+  const Rows rw = block_rows(n, rpb);
+  double acc = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  const double s = block_sum256(acc, lds4);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+  if (lead()) st->iterations = it;
+}
+
+// Stop test after the last permitted iteration.
+__global__ void __launch_bounds__(256) k_cg_check(const double* __restrict__ part_in, int nparts,
+                                                  CgStatus* st, StopRule sr, long long it) {
+  __shared__ double lds4[4];
+  if (already_converged(st)) return;
+  const double rr = reduce_partials(part_in, nparts, lds4);
+  const bool conv = (sr.rule == MPS_STOP_RHS_NORM) ? (rr <= st->tol) : (rr < st->tol);
+  if (threadIdx.x == 0) {
+    st->rr[it & 1] = rr;
+    if (conv) {
+      st->converged = 1;
+      st->iterations = it - 1;
+    }
+  }
+}
+
+// Final reduction of two partial arrays into out[0], out[1] (single block).
+__global__ void __launch_bounds__(256) k_reduce2(const double* __restrict__ a,
+                                                 const double* __restrict__ b, int nparts,
+                                                 double* __restrict__ out) {
+  __shared__ double lds4[4];
+  const double sa = reduce_partials(a, nparts, lds4);
+  const double sb = reduce_partials(b, nparts, lds4);
+  if (threadIdx.x == 0) {
+    out[0] = sa;
+    out[1] = sb;
+  }
+}
+
+// ---------------- mixed precision: FP32 CG inside FP64 refinement ----------------
+
+// r32 = r64 / |r64|, d32 = 0, partial sums of r32.r32 and r32.z (z = Dinv r).
+template <bool JAC>
+__global__ void __launch_bounds__(256) k_ir_prep(size_t n, size_t rpb, const double* __restrict__ r64,
+                                                 double inv_norm, const float* __restrict__ dinv,
+                                                 float* __restrict__ r32, float* __restrict__ d32,
+                                                 double* __restrict__ part_rr,
+                                                 double* __restrict__ part_rz) {
+  __shared__ double lds4[4];
+  const Rows rw = block_rows(n, rpb);
+  double arr = 0, arz = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    const float ri = static_cast<float>(r64[i] * inv_norm);
+    r32[i] = ri;
+    d32[i] = 0.0f;
+    arr += static_cast<double>(ri) * ri;
+    if (JAC) arz += static_cast<double>(ri) * static_cast<double>(ri * dinv[i]);
+  }
+  const double s1 = block_sum256(arr, lds4);
+  const double s2 = JAC ? block_sum256(arz, lds4) : s1;
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = s1;
+    part_rz[blockIdx.x] = s2;
+  }
+}
+
+// FP32 SpMV half-iteration.  Stop test on r.r relative to the sweep's initial
+// r.r; beta from r.z (== r.r without preconditioner); dots in FP64.
+template <bool FIRST, bool JAC>
+__global__ void __launch_bounds__(256) k_cg32_spmv(Sell<float> A, const float* __restrict__ r,
+                                                   const float* __restrict__ dinv,
+                                                   const float* __restrict__ pold,
+                                                   float* __restrict__ pnew, float* __restrict__ q,
+                                                   const double* __restrict__ part_rr,
+                                                   const double* __restrict__ part_rz, int nparts,
+                                                   double* __restrict__ part_out, CgStatus* st,
+                                                   double tol_rel, long long it) {
+  __shared__ double lds4[4];
+  if (already_converged(st)) return;
+  const int par = static_cast<int>(it & 1);
+  const double rr = reduce_partials(part_rr, nparts, lds4);
+  const double rz = JAC ? reduce_partials(part_rz, nparts, lds4) : rr;
+  double tol;
+  bool conv;
+  if (FIRST) {
+    tol = rr * tol_rel * tol_rel;
+    conv = (tol == 0);
+  } else {
+    tol = st->tol;
+    conv = rr < tol;
+  }
+  if (lead()) {
+    st->rr[par] = rz;
+    if (FIRST) {
+      st->tol = tol;
+      st->rr0 = rr;
+    }
+    if (conv) {
+      st->converged = 1;
+      st->iterations = it - 1;
+    }
+  }
+  if (conv) return;
+  const float beta = FIRST ? 0.0f : static_cast<float>(rz / st->rr[par ^ 1]);
+  const Rows rw = block_rows(A.n, A.rpb);
+  double acc = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    const float zi = JAC ? r[i] * dinv[i] : r[i];
+    const float pi = FIRST ? zi : zi + beta * pold[i];
+    pnew[i] = pi;
+    float s = A.diag[i] * pi;
+    const size_t base = A.off[i >> 6] + (i & 63);
+    const uint32_t wdt = A.width[i >> 6];
+    for (uint32_t k = 0; k < wdt; k++) {
+      const size_t o = base + static_cast<size_t>(k) * 64;
+      const uint32_t j = A.col[o];
+      const float zj = JAC ? r[j] * dinv[j] : r[j];
+      const float pj = FIRST ? zj : zj + beta * pold[j];
+      s += A.val[o] * pj;
+    }
+    q[i] = s;
+    acc += static_cast<double>(pi) * s;
+  }
+  const double sp = block_sum256(acc, lds4);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
+}
+
+template <bool JAC>
+__global__ void __launch_bounds__(256) k_cg32_update(size_t n, size_t rpb, float* __restrict__ d,
+                                                     float* __restrict__ r, const float* __restrict__ p,
+                                                     const float* __restrict__ q,
+                                                     const float* __restrict__ dinv,
+                                                     const double* __restrict__ part_in, int nparts,
+                                                     double* __restrict__ part_rr,
+                                                     double* __restrict__ part_rz, CgStatus* st,
+                                                     long long it) {
+  __shared__ double lds4[4];
+  if (already_converged(st)) return;
+  const int par = static_cast<int>(it & 1);
+  const double pq = reduce_partials(part_in, nparts, lds4);
+  const float alpha = static_cast<float>(st->rr[par] / pq);
+  const Rows rw = block_rows(n, rpb);
+  double arr = 0, arz = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+    d[i] = d[i] + alpha * p[i];
+    const float ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    arr += static_cast<double>(ri) * ri;
+    if (JAC) arz += static_cast<double>(ri) * static_cast<double>(ri * dinv[i]);
+  }
+  const double s1 = block_sum256(arr, lds4);
+  const double s2 = JAC ? block_sum256(arz, lds4) : s1;
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = s1;
+    part_rz[blockIdx.x] = s2;
+  }
+  if (lead()) st->iterations = it;
+}
+
+// x += |r| d (FP64 accumulation of the FP32 correction).
+__global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict__ x,
+                                                    const float* __restrict__ d, double norm) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) x[i] = x[i] + static_cast<double>(d[i]) * norm;
+}
+
+__global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const float* __restrict__ diag,
+                                                    float* __restrict__ dinv) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    const float d = diag[i];
+    dinv[i] = (d != 0.0f) ? 1.0f / d : 1.0f;
+  }
+}
+
+}  // namespace
+
+namespace {
+
+struct Launch {
+  int grid;
+  size_t rpb;
+};
+Launch cg_grid(size_t n) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > static_cast<size_t>(kRedBlocks)) blocks = kRedBlocks;
+  if (blocks == 0) blocks = 1;
+  size_t rpb = (n + blocks - 1) / blocks;
+  rpb = (rpb + 255) / 256 * 256;
+  blocks = (n + rpb - 1) / rpb;
+  if (blocks == 0) blocks = 1;
+  return Launch{static_cast<int>(blocks), rpb};
+}
+
+template <class T>
+Sell<T> sell(const Context& c, const T* val, const T* diag, size_t rpb) {
+  return Sell<T>{c.slice_off, c.slice_width, val, c.scol, diag, c.ppe_n, rpb};
+}
+
+// Reads the device status; folds the timed launches, dropping those that
+// ran after convergence (iteration tags above the converged count).
+void read_status(Context& c, CgStatus* dev, CgStatus* host) {
+  MPS_HIP(hipMemcpyAsync(host, dev, sizeof(CgStatus), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  collect_events(c, host->converged ? host->iterations : 0x7fffffffffffffffLL);
+}
+
+long long next_chunk(long long done, long long predicted) {
+  if (done == 0) return std::max<long long>(16, predicted > 0 ? predicted * 15 / 16 : 32);
+  return std::max<long long>(16, std::min<long long>(256, done / 4));
+}
+
+double spmv_bytes(const Context& c, int vbytes, double extra_per_row) {
+  return static_cast<double>(c.nnz_offdiag) * (vbytes + 4) + c.ppe_n * (vbytes + extra_per_row) +
+         c.nslices * 12.0;
+}
+
+// Plain FP64 CG (the reference algorithm); x = c.sol holds the warm start.
+void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
+  MPS_HIP(hipMemsetAsync(c.st, 0, sizeof(CgStatus), c.stream));
+  tracked(c, MPS_K_CG_INIT, spmv_bytes(c, 8, 8.0 + 8 + 8), [&] {
+    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(256), 0, c.stream, A, static_cast<const double*>(c.b),
+                       static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
+  });
+  check_launch();
+  double* pbuf[2] = {c.pa, c.pb};
+  long long it = 1;
+  const double b_spmv = spmv_bytes(c, 8, 8.0 * 4);  // diag, own r and p_old, writes p_new and q
+  const double b_upd = c.ppe_n * 48.0;
+  while (true) {
+    const long long chunk = next_chunk(it - 1, c.last_iters);
+    for (long long m = 0; m < chunk && it <= max_iter; m++, it++) {
+      const double* pold = pbuf[(it + 1) & 1];
+      double* pnew = pbuf[it & 1];
+      tracked(c, MPS_K_CG_SPMV, b_spmv, [&] {
+        if (it == 1)
+          hipLaunchKernelGGL(k_cg_spmv<true>, dim3(L.grid), dim3(256), 0, c.stream, A,
+                             static_cast<const double*>(c.r), pold, pnew, c.q,
+                             static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
+                             L.grid, c.partA, c.st, sr, it);
+        else
+          hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(256), 0, c.stream, A,
+                             static_cast<const double*>(c.r), pold, pnew, c.q,
+                             static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
+                             L.grid, c.partA, c.st, sr, it);
+      }, it);
+      tracked(c, MPS_K_CG_UPDATE, b_upd, [&] {
+        hipLaunchKernelGGL(k_cg_update, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.sol, c.r,
+                           static_cast<const double*>(pnew), static_cast<const double*>(c.q),
+                           static_cast<const double*>(c.partA), L.grid, c.partB, c.st, it);
+      }, it);
+    }
+    check_launch();
+    if (it > max_iter) {
+      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.partB),
+                         L.grid, c.st, sr, it);
+      check_launch();
+      read_status(c, c.st, c.st_host);
+      break;
+    }
+    read_status(c, c.st, c.st_host);
+    if (c.st_host->converged) break;
+  }
+  const long long launched = std::min(it - 1, max_iter);
+  uncount(c, MPS_K_CG_SPMV, launched - c.st_host->iterations, b_spmv);
+  uncount(c, MPS_K_CG_UPDATE, launched - c.st_host->iterations, b_upd);
+  c.stats.iterations = c.st_host->iterations;
+  c.stats.outer_iterations = 1;
+  c.stats.rr0 = c.st_host->rr0;
+  c.stats.rr = c.st_host->rr[(c.st_host->iterations + 1) & 1];
+  c.last_iters = c.st_host->iterations;
+  if (!c.st_host->converged)
+    throw Failure{MPS_ERR_CG_NOT_CONVERGED,
+                  "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
+}
+
+// FP64 residual |b - A x|^2 and |b|^2 (reuses the init kernel on scratch r).
+void residual64(Context& c, double& rr, double& bb, bool track) {
+  const Launch L = cg_grid(c.ppe_n);
+  const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
+  auto launch = [&] {
+    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(256), 0, c.stream, A, static_cast<const double*>(c.b),
+                       static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
+    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.partB),
+                       static_cast<const double*>(c.partC), L.grid, c.red_out);
+  };
+  if (track) tracked(c, MPS_K_IR_RESIDUAL, spmv_bytes(c, 8, 8.0 + 8 + 8), launch);
+  else launch();
+  check_launch();
+  MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  rr = c.red_host[0];
+  bb = c.red_host[1];
+}
+
+// Mixed precision: FP32 CG sweeps on A d = r, FP64 residual and accumulation.
+void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  ppe_make_fp32(c);
+  const bool jac = c.solver.jacobi != 0;
+  if (jac)
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                       static_cast<const float*>(c.diag32), c.dinv32);
+  const Sell<float> A = sell<float>(c, c.sval32, c.diag32, L.rpb);
+  const double inner_tol = c.solver.inner_tol > 0 ? c.solver.inner_tol : 1e-4;
+  const long long inner_max = c.solver.inner_max_iter > 0 ? c.solver.inner_max_iter : max_iter;
+  float* pbuf[2] = {c.pa32, c.pb32};
+  long long total = 0;
+  double rr0 = 0, bb = 0, rr = 0;
+  const double b_spmv = spmv_bytes(c, 4, 4.0 * 4 + (jac ? 4.0 : 0.0));
+  const double b_upd = n * (24.0 + (jac ? 4.0 : 0.0));
+  int outer = 0;
+  const int max_outer = 64;
+  bool done = false;
+  for (;; outer++) {
+    residual64(c, rr, bb, true);
+    if (outer == 0) rr0 = rr;
+    if (sr.rule == MPS_STOP_RHS_NORM)
+      done = rr <= (sr.tol * sr.tol) * bb;
+    else
+      done = (outer == 0) ? (rr * sr.tol * sr.tol == 0) : (rr < rr0 * sr.tol * sr.tol);
+    if (done || outer >= max_outer || total >= max_iter || !(rr == rr)) break;
+    const double norm = std::sqrt(rr);
+    MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
+    if (jac)
+      hipLaunchKernelGGL(k_ir_prep<true>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb,
+                         static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
+                         c.r32, c.d32, c.partB, c.partC);
+    else
+      hipLaunchKernelGGL(k_ir_prep<false>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb,
+                         static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
+                         c.r32, c.d32, c.partB, c.partC);
+    check_launch();
+    long long it = 1;
+    while (true) {
+      const long long chunk = next_chunk(it - 1, c.last_inner);
+      for (long long m = 0; m < chunk && it <= inner_max; m++, it++) {
+        const float* pold = pbuf[(it + 1) & 1];
+        float* pnew = pbuf[it & 1];
+        tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
+#define MPS_S32(F, J)                                                                                   \
+  hipLaunchKernelGGL((k_cg32_spmv<F, J>), dim3(L.grid), dim3(256), 0, c.stream, A,                      \
+                     static_cast<const float*>(c.r32), static_cast<const float*>(c.dinv32), pold, pnew, \
+                     c.q32, static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),  \
+                     L.grid, c.partA, c.st32, inner_tol, it)
+          if (it == 1) {
+            if (jac) MPS_S32(true, true);
+            else MPS_S32(true, false);
+          } else {
+            if (jac) MPS_S32(false, true);
+            else MPS_S32(false, false);
+          }
+#undef MPS_S32
+        }, it);
+        tracked(c, MPS_K_CG_UPDATE32, b_upd, [&] {
+          if (jac)
+            hipLaunchKernelGGL(k_cg32_update<true>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.d32,
+                               c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
+                               static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
+                               L.grid, c.partB, c.partC, c.st32, it);
+          else
+            hipLaunchKernelGGL(k_cg32_update<false>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.d32,
+                               c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
+                               static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
+                               L.grid, c.partB, c.partC, c.st32, it);
+        }, it);
+      }
+      check_launch();
+      read_status(c, c.st32, c.st32_host);
+      if (c.st32_host->converged || it > inner_max) break;
+    }
+    const long long inner = c.st32_host->converged ? c.st32_host->iterations : inner_max;
+    const long long launched = std::min(it - 1, inner_max);
+    uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
+    uncount(c, MPS_K_CG_UPDATE32, launched - inner, b_upd);
+    c.last_inner = inner;
+    total += inner;
+    hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
+                       static_cast<const float*>(c.d32), norm);
+    check_launch();
+  }
+  c.stats.iterations = total;
+  c.stats.outer_iterations = outer;
+  c.stats.rr0 = rr0;
+  c.stats.rr = rr;
+  c.last_iters = total;
+  if (!done)
+    throw Failure{MPS_ERR_CG_NOT_CONVERGED,
+                  "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
+}
+
+}  // namespace
+
+void ppe_solve(Context& c) {
+  if (!c.have_ppe) throw Failure{MPS_ERR_STATE, "no pressure Poisson system assembled"};
+  const size_t n = c.ppe_n;
+  c.stats = mps_solve_stats{};
+  if (n == 0) return;
+  StopRule sr;
+  sr.rule = c.solver.stop_rule;
+  sr.tol = (c.solver.tol > 0) ? c.solver.tol : c.eps;
+  const long long max_iter = (c.solver.max_iter > 0) ? c.solver.max_iter : static_cast<long long>(n);
+  MPS_HIP(hipEventRecord(c.solve_a, c.stream));
+  int rc = MPS_OK;
+  std::string msg;
+  try {
+    if (c.solver.kind == MPS_SOLVER_MIXED_IR) solve_mixed(c, sr, max_iter);
+    else solve_cg64(c, sr, max_iter);
+  } catch (Failure& f) {
+    if (f.code != MPS_ERR_CG_NOT_CONVERGED) throw;
+    rc = f.code;
+    msg = f.msg;
+  }
+  MPS_HIP(hipEventRecord(c.solve_b, c.stream));
+  MPS_HIP(hipEventSynchronize(c.solve_b));
+  float ms = 0;
+  MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
+  c.stats.solve_ms = ms;
+  // true FP64 residual of the returned x (reported, not part of the solve)
+  double rr = 0, bb = 0;
+  residual64(c, rr, bb, false);
+  c.stats.rel_residual = (bb > 0) ? std::sqrt(rr / bb) : std::sqrt(rr);
+  c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
+  if (rc != MPS_OK) throw Failure{rc, msg};
+}
+
+}  // namespace mps

@@ -1,0 +1,69 @@
+// Device-side helpers shared by the particle, PPE and CG kernels.
+//
+// All FP64 arithmetic here must round exactly as the reference's host code:
+// the library is compiled with -ffp-contract=off, and every expression keeps
+// the association order of the reference source it cites.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace mps {
+
+// Particle::W, Particle.hpp:71-75
+__device__ __forceinline__ double weight(double r, double re) {
+  return ((0 < r) && (r < re)) ? (re / r - 1) : 0;
+}
+
+// Slot k of particle i in the sliced neighbour layout.
+__device__ __forceinline__ size_t nbr_slot(uint32_t i, uint32_t k, uint32_t kmax) {
+  return (static_cast<size_t>(i >> 6) * kmax + k) * 64 + (i & 63u);
+}
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
+// round-robin; remapping makes each XCD walk one contiguous range of
+// particles, so gathered neighbour data stays in that XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb / 8u;
+  if (per == 0u || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
+// Wavefront (64-lane) sum / max.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned t = __shfl_xor(v, o, 64);
+    v = (t > v) ? t : v;
+  }
+  return v;
+}
+
+// Deterministic block sum of one double per thread (blockDim.x == 256):
+// fixed wavefront butterflies then the 4 wavefront sums in order.
+__device__ __forceinline__ double block_sum256(double v, double* lds4) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) lds4[wv] = v;
+  __syncthreads();
+  const double s = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+  __syncthreads();
+  return s;
+}
+
+// Sum of `count` partials in a fixed order that every block reproduces
+// identically (thread t takes t, t+256, ...; then block_sum256).
+__device__ __forceinline__ double reduce_partials(const double* part, int count, double* lds4) {
+  double s = 0;
+  for (int k = threadIdx.x; k < count; k += 256) s += part[k];
+  return block_sum256(s, lds4);
+}
+
+}  // namespace mps

@@ -1,0 +1,230 @@
+// Internal declarations of the MI355X MPS library: the context, device views
+// and the launch helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mps.h"
+
+namespace mps {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlock = 256;        // threads per block of the particle passes
+constexpr int kRedBlocks = 1024;   // fixed partial-sum slots of the reduction kernels
+
+// Particle type codes, Particle.hpp:16-29.
+constexpr uint8_t kFluid = 0, kWall = 1, kDummy = 2, kDisabled = 3;
+
+// Device-side CG scalars.  rr is double-buffered by iteration parity so that
+// block 0 of a kernel can publish the next value while the other blocks of
+// the same kernel still read the previous one.
+struct CgStatus {
+  double rr[2];
+  double tol;
+  double rr0;
+  double b2;          // |b|^2 (stop rule on |b|)
+  int converged;      // 1 once the stop rule holds
+  int pad;
+  long long iterations;
+};
+
+// Mixed-precision refinement scalars.
+struct IrStatus {
+  double rr_outer;    // |b - A x|^2 in FP64
+  double b2;
+  double rr0;
+  int done;
+  int pad;
+};
+
+struct KStat {
+  long long launches = 0;
+  double ms = 0, bytes = 0;
+};
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  int kid = 0;
+  long long tag = 0;   // CG iteration of the launch (0 outside the solver)
+  double bytes = 0;
+};
+
+struct Context {
+  mps_env env{};
+  double eps = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double t = 0, dt = 0;
+
+  // particles (SoA, FP64 as the reference)
+  size_t n = 0, alloc_n = 0;
+  double *x = nullptr, *z = nullptr, *u = nullptr, *w = nullptr, *p = nullptr, *nd = nullptr;
+  uint8_t* type = nullptr;
+  double *nws = nullptr, *ecs = nullptr, *ox = nullptr, *oz = nullptr;
+  double *dux = nullptr, *duz = nullptr, *wall_x = nullptr, *wall_z = nullptr;
+  double* hs_out = nullptr;
+
+  // uniform grid of cells of side NeighborLength (Grid.hpp:137-150)
+  long long gnx = 0, gnz = 0;
+  uint32_t cap = 0, kmax = 0;
+  size_t ncells = 0;
+  uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
+  uint32_t *key = nullptr, *sorted = nullptr;
+  uint64_t* scan_tmp = nullptr;   // block sums of the scans
+  int* flags = nullptr;           // [0] cell overflow, [1] neighbour overflow
+  int* flags_host = nullptr;      // pinned mirror
+
+  // neighbour lists: sliced layout, slot k of particle i at
+  // ((i/64)*kmax + k)*64 + i%64 (coalesced across a wavefront)
+  uint32_t *nbr = nullptr, *ncount = nullptr;
+  bool have_neighbors = false;
+  unsigned long long* nbr_total_dev = nullptr;
+  unsigned long long* nbr_total_host = nullptr;  // pinned
+  double nbr_total = 0;                          // sum of list lengths (byte accounting)
+
+  // PPE in sliced ELL (SELL-64): diagonal kept dense, off-diagonals of row i
+  // at slice_off[i/64] + k*64 + i%64 for k < slice_width[i/64]
+  size_t nslices = 0, sell_cap = 0;
+  uint32_t *row_len = nullptr, *slice_width = nullptr;
+  uint64_t* slice_off = nullptr;
+  double* sval = nullptr;
+  float* sval32 = nullptr;
+  uint32_t* scol = nullptr;
+  double *diag = nullptr, *b = nullptr, *sol = nullptr;
+  float* diag32 = nullptr;
+  size_t ppe_n = 0;        // unknowns of the installed system
+  size_t ppe_alloc = 0;    // capacity of the PPE / CG vectors
+  bool have_ppe = false;
+  bool have_ppe32 = false;
+  unsigned long long* ppe_counters = nullptr;       // [0] off-diagonal nnz
+  unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots
+  double nnz_offdiag = 0, sell_slots = 0;
+
+  // CG work vectors
+  double *r = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr;
+  double *partA = nullptr, *partB = nullptr, *partC = nullptr;
+  double* red_out = nullptr;
+  double* red_host = nullptr;  // pinned
+  CgStatus* st = nullptr;
+  CgStatus* st_host = nullptr;
+  CgStatus* st32 = nullptr;
+  CgStatus* st32_host = nullptr;
+  long long last_inner = 0;
+  // mixed-precision work vectors
+  float *r32 = nullptr, *pa32 = nullptr, *pb32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+  float* dinv32 = nullptr;
+  IrStatus* ir = nullptr;
+  IrStatus* ir_host = nullptr;
+  long long last_iters = 0;
+
+  mps_solver_opts solver{};
+  mps_solve_stats stats{};
+
+  // wall kinematics (Computer.hpp:993, :1013-1019)
+  mps_wall_fn wall_fn = nullptr;
+  mps_wall_pre_fn wall_pre = nullptr;
+  void* wall_user = nullptr;
+
+  // measurement
+  bool profiling = false;
+  KStat kstat[MPS_K_COUNT];
+  std::vector<EventPair> events;
+  size_t events_used = 0;
+  hipEvent_t solve_a = nullptr, solve_b = nullptr;
+
+  std::string err;
+};
+
+// ---- error plumbing ----------------------------------------------------------
+struct Failure {
+  int code;
+  std::string msg;
+};
+
+#define MPS_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::mps::Failure{MPS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+template <class T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  MPS_HIP(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// ---- measured launches ---------------------------------------------------------
+// Counts the launch and its algorithmic bytes; when profiling is on, also
+// records a HIP event pair around it on the context stream (the statistics
+// are then folded in by collect_events, which can drop solver launches that
+// ran after convergence: those exit at once and do no work).
+template <class F>
+void tracked(Context& c, int kid, double bytes, F&& launch, long long tag = 0) {
+  if (!c.profiling) {
+    KStat& s = c.kstat[kid];
+    s.launches++;
+    s.bytes += bytes;
+    launch();
+    return;
+  }
+  if (c.events_used == c.events.size()) {
+    EventPair e;
+    MPS_HIP(hipEventCreate(&e.a));
+    MPS_HIP(hipEventCreate(&e.b));
+    c.events.push_back(e);
+  }
+  EventPair& e = c.events[c.events_used++];
+  e.kid = kid;
+  e.tag = tag;
+  e.bytes = bytes;
+  MPS_HIP(hipEventRecord(e.a, c.stream));
+  launch();
+  MPS_HIP(hipEventRecord(e.b, c.stream));
+}
+// Folds the recorded event pairs into the kernel statistics (stream must be
+// idle); pairs tagged above max_tag are discarded.
+void collect_events(Context& c, long long max_tag = 0x7fffffffffffffffLL);
+// Removes `count` launches of `bytes` each from a kernel's statistics (solver
+// launches found to have run after convergence, when not profiling).
+void uncount(Context& c, int kid, long long count, double bytes);
+void check_launch();
+
+inline unsigned grid_for(size_t n, int block = kBlock) {
+  return static_cast<unsigned>((n + block - 1) / block);
+}
+
+// ---- stage entry points (defined in the .hip units) ------------------------------
+void search(Context& c);                 // mps_search.hip
+void density(Context& c, bool ecs, bool savex, bool dens);  // mps_particles.hip
+void density_variation(Context& c, double* out_dev);
+void explicit_forces(Context& c);
+void wall_targets_from_callback(Context& c);
+void pressure_writeback(Context& c);
+void pressure_gradient(Context& c);
+void dynamic_stabilize(Context& c);
+void ppe_assemble(Context& c);           // mps_ppe.hip
+void ppe_make_fp32(Context& c);
+void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,
+                    const double* val, const double* b, const double* x0);
+void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uint32_t>& col,
+                      std::vector<double>& val);
+void ensure_sell_capacity(Context& c, size_t nslices, size_t slots);
+void ensure_ppe_vectors(Context& c, size_t n);  // mps_api.cpp
+void ppe_solve(Context& c);              // mps_cg.hip
+double max_speed_dt(Context& c);         // mps_particles.hip (DetermineDt)
+void device_math_check(size_t n, const double* a, const double* b, double* s, double* d);
+
+}  // namespace mps

@@ -1,0 +1,343 @@
+// Pressure Poisson equation assembly: Computer::SetPressurePoissonEquation
+// (Computer.hpp:1145-1356) rebuilt as two neighbour passes that write a
+// device-resident sliced ELL matrix (SELL-64: one slice = one wavefront of
+// 64 rows, slot k of row i at slice_off[i/64] + 64k + i%64).
+//
+//   k_ppe_count  RHS b_i (HS + ECS, FP64), warm start x0_i = p_i, the
+//                off-diagonal count of row i and, per slice, the wavefront
+//                maximum of those counts;
+//   scan         slice widths -> slice offsets;
+//   k_ppe_fill   HL off-diagonals 3 r_e / (n0 r^3) in neighbour-list order,
+//                zero padding, and the diagonal -sum(a_ij) including the SPP
+//                ghost (Computer.hpp:1290-1292).
+//
+// The reference builds per-row vectors in parallel and scatters them serially
+// into a uBLAS compressed matrix before copying to ViennaCL CSR
+// (:1330-1354); nothing here is serial.
+#include <algorithm>
+
+#include "mps_device.hpp"
+#include "mps_internal.hpp"
+
+namespace mps {
+
+void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* offsets);
+
+namespace {
+
+struct NV {
+  const uint32_t* __restrict__ ids;
+  const uint32_t* __restrict__ count;
+  uint32_t kmax;
+};
+
+struct PpeConsts {
+  double re, re2, rl, rl2, n0, n0l, rho, dt;
+  int warm;  // mps_warm_start
+};
+
+__device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, double sz, double nws_i,
+                                          double n0, double re, double& gx, double& gz) {
+  sx /= n0;
+  sz /= n0;
+  if (!(nws_i < n0)) return false;
+  const double rg = sqrt(sx * sx + sz * sz);
+  if (!(rg > DBL_EPSILON)) return false;
+  const double wspp = n0 - nws_i;
+  const double rspp = re / (wspp + 1);
+  gx = xi - (rspp / rg) * sx;
+  gz = zi - (rspp / rg) * sz;
+  return true;
+}
+
+// b and x0 (Computer.hpp:1185-1221), row lengths and slice widths.
+__global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ w,
+                                                      const double* __restrict__ p,
+                                                      const uint8_t* __restrict__ type, NV nb,
+                                                      PpeConsts k, const double* __restrict__ nws,
+                                                      const double* __restrict__ ecs,
+                                                      double* __restrict__ b, double* __restrict__ sol,
+                                                      uint32_t* __restrict__ row_len,
+                                                      uint32_t* __restrict__ slice_width,
+                                                      unsigned long long* nnz_total) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  uint32_t cnt = 0;
+  if (i < n) {
+    const uint8_t ti = type[i];
+    if (ti == kDummy || ti == kDisabled) {
+      b[i] = 0;
+      sol[i] = 0;
+    } else {
+      const double xi = x[i], zi = z[i], ui = u[i], wi = w[i];
+      double h = 0.0;
+      const uint32_t m = nb.count[i];
+      for (uint32_t q = 0; q < m; q++) {
+        const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
+        const double dx = x[j] - xi, dz = z[j] - zi;
+        const double r2 = dx * dx + dz * dz;
+        if (r2 < k.re2) {
+          const double r = sqrt(r2);
+          const double du = u[j] - ui, dw = w[j] - wi;
+          h += (dx * du + dz * dw) / (r * r * r);
+        }
+        if (r2 < k.rl2 && type[j] != kDummy) cnt++;
+      }
+      const double speed = (nws[i] < k.n0) ? 0.0 : -k.re * h;  // HS, :838-872
+      b[i] = -k.rho / (k.n0 * k.dt) * (speed + ecs[i]);
+      sol[i] = (k.warm == MPS_WARM_ZERO) ? 0.0 : p[i];
+    }
+    row_len[i] = cnt;
+  }
+  const unsigned wmax = wave_max_u32(cnt);
+  const unsigned long long wsum = static_cast<unsigned long long>(wave_sum(static_cast<double>(cnt)));
+  if ((threadIdx.x & 63) == 0) {
+    const size_t s = (blk * static_cast<size_t>(blockDim.x) + threadIdx.x) >> 6;
+    if (s * 64 < n) {
+      slice_width[s] = wmax;
+      atomicAdd(nnz_total, wsum);
+    }
+  }
+}
+
+// HL off-diagonals and diagonal (Computer.hpp:1232-1328).
+template <bool F32>
+__global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ z,
+                                                     const uint8_t* __restrict__ type, NV nb,
+                                                     PpeConsts k, const double* __restrict__ nws,
+                                                     const uint64_t* __restrict__ slice_off,
+                                                     const uint32_t* __restrict__ slice_width,
+                                                     double* __restrict__ val, uint32_t* __restrict__ col,
+                                                     double* __restrict__ diag, float* __restrict__ val32,
+                                                     float* __restrict__ diag32) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const size_t base = slice_off[i >> 6] + (i & 63);
+  const uint32_t width = slice_width[i >> 6];
+  const uint32_t self = static_cast<uint32_t>(i);
+  const uint8_t ti = type[i];
+  uint32_t kk = 0;
+  double d;
+  if (ti == kDummy || ti == kDisabled) {
+    d = 1.0;
+  } else {
+    const double xi = x[i], zi = z[i];
+    double s = 0.0, sx = 0, sz = 0;
+    const uint32_t m = nb.count[i];
+    for (uint32_t q = 0; q < m; q++) {
+      const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
+      const double dx = x[j] - xi, dz = z[j] - zi;
+      const double r2 = dx * dx + dz * dz;
+      if (r2 < k.rl2 && type[j] != kDummy) {
+        const double r = sqrt(r2);
+        const double a = 3 * k.rl / k.n0l / (r * r * r);
+        const size_t o = base + static_cast<size_t>(kk) * 64;
+        val[o] = a;
+        col[o] = j;
+        if (F32) val32[o] = static_cast<float>(a);
+        kk++;
+        s += -a;
+      }
+      if (r2 < k.re2) {
+        const double wg = weight(sqrt(r2), k.re);
+        sx += wg * dx;
+        sz += wg * dz;
+      }
+    }
+    double gxp, gzp;
+    if (ghost_pos(xi, zi, sx, sz, nws[i], k.n0, k.re, gxp, gzp)) {
+      const double rx = xi - gxp, rz = zi - gzp;
+      const double r = sqrt(rx * rx + rz * rz);
+      const double a = 3 * k.rl / k.n0l / (r * r * r);
+      s += -a;
+    }
+    d = s;
+  }
+  for (; kk < width; kk++) {
+    const size_t o = base + static_cast<size_t>(kk) * 64;
+    val[o] = 0.0;
+    col[o] = self;
+    if (F32) val32[o] = 0.0f;
+  }
+  diag[i] = d;
+  if (F32) diag32[i] = static_cast<float>(d);
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_f32(size_t n, const double* __restrict__ a,
+                                                   float* __restrict__ b) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) b[i] = static_cast<float>(a[i]);
+}
+
+}  // namespace
+
+void ensure_sell_capacity(Context& c, size_t nslices, size_t slots) {
+  if (slots > c.sell_cap) {
+    dfree(c.sval);
+    dfree(c.scol);
+    dfree(c.sval32);
+    const size_t cap = slots + slots / 8 + 64;
+    c.sval = dalloc<double>(cap);
+    c.scol = dalloc<uint32_t>(cap);
+    c.sell_cap = cap;
+    c.have_ppe32 = false;
+  }
+  (void)nslices;
+}
+
+static bool want_fp32(const Context& c) { return c.solver.kind == MPS_SOLVER_MIXED_IR; }
+
+void ppe_assemble(Context& c) {
+  const size_t n = c.n;
+  c.ppe_n = n;
+  c.nslices = (n + 63) / 64;
+  c.have_ppe = true;
+  c.have_ppe32 = false;
+  if (n == 0) return;
+  // worst case: every row as long as the longest neighbour list
+  ensure_sell_capacity(c, c.nslices, c.nslices * 64 * static_cast<size_t>(c.kmax));
+  const bool f32 = want_fp32(c);
+  if (f32 && !c.sval32) c.sval32 = dalloc<float>(c.sell_cap);
+  PpeConsts k{c.env.r_e, c.env.r_e * c.env.r_e, c.env.r_lap, c.env.r_lap * c.env.r_lap, c.env.n0,
+              c.env.n0_lap, c.env.rho, c.dt, c.solver.warm_start};
+  const NV nb{c.nbr, c.ncount, c.kmax};
+  MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 2 * sizeof(unsigned long long), c.stream));
+  tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 20.0, [&] {
+    hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
+                       c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                       static_cast<const double*>(c.ecs), c.b, c.sol, c.row_len, c.slice_width,
+                       c.ppe_counters);
+  });
+  check_launch();
+  scan_slices(c, c.slice_width, c.nslices, c.slice_off);
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host, c.ppe_counters, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 1, c.slice_off + c.nslices, sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.nnz_offdiag = c.ppe_counters_host[0];
+  c.sell_slots = c.ppe_counters_host[1];
+  const double bytes = n * (16.0 + 1 + 8 + 4 + 8) + 4.0 * c.nbr_total +
+                       c.sell_slots * (12.0 + (f32 ? 4.0 : 0.0)) + n * (f32 ? 4.0 : 0.0);
+  tracked(c, MPS_K_PPE_FILL, bytes, [&] {
+    if (f32)
+      hipLaunchKernelGGL(k_ppe_fill<true>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
+                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                         static_cast<const uint64_t*>(c.slice_off),
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.sval32,
+                         c.diag32);
+    else
+      hipLaunchKernelGGL(k_ppe_fill<false>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
+                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                         static_cast<const uint64_t*>(c.slice_off),
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag,
+                         static_cast<float*>(nullptr), static_cast<float*>(nullptr));
+  });
+  check_launch();
+  c.have_ppe32 = f32;
+}
+
+// FP32 copies of an already assembled system (the mixed solver on a system
+// installed through mps_set_ppe, or after switching solvers).
+void ppe_make_fp32(Context& c) {
+  if (c.have_ppe32 || c.ppe_n == 0) return;
+  if (!c.sval32) c.sval32 = dalloc<float>(c.sell_cap);
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(c.sell_slots)), dim3(kBlock), 0, c.stream, c.sell_slots,
+                     static_cast<const double*>(c.sval), c.sval32);
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(c.ppe_n)), dim3(kBlock), 0, c.stream, c.ppe_n,
+                     static_cast<const double*>(c.diag), c.diag32);
+  check_launch();
+  c.have_ppe32 = true;
+}
+
+// CSR (any column order; diagonal entries may be absent) -> device SELL-64.
+void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,
+                    const double* val, const double* b, const double* x0) {
+  const size_t ns = (n + 63) / 64;
+  std::vector<double> dg(n, 0.0);
+  std::vector<uint32_t> len(n, 0), width(ns, 0);
+  for (size_t i = 0; i < n; i++) {
+    for (uint64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      if (col[k] >= n) throw Failure{MPS_ERR_BAD_ARG, "column index out of range"};
+      if (col[k] == i) dg[i] += val[k];
+      else len[i]++;
+    }
+    width[i / 64] = std::max(width[i / 64], len[i]);
+  }
+  std::vector<uint64_t> off(ns + 1, 0);
+  for (size_t s = 0; s < ns; s++) off[s + 1] = off[s] + 64ull * width[s];
+  const size_t slots = off[ns];
+  std::vector<double> sv(slots, 0.0);
+  std::vector<uint32_t> sc(slots, 0);
+  for (size_t i = 0; i < n; i++) {
+    const size_t base = off[i / 64] + (i % 64);
+    uint32_t kk = 0;
+    for (uint64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      if (col[k] == i) continue;
+      sv[base + 64ull * kk] = val[k];
+      sc[base + 64ull * kk] = col[k];
+      kk++;
+    }
+    for (; kk < width[i / 64]; kk++) sc[base + 64ull * kk] = static_cast<uint32_t>(i);
+  }
+  ensure_ppe_vectors(c, n);
+  ensure_sell_capacity(c, ns, slots);
+  MPS_HIP(hipMemcpyAsync(c.sval, sv.data(), slots * 8, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.scol, sc.data(), slots * 4, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.diag, dg.data(), n * 8, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.row_len, len.data(), n * 4, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.slice_width, width.data(), ns * 4, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.slice_off, off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.b, b, n * 8, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.sol, x0, n * 8, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.ppe_n = n;
+  c.nslices = ns;
+  c.sell_slots = slots;
+  uint64_t nnz = 0;
+  for (size_t i = 0; i < n; i++) nnz += len[i];
+  c.nnz_offdiag = nnz;
+  c.have_ppe = true;
+  c.have_ppe32 = false;
+}
+
+// Device SELL-64 -> CSR with ascending columns, diagonal included.
+void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uint32_t>& col,
+                      std::vector<double>& val) {
+  const size_t n = c.ppe_n, ns = c.nslices;
+  std::vector<double> dg(n), sv(c.sell_slots);
+  std::vector<uint32_t> len(n), sc(c.sell_slots);
+  std::vector<uint64_t> off(ns + 1);
+  MPS_HIP(hipMemcpyAsync(dg.data(), c.diag, n * 8, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(len.data(), c.row_len, n * 4, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(off.data(), c.slice_off, (ns + 1) * 8, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(sv.data(), c.sval, c.sell_slots * 8, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(sc.data(), c.scol, c.sell_slots * 4, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  rowptr.assign(n + 1, 0);
+  col.clear();
+  val.clear();
+  std::vector<std::pair<uint32_t, double>> row;
+  for (size_t i = 0; i < n; i++) {
+    row.clear();
+    const size_t base = off[i / 64] + (i % 64);
+    for (uint32_t k = 0; k < len[i]; k++) row.emplace_back(sc[base + 64ull * k], sv[base + 64ull * k]);
+    row.emplace_back(static_cast<uint32_t>(i), dg[i]);
+    std::stable_sort(row.begin(), row.end(),
+                     [](const std::pair<uint32_t, double>& a, const std::pair<uint32_t, double>& b) {
+                       return a.first < b.first;
+                     });
+    for (auto& e : row) {
+      col.push_back(e.first);
+      val.push_back(e.second);
+    }
+    rowptr[i + 1] = col.size();
+  }
+}
+
+}  // namespace mps

@@ -1,0 +1,288 @@
+// Neighbour search: Computer::SearchNeighbor (Computer.hpp:698-756) over the
+// uniform grid of Grid.hpp, rebuilt for the GPU.
+//
+// The reference inserts particles serially into fixed-capacity cells and then
+// walks, for each particle, the 9 surrounding cells in x-major order
+// (Grid.hpp:393-401) with ascending particle index inside a cell.  Here:
+//   1. k_cell_keys     cell key per particle (out-of-grid -> Disabled) and an
+//                      atomic per-cell histogram, which also detects the
+//                      "Too many particle in a block" overflow (Grid.hpp:314-318);
+//   2. exclusive scan  of the histogram -> cell start offsets;
+//   3. k_cell_scatter  counting-sort scatter (a one-digit radix sort whose digit
+//                      is the whole cell key);
+//   4. k_cell_sort     orders each cell's few entries by particle index, which
+//                      restores exactly the serial insertion order;
+//   5. k_neighbors     one thread per particle walks the 9 cells in the
+//                      reference order and writes its list into the sliced
+//                      (wavefront-coalesced) neighbour layout.
+#include "mps_device.hpp"
+#include "mps_internal.hpp"
+
+namespace mps {
+namespace {
+
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+struct GridParams {
+  double min_x, min_z, nl;
+  long long gnx, gnz;
+  uint32_t cap;
+};
+
+// Grid::Block (Grid.hpp:250-254) with the range test of Grid::Store
+// (Grid.hpp:278-290).  The floored quotient is range-checked as a double, so
+// NaN / inf / huge coordinates fall outside the grid as on the reference host.
+__device__ __forceinline__ bool cell_of(const GridParams& g, double px, double pz, long long& bx,
+                                        long long& bz) {
+  const double fx = floor((px - g.min_x) / g.nl);
+  const double fz = floor((pz - g.min_z) / g.nl);
+  if (!(fx >= 0 && fx < static_cast<double>(g.gnx) && fz >= 0 && fz < static_cast<double>(g.gnz)))
+    return false;
+  bx = static_cast<long long>(fx);
+  bz = static_cast<long long>(fz);
+  return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_keys(GridParams g, size_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ z, uint8_t* __restrict__ type,
+                                                      uint32_t* __restrict__ key, uint32_t* cell_count,
+                                                      int* flags) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = kNoKey;
+  if (type[i] != kDisabled) {
+    long long bx, bz;
+    if (cell_of(g, x[i], z[i], bx, bz)) {
+      k = static_cast<uint32_t>(bx * g.gnz + bz);
+      const uint32_t l = atomicAdd(&cell_count[k], 1u);
+      if (l >= g.cap) flags[0] = 1;
+    } else {
+      type[i] = kDisabled;  // Computer.hpp:711-715
+    }
+  }
+  key[i] = k;
+}
+
+// ---- exclusive scan of u32 counts (x mult) into u32 or u64 offsets --------------
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = 256 * kScanItems;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
+                                                     uint64_t* __restrict__ bsum) {
+  __shared__ uint64_t s[4];
+  const size_t base = blockIdx.x * static_cast<size_t>(kScanChunk) + threadIdx.x * kScanItems;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (base + k < n) acc += in[base + k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+// Inclusive scan of one value per thread across a 256-thread block.
+__device__ __forceinline__ uint64_t block_inclusive_scan(uint64_t v, uint64_t* lds4) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) lds4[wv] = v;
+  __syncthreads();
+  uint64_t off = 0;
+  for (int k = 0; k < wv; k++) off += lds4[k];
+  __syncthreads();
+  return v + off;
+}
+
+// Single block: exclusive scan of the block sums in place, grand total to *total.
+__global__ void __launch_bounds__(256) k_scan_top(uint64_t* bsum, int nb, uint64_t* total) {
+  __shared__ uint64_t lds4[4];
+  __shared__ uint64_t tot;
+  uint64_t carry = 0;
+  for (int base = 0; base < nb; base += 256) {
+    const int idx = base + threadIdx.x;
+    const uint64_t v = (idx < nb) ? bsum[idx] : 0;
+    const uint64_t inc = block_inclusive_scan(v, lds4);
+    if (idx < nb) bsum[idx] = carry + inc - v;
+    if (threadIdx.x == 255) tot = inc;
+    __syncthreads();
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <class TO>
+__global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* __restrict__ in, size_t n,
+                                                    const uint64_t* __restrict__ bsum,
+                                                    TO* __restrict__ out, uint64_t mult) {
+  __shared__ uint64_t lds4[4];
+  const size_t base = blockIdx.x * static_cast<size_t>(kScanChunk) + threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = (base + k < n) ? static_cast<uint64_t>(in[base + k]) * mult : 0;
+    acc += v[k];
+  }
+  const uint64_t inc = block_inclusive_scan(acc, lds4);
+  uint64_t run = bsum[blockIdx.x] + inc - acc;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n) out[base + k] = static_cast<TO>(run);
+    run += v[k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_scatter(size_t n, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ cell_start,
+                                                         uint32_t* cell_fill,
+                                                         uint32_t* __restrict__ sorted) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = key[i];
+  if (k == kNoKey) return;
+  const uint32_t slot = atomicAdd(&cell_fill[k], 1u);
+  sorted[cell_start[k] + slot] = static_cast<uint32_t>(i);
+}
+
+// Restores ascending particle index inside each cell (the reference inserts
+// serially in index order, Computer.hpp:704-717).  Overflowed cells are left
+// alone: the search reports an error for them.
+__global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t cap,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ cell_count,
+                                                      uint32_t* __restrict__ sorted) {
+  const size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (c >= ncells) return;
+  const uint32_t cnt = cell_count[c];
+  if (cnt < 2 || cnt > cap) return;
+  uint32_t* s = sorted + cell_start[c];
+  for (uint32_t a = 1; a < cnt; a++) {
+    const uint32_t v = s[a];
+    uint32_t b = a;
+    while (b > 0 && s[b - 1] > v) {
+      s[b] = s[b - 1];
+      b--;
+    }
+    s[b] = v;
+  }
+}
+
+// Computer::SearchNeighbor's per-particle walk, Computer.hpp:720-755: the 9
+// cells in Grid.hpp:393-401 order, j != i, R(x_i, x_j) < NeighborLength.
+// Disabled particles keep their previous list, as in the reference.
+__global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ z,
+                                                      const uint8_t* __restrict__ type,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ cell_count,
+                                                      const uint32_t* __restrict__ sorted, uint32_t kmax,
+                                                      uint32_t* __restrict__ nbr,
+                                                      uint32_t* __restrict__ ncount, int* flags,
+                                                      unsigned long long* total) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  uint32_t cnt = 0;
+  if (i < n && type[i] != kDisabled) {
+    const double xi = x[i], zi = z[i];
+    long long bx = 0, bz = 0;
+    cell_of(g, xi, zi, bx, bz);
+    const double nl = g.nl;
+    for (int ox = -1; ox <= 1; ox++) {
+      const long long cx = bx + ox;
+      if (cx < 0 || cx >= g.gnx) continue;
+      for (int oz = -1; oz <= 1; oz++) {
+        const long long cz = bz + oz;
+        if (cz < 0 || cz >= g.gnz) continue;
+        const size_t c = static_cast<size_t>(cx * g.gnz + cz);
+        const uint32_t s = cell_start[c], m = cell_count[c];
+        for (uint32_t l = 0; l < m; l++) {
+          const uint32_t j = sorted[s + l];
+          if (j == i) continue;
+          const double rx = xi - x[j], rz = zi - z[j];
+          const double r = sqrt(rx * rx + rz * rz);  // Computer::R, Computer.hpp:568-572
+          if (r < nl) {
+            if (cnt < kmax) nbr[nbr_slot(static_cast<uint32_t>(i), cnt, kmax)] = j;
+            cnt++;
+          }
+        }
+      }
+    }
+    if (cnt > kmax) {
+      flags[1] = 1;
+      cnt = kmax;
+    }
+    ncount[i] = cnt;
+  }
+  const double ws = wave_sum(static_cast<double>(cnt));
+  if ((threadIdx.x & 63) == 0 && ws > 0) atomicAdd(total, static_cast<unsigned long long>(ws));
+}
+
+}  // namespace
+
+// Exclusive scan out[k] = sum_{m<k} in[m] * mult over n entries; optional total.
+template <class TO>
+static void scan_u32(Context& c, const uint32_t* in, size_t n, TO* out, uint64_t mult, uint64_t* total) {
+  const int nb = static_cast<int>((n + kScanChunk - 1) / kScanChunk);
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c.stream, in, n, c.scan_tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c.stream, c.scan_tmp, nb, total);
+  hipLaunchKernelGGL(k_scan_apply<TO>, dim3(nb), dim3(256), 0, c.stream, in, n,
+                     static_cast<const uint64_t*>(c.scan_tmp), out, mult);
+}
+
+void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* offsets) {
+  scan_u32<uint64_t>(c, widths, nslices, offsets, 64, offsets + nslices);
+}
+
+void search(Context& c) {
+  const size_t n = c.n;
+  GridParams g{c.env.min_x, c.env.min_z, c.env.neighbor_length, c.gnx, c.gnz, c.cap};
+  MPS_HIP(hipMemsetAsync(c.cell_count, 0, c.ncells * sizeof(uint32_t), c.stream));
+  MPS_HIP(hipMemsetAsync(c.cell_fill, 0, c.ncells * sizeof(uint32_t), c.stream));
+  MPS_HIP(hipMemsetAsync(c.flags, 0, 4 * sizeof(int), c.stream));
+  MPS_HIP(hipMemsetAsync(c.nbr_total_dev, 0, sizeof(unsigned long long), c.stream));
+  if (n == 0) {
+    c.have_neighbors = true;
+    return;
+  }
+  tracked(c, MPS_K_CELL_KEYS, 26.0 * n, [&] {
+    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
+                       c.type, c.key, c.cell_count, c.flags);
+  });
+  tracked(c, MPS_K_CELL_SCAN, 8.0 * c.ncells, [&] {
+    scan_u32<uint32_t>(c, c.cell_count, c.ncells, c.cell_start, 1, nullptr);
+  });
+  tracked(c, MPS_K_CELL_SCATTER, 16.0 * n, [&] {
+    hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       static_cast<const uint32_t*>(c.key), static_cast<const uint32_t*>(c.cell_start),
+                       c.cell_fill, c.sorted);
+  });
+  tracked(c, MPS_K_CELL_SORT, 8.0 * n + 8.0 * c.ncells, [&] {
+    hipLaunchKernelGGL(k_cell_sort, dim3(grid_for(c.ncells)), dim3(kBlock), 0, c.stream, c.ncells,
+                       c.cap, static_cast<const uint32_t*>(c.cell_start),
+                       static_cast<const uint32_t*>(c.cell_count), c.sorted);
+  });
+  tracked(c, MPS_K_NEIGHBORS, 25.0 * n, [&] {
+    hipLaunchKernelGGL(k_neighbors, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
+                       static_cast<const uint8_t*>(c.type), static_cast<const uint32_t*>(c.cell_start),
+                       static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.sorted),
+                       c.kmax, c.nbr, c.ncount, c.flags, c.nbr_total_dev);
+  });
+  check_launch();
+  MPS_HIP(hipMemcpyAsync(c.flags_host, c.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.nbr_total_host, c.nbr_total_dev, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.nbr_total = *c.nbr_total_host;
+  c.have_neighbors = true;
+  if (c.flags_host[0] || c.flags_host[1])
+    throw Failure{MPS_ERR_CELL_OVERFLOW, "Too many particle in a block"};
+}
+
+}  // namespace mps

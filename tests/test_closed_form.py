@@ -1,0 +1,155 @@
+"""Closed-form checks for the terms the reference's own tests leave open: the
+SPP ghost particle (Computer.hpp:664-693), the HL viscosity (:963-965, its
+tests are disabled in test_ComputerExplicitForces.cpp:115-247), the PPE
+diagonal with the ghost (:1290-1292), the midpoint pressure gradient with the
+ghost (:1501-1521) and the DS repulsion (:1576-1635).
+
+Two fluid particles on the x axis: every stage reduces to one pair term plus
+the ghost, whose position follows by hand: particle i sees j at distance d,
+nWithoutSpp = W(d) < n0, the centroid offset points at j, so the ghost sits
+at distance r_e / (n0 - W(d) + 1) on the far side of i.  The expected values
+below are computed from those formulas, not from the implementation.
+Also: neighbour lists against an independent brute-force cell walk (order
+included), and the PPE solve against a direct sparse solve.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse
+import scipy.sparse.linalg
+
+import cases
+from openmps_amd import scenes
+
+KINDS = ["oracle", pytest.param("gpu", marks=pytest.mark.gpu)]
+
+L0 = 0.01
+ENV = cases.env(1e-3, 0.1, 9.8, 998.2, 1e-6, 2.1, L0, -0.2, -0.2, 0.2, 0.2)
+DT = 1e-3
+
+
+def _pair(kind, d, u1=0.0, p0=0.0, p1=0.0):
+    c = cases.make(kind, ENV, 1e-10)
+    c.set_time(DT, DT)
+    s = scenes._pack([0.0, d], [0.0, 0.0], [0, 0])
+    s["u"] = np.array([0.0, u1])
+    s["p"] = np.array([p0, p1])
+    c.add_particles(s)
+    c.stage("search")
+    c.stage("density")
+    return c
+
+
+def _hand(d):
+    re = 2.1 * L0
+    n0 = cases.make("oracle", ENV, 1e-10).env()["n0"]
+    wd = re / d - 1
+    rspp = re / (n0 - wd + 1)
+    return re, n0, wd, rspp
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pair_density_and_ghost_matrix(kind):
+    d = 1.0 * L0
+    c = _pair(kind, d)
+    re, n0, wd, rspp = _hand(d)
+    assert np.allclose(c.aux("n_without_spp"), [wd, wd], rtol=1e-14)
+    assert np.allclose(c.particles()["n"], [n0, n0], rtol=1e-14)
+    c.stage("ppe_assemble")
+    rp, col, val, b, x = c.ppe()
+    A = cases.dense(rp, col, val, 2)
+    a = 3 * re / n0 / d ** 3
+    ag = 3 * re / n0 / rspp ** 3
+    assert A[0, 1] == pytest.approx(a, rel=1e-13) and A[1, 0] == pytest.approx(a, rel=1e-13)
+    assert A[0, 0] == pytest.approx(-(a + ag), rel=1e-12)
+    assert A[1, 1] == pytest.approx(-(a + ag), rel=1e-12)
+    assert np.all(b == 0.0)  # HS is zero below n0, ECS is zero at n = n0
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pair_viscosity(kind):
+    d, U = 1.0 * L0, 0.3
+    c = _pair(kind, d, u1=U)
+    c.stage("explicit")
+    re, n0, _, _ = _hand(d)
+    cvis = 1e-6 * 3 * re / n0 / d ** 3
+    assert c.aux("du_x") == pytest.approx([cvis * U, -cvis * U], rel=1e-13)
+    assert c.aux("du_z") == pytest.approx([-9.8, -9.8], rel=1e-15)
+    P = c.particles()
+    assert P["u"][0] == pytest.approx(cvis * U * DT, rel=1e-13)
+    assert P["w"][0] == pytest.approx(-9.8 * DT, rel=1e-15)
+    assert P["z"][0] == pytest.approx(-9.8 * DT * DT, rel=1e-15)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pair_pressure_gradient_with_ghost(kind):
+    d, P0, P1 = 1.0 * L0, 100.0, 40.0
+    c = _pair(kind, d, p0=P0, p1=P1)
+    c.stage("pressure_gradient")
+    re, n0, wd, rspp = _hand(d)
+    w_g = re / rspp - 1
+    coef = -DT / 998.2 * 2 / n0
+    s0 = (P1 + P0) / d ** 2 * wd * d + P0 / rspp ** 2 * w_g * (-rspp)
+    s1 = (P0 + P1) / d ** 2 * wd * (-d) + P1 / rspp ** 2 * w_g * rspp
+    assert c.aux("du_x") == pytest.approx([coef * s0, coef * s1], rel=1e-11)
+    assert np.all(c.aux("du_z") == 0.0)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pair_dynamic_stabilize(kind):
+    d = 0.8 * L0  # closer than l0 (1 - C) = 0.9 l0
+    c = _pair(kind, d)
+    c.stage("save_x")
+    c.stage("dynamic_stabilize")
+    n0 = _hand(d)[1]
+    dd = L0 - 0.1 * L0
+    m = math.sqrt(dd * dd) - d
+    coef = -1.0 / (2 * DT * n0)
+    assert c.aux("du_x") == pytest.approx([coef * m, -coef * m], rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_neighbors_brute_force(kind):
+    """Lists (values and order) equal an independent cell walk on a jittered
+    dam break, 2.2k particles."""
+    cfg = scenes.config("dambreak_1k")
+    s, _ = scenes.dam_break(1.0, 0.6, 0.25, 0.5, 0.01, jitter=0.2, seed=7)
+    c = cases.make(kind, cfg["env"], cfg["eps"])
+    c.add_particles(s)
+    c.stage("search")
+    cnt, ids = c.neighbors()
+    g = c.grid()
+    nl = c.env()["neighbor_length"]
+    lists, ok = cases.brute_neighbors(s, nl, cfg["env"], (g["nx"], g["nz"]))
+    off = np.concatenate([[0], np.cumsum(cnt.astype(np.int64))])
+    for i, li in enumerate(lists):
+        if li is None:
+            continue
+        assert ids[off[i]:off[i + 1]].tolist() == li
+    # set check against all pairs (O(N^2)) for the radius rule
+    x, z = s["x"], s["z"]
+    r = np.sqrt((x[:, None] - x[None, :]) ** 2 + (z[:, None] - z[None, :]) ** 2)
+    np.fill_diagonal(r, np.inf)
+    assert np.array_equal(cnt, (r < nl).sum(axis=1).astype(np.uint32))
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_ppe_solve_direct(kind):
+    """The CG solution of the first dam-break step equals a direct sparse
+    solve of the same system within the conditioning of the tolerance."""
+    cfg = scenes.config("dambreak_1k")
+    c = cases.make(kind, cfg["env"], cfg["eps"])
+    c.add_particles(cfg["particles"])
+    c.set_time(cfg["dt"], cfg["dt"])
+    for st in ("search", "density", "error_correction", "explicit", "density", "save_x",
+               "ppe_assemble"):
+        c.stage(st)
+    rp, col, val, b, x0 = c.ppe()
+    c.stage("ppe_solve")
+    x = c.ppe()[4]
+    n = len(b)
+    A = scipy.sparse.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    xd = scipy.sparse.linalg.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b - A @ x0)
