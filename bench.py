@@ -75,6 +75,7 @@ def cpu_baseline(cfg, tol: float, budget_s: float):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.Oracle(cfg["env"], cfg["eps"])
     o.set_stop_rule(1, tol)
+    o.set_verbose(True)
     o.add_particles(cfg["particles"])
     n = o.size()
     t0 = time.perf_counter()

@@ -68,13 +68,13 @@ constexpr int kScanItems = 8;
 constexpr int kScanChunk = 256 * kScanItems;
 
 __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
-                                                     uint64_t* __restrict__ bsum) {
+                                                     uint64_t* __restrict__ bsum, uint64_t mult) {
   __shared__ uint64_t s[4];
   const size_t base = blockIdx.x * static_cast<size_t>(kScanChunk) + threadIdx.x * kScanItems;
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++)
-    if (base + k < n) acc += in[base + k];
+    if (base + k < n) acc += static_cast<uint64_t>(in[base + k]) * mult;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
@@ -230,7 +230,7 @@ template <class TO>
 static void scan_u32(Context& c, const uint32_t* in, size_t n, TO* out, uint64_t mult, uint64_t* total) {
   const int nb = static_cast<int>((n + kScanChunk - 1) / kScanChunk);
   if (nb == 0) return;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c.stream, in, n, c.scan_tmp);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c.stream, in, n, c.scan_tmp, mult);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, c.stream, c.scan_tmp, nb, total);
   hipLaunchKernelGGL(k_scan_apply<TO>, dim3(nb), dim3(256), 0, c.stream, in, n,
                      static_cast<const uint64_t*>(c.scan_tmp), out, mult);

@@ -83,6 +83,7 @@ def lib():
         L.oracle_set_ppe.argtypes = [_P, _SZ, _P, _P, _P, _P, _P]
         L.oracle_cg_stats.argtypes = [_P, _P]
         L.oracle_set_stop_rule.argtypes = [_P, ctypes.c_int, _D]
+        L.oracle_set_verbose.argtypes = [_P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -245,6 +246,9 @@ class Oracle:
     def set_stop_rule(self, rule: int, tol: float) -> None:
         """0: the reference rule (relative to the warm start); 1: |b-Ax| <= tol |b|."""
         lib().oracle_set_stop_rule(self._h, rule, tol)
+
+    def set_verbose(self, on: bool) -> None:
+        lib().oracle_set_verbose(self._h, int(on))
 
     def cg_stats(self) -> dict:
         out = np.zeros(3)

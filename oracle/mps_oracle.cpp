@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdio>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -129,6 +130,7 @@ class Oracle {
   std::vector<double> val, b, sol;
   long cg_iters = -1;
   double cg_rr0 = 0, cg_rr = 0;
+  int verbose = 0;        // 1: progress lines on stderr from the CG loop
   int stop_rule = 0;      // 0: r.r < eps^2 r0.r0 (reference); 1: |r| <= tol |b|
   double stop_tol = 0;    // tolerance of rule 1
   std::string err;
@@ -469,6 +471,10 @@ class Oracle {
       const double rrNew = det_dot(r, r);
       it++;
       conv = (stop_rule == 1) ? (rrNew <= residual0) : (rrNew < residual0);
+      if (verbose && it % 500 == 0) {
+        std::fprintf(stderr, "    oracle CG iteration %ld, r.r / target %.3e\n", it, rrNew / residual0);
+        std::fflush(stderr);
+      }
       if (!conv) {
         const double beta = rrNew / rr;
 #pragma omp parallel for schedule(static)
@@ -754,6 +760,7 @@ void oracle_set_ppe(void* h, size_t n, const uint64_t* rowptr, const uint32_t* c
   o->sol.assign(x0, x0 + n);
 }
 // Extension used by the CPU baseline: rule 1 stops at |b - A x| <= tol |b|.
+void oracle_set_verbose(void* h, int on) { static_cast<Oracle*>(h)->verbose = on; }
 void oracle_set_stop_rule(void* h, int rule, double tol) {
   static_cast<Oracle*>(h)->stop_rule = rule;
   static_cast<Oracle*>(h)->stop_tol = tol;
