@@ -128,26 +128,35 @@ def main() -> int:
 
     solve = {args.solver: {"ms_per_step": solve_ms, "cg_iterations": iters,
                            "rel_residual": max(s["rel_residual"] for s in st)}}
+    kt64 = kt if args.solver == "fp64" else {}
     if not args.no_compare and args.solver != "fp64":
         ref_c = make_computer(cfg, "fp64", args.tol)
         ref_c._bench_dt = cfg["dt"]
         if args.warmup:
             run(ref_c, args.warmup, "fp64 warmup")
+        ref_c.reset_kernel_stats()
+        ref_c.set_profiling(True)
         _, st64 = run(ref_c, args.steps, "fp64 timed")
+        ref_c.set_profiling(False)
+        kt64 = kernel_table(ref_c)
         solve["fp64"] = {"ms_per_step": sum(s["solve_ms"] for s in st64) / len(st64),
                          "cg_iterations": sum(s["iterations"] for s in st64) / len(st64),
                          "rel_residual": max(s["rel_residual"] for s in st64)}
         del ref_c
 
+    def roof(table, name):
+        d = table.get(name)
+        if not d:
+            return None
+        return {"kernel": name, "bound": "hbm", "achieved": round(d["GBps"], 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": d["bytes_per_launch"],
+                "ms_per_launch": d["ms_per_launch"]}
+
     # dominant kernel: the SpMV of the solver that ran the timed steps
     dom = "cg_spmv32" if args.solver.startswith("mixed") else "cg_spmv"
-    d = kt.get(dom)
-    roofline = None
-    if d:
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(d["GBps"], 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                    "traffic": None, "bytes_per_launch": d["bytes_per_launch"],
-                    "ms_per_launch": d["ms_per_launch"]}
+    roofline = roof(kt, dom)
+    roofline_fp64 = roof(kt64, "cg_spmv")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -172,8 +181,12 @@ def main() -> int:
         "solve_speedup_vs_fp64": (solve["fp64"]["ms_per_step"] / solve_ms
                                   if "fp64" in solve and args.solver != "fp64" else None),
         "roofline": roofline,
+        "roofline_fp64_spmv": roofline_fp64,
         "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
                         "launches": v["launches"]} for k, v in kt.items()},
+        "kernels_fp64": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
+                             "launches": v["launches"]} for k, v in kt64.items()
+                         if k.startswith("cg_")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -207,6 +207,11 @@ int mps_set_profiling(mps_ctx* ctx, int on);
 int mps_get_kernel_stat(mps_ctx* ctx, int id, mps_kernel_stat* out);
 int mps_reset_kernel_stats(mps_ctx* ctx);
 
+/* Copies `bytes` bytes between two device buffers with a plain streaming
+ * kernel and returns its duration: the known-size reference for scaling HBM
+ * traffic counters (FETCH_SIZE / WRITE_SIZE) to bytes. */
+int mps_device_copy_probe(size_t bytes, double* ms_out);
+
 /* Device arithmetic self-check: out_sqrt[i] = sqrt(a[i]), out_div[i] = a[i]/b[i]
  * computed by the same code the kernels use (pins rounding against the host). */
 int mps_device_math_check(size_t n, const double* a, const double* b, double* out_sqrt,

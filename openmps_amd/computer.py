@@ -380,3 +380,10 @@ def device_math_check(a, b):
     _raise(L.lib().mps_device_math_check(len(a), a.ctypes.data, b.ctypes.data, s.ctypes.data,
                                          d.ctypes.data), None)
     return s, d
+
+
+def device_copy_probe(nbytes: int) -> float:
+    """Duration (ms) of a plain device-to-device copy kernel moving nbytes."""
+    out = ctypes.c_double()
+    _raise(L.lib().mps_device_copy_probe(nbytes, ctypes.byref(out)), None)
+    return out.value

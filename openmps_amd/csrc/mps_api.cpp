@@ -690,6 +690,17 @@ int mps_reset_kernel_stats(mps_ctx* ctx) {
   });
 }
 
+int mps_device_copy_probe(size_t bytes, double* ms_out) {
+  try {
+    const double ms = mps::device_copy_probe(bytes);
+    if (ms_out) *ms_out = ms;
+  } catch (Failure& f) {
+    std::fprintf(stderr, "mps_device_copy_probe: %s\n", f.msg.c_str());
+    return f.code;
+  }
+  return MPS_OK;
+}
+
 int mps_device_math_check(size_t n, const double* a, const double* b, double* out_sqrt, double* out_div) {
   try {
     mps::device_math_check(n, a, b, out_sqrt, out_div);

@@ -13,6 +13,13 @@
 // blocks agree bitwise on alpha, beta and the stop decision without atomics or
 // grid-wide fences.  The host enqueues iterations in chunks and reads the
 // device status once per chunk; kernels after convergence exit immediately.
+//
+// Launch shape: at most 1024 blocks of 512 threads (8 waves per SIMD on 256
+// CUs); block b owns one contiguous row range, and blocks are remapped so that
+// each XCD walks a contiguous eighth of the rows (XCD-local L2 hits on the
+// gathered vector entries).  The SELL row sum keeps four slots in flight per
+// lane.  These kernels use explicit FMA: the CG is checked against the
+// tolerance, not bit for bit.
 #include <algorithm>
 #include <cmath>
 
@@ -22,6 +29,8 @@
 namespace mps {
 namespace {
 
+constexpr int NT = 512;  // threads per block of the solver kernels
+
 template <class T>
 struct Sell {
   const uint64_t* __restrict__ off;
@@ -30,7 +39,7 @@ struct Sell {
   const uint32_t* __restrict__ col;
   const T* __restrict__ diag;
   size_t n;
-  size_t rpb;  // rows per block (multiple of 256)
+  size_t rpb;  // rows per block (multiple of NT)
 };
 
 struct Rows {
@@ -53,29 +62,49 @@ __device__ __forceinline__ bool already_converged(const CgStatus* st) {
   return s_conv != 0;
 }
 
+// s + sum_k val[k] * v(col[k]) over the stored off-diagonal slots of row i,
+// four independent slots (loads, gathers) in flight per iteration.
+template <class T, class V>
+__device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
+  const size_t base = A.off[i >> 6] + (i & 63);
+  const uint32_t wdt = A.width[i >> 6];
+  const uint32_t* __restrict__ cp = A.col + base;
+  const T* __restrict__ vp = A.val + base;
+  uint32_t k = 0;
+  for (; k + 4 <= wdt; k += 4) {
+    const size_t o = static_cast<size_t>(k) * 64;
+    const uint32_t j0 = cp[o], j1 = cp[o + 64], j2 = cp[o + 128], j3 = cp[o + 192];
+    const T a0 = vp[o], a1 = vp[o + 64], a2 = vp[o + 128], a3 = vp[o + 192];
+    const T x0 = v(j0), x1 = v(j1), x2 = v(j2), x3 = v(j3);
+    s = fma(a0, x0, s);
+    s = fma(a1, x1, s);
+    s = fma(a2, x2, s);
+    s = fma(a3, x3, s);
+  }
+  for (; k < wdt; k++) {
+    const size_t o = static_cast<size_t>(k) * 64;
+    s = fma(vp[o], v(cp[o]), s);
+  }
+  return s;
+}
+
 // r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
-__global__ void __launch_bounds__(256) k_cg_init(Sell<double> A, const double* __restrict__ b,
-                                                 const double* __restrict__ x, double* __restrict__ r,
-                                                 double* __restrict__ part_rr,
-                                                 double* __restrict__ part_bb) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg_init(Sell<double> A, const double* __restrict__ b,
+                                                const double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ part_rr,
+                                                double* __restrict__ part_bb) {
+  __shared__ double lds[NT / 64];
   const Rows rw = block_rows(A.n, A.rpb);
   double arr = 0, abb = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
-    double s = A.diag[i] * x[i];
-    const size_t base = A.off[i >> 6] + (i & 63);
-    const uint32_t wdt = A.width[i >> 6];
-    for (uint32_t k = 0; k < wdt; k++) {
-      const size_t o = base + static_cast<size_t>(k) * 64;
-      s += A.val[o] * x[A.col[o]];
-    }
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    const double s = sell_row(A, i, A.diag[i] * x[i], [&](uint32_t j) { return x[j]; });
     const double ri = b[i] - s;
     r[i] = ri;
-    arr += ri * ri;
-    abb += b[i] * b[i];
+    arr = fma(ri, ri, arr);
+    abb = fma(b[i], b[i], abb);
   }
-  const double srr = block_sum256(arr, lds4);
-  const double sbb = block_sum256(abb, lds4);
+  const double srr = block_sum<NT>(arr, lds);
+  const double sbb = block_sum<NT>(abb, lds);
   if (threadIdx.x == 0) {
     part_rr[blockIdx.x] = srr;
     part_bb[blockIdx.x] = sbb;
@@ -90,23 +119,23 @@ struct StopRule {
 // One CG iteration, first half (see file comment).  `it` is the 1-based
 // iteration number; FIRST marks it == 1, where p_1 = r_0.
 template <bool FIRST>
-__global__ void __launch_bounds__(256) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
-                                                 const double* __restrict__ pold,
-                                                 double* __restrict__ pnew, double* __restrict__ q,
-                                                 const double* __restrict__ part_in,
-                                                 const double* __restrict__ part_bb, int nparts,
-                                                 double* __restrict__ part_out, CgStatus* st,
-                                                 StopRule sr, long long it) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
+                                                const double* __restrict__ pold,
+                                                double* __restrict__ pnew, double* __restrict__ q,
+                                                const double* __restrict__ part_in,
+                                                const double* __restrict__ part_bb, int nparts,
+                                                double* __restrict__ part_out, CgStatus* st,
+                                                StopRule sr, long long it) {
+  __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double rr = reduce_partials(part_in, nparts, lds4);
+  const double rr = reduce_partials<NT>(part_in, nparts, lds);
   double beta = 0, tol;
   bool conv;
   double bb = 0;
   if (FIRST) {
     if (sr.rule == MPS_STOP_RHS_NORM) {
-      bb = reduce_partials(part_bb, nparts, lds4);
+      bb = reduce_partials<NT>(part_bb, nparts, lds);
       tol = (sr.tol * sr.tol) * bb;
       conv = (rr <= tol);
     } else {
@@ -133,56 +162,50 @@ __global__ void __launch_bounds__(256) k_cg_spmv(Sell<double> A, const double* _
   if (!FIRST) beta = rr / st->rr[par ^ 1];
   const Rows rw = block_rows(A.n, A.rpb);
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
-    const double pi = FIRST ? r[i] : r[i] + beta * pold[i];
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    const double pi = FIRST ? r[i] : fma(beta, pold[i], r[i]);
     pnew[i] = pi;
-    double s = A.diag[i] * pi;
-    const size_t base = A.off[i >> 6] + (i & 63);
-    const uint32_t wdt = A.width[i >> 6];
-    for (uint32_t k = 0; k < wdt; k++) {
-      const size_t o = base + static_cast<size_t>(k) * 64;
-      const uint32_t j = A.col[o];
-      const double pj = FIRST ? r[j] : r[j] + beta * pold[j];
-      s += A.val[o] * pj;
-    }
+    const double s = sell_row(A, i, A.diag[i] * pi, [&](uint32_t j) {
+      return FIRST ? r[j] : fma(beta, pold[j], r[j]);
+    });
     q[i] = s;
-    acc += pi * s;
+    acc = fma(pi, s, acc);
   }
-  const double sp = block_sum256(acc, lds4);
+  const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
 }
 
 // One CG iteration, second half (Computer.hpp:1398-1404).
-__global__ void __launch_bounds__(256) k_cg_update(size_t n, size_t rpb, double* __restrict__ x,
-                                                   double* __restrict__ r, const double* __restrict__ p,
-                                                   const double* __restrict__ q,
-                                                   const double* __restrict__ part_in, int nparts,
-                                                   double* __restrict__ part_out, CgStatus* st,
-                                                   long long it) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg_update(size_t n, size_t rpb, double* __restrict__ x,
+                                                  double* __restrict__ r, const double* __restrict__ p,
+                                                  const double* __restrict__ q,
+                                                  const double* __restrict__ part_in, int nparts,
+                                                  double* __restrict__ part_out, CgStatus* st,
+                                                  long long it) {
+  __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double pq = reduce_partials(part_in, nparts, lds4);
+  const double pq = reduce_partials<NT>(part_in, nparts, lds);
   const double alpha = st->rr[par] / pq;
   const Rows rw = block_rows(n, rpb);
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    x[i] = fma(alpha, p[i], x[i]);
+    const double ri = fma(-alpha, q[i], r[i]);
     r[i] = ri;
-    acc += ri * ri;
+    acc = fma(ri, ri, acc);
   }
-  const double s = block_sum256(acc, lds4);
+  const double s = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
   if (lead()) st->iterations = it;
 }
 
 // Stop test after the last permitted iteration.
-__global__ void __launch_bounds__(256) k_cg_check(const double* __restrict__ part_in, int nparts,
-                                                  CgStatus* st, StopRule sr, long long it) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg_check(const double* __restrict__ part_in, int nparts,
+                                                 CgStatus* st, StopRule sr, long long it) {
+  __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
-  const double rr = reduce_partials(part_in, nparts, lds4);
+  const double rr = reduce_partials<NT>(part_in, nparts, lds);
   const bool conv = (sr.rule == MPS_STOP_RHS_NORM) ? (rr <= st->tol) : (rr < st->tol);
   if (threadIdx.x == 0) {
     st->rr[it & 1] = rr;
@@ -194,12 +217,12 @@ __global__ void __launch_bounds__(256) k_cg_check(const double* __restrict__ par
 }
 
 // Final reduction of two partial arrays into out[0], out[1] (single block).
-__global__ void __launch_bounds__(256) k_reduce2(const double* __restrict__ a,
-                                                 const double* __restrict__ b, int nparts,
-                                                 double* __restrict__ out) {
-  __shared__ double lds4[4];
-  const double sa = reduce_partials(a, nparts, lds4);
-  const double sb = reduce_partials(b, nparts, lds4);
+__global__ void __launch_bounds__(NT) k_reduce2(const double* __restrict__ a,
+                                                const double* __restrict__ b, int nparts,
+                                                double* __restrict__ out) {
+  __shared__ double lds[NT / 64];
+  const double sa = reduce_partials<NT>(a, nparts, lds);
+  const double sb = reduce_partials<NT>(b, nparts, lds);
   if (threadIdx.x == 0) {
     out[0] = sa;
     out[1] = sb;
@@ -210,23 +233,23 @@ __global__ void __launch_bounds__(256) k_reduce2(const double* __restrict__ a,
 
 // r32 = r64 / |r64|, d32 = 0, partial sums of r32.r32 and r32.z (z = Dinv r).
 template <bool JAC>
-__global__ void __launch_bounds__(256) k_ir_prep(size_t n, size_t rpb, const double* __restrict__ r64,
-                                                 double inv_norm, const float* __restrict__ dinv,
-                                                 float* __restrict__ r32, float* __restrict__ d32,
-                                                 double* __restrict__ part_rr,
-                                                 double* __restrict__ part_rz) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_ir_prep(size_t n, size_t rpb, const double* __restrict__ r64,
+                                                double inv_norm, const float* __restrict__ dinv,
+                                                float* __restrict__ r32, float* __restrict__ d32,
+                                                double* __restrict__ part_rr,
+                                                double* __restrict__ part_rz) {
+  __shared__ double lds[NT / 64];
   const Rows rw = block_rows(n, rpb);
   double arr = 0, arz = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
     const float ri = static_cast<float>(r64[i] * inv_norm);
     r32[i] = ri;
     d32[i] = 0.0f;
-    arr += static_cast<double>(ri) * ri;
-    if (JAC) arz += static_cast<double>(ri) * static_cast<double>(ri * dinv[i]);
+    arr = fma(static_cast<double>(ri), static_cast<double>(ri), arr);
+    if (JAC) arz = fma(static_cast<double>(ri), static_cast<double>(ri * dinv[i]), arz);
   }
-  const double s1 = block_sum256(arr, lds4);
-  const double s2 = JAC ? block_sum256(arz, lds4) : s1;
+  const double s1 = block_sum<NT>(arr, lds);
+  const double s2 = JAC ? block_sum<NT>(arz, lds) : s1;
   if (threadIdx.x == 0) {
     part_rr[blockIdx.x] = s1;
     part_rz[blockIdx.x] = s2;
@@ -236,19 +259,19 @@ __global__ void __launch_bounds__(256) k_ir_prep(size_t n, size_t rpb, const dou
 // FP32 SpMV half-iteration.  Stop test on r.r relative to the sweep's initial
 // r.r; beta from r.z (== r.r without preconditioner); dots in FP64.
 template <bool FIRST, bool JAC>
-__global__ void __launch_bounds__(256) k_cg32_spmv(Sell<float> A, const float* __restrict__ r,
-                                                   const float* __restrict__ dinv,
-                                                   const float* __restrict__ pold,
-                                                   float* __restrict__ pnew, float* __restrict__ q,
-                                                   const double* __restrict__ part_rr,
-                                                   const double* __restrict__ part_rz, int nparts,
-                                                   double* __restrict__ part_out, CgStatus* st,
-                                                   double tol_rel, long long it) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg32_spmv(Sell<float> A, const float* __restrict__ r,
+                                                  const float* __restrict__ dinv,
+                                                  const float* __restrict__ pold,
+                                                  float* __restrict__ pnew, float* __restrict__ q,
+                                                  const double* __restrict__ part_rr,
+                                                  const double* __restrict__ part_rz, int nparts,
+                                                  double* __restrict__ part_out, CgStatus* st,
+                                                  double tol_rel, long long it) {
+  __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double rr = reduce_partials(part_rr, nparts, lds4);
-  const double rz = JAC ? reduce_partials(part_rz, nparts, lds4) : rr;
+  const double rr = reduce_partials<NT>(part_rr, nparts, lds);
+  const double rz = JAC ? reduce_partials<NT>(part_rz, nparts, lds) : rr;
   double tol;
   bool conv;
   if (FIRST) {
@@ -273,52 +296,46 @@ __global__ void __launch_bounds__(256) k_cg32_spmv(Sell<float> A, const float* _
   const float beta = FIRST ? 0.0f : static_cast<float>(rz / st->rr[par ^ 1]);
   const Rows rw = block_rows(A.n, A.rpb);
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
     const float zi = JAC ? r[i] * dinv[i] : r[i];
-    const float pi = FIRST ? zi : zi + beta * pold[i];
+    const float pi = FIRST ? zi : fmaf(beta, pold[i], zi);
     pnew[i] = pi;
-    float s = A.diag[i] * pi;
-    const size_t base = A.off[i >> 6] + (i & 63);
-    const uint32_t wdt = A.width[i >> 6];
-    for (uint32_t k = 0; k < wdt; k++) {
-      const size_t o = base + static_cast<size_t>(k) * 64;
-      const uint32_t j = A.col[o];
+    const float s = sell_row(A, i, A.diag[i] * pi, [&](uint32_t j) {
       const float zj = JAC ? r[j] * dinv[j] : r[j];
-      const float pj = FIRST ? zj : zj + beta * pold[j];
-      s += A.val[o] * pj;
-    }
+      return FIRST ? zj : fmaf(beta, pold[j], zj);
+    });
     q[i] = s;
-    acc += static_cast<double>(pi) * s;
+    acc = fma(static_cast<double>(pi), static_cast<double>(s), acc);
   }
-  const double sp = block_sum256(acc, lds4);
+  const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
 }
 
 template <bool JAC>
-__global__ void __launch_bounds__(256) k_cg32_update(size_t n, size_t rpb, float* __restrict__ d,
-                                                     float* __restrict__ r, const float* __restrict__ p,
-                                                     const float* __restrict__ q,
-                                                     const float* __restrict__ dinv,
-                                                     const double* __restrict__ part_in, int nparts,
-                                                     double* __restrict__ part_rr,
-                                                     double* __restrict__ part_rz, CgStatus* st,
-                                                     long long it) {
-  __shared__ double lds4[4];
+__global__ void __launch_bounds__(NT) k_cg32_update(size_t n, size_t rpb, float* __restrict__ d,
+                                                    float* __restrict__ r, const float* __restrict__ p,
+                                                    const float* __restrict__ q,
+                                                    const float* __restrict__ dinv,
+                                                    const double* __restrict__ part_in, int nparts,
+                                                    double* __restrict__ part_rr,
+                                                    double* __restrict__ part_rz, CgStatus* st,
+                                                    long long it) {
+  __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double pq = reduce_partials(part_in, nparts, lds4);
+  const double pq = reduce_partials<NT>(part_in, nparts, lds);
   const float alpha = static_cast<float>(st->rr[par] / pq);
   const Rows rw = block_rows(n, rpb);
   double arr = 0, arz = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += 256) {
-    d[i] = d[i] + alpha * p[i];
-    const float ri = r[i] - alpha * q[i];
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    d[i] = fmaf(alpha, p[i], d[i]);
+    const float ri = fmaf(-alpha, q[i], r[i]);
     r[i] = ri;
-    arr += static_cast<double>(ri) * ri;
-    if (JAC) arz += static_cast<double>(ri) * static_cast<double>(ri * dinv[i]);
+    arr = fma(static_cast<double>(ri), static_cast<double>(ri), arr);
+    if (JAC) arz = fma(static_cast<double>(ri), static_cast<double>(ri * dinv[i]), arz);
   }
-  const double s1 = block_sum256(arr, lds4);
-  const double s2 = JAC ? block_sum256(arz, lds4) : s1;
+  const double s1 = block_sum<NT>(arr, lds);
+  const double s2 = JAC ? block_sum<NT>(arz, lds) : s1;
   if (threadIdx.x == 0) {
     part_rr[blockIdx.x] = s1;
     part_rz[blockIdx.x] = s2;
@@ -330,7 +347,7 @@ __global__ void __launch_bounds__(256) k_cg32_update(size_t n, size_t rpb, float
 __global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict__ x,
                                                     const float* __restrict__ d, double norm) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i < n) x[i] = x[i] + static_cast<double>(d[i]) * norm;
+  if (i < n) x[i] = fma(static_cast<double>(d[i]), norm, x[i]);
 }
 
 __global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const float* __restrict__ diag,
@@ -342,20 +359,16 @@ __global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const float* __res
   }
 }
 
-}  // namespace
-
-namespace {
-
 struct Launch {
   int grid;
   size_t rpb;
 };
 Launch cg_grid(size_t n) {
-  size_t blocks = (n + 255) / 256;
+  size_t blocks = (n + NT - 1) / NT;
   if (blocks > static_cast<size_t>(kRedBlocks)) blocks = kRedBlocks;
   if (blocks == 0) blocks = 1;
   size_t rpb = (n + blocks - 1) / blocks;
-  rpb = (rpb + 255) / 256 * 256;
+  rpb = (rpb + NT - 1) / NT * NT;
   blocks = (n + rpb - 1) / rpb;
   if (blocks == 0) blocks = 1;
   return Launch{static_cast<int>(blocks), rpb};
@@ -379,6 +392,9 @@ long long next_chunk(long long done, long long predicted) {
   return std::max<long long>(16, std::min<long long>(256, done / 4));
 }
 
+// Algorithmic HBM bytes of one SpMV launch: every stored non-zero (value +
+// column index) once, the diagonal once, `extra_per_row` bytes of streamed
+// per-row vectors, and the slice table.
 double spmv_bytes(const Context& c, int vbytes, double extra_per_row) {
   return static_cast<double>(c.nnz_offdiag) * (vbytes + 4) + c.ppe_n * (vbytes + extra_per_row) +
          c.nslices * 12.0;
@@ -391,13 +407,14 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
   const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
   MPS_HIP(hipMemsetAsync(c.st, 0, sizeof(CgStatus), c.stream));
   tracked(c, MPS_K_CG_INIT, spmv_bytes(c, 8, 8.0 + 8 + 8), [&] {
-    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(256), 0, c.stream, A, static_cast<const double*>(c.b),
+    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
                        static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
   });
   check_launch();
   double* pbuf[2] = {c.pa, c.pb};
   long long it = 1;
-  const double b_spmv = spmv_bytes(c, 8, 8.0 * 4);  // diag, own r and p_old, writes p_new and q
+  // reads r and p_old, writes p_new and q (the diagonal is in spmv_bytes)
+  const double b_spmv = spmv_bytes(c, 8, 8.0 * 4);
   const double b_upd = c.ppe_n * 48.0;
   while (true) {
     const long long chunk = next_chunk(it - 1, c.last_iters);
@@ -406,25 +423,25 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
       double* pnew = pbuf[it & 1];
       tracked(c, MPS_K_CG_SPMV, b_spmv, [&] {
         if (it == 1)
-          hipLaunchKernelGGL(k_cg_spmv<true>, dim3(L.grid), dim3(256), 0, c.stream, A,
+          hipLaunchKernelGGL(k_cg_spmv<true>, dim3(L.grid), dim3(NT), 0, c.stream, A,
                              static_cast<const double*>(c.r), pold, pnew, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
         else
-          hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(256), 0, c.stream, A,
+          hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(NT), 0, c.stream, A,
                              static_cast<const double*>(c.r), pold, pnew, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
       }, it);
       tracked(c, MPS_K_CG_UPDATE, b_upd, [&] {
-        hipLaunchKernelGGL(k_cg_update, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.sol, c.r,
+        hipLaunchKernelGGL(k_cg_update, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.sol, c.r,
                            static_cast<const double*>(pnew), static_cast<const double*>(c.q),
                            static_cast<const double*>(c.partA), L.grid, c.partB, c.st, it);
       }, it);
     }
     check_launch();
     if (it > max_iter) {
-      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.partB),
+      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(NT), 0, c.stream, static_cast<const double*>(c.partB),
                          L.grid, c.st, sr, it);
       check_launch();
       read_status(c, c.st, c.st_host);
@@ -451,9 +468,9 @@ void residual64(Context& c, double& rr, double& bb, bool track) {
   const Launch L = cg_grid(c.ppe_n);
   const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
   auto launch = [&] {
-    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(256), 0, c.stream, A, static_cast<const double*>(c.b),
+    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
                        static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.partB),
+    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(NT), 0, c.stream, static_cast<const double*>(c.partB),
                        static_cast<const double*>(c.partC), L.grid, c.red_out);
   };
   if (track) tracked(c, MPS_K_IR_RESIDUAL, spmv_bytes(c, 8, 8.0 + 8 + 8), launch);
@@ -488,19 +505,26 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   for (;; outer++) {
     residual64(c, rr, bb, true);
     if (outer == 0) rr0 = rr;
-    if (sr.rule == MPS_STOP_RHS_NORM)
-      done = rr <= (sr.tol * sr.tol) * bb;
-    else
-      done = (outer == 0) ? (rr * sr.tol * sr.tol == 0) : (rr < rr0 * sr.tol * sr.tol);
+    double target;  // outer r.r to reach
+    if (sr.rule == MPS_STOP_RHS_NORM) {
+      target = (sr.tol * sr.tol) * bb;
+      done = rr <= target;
+    } else {
+      target = rr0 * sr.tol * sr.tol;
+      done = (outer == 0) ? (target == 0) : (rr < target);
+    }
     if (done || outer >= max_outer || total >= max_iter || !(rr == rr)) break;
     const double norm = std::sqrt(rr);
+    // this sweep reduces the residual by inner_tol, or only as far as the
+    // outer target needs (with a factor 2 of margin), whichever is less work
+    const double sweep_tol = std::max(inner_tol, 0.5 * std::sqrt(target / rr));
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
     if (jac)
-      hipLaunchKernelGGL(k_ir_prep<true>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb,
+      hipLaunchKernelGGL(k_ir_prep<true>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb,
                          static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
                          c.r32, c.d32, c.partB, c.partC);
     else
-      hipLaunchKernelGGL(k_ir_prep<false>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb,
+      hipLaunchKernelGGL(k_ir_prep<false>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb,
                          static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
                          c.r32, c.d32, c.partB, c.partC);
     check_launch();
@@ -512,10 +536,10 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
         float* pnew = pbuf[it & 1];
         tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
 #define MPS_S32(F, J)                                                                                   \
-  hipLaunchKernelGGL((k_cg32_spmv<F, J>), dim3(L.grid), dim3(256), 0, c.stream, A,                      \
+  hipLaunchKernelGGL((k_cg32_spmv<F, J>), dim3(L.grid), dim3(NT), 0, c.stream, A,                       \
                      static_cast<const float*>(c.r32), static_cast<const float*>(c.dinv32), pold, pnew, \
                      c.q32, static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),  \
-                     L.grid, c.partA, c.st32, inner_tol, it)
+                     L.grid, c.partA, c.st32, sweep_tol, it)
           if (it == 1) {
             if (jac) MPS_S32(true, true);
             else MPS_S32(true, false);
@@ -527,12 +551,12 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
         }, it);
         tracked(c, MPS_K_CG_UPDATE32, b_upd, [&] {
           if (jac)
-            hipLaunchKernelGGL(k_cg32_update<true>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.d32,
+            hipLaunchKernelGGL(k_cg32_update<true>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.d32,
                                c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
                                static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
                                L.grid, c.partB, c.partC, c.st32, it);
           else
-            hipLaunchKernelGGL(k_cg32_update<false>, dim3(L.grid), dim3(256), 0, c.stream, n, L.rpb, c.d32,
+            hipLaunchKernelGGL(k_cg32_update<false>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.d32,
                                c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
                                static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
                                L.grid, c.partB, c.partC, c.st32, it);
@@ -546,7 +570,7 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     const long long launched = std::min(it - 1, inner_max);
     uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
     uncount(c, MPS_K_CG_UPDATE32, launched - inner, b_upd);
-    c.last_inner = inner;
+    if (outer == 0) c.last_inner = inner;
     total += inner;
     hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
                        static_cast<const float*>(c.d32), norm);

@@ -46,24 +46,28 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
   return v;
 }
 
-// Deterministic block sum of one double per thread (blockDim.x == 256):
-// fixed wavefront butterflies then the 4 wavefront sums in order.
-__device__ __forceinline__ double block_sum256(double v, double* lds4) {
+// Deterministic block sum of one double per thread (blockDim.x == NT):
+// fixed wavefront butterflies, then the NT/64 wavefront sums in order.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) lds4[wv] = v;
+  if (lane == 0) lds[wv] = v;
   __syncthreads();
-  const double s = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+  double s = lds[0];
+#pragma unroll
+  for (int k = 1; k < NT / 64; k++) s += lds[k];
   __syncthreads();
   return s;
 }
 
 // Sum of `count` partials in a fixed order that every block reproduces
-// identically (thread t takes t, t+256, ...; then block_sum256).
-__device__ __forceinline__ double reduce_partials(const double* part, int count, double* lds4) {
+// identically (thread t takes t, t+NT, ...; then block_sum).
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double* part, int count, double* lds) {
   double s = 0;
-  for (int k = threadIdx.x; k < count; k += 256) s += part[k];
-  return block_sum256(s, lds4);
+  for (int k = threadIdx.x; k < count; k += NT) s += part[k];
+  return block_sum<NT>(s, lds);
 }
 
 }  // namespace mps

@@ -226,5 +226,6 @@ void ensure_ppe_vectors(Context& c, size_t n);  // mps_api.cpp
 void ppe_solve(Context& c);              // mps_cg.hip
 double max_speed_dt(Context& c);         // mps_particles.hip (DetermineDt)
 void device_math_check(size_t n, const double* a, const double* b, double* s, double* d);
+double device_copy_probe(size_t bytes);
 
 }  // namespace mps

@@ -358,6 +358,11 @@ __global__ void __launch_bounds__(256) k_max_speed2(size_t n, const double* __re
   }
 }
 
+__global__ void __launch_bounds__(256) k_copy_probe(size_t n4, const float4* __restrict__ a,
+                                                    float4* __restrict__ b) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n4; i += gridDim.x * 256ull) b[i] = a[i];
+}
+
 __global__ void k_math_check(size_t n, const double* a, const double* b, double* s, double* d) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -522,6 +527,27 @@ double max_speed_dt(Context& c) {
   if (maxU == 0) return dtmax;
   const double a = dxmax / maxU;
   return (dtmax < a) ? dtmax : a;
+}
+
+double device_copy_probe(size_t bytes) {
+  const size_t n4 = bytes / 16;
+  float4* a = dalloc<float4>(n4);
+  float4* b = dalloc<float4>(n4);
+  MPS_HIP(hipMemset(a, 0, n4 * 16));
+  hipEvent_t e0, e1;
+  MPS_HIP(hipEventCreate(&e0));
+  MPS_HIP(hipEventCreate(&e1));
+  MPS_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, 0, n4, static_cast<const float4*>(a), b);
+  MPS_HIP(hipEventRecord(e1, 0));
+  MPS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MPS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(a);
+  dfree(b);
+  return ms;
 }
 
 void device_math_check(size_t n, const double* a, const double* b, double* s, double* d) {
