@@ -58,5 +58,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_facade_driver(verbose: bool = False) -> str:
+    """Compiles tests/cpp/facade_main.cpp (the C++ Computer facade driven like
+    the reference's Main.cpp) against the library with the host compiler."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "facade_main.cpp")
+    out = os.path.join(root, "tests", "cpp", "facade_main")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"), src,
+           "-L", HERE, "-lmps_hip", "-Wl,-rpath,$ORIGIN/../../openmps_amd", "-o", out + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_facade_driver(verbose=True)
