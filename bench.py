@@ -31,13 +31,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_computer(cfg, solver: str, tol: float):
+# solver variants: name -> (mixed precision, warm start, Jacobi in the FP32 CG)
+VARIANTS = {
+    "fp64": (False, "previous", False),          # the reference algorithm, previous-step guess
+    "fp64_zero": (False, "zero", False),
+    "fp64_learned": (False, "learned", False),
+    "mixed": (True, "previous", False),
+    "mixed_jacobi": (True, "previous", True),
+    "mixed_zero": (True, "zero", True),
+    "learned_mixed": (True, "learned", True),
+}
+
+
+def make_computer(cfg, solver: str, tol: float, weights: str | None = None):
     from openmps_amd import Computer, make_environment
     from openmps_amd import _lib as L
+    mixed, warm, jac = VARIANTS[solver]
     c = Computer(cfg["eps"], make_environment(cfg["env"]))
-    kind = L.SOLVER_MIXED_IR if solver.startswith("mixed") else L.SOLVER_CG_FP64
-    c.set_solver(kind=kind, stop_rule=L.STOP_RHS_NORM, warm_start=L.WARM_PREVIOUS,
-                 jacobi=solver.endswith("jacobi"), tol=tol, inner_tol=1e-4)
+    ws = {"previous": L.WARM_PREVIOUS, "zero": L.WARM_ZERO, "learned": L.WARM_LEARNED}[warm]
+    c.set_solver(kind=L.SOLVER_MIXED_IR if mixed else L.SOLVER_CG_FP64, stop_rule=L.STOP_RHS_NORM,
+                 warm_start=ws, jacobi=jac, tol=tol, inner_tol=1e-4)
+    if warm == "learned":
+        from openmps_amd import learned
+        learned.attach_model(c, learned.Model.load(weights or learned.DEFAULT_WEIGHTS))
     c.add_particles(cfg["particles"])
     return c
 
@@ -94,8 +110,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dambreak_1m")
-    ap.add_argument("--solver", default="mixed_jacobi",
-                    choices=["mixed_jacobi", "mixed", "fp64"])
+    ap.add_argument("--solver", default=None, choices=sorted(VARIANTS),
+                    help="headline solver (default: learned_mixed when trained weights exist, "
+                         "else mixed_jacobi)")
+    ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
+    ap.add_argument("--ablation", action="store_true",
+                    help="also time zero / previous / learned guesses for both solvers")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,13 +128,17 @@ def main() -> int:
         return 2
 
     from openmps_amd import scenes
+    from openmps_amd import learned as _learned
+    have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)
+    if args.solver is None:
+        args.solver = "learned_mixed" if have_weights else "mixed_jacobi"
     log(f"building scene {args.workload} ...")
     cfg = scenes.config(args.workload)
     N = len(cfg["particles"]["x"])
     nfluid = int((cfg["particles"]["type"] == 0).sum())
     log(f"  {N} particles ({nfluid} fluid)")
 
-    main_c = make_computer(cfg, args.solver, args.tol)
+    main_c = make_computer(cfg, args.solver, args.tol, args.weights)
     main_c._bench_dt = cfg["dt"]
     if args.warmup:
         run(main_c, args.warmup, f"{args.solver} warmup")
@@ -126,22 +150,35 @@ def main() -> int:
     solve_ms = sum(s["solve_ms"] for s in st) / len(st)
     iters = sum(s["iterations"] for s in st) / len(st)
 
-    solve = {args.solver: {"ms_per_step": solve_ms, "cg_iterations": iters,
-                           "rel_residual": max(s["rel_residual"] for s in st)}}
+    def summary(stats):
+        return {"ms_per_step": sum(s["solve_ms"] for s in stats) / len(stats),
+                "cg_iterations": sum(s["iterations"] for s in stats) / len(stats),
+                "initial_rel_residual": sum(s["initial_rel_residual"] for s in stats) / len(stats),
+                "rel_residual": max(s["rel_residual"] for s in stats),
+                "learned_used": sum(s["learned_used"] for s in stats)}
+
+    solve = {args.solver: summary(st)}
     kt64 = kt if args.solver == "fp64" else {}
+    others = []
     if not args.no_compare and args.solver != "fp64":
-        ref_c = make_computer(cfg, "fp64", args.tol)
+        others.append("fp64")
+    if args.ablation:
+        others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_zero", "learned_mixed",
+                               "fp64_learned")
+                   if v != args.solver and (have_weights or "learned" not in v)]
+    del main_c
+    for v in others:
+        ref_c = make_computer(cfg, v, args.tol, args.weights)
         ref_c._bench_dt = cfg["dt"]
         if args.warmup:
-            run(ref_c, args.warmup, "fp64 warmup")
+            run(ref_c, args.warmup, f"{v} warmup")
         ref_c.reset_kernel_stats()
-        ref_c.set_profiling(True)
-        _, st64 = run(ref_c, args.steps, "fp64 timed")
+        ref_c.set_profiling(v == "fp64")
+        _, stv = run(ref_c, args.steps, f"{v} timed")
         ref_c.set_profiling(False)
-        kt64 = kernel_table(ref_c)
-        solve["fp64"] = {"ms_per_step": sum(s["solve_ms"] for s in st64) / len(st64),
-                         "cg_iterations": sum(s["iterations"] for s in st64) / len(st64),
-                         "rel_residual": max(s["rel_residual"] for s in st64)}
+        if v == "fp64":
+            kt64 = kernel_table(ref_c)
+        solve[v] = summary(stv)
         del ref_c
 
     def roof(table, name):
@@ -154,7 +191,7 @@ def main() -> int:
                 "ms_per_launch": d["ms_per_launch"]}
 
     # dominant kernel: the SpMV of the solver that ran the timed steps
-    dom = "cg_spmv32" if args.solver.startswith("mixed") else "cg_spmv"
+    dom = "cg_spmv32" if VARIANTS[args.solver][0] else "cg_spmv"
     roofline = roof(kt, dom)
     roofline_fp64 = roof(kt64, "cg_spmv")
 
@@ -170,7 +207,8 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.solver == "fp64" else "f64+f32",
+        "dtype": ("f64" if not VARIANTS[args.solver][0] else "f64+f32") +
+                 ("+bf16" if VARIANTS[args.solver][1] == "learned" else ""),
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": N, "fluid_particles": nfluid,
                    "l0": cfg["env"]["l0"], "dt": cfg["dt"], "r_e_by_l0": cfg["env"]["r_e_by_l0"],

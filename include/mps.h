@@ -144,7 +144,9 @@ typedef struct mps_solve_stats {
   double rr;                /* final r.r of the recursion */
   double rel_residual;      /* |b - A x| / |b| recomputed in FP64 after the solve */
   double initial_rel_residual; /* |b - A x0| / |b| */
-  double solve_ms;          /* device time of the solve (HIP events) */
+  double solve_ms;          /* device time of the solve (HIP events), warm start included */
+  int64_t learned_used;     /* 1: the learned guess replaced the previous-step guess */
+  double prev_rel_residual; /* |b - A p_prev| / |b| when the learned guess was evaluated */
 } mps_solve_stats;
 int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out);
 
@@ -188,12 +190,47 @@ int mps_get_ppe(mps_ctx* ctx, uint64_t* rowptr, uint32_t* col, double* val);
 int mps_set_ppe(mps_ctx* ctx, size_t n, const uint64_t* rowptr, const uint32_t* col,
                 const double* val, const double* b, const double* x0);
 
+/* ---- learned warm start (no reference counterpart) ----
+ * A per-particle MLP (132 -> 256 -> 256 -> 1, GELU, BF16 MFMA with FP32
+ * accumulation) predicts p - p_prev from the features of the 32 nearest
+ * stencil neighbours; see mps_mlp.hip.  Parameter layouts on the host:
+ * w1 [256][132], b1 [256], w2 [256][256], b2 [256], w3 [256], b3 [1]. */
+typedef struct mps_model mps_model;
+typedef struct mps_dataset mps_dataset;
+int mps_model_create(int device, int seed, mps_model** out);
+void mps_model_destroy(mps_model* m);
+int mps_model_get_params(mps_model* m, float* w1, float* b1, float* w2, float* b2, float* w3,
+                         float* b3);
+int mps_model_set_params(mps_model* m, const float* w1, const float* b1, const float* w2,
+                         const float* b2, const float* w3, const float* b3);
+/* y = MLP(x) for `rows` feature rows x [rows][132] (host arrays). */
+int mps_model_forward(mps_model* m, size_t rows, const float* x, float* y);
+/* Mean-squared-error loss of (x, t) and its gradient for every parameter. */
+int mps_model_gradients(mps_model* m, size_t rows, const float* x, const float* t, float* loss,
+                        float* g_w1, float* g_b1, float* g_w2, float* g_b2, float* g_w3,
+                        float* g_b3);
+/* Adam over shuffled mini-batches of the dataset; losses_out[epochs]. */
+int mps_model_train(mps_model* m, mps_dataset* d, int epochs, int batch, double lr, uint64_t seed,
+                    double* losses_out);
+int mps_dataset_create(int device, size_t capacity, mps_dataset** out);
+void mps_dataset_destroy(mps_dataset* d);
+int mps_dataset_size(mps_dataset* d, size_t* n);
+int mps_dataset_get(mps_dataset* d, size_t first, size_t count, float* x, float* t);
+/* Use the model for MPS_WARM_LEARNED (NULL detaches). */
+int mps_attach_model(mps_ctx* ctx, mps_model* m);
+/* At every pressure solve, append (features, (p - p_prev) / s_d) of a
+ * `fraction` of the alive rows to the dataset (NULL detaches). */
+int mps_attach_dataset(mps_ctx* ctx, mps_dataset* d, double fraction, uint64_t seed);
+/* Features of the current state (computes them): out [n][132]. */
+int mps_get_features(mps_ctx* ctx, float* out);
+
 /* ---- measurement (no reference counterpart) ---- */
 enum mps_kernel_id {
   MPS_K_CELL_KEYS = 0, MPS_K_CELL_SCAN, MPS_K_CELL_SCATTER, MPS_K_CELL_SORT, MPS_K_NEIGHBORS,
   MPS_K_DENSITY_ECS, MPS_K_EXPLICIT, MPS_K_UPDATE, MPS_K_DENSITY_SAVE, MPS_K_PPE_COUNT,
   MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_SPMV32,
   MPS_K_CG_UPDATE32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
+  MPS_K_FEATURES, MPS_K_MLP,
   MPS_K_COUNT
 };
 typedef struct mps_kernel_stat {

@@ -6,7 +6,8 @@ mirror of the reference's ``Environment`` / ``Computer`` interface over that
 ABI, plus the synthetic scene generator used by the benchmarks.
 """
 from .computer import (Computer, ComputerException, CreateComputer, Environment, Grid,
-                       GridException, MpsError, ParticleType, device_math_check)
+                       GridException, MpsError, ParticleType, device_math_check,
+                       make_environment_from)
 from . import scenes
 
 __all__ = ["Computer", "ComputerException", "CreateComputer", "Environment", "Grid",
@@ -14,8 +15,4 @@ __all__ = ["Computer", "ComputerException", "CreateComputer", "Environment", "Gr
            "make_environment"]
 
 
-def make_environment(env: dict) -> Environment:
-    """Environment from a scene dictionary (keys of ``scenes.config(...)["env"]``)."""
-    return Environment(env["max_dt"], env["courant"], env["g"], env["rho"], env["nu"],
-                       env["r_e_by_l0"], env["l0"], env["min_x"], env["min_z"], env["max_x"],
-                       env["max_z"], env.get("r_lap_by_l0"))
+make_environment = make_environment_from

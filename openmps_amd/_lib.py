@@ -34,7 +34,8 @@ AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_
 
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
            "explicit", "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv",
-           "cg_update", "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other")
+           "cg_update", "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other",
+           "features", "mlp")
 
 
 class MpsEnv(ctypes.Structure):
@@ -51,7 +52,8 @@ class SolverOpts(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", _I64), ("outer_iterations", _I64), ("rr0", _D), ("rr", _D),
-                ("rel_residual", _D), ("initial_rel_residual", _D), ("solve_ms", _D)]
+                ("rel_residual", _D), ("initial_rel_residual", _D), ("solve_ms", _D),
+                ("learned_used", _I64), ("prev_rel_residual", _D)]
 
 
 class KernelStat(ctypes.Structure):
@@ -108,6 +110,20 @@ SIGNATURES = {
     "mps_reset_kernel_stats": (_I, [_P]),
     "mps_device_math_check": (_I, [_SZ, _P, _P, _P, _P]),
     "mps_device_copy_probe": (_I, [_SZ, ctypes.POINTER(_D)]),
+    "mps_model_create": (_I, [_I, _I, ctypes.POINTER(_P)]),
+    "mps_model_destroy": (None, [_P]),
+    "mps_model_get_params": (_I, [_P] + [_P] * 6),
+    "mps_model_set_params": (_I, [_P] + [_P] * 6),
+    "mps_model_forward": (_I, [_P, _SZ, _P, _P]),
+    "mps_model_gradients": (_I, [_P, _SZ, _P, _P, _P] + [_P] * 6),
+    "mps_model_train": (_I, [_P, _P, _I, _I, _D, ctypes.c_uint64, _P]),
+    "mps_dataset_create": (_I, [_I, _SZ, ctypes.POINTER(_P)]),
+    "mps_dataset_destroy": (None, [_P]),
+    "mps_dataset_size": (_I, [_P, ctypes.POINTER(_SZ)]),
+    "mps_dataset_get": (_I, [_P, _SZ, _SZ, _P, _P]),
+    "mps_attach_model": (_I, [_P, _P]),
+    "mps_attach_dataset": (_I, [_P, _P, _D, ctypes.c_uint64]),
+    "mps_get_features": (_I, [_P, _P]),
 }
 
 _lib = None

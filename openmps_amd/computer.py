@@ -365,6 +365,13 @@ class Computer:
         return out
 
 
+def make_environment_from(env: dict) -> Environment:
+    """Environment from a dictionary with the keys of scenes.config(...)["env"]."""
+    return Environment(env["max_dt"], env["courant"], env["g"], env["rho"], env["nu"],
+                       env["r_e_by_l0"], env["l0"], env["min_x"], env["min_z"], env["max_x"],
+                       env["max_z"], env.get("r_lap_by_l0"))
+
+
 def CreateComputer(allowableResidual: float, env: Environment, positionWall=None,
                    positionWallPre=None, device: int = 0) -> Computer:
     """CreateComputer, Computer.hpp:1800-1815."""

@@ -12,6 +12,14 @@
 struct mps_ctx {
   mps::Context c;
 };
+struct mps_model {
+  int device = 0;
+  mps::Model m;
+};
+struct mps_dataset {
+  int device = 0;
+  mps::Dataset d;
+};
 
 namespace mps {
 
@@ -129,6 +137,7 @@ void free_all(Context& c) {
   dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.r32); dfree(c.pa32); dfree(c.pb32); dfree(c.q32); dfree(c.d32); dfree(c.dinv32);
+  dfree(c.feats); dfree(c.ypred);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
@@ -189,7 +198,7 @@ void forward(Context& c, double dt) {
 const char* kKernelNames[MPS_K_COUNT] = {
     "cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs", "explicit",
     "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv", "cg_update",
-    "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other"};
+    "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other", "features", "mlp"};
 
 }  // namespace
 
@@ -687,6 +696,200 @@ int mps_reset_kernel_stats(mps_ctx* ctx) {
   return mps::guard(ctx, [&](Context& c) {
     mps::sync(c);
     for (auto& s : c.kstat) s = mps::KStat{};
+  });
+}
+
+// ---- learned warm start ----------------------------------------------------------
+
+}  // extern "C"
+
+namespace mps {
+namespace {
+template <class H, class F>
+int guard_obj(H* h, F&& f) {
+  if (!h) return MPS_ERR_BAD_ARG;
+  try {
+    MPS_HIP(hipSetDevice(h->device));
+    f();
+    return MPS_OK;
+  } catch (Failure& e) {
+    std::fprintf(stderr, "mps: %s\n", e.msg.c_str());
+    if (e.code == MPS_ERR_HIP) (void)hipGetLastError();
+    return e.code;
+  } catch (std::exception& e) {
+    std::fprintf(stderr, "mps: %s\n", e.what());
+    return MPS_ERR_BAD_ARG;
+  }
+}
+// host [256][132] <-> device [256][160] for w1; other parameters are dense
+void params_to_device(mps::Model& m, const float* const* src) {
+  for (int k = 0; k < 6; k++) {
+    const size_t n = mps::Model::param_count(k);
+    std::vector<float> h(n, 0.0f);
+    if (k == 0) {
+      for (int r = 0; r < mps::kHid; r++)
+        for (int q = 0; q < mps::kFeat; q++) h[r * mps::kFeatPad + q] = src[0][r * mps::kFeat + q];
+    } else {
+      std::memcpy(h.data(), src[k], n * 4);
+    }
+    MPS_HIP(hipMemcpy(m.w[k], h.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  m.refresh_bf16();
+}
+void params_to_host(float* const* base, float* const* dst) {
+  for (int k = 0; k < 6; k++) {
+    const size_t n = mps::Model::param_count(k);
+    std::vector<float> h(n);
+    MPS_HIP(hipMemcpy(h.data(), base[k], n * 4, hipMemcpyDeviceToHost));
+    if (k == 0) {
+      for (int r = 0; r < mps::kHid; r++)
+        for (int q = 0; q < mps::kFeat; q++) dst[0][r * mps::kFeat + q] = h[r * mps::kFeatPad + q];
+    } else {
+      std::memcpy(dst[k], h.data(), n * 4);
+    }
+  }
+}
+}  // namespace
+}  // namespace mps
+
+extern "C" {
+
+int mps_model_create(int device, int seed, mps_model** out) {
+  if (!out) return MPS_ERR_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return MPS_ERR_NO_DEVICE;
+  }
+  mps_model* h = new (std::nothrow) mps_model;
+  if (!h) return MPS_ERR_BAD_ARG;
+  h->device = device;
+  const int rc = mps::guard_obj(h, [&] { h->m.init(seed); });
+  if (rc != MPS_OK) {
+    h->m.release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return MPS_OK;
+}
+
+void mps_model_destroy(mps_model* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  m->m.release();
+  delete m;
+}
+
+int mps_model_get_params(mps_model* m, float* w1, float* b1, float* w2, float* b2, float* w3, float* b3) {
+  return mps::guard_obj(m, [&] {
+    float* dst[6] = {w1, b1, w2, b2, w3, b3};
+    for (float* p : dst)
+      if (!p) throw Failure{MPS_ERR_BAD_ARG, "null parameter array"};
+    mps::params_to_host(m->m.w, dst);
+  });
+}
+
+int mps_model_set_params(mps_model* m, const float* w1, const float* b1, const float* w2, const float* b2,
+                         const float* w3, const float* b3) {
+  return mps::guard_obj(m, [&] {
+    const float* src[6] = {w1, b1, w2, b2, w3, b3};
+    for (const float* p : src)
+      if (!p) throw Failure{MPS_ERR_BAD_ARG, "null parameter array"};
+    mps::params_to_device(m->m, src);
+  });
+}
+
+int mps_model_forward(mps_model* m, size_t rows, const float* x, float* y) {
+  return mps::guard_obj(m, [&] {
+    if (rows == 0) return;
+    mps::model_forward_host(m->m, rows, x, y);
+  });
+}
+
+int mps_model_gradients(mps_model* m, size_t rows, const float* x, const float* t, float* loss, float* g_w1,
+                        float* g_b1, float* g_w2, float* g_b2, float* g_w3, float* g_b3) {
+  return mps::guard_obj(m, [&] {
+    if (rows == 0) throw Failure{MPS_ERR_BAD_ARG, "no rows"};
+    mps::model_grads_host(m->m, rows, x, t, loss);
+    float* dst[6] = {g_w1, g_b1, g_w2, g_b2, g_w3, g_b3};
+    mps::params_to_host(m->m.g, dst);
+  });
+}
+
+int mps_model_train(mps_model* m, mps_dataset* d, int epochs, int batch, double lr, uint64_t seed,
+                    double* losses_out) {
+  if (!d) return MPS_ERR_BAD_ARG;
+  return mps::guard_obj(m, [&] { m->m.train(d->d, epochs, batch, static_cast<float>(lr), seed, losses_out); });
+}
+
+int mps_dataset_create(int device, size_t capacity, mps_dataset** out) {
+  if (!out || capacity == 0) return MPS_ERR_BAD_ARG;
+  *out = nullptr;
+  mps_dataset* h = new (std::nothrow) mps_dataset;
+  if (!h) return MPS_ERR_BAD_ARG;
+  h->device = device;
+  const int rc = mps::guard_obj(h, [&] { h->d.init(capacity); });
+  if (rc != MPS_OK) {
+    h->d.release();
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return MPS_OK;
+}
+
+void mps_dataset_destroy(mps_dataset* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->device);
+  d->d.release();
+  delete d;
+}
+
+int mps_dataset_size(mps_dataset* d, size_t* n) {
+  return mps::guard_obj(d, [&] { *n = d->d.size(); });
+}
+
+int mps_dataset_get(mps_dataset* d, size_t first, size_t count, float* x, float* t) {
+  return mps::guard_obj(d, [&] {
+    if (first + count > d->d.size()) throw Failure{MPS_ERR_BAD_ARG, "range beyond the dataset"};
+    std::vector<__bf16> hx(count * mps::kFeatPad);
+    MPS_HIP(hipMemcpy(hx.data(), d->d.x + first * mps::kFeatPad, hx.size() * 2, hipMemcpyDeviceToHost));
+    if (t) MPS_HIP(hipMemcpy(t, d->d.t + first, count * 4, hipMemcpyDeviceToHost));
+    if (x)
+      for (size_t r = 0; r < count; r++)
+        for (int q = 0; q < mps::kFeat; q++) x[r * mps::kFeat + q] = static_cast<float>(hx[r * mps::kFeatPad + q]);
+  });
+}
+
+int mps_attach_model(mps_ctx* ctx, mps_model* m) {
+  if (!ctx) return MPS_ERR_BAD_ARG;
+  ctx->c.model = m ? &m->m : nullptr;
+  return MPS_OK;
+}
+
+int mps_attach_dataset(mps_ctx* ctx, mps_dataset* d, double fraction, uint64_t seed) {
+  if (!ctx || !(fraction >= 0 && fraction <= 1)) return MPS_ERR_BAD_ARG;
+  ctx->c.dataset = d ? &d->d : nullptr;
+  ctx->c.collect_frac = fraction;
+  ctx->c.collect_seed = seed;
+  ctx->c.collect_step = 0;
+  return MPS_OK;
+}
+
+int mps_get_features(mps_ctx* ctx, float* out) {
+  return mps::guard(ctx, [&](Context& c) {
+    if (!c.have_ppe || c.ppe_n != c.n) throw Failure{MPS_ERR_STATE, "assemble the PPE first"};
+    double s_r = 0, rr = 0;
+    const size_t n = c.n;
+    std::vector<__bf16> h(n * mps::kFeatPad);
+    if (mps::learned_features(c, s_r, rr)) {
+      MPS_HIP(hipMemcpyAsync(h.data(), c.feats, h.size() * 2, hipMemcpyDeviceToHost, c.stream));
+      mps::sync(c);
+    }
+    for (size_t r = 0; r < n; r++)
+      for (int q = 0; q < mps::kFeat; q++) out[r * mps::kFeat + q] = static_cast<float>(h[r * mps::kFeatPad + q]);
   });
 }
 

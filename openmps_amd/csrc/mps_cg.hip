@@ -463,6 +463,16 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
 }
 
+}  // namespace
+
+// Single-block reduction of two partial arrays into c.red_host[0..1] (synchronous).
+void reduce2(Context& c, const double* a, const double* b, int nparts) {
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(NT), 0, c.stream, a, b, nparts, c.red_out);
+  check_launch();
+  MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+}
+
 // FP64 residual |b - A x|^2 and |b|^2 (reuses the init kernel on scratch r).
 void residual64(Context& c, double& rr, double& bb, bool track) {
   const Launch L = cg_grid(c.ppe_n);
@@ -481,6 +491,8 @@ void residual64(Context& c, double& rr, double& bb, bool track) {
   rr = c.red_host[0];
   bb = c.red_host[1];
 }
+
+namespace {
 
 // Mixed precision: FP32 CG sweeps on A d = r, FP64 residual and accumulation.
 void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
@@ -597,10 +609,14 @@ void ppe_solve(Context& c) {
   sr.rule = c.solver.stop_rule;
   sr.tol = (c.solver.tol > 0) ? c.solver.tol : c.eps;
   const long long max_iter = (c.solver.max_iter > 0) ? c.solver.max_iter : static_cast<long long>(n);
+  // snapshot features use x0 = p_prev, so they are taken before any warm start
+  const bool on_particles = (c.ppe_n == c.n);
+  if (c.dataset && on_particles) collect_before(c);
   MPS_HIP(hipEventRecord(c.solve_a, c.stream));
   int rc = MPS_OK;
   std::string msg;
   try {
+    if (c.solver.warm_start == MPS_WARM_LEARNED && on_particles) learned_warm_start(c);
     if (c.solver.kind == MPS_SOLVER_MIXED_IR) solve_mixed(c, sr, max_iter);
     else solve_cg64(c, sr, max_iter);
   } catch (Failure& f) {
@@ -619,6 +635,7 @@ void ppe_solve(Context& c) {
   c.stats.rel_residual = (bb > 0) ? std::sqrt(rr / bb) : std::sqrt(rr);
   c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
   if (rc != MPS_OK) throw Failure{rc, msg};
+  if (c.dataset && on_particles) collect_after(c);
 }
 
 }  // namespace mps

@@ -42,6 +42,52 @@ struct IrStatus {
   int pad;
 };
 
+constexpr int kFeat = 132;     // warm-start feature width
+constexpr int kFeatPad = 160;  // padded to the MFMA K step (32)
+constexpr int kHid = 256;      // hidden width
+constexpr int kNbr = 32;       // neighbours per feature row
+
+struct Dataset {
+  size_t cap = 0;
+  __bf16* x = nullptr;             // cap x kFeatPad
+  float* t = nullptr;              // cap
+  unsigned long long* count = nullptr;
+  std::string err;
+  void init(size_t capacity);
+  size_t size() const;
+  void release();
+};
+
+// MLP 132 -> 256 -> 256 -> 1 (GELU).  Parameters k = 0..5: w1 [256][160]
+// (columns >= 132 zero), b1 [256], w2 [256][256], b2 [256], w3 [256], b3 [1],
+// FP32 masters with Adam moments, BF16 copies of w1 / w2 for the GEMMs.
+struct Model {
+  hipStream_t stream = nullptr;
+  float* w[6] = {};
+  float* g[6] = {};
+  float* m1[6] = {};
+  float* m2[6] = {};
+  __bf16 *w1b = nullptr, *w2b = nullptr;
+  float* loss = nullptr;
+  long long adam_t = 0;
+  size_t cap = 0;  // batch work buffers
+  __bf16 *xb = nullptr, *h1b = nullptr, *da2b = nullptr, *da1b = nullptr;
+  float *tb = nullptr, *z1 = nullptr, *z2 = nullptr, *a2 = nullptr, *y = nullptr, *dy = nullptr,
+        *dh1 = nullptr;
+  uint32_t* idx = nullptr;
+  std::string err;
+  static size_t param_count(int k);
+  void init(int seed);
+  void release();
+  void refresh_bf16();
+  void ensure_batch(size_t m);
+  void forward(hipStream_t s, size_t m, const __bf16* x, float* out, bool train);
+  void backward(hipStream_t s, size_t m);
+  void adam(hipStream_t s, float lr);
+  void pack_host_rows(size_t m, const float* x_host);
+  void train(Dataset& ds, int epochs, int batch, float lr, uint64_t seed, double* losses);
+};
+
 struct KStat {
   long long launches = 0;
   double ms = 0, bytes = 0;
@@ -124,6 +170,17 @@ struct Context {
 
   mps_solver_opts solver{};
   mps_solve_stats stats{};
+
+  // learned warm start and snapshot collection
+  Model* model = nullptr;
+  Dataset* dataset = nullptr;
+  double collect_frac = 0, collect_sr = 0;
+  uint64_t collect_seed = 0;
+  long long collect_step = 0;
+  bool collect_ok = false;
+  __bf16* feats = nullptr;  // n x kFeatPad of the last feature pass
+  float* ypred = nullptr;
+  size_t feats_cap = 0;
 
   // wall kinematics (Computer.hpp:993, :1013-1019)
   mps_wall_fn wall_fn = nullptr;
@@ -224,6 +281,14 @@ void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uin
 void ensure_sell_capacity(Context& c, size_t nslices, size_t slots);
 void ensure_ppe_vectors(Context& c, size_t n);  // mps_api.cpp
 void ppe_solve(Context& c);              // mps_cg.hip
+void residual64(Context& c, double& rr, double& bb, bool track);
+void reduce2(Context& c, const double* a, const double* b, int nparts);  // -> c.red_host[0..1]
+bool learned_features(Context& c, double& s_r, double& rr);             // mps_mlp.hip
+void learned_warm_start(Context& c);
+void collect_before(Context& c);
+void collect_after(Context& c);
+void model_forward_host(Model& m, size_t rows, const float* x, float* y);
+void model_grads_host(Model& m, size_t rows, const float* x, const float* t, float* loss_out);
 double max_speed_dt(Context& c);         // mps_particles.hip (DetermineDt)
 void device_math_check(size_t n, const double* a, const double* b, double* s, double* d);
 double device_copy_probe(size_t bytes);

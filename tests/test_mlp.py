@@ -1,0 +1,200 @@
+"""Learned warm start: the FP64 MLP oracle, the GPU forward / backward kernels
+against it, the feature definition, training and the solve it warm-starts."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import mlp as omlp
+from openmps_amd import scenes
+
+FEAT, HID = 132, 256
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = {"w1": rng.uniform(-1, 1, (HID, FEAT)) / np.sqrt(FEAT),
+         "b1": rng.uniform(-1, 1, HID) / np.sqrt(FEAT),
+         "w2": rng.uniform(-1, 1, (HID, HID)) / np.sqrt(HID),
+         "b2": rng.uniform(-1, 1, HID) / np.sqrt(HID),
+         "w3": rng.uniform(-1, 1, HID) / np.sqrt(HID) * scale,
+         "b3": rng.uniform(-1, 1, 1) / np.sqrt(HID)}
+    return {k: v.astype(np.float32).astype(np.float64) for k, v in p.items()}
+
+
+def test_oracle_gradients_match_finite_differences():
+    """Pins the oracle's analytic backward pass (FP64) entry by entry."""
+    p = random_params(1)
+    rng = np.random.Generator(np.random.PCG64(2))
+    x = rng.normal(size=(7, FEAT))
+    t = rng.normal(size=7)
+    _, g = omlp.loss_and_grads(p, x, t)
+    picks = {"w1": [(0, 0), (17, 100), (255, 131)], "b1": [(3,), (200,)],
+             "w2": [(5, 7), (250, 1)], "b2": [(9,)], "w3": [(0,), (128,)], "b3": [(0,)]}
+    for name, idxs in picks.items():
+        for idx in idxs:
+            num = omlp.numeric_grad(p, x, t, name, idx)
+            assert abs(num - g[name][idx]) <= 1e-6 * max(1.0, abs(num)), (name, idx)
+
+
+def test_oracle_gelu_derivative():
+    z = np.linspace(-6, 6, 1001)
+    h = 1e-6
+    num = (omlp.gelu(z + h) - omlp.gelu(z - h)) / (2 * h)
+    assert np.allclose(omlp.gelu_grad(z), num, atol=1e-8)
+    assert omlp.gelu(0.0) == 0.0 and abs(omlp.gelu(10.0) - 10.0) < 1e-12
+
+
+def test_bf16_rounding():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 3.0e-39, 65504.0])
+    got = omlp.to_bf16(vals)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: rounds to even (1.0);
+    # 1 + 3 * 2^-9 rounds up to 1 + 2^-7
+    assert got.tolist()[:4] == [1.0, 1.0, 1.0 + 2 ** -7, -2.5]
+    assert got[5] == 65536.0
+
+
+def emulate_gpu_forward(p, x):
+    """The GPU's rounding points: BF16 x, w1, w2 and h1; FP32 elsewhere."""
+    xb = omlp.to_bf16(x)
+    z1 = xb @ omlp.to_bf16(p["w1"]).T + p["b1"]
+    h1 = omlp.to_bf16(omlp.gelu(z1))
+    z2 = h1 @ omlp.to_bf16(p["w2"]).T + p["b2"]
+    return omlp.gelu(z2) @ p["w3"] + p["b3"][0]
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_oracle():
+    from openmps_amd.learned import Model
+    p = random_params(3)
+    m = Model(0)
+    m.set_params(p)
+    back = m.params()
+    for k in p:
+        assert np.array_equal(back[k].astype(np.float64), p[k]), k
+    rng = np.random.Generator(np.random.PCG64(4))
+    x = rng.normal(size=(1000, FEAT)).astype(np.float32).astype(np.float64)
+    y = m.forward(x).astype(np.float64)
+    ye = emulate_gpu_forward(p, x)
+    assert np.abs(y - ye).max() <= 2e-3 * (np.abs(ye).max() + 1e-3)
+    y64, _ = omlp.forward(p, x)  # unrounded FP64 reference: BF16-level agreement
+    assert np.linalg.norm(y - y64) <= 3e-2 * np.linalg.norm(y64)
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_oracle():
+    from openmps_amd.learned import Model
+    p = random_params(5)
+    pb = dict(p, w1=omlp.to_bf16(p["w1"]), w2=omlp.to_bf16(p["w2"]))
+    m = Model(0)
+    m.set_params(p)
+    rng = np.random.Generator(np.random.PCG64(6))
+    x = omlp.to_bf16(rng.normal(size=(4096, FEAT)))
+    t = rng.normal(size=4096)
+    loss, g = m.gradients(x, t)
+    lo, go = omlp.loss_and_grads(pb, x, t)
+    assert abs(loss - lo) <= 1e-2 * lo
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        err = np.linalg.norm(g[k] - go[k]) / np.linalg.norm(go[k])
+        assert err < 3e-2, (k, err)
+
+
+def _small_scene(seed=5):
+    parts, dom = scenes.dam_break(0.1, 0.06, 0.025, 0.05, 0.001, jitter=0.02, seed=seed)
+    env = dict(max_dt=1e-4, courant=0.1, g=9.8, rho=998.2, nu=1e-6, r_e_by_l0=2.1, l0=0.001,
+               r_lap_by_l0=3.1, **dom)
+    return parts, env
+
+
+@pytest.mark.gpu
+def test_features_definition():
+    """The feature rows equal the definition evaluated in numpy from the
+    particle state, the assembled system and r0 = b - A p_prev."""
+    import scipy.sparse
+    from openmps_amd import learned
+    parts, env = _small_scene()
+    c = cases.make("gpu", env, 1e-10)
+    c.add_particles(parts)
+    for _ in range(3):
+        c.forward_time(1e-4)
+    c.set_time(4e-4, 1e-4)
+    for st in ("search", "density", "error_correction", "explicit", "density", "save_x",
+               "ppe_assemble"):
+        c.stage(st)
+    F = learned.features(c)
+    rp, col, val, b, _ = c.ppe()
+    P = c.particles()
+    n = len(b)
+    A = scipy.sparse.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    r0 = b - A @ P["p"]
+    alive = (P["type"] != 2) & (P["type"] != 3)
+    s_r = np.sqrt((r0[alive] ** 2).sum() / alive.sum())
+    s_p = np.sqrt((P["p"][alive] ** 2).sum() / alive.sum())
+    cnt, ids = c.neighbors()
+    off = np.concatenate([[0], np.cumsum(cnt.astype(np.int64))])
+    e = c.env()
+    rl2 = e["r_lap"] ** 2
+    bad = 0
+    rng = np.random.Generator(np.random.PCG64(9))
+    for i in rng.choice(np.nonzero(alive)[0], 300, replace=False):
+        js = ids[off[i]:off[i + 1]]
+        dx, dz = P["x"][js] - P["x"][i], P["z"][js] - P["z"][i]
+        r2 = dx * dx + dz * dz
+        keep = (P["type"][js] != 2) & (r2 < rl2)
+        js, dx, dz, r2 = js[keep], dx[keep], dz[keep], r2[keep]
+        order = np.argsort(r2.astype(np.float32), kind="stable")[:32]
+        want = np.zeros(FEAT)
+        want[0] = len(js) / 32.0
+        want[1] = (P["n"][i] - e["n0"]) / e["n0"]
+        want[2] = P["p"][i] / s_p
+        want[3] = r0[i] / s_r
+        for k, q in enumerate(order):
+            want[4 + 4 * k:8 + 4 * k] = [dx[q] / e["l0"], dz[q] / e["l0"], P["p"][js[q]] / s_p,
+                                         r0[js[q]] / s_r]
+        if not np.allclose(F[i], omlp.to_bf16(want), rtol=1e-2, atol=2e-2):
+            bad += 1
+    assert bad <= 3  # ties in float distance may swap two equal-distance neighbours
+    assert np.all(F[~alive] == 0)
+
+
+@pytest.mark.gpu
+def test_snapshots_and_training_reduce_loss():
+    from openmps_amd import learned
+    from openmps_amd import _lib as L
+    parts, env = _small_scene(7)
+    ds = learned.Dataset(200_000)
+    c = cases.make("gpu", env, 1e-10)
+    c.set_solver(kind=L.SOLVER_CG_FP64, stop_rule=L.STOP_RHS_NORM, tol=1e-10)
+    c.add_particles(parts)
+    learned.attach_dataset(c, ds, 0.25, 11)
+    for _ in range(20):
+        c.forward_time(1e-4)
+    n = len(ds)
+    assert n > 5000
+    x, t = ds.rows(0, 10)
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(t))
+    m = learned.Model(3)
+    losses = m.train(ds, 6, 1024, 2e-3, 5)
+    assert losses[-1] < 0.7 * losses[0], losses
+
+
+@pytest.mark.gpu
+def test_learned_warm_start_is_judged_by_the_solver():
+    """With any model (here untrained) the solve still reaches the stop rule:
+    a worse guess is discarded for the previous-step guess."""
+    from openmps_amd import learned
+    from openmps_amd import _lib as L
+    parts, env = _small_scene(8)
+    c = cases.make("gpu", env, 1e-8)
+    c.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, warm_start=L.WARM_LEARNED,
+                 jacobi=True, tol=1e-8)
+    m = learned.Model(1)
+    learned.attach_model(c, m)
+    c.add_particles(parts)
+    for _ in range(5):
+        c.forward_time(1e-4)
+        s = c.solve_stats()
+        assert s["rel_residual"] <= 1e-8 * 1.5
+        if not s["learned_used"]:
+            assert s["initial_rel_residual"] == pytest.approx(s["prev_rel_residual"], rel=1e-9)
