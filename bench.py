@@ -84,6 +84,15 @@ def kernel_table(c):
     return out
 
 
+def profiled_table(c, steps: int, label: str):
+    """Per-kernel HIP-event timings over `steps` further steps."""
+    c.reset_kernel_stats()
+    c.set_profiling(True)
+    run(c, steps, f"{label} profiled")
+    c.set_profiling(False)
+    return kernel_table(c)
+
+
 def cpu_baseline(cfg, tol: float, budget_s: float):
     """The CPU oracle (C++/OpenMP FP64 restatement of the reference path) on
     one step of the same scene with the same stop rule."""
@@ -142,11 +151,10 @@ def main() -> int:
     main_c._bench_dt = cfg["dt"]
     if args.warmup:
         run(main_c, args.warmup, f"{args.solver} warmup")
-    main_c.reset_kernel_stats()
-    main_c.set_profiling(True)
+    # timed steps run without per-kernel events; the kernel table comes from
+    # the same number of further steps with events on
     elapsed, st = run(main_c, args.steps, f"{args.solver} timed")
-    main_c.set_profiling(False)
-    kt = kernel_table(main_c)
+    kt = profiled_table(main_c, args.steps, args.solver)
     solve_ms = sum(s["solve_ms"] for s in st) / len(st)
     iters = sum(s["iterations"] for s in st) / len(st)
 
@@ -172,12 +180,9 @@ def main() -> int:
         ref_c._bench_dt = cfg["dt"]
         if args.warmup:
             run(ref_c, args.warmup, f"{v} warmup")
-        ref_c.reset_kernel_stats()
-        ref_c.set_profiling(v == "fp64")
         _, stv = run(ref_c, args.steps, f"{v} timed")
-        ref_c.set_profiling(False)
         if v == "fp64":
-            kt64 = kernel_table(ref_c)
+            kt64 = profiled_table(ref_c, args.steps, v)
         solve[v] = summary(stv)
         del ref_c
 
@@ -191,9 +196,16 @@ def main() -> int:
                 "ms_per_launch": d["ms_per_launch"]}
 
     # dominant kernel: the SpMV of the solver that ran the timed steps
-    dom = "cg_spmv32" if VARIANTS[args.solver][0] else "cg_spmv"
+    dom = "cg_iter32" if VARIANTS[args.solver][0] else "cg_spmv"
     roofline = roof(kt, dom)
     roofline_fp64 = roof(kt64, "cg_spmv")
+
+    # a plain copy kernel of known size: the practical HBM ceiling beside the 8 TB/s peak
+    from openmps_amd import device_copy_probe
+    copy_bytes = 1 << 30
+    copy_ms = device_copy_probe(copy_bytes)
+    copy = {"bytes_moved": 2 * copy_bytes, "ms": copy_ms, "GBps": 2 * copy_bytes / copy_ms / 1e6}
+    log(f"  copy kernel: {copy['GBps']:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -225,6 +237,7 @@ def main() -> int:
         "kernels_fp64": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
                              "launches": v["launches"]} for k, v in kt64.items()
                          if k.startswith("cg_")},
+        "copy_kernel": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -228,8 +228,8 @@ int mps_get_features(mps_ctx* ctx, float* out);
 enum mps_kernel_id {
   MPS_K_CELL_KEYS = 0, MPS_K_CELL_SCAN, MPS_K_CELL_SCATTER, MPS_K_CELL_SORT, MPS_K_NEIGHBORS,
   MPS_K_DENSITY_ECS, MPS_K_EXPLICIT, MPS_K_UPDATE, MPS_K_DENSITY_SAVE, MPS_K_PPE_COUNT,
-  MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_SPMV32,
-  MPS_K_CG_UPDATE32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
+  MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_ITER32,
+  MPS_K_CG_INIT32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
   MPS_K_FEATURES, MPS_K_MLP,
   MPS_K_COUNT
 };

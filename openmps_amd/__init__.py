@@ -6,12 +6,12 @@ mirror of the reference's ``Environment`` / ``Computer`` interface over that
 ABI, plus the synthetic scene generator used by the benchmarks.
 """
 from .computer import (Computer, ComputerException, CreateComputer, Environment, Grid,
-                       GridException, MpsError, ParticleType, device_math_check,
-                       make_environment_from)
+                       GridException, MpsError, ParticleType, device_copy_probe,
+                       device_math_check, make_environment_from)
 from . import scenes
 
 __all__ = ["Computer", "ComputerException", "CreateComputer", "Environment", "Grid",
-           "GridException", "MpsError", "ParticleType", "device_math_check", "scenes",
+           "GridException", "MpsError", "ParticleType", "device_copy_probe", "device_math_check", "scenes",
            "make_environment"]
 
 

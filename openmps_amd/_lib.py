@@ -34,7 +34,7 @@ AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_
 
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
            "explicit", "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv",
-           "cg_update", "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other",
+           "cg_update", "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other",
            "features", "mlp")
 
 

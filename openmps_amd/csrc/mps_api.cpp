@@ -136,8 +136,10 @@ void free_all(Context& c) {
   dfree(c.scol); dfree(c.diag); dfree(c.diag32); dfree(c.b); dfree(c.sol);
   dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
-  dfree(c.r32); dfree(c.pa32); dfree(c.pb32); dfree(c.q32); dfree(c.d32); dfree(c.dinv32);
+  dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
   dfree(c.feats); dfree(c.ypred);
+  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
+  dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
@@ -198,7 +200,7 @@ void forward(Context& c, double dt) {
 const char* kKernelNames[MPS_K_COUNT] = {
     "cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs", "explicit",
     "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv", "cg_update",
-    "cg_spmv32", "cg_update32", "ir_residual", "gradient", "ds", "other", "features", "mlp"};
+    "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other", "features", "mlp"};
 
 }  // namespace
 
@@ -208,7 +210,9 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   const size_t ns = m / 64 + 1;
   dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.diag); dfree(c.diag32);
   dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q);
-  dfree(c.r32); dfree(c.pa32); dfree(c.pb32); dfree(c.q32); dfree(c.d32); dfree(c.dinv32);
+  dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
+  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16);
+  dfree(c.g32a); dfree(c.g32b);
   c.row_len = dalloc<uint32_t>(m);
   c.slice_width = dalloc<uint32_t>(ns);
   c.slice_off = dalloc<uint64_t>(ns + 1);
@@ -220,21 +224,26 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.pa = dalloc<double>(m);
   c.pb = dalloc<double>(m);
   c.q = dalloc<double>(m);
-  c.r32 = dalloc<float>(m);
   c.pa32 = dalloc<float>(m);
-  c.pb32 = dalloc<float>(m);
-  c.q32 = dalloc<float>(m);
   c.d32 = dalloc<float>(m);
   c.dinv32 = dalloc<float>(m);
+  c.ghost32 = dalloc<float>(m);
+  c.g32a = dalloc<float4>(m);
+  c.g32b = dalloc<float4>(m);
+  c.qpos = dalloc<uint2>(m);
+  c.slice_wide = dalloc<uint8_t>(ns);
+  c.w16 = dalloc<uint32_t>(ns);
+  c.off16 = dalloc<uint64_t>(ns + 1);
   MPS_HIP(hipMemsetAsync(c.sol, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pb, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa32, 0, m * 4, c.stream));
-  MPS_HIP(hipMemsetAsync(c.pb32, 0, m * 4, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.ppe_alloc = n;
   c.have_ppe = false;
   c.have_ppe32 = false;
+  c.ppe_from_particles = false;
+  c.have_col16 = false;
 }
 
 }  // namespace mps
@@ -314,6 +323,8 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.partA = mps::dalloc<double>(mps::kRedBlocks);
     c.partB = mps::dalloc<double>(mps::kRedBlocks);
     c.partC = mps::dalloc<double>(mps::kRedBlocks);
+    c.part3a = mps::dalloc<double>(3 * mps::kRedBlocks);
+    c.part3b = mps::dalloc<double>(3 * mps::kRedBlocks);
     c.red_out = mps::dalloc<double>(2);
     c.st = mps::dalloc<mps::CgStatus>(1);
     c.st32 = mps::dalloc<mps::CgStatus>(1);
@@ -452,6 +463,7 @@ int mps_set_particles(mps_ctx* ctx, const double* x, const double* z, const doub
     if (n) MPS_HIP(hipMemcpyAsync(c.nd, n, N * 8, h2d, c.stream));
     if (type) MPS_HIP(hipMemcpyAsync(c.type, type, N, h2d, c.stream));
     MPS_HIP(hipStreamSynchronize(c.stream));
+    c.ppe_from_particles = false;  // positions no longer those the system was assembled from
   });
 }
 

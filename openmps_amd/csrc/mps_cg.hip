@@ -21,6 +21,7 @@
 // lane.  These kernels use explicit FMA: the CG is checked against the
 // tolerance, not bit for bit.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "mps_device.hpp"
@@ -63,7 +64,9 @@ __device__ __forceinline__ bool already_converged(const CgStatus* st) {
 }
 
 // s + sum_k val[k] * v(col[k]) over the stored off-diagonal slots of row i,
-// four independent slots (loads, gathers) in flight per iteration.
+// four independent slots (loads, gathers) in flight per iteration.  The
+// value and column streams are read once per launch: nontemporal loads keep
+// them from evicting the gathered vector entries from the caches.
 template <class T, class V>
 __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
   const size_t base = A.off[i >> 6] + (i & 63);
@@ -73,8 +76,11 @@ __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
   uint32_t k = 0;
   for (; k + 4 <= wdt; k += 4) {
     const size_t o = static_cast<size_t>(k) * 64;
-    const uint32_t j0 = cp[o], j1 = cp[o + 64], j2 = cp[o + 128], j3 = cp[o + 192];
-    const T a0 = vp[o], a1 = vp[o + 64], a2 = vp[o + 128], a3 = vp[o + 192];
+    const uint32_t j0 = __builtin_nontemporal_load(cp + o), j1 = __builtin_nontemporal_load(cp + o + 64),
+                   j2 = __builtin_nontemporal_load(cp + o + 128),
+                   j3 = __builtin_nontemporal_load(cp + o + 192);
+    const T a0 = __builtin_nontemporal_load(vp + o), a1 = __builtin_nontemporal_load(vp + o + 64),
+            a2 = __builtin_nontemporal_load(vp + o + 128), a3 = __builtin_nontemporal_load(vp + o + 192);
     const T x0 = v(j0), x1 = v(j1), x2 = v(j2), x3 = v(j3);
     s = fma(a0, x0, s);
     s = fma(a1, x1, s);
@@ -83,9 +89,89 @@ __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
   }
   for (; k < wdt; k++) {
     const size_t o = static_cast<size_t>(k) * 64;
-    s = fma(vp[o], v(cp[o]), s);
+    s = fma(__builtin_nontemporal_load(vp + o), v(__builtin_nontemporal_load(cp + o)), s);
   }
   return s;
+}
+
+// The FP32 operator of a system assembled from particles, recomputed from
+// positions instead of read back (matrix-free): a_ij = 3 r_lap / (n0_lap r_ij^3)
+// (Computer.hpp:1290), r_ij from fixed-point positions with step h = extent /
+// 2^30 (exact integer differences), the stored columns as int16 offsets j - i
+// ("narrow" slices) or the SELL u32 columns ("wide" slices, slice-uniform
+// branch), loops to each row's own length, and
+//   (A p)_i = ghost_i p_i + sum_j a_ij (p_j - p_i),
+// i.e. the assembled diagonal -sum_j a_ij plus the SPP ghost share.
+struct MatFree {
+  const uint64_t* __restrict__ off;
+  const uint32_t* __restrict__ col;
+  const uint64_t* __restrict__ off16;
+  const int16_t* __restrict__ col16;
+  const uint8_t* __restrict__ wide;
+  const uint32_t* __restrict__ len;
+  const uint2* __restrict__ pos;
+  const float* __restrict__ ghost;
+  float c0;  // 3 r_lap / n0_lap / h^3 for r in fixed-point steps h
+  size_t n;
+  size_t rpb;
+};
+
+template <int UNROLL = 4, class V>
+__device__ __forceinline__ float op_apply(const Sell<float>& A, size_t i, float pi, V v) {
+  return sell_row(A, i, A.diag[i] * pi, v);
+}
+
+// a_ij (u_j - u_i) + s.  r^2 in fixed-point units is >= 1 for distinct
+// positions (never denormal), so the hardware reciprocal square root is used
+// without the denormal scaling rsqrtf adds; A.c0 carries the unit conversion.
+template <class J, class V>
+__device__ __forceinline__ float mf_term(const MatFree& A, uint2 qi, uint2 q, float vj, float pi, float s) {
+  const float fx = static_cast<float>(static_cast<int>(q.x - qi.x));
+  const float fz = static_cast<float>(static_cast<int>(q.y - qi.y));
+  const float t = __builtin_amdgcn_rsqf(fmaf(fx, fx, fz * fz));
+  return fmaf(A.c0 * (t * t * t), vj - pi, s);
+}
+
+// UNROLL stored columns (index loads, position and value gathers) in flight.
+template <int UNROLL, class J, class V>
+__device__ __forceinline__ float mf_sum(const MatFree& A, uint2 qi, uint32_t m, float pi, float s,
+                                        J colk, V v) {
+  uint32_t k = 0;
+  for (; k + UNROLL <= m; k += UNROLL) {
+    uint32_t j[UNROLL];
+    uint2 q[UNROLL];
+    float vj[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) j[u] = colk(k + u);
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) q[u] = A.pos[j[u]];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) vj[u] = v(j[u]);
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) s = mf_term<J, V>(A, qi, q[u], vj[u], pi, s);
+  }
+  for (; k < m; k++) {
+    const uint32_t j = colk(k);
+    s = mf_term<J, V>(A, qi, A.pos[j], v(j), pi, s);
+  }
+  return s;
+}
+
+template <int UNROLL = 4, class V>
+__device__ __forceinline__ float op_apply(const MatFree& A, size_t i, float pi, V v) {
+  const size_t sl = i >> 6;
+  const uint32_t m = A.len[i];
+  const uint2 qi = A.pos[i];
+  const float s = A.ghost[i] * pi;
+  if (A.wide[sl]) {
+    const uint32_t* __restrict__ cp = A.col + A.off[sl] + (i & 63);
+    return mf_sum<UNROLL>(A, qi, m, pi, s, [&](uint32_t k) { return cp[static_cast<size_t>(k) * 64]; }, v);
+  }
+  const int16_t* __restrict__ cp = A.col16 + A.off16[sl] + (i & 63);
+  const int ii = static_cast<int>(i);
+  return mf_sum<UNROLL>(A, qi, m, pi, s,
+                        [&](uint32_t k) { return static_cast<uint32_t>(ii + cp[static_cast<size_t>(k) * 64]); },
+                        v);
 }
 
 // r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
@@ -119,7 +205,7 @@ struct StopRule {
 // One CG iteration, first half (see file comment).  `it` is the 1-based
 // iteration number; FIRST marks it == 1, where p_1 = r_0.
 template <bool FIRST>
-__global__ void __launch_bounds__(NT) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
+__global__ void __launch_bounds__(NT, 8) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
                                                 const double* __restrict__ pold,
                                                 double* __restrict__ pnew, double* __restrict__ q,
                                                 const double* __restrict__ part_in,
@@ -230,62 +316,126 @@ __global__ void __launch_bounds__(NT) k_reduce2(const double* __restrict__ a,
 }
 
 // ---------------- mixed precision: FP32 CG inside FP64 refinement ----------------
+//
+// Each refinement sweep solves A d = r / |r| in FP32 with the Chronopoulos-Gear
+// form of preconditioned CG (Jacobi or none): one global reduction phase per
+// iteration, so one kernel per iteration.  Iteration i, given the reduced
+// gamma_i = (r_i, u_i), delta_i = (w_i, u_i) with u = M r, w = A u:
+//   beta_i  = gamma_i / gamma_{i-1},  alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})
+//   s_i = w_i + beta_i s_{i-1},  p_i = u_i + beta_i p_{i-1},  d += alpha_i p_i,
+//   r_{i+1} = r_i - alpha_i s_i,  u_{i+1} = M r_{i+1},  w_{i+1} = A u_{i+1},
+// then the partial sums of gamma_{i+1}, delta_{i+1} and r_{i+1}.r_{i+1}.  The
+// SpMV gathers u_{i+1,j} of other blocks' rows by recomputing them from the
+// previous iteration's (r, w, s, M) of row j, packed in one float4 that is
+// double-buffered by iteration parity; p and d are only touched by their own
+// row.  Dots accumulate in FP64.
 
-// r32 = r64 / |r64|, d32 = 0, partial sums of r32.r32 and r32.z (z = Dinv r).
-template <bool JAC>
-__global__ void __launch_bounds__(NT) k_ir_prep(size_t n, size_t rpb, const double* __restrict__ r64,
-                                                double inv_norm, const float* __restrict__ dinv,
-                                                float* __restrict__ r32, float* __restrict__ d32,
-                                                double* __restrict__ part_rr,
-                                                double* __restrict__ part_rz) {
-  __shared__ double lds[NT / 64];
-  const Rows rw = block_rows(n, rpb);
-  double arr = 0, arz = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const float ri = static_cast<float>(r64[i] * inv_norm);
-    r32[i] = ri;
-    d32[i] = 0.0f;
-    arr = fma(static_cast<double>(ri), static_cast<double>(ri), arr);
-    if (JAC) arz = fma(static_cast<double>(ri), static_cast<double>(ri * dinv[i]), arz);
+struct Dots3 {
+  double a, b, c;
+};
+// Deterministic block sums of three doubles per thread (one LDS round).
+__device__ __forceinline__ Dots3 block_sum3(double a, double b, double c, double* lds) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[wv] = a;
+    lds[NT / 64 + wv] = b;
+    lds[2 * (NT / 64) + wv] = c;
   }
-  const double s1 = block_sum<NT>(arr, lds);
-  const double s2 = JAC ? block_sum<NT>(arz, lds) : s1;
+  __syncthreads();
+  Dots3 s{lds[0], lds[NT / 64], lds[2 * (NT / 64)]};
+#pragma unroll
+  for (int k = 1; k < NT / 64; k++) {
+    s.a += lds[k];
+    s.b += lds[NT / 64 + k];
+    s.c += lds[2 * (NT / 64) + k];
+  }
+  __syncthreads();
+  return s;
+}
+__device__ __forceinline__ Dots3 reduce_partials3(const double* __restrict__ part, int count,
+                                                  double* lds) {
+  double a = 0, b = 0, c = 0;
+  for (int k = threadIdx.x; k < count; k += NT) {
+    a += part[k];
+    b += part[kRedBlocks + k];
+    c += part[2 * kRedBlocks + k];
+  }
+  return block_sum3(a, b, c, lds);
+}
+__device__ __forceinline__ void write_partials3(double* __restrict__ part, Dots3 s) {
   if (threadIdx.x == 0) {
-    part_rr[blockIdx.x] = s1;
-    part_rz[blockIdx.x] = s2;
+    part[blockIdx.x] = s.a;
+    part[kRedBlocks + blockIdx.x] = s.b;
+    part[2 * kRedBlocks + blockIdx.x] = s.c;
   }
 }
 
-// FP32 SpMV half-iteration.  Stop test on r.r relative to the sweep's initial
-// r.r; beta from r.z (== r.r without preconditioner); dots in FP64.
-template <bool FIRST, bool JAC>
-__global__ void __launch_bounds__(NT) k_cg32_spmv(Sell<float> A, const float* __restrict__ r,
-                                                  const float* __restrict__ dinv,
-                                                  const float* __restrict__ pold,
-                                                  float* __restrict__ pnew, float* __restrict__ q,
-                                                  const double* __restrict__ part_rr,
-                                                  const double* __restrict__ part_rz, int nparts,
-                                                  double* __restrict__ part_out, CgStatus* st,
-                                                  double tol_rel, long long it) {
-  __shared__ double lds[NT / 64];
+// r_0 = r64 / |r64|, u_0 = M r_0, w_0 = A u_0; G_out = (r_0, w_0, 0, M), p = d = 0,
+// partials of gamma_0, delta_0, r_0.r_0.  Neighbour u_0,j straight from r64.
+template <class Op>
+__global__ void __launch_bounds__(NT, 8) k_cgf_init(Op A, const double* __restrict__ r64, double inv_norm,
+                                                 const float* __restrict__ dinv, float4* __restrict__ gout,
+                                                 float* __restrict__ p, float* __restrict__ d,
+                                                 double* __restrict__ part_out) {
+  __shared__ double lds[3 * (NT / 64)];
+  const Rows rw = block_rows(A.n, A.rpb);
+  double ag = 0, ad = 0, ar = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    const float dv = dinv ? dinv[i] : 1.0f;
+    const float ri = static_cast<float>(r64[i] * inv_norm);
+    const float ui = dv * ri;
+    const float wi = op_apply(A, i, ui, [&](uint32_t j) {
+      return (dinv ? dinv[j] : 1.0f) * static_cast<float>(r64[j] * inv_norm);
+    });
+    gout[i] = make_float4(ri, wi, 0.0f, dv);
+    p[i] = 0.0f;
+    d[i] = 0.0f;
+    ag = fma(static_cast<double>(ri), static_cast<double>(ui), ag);
+    ad = fma(static_cast<double>(wi), static_cast<double>(ui), ad);
+    ar = fma(static_cast<double>(ri), static_cast<double>(ri), ar);
+  }
+  write_partials3(part_out, block_sum3(ag, ad, ar, lds));
+}
+
+// Columns in flight per lane in the iteration kernel (each gathers a float4
+// and a position): 3 keeps it within 64 VGPRs, i.e. 8 waves per SIMD.
+constexpr int kIterUnroll = 3;
+
+// Iteration i (0-based): stop test on r_i.r_i < tol_rel^2 r_0.r_0, then the
+// updates above.  `it` = i + 1 is the iteration number the launch completes.
+template <class Op>
+__global__ void __launch_bounds__(NT, 8) k_cgf_iter(Op A, const float4* __restrict__ gin,
+                                                 float4* __restrict__ gout, float* __restrict__ p,
+                                                 float* __restrict__ d, const double* __restrict__ part_in,
+                                                 int nparts, double* __restrict__ part_out, CgStatus* st,
+                                                 double tol_rel, long long it) {
+  __shared__ double lds[3 * (NT / 64)];
   if (already_converged(st)) return;
+  const Dots3 g = reduce_partials3(part_in, nparts, lds);  // gamma_i, delta_i, rr_i
   const int par = static_cast<int>(it & 1);
-  const double rr = reduce_partials<NT>(part_rr, nparts, lds);
-  const double rz = JAC ? reduce_partials<NT>(part_rz, nparts, lds) : rr;
-  double tol;
+  const bool first = (it == 1);
+  double tol, beta = 0.0, alpha;
   bool conv;
-  if (FIRST) {
-    tol = rr * tol_rel * tol_rel;
+  if (first) {
+    tol = g.c * tol_rel * tol_rel;
     conv = (tol == 0);
+    alpha = g.a / g.b;
   } else {
     tol = st->tol;
-    conv = rr < tol;
+    conv = g.c < tol;
+    beta = g.a / st->gamma[par ^ 1];
+    alpha = g.a / (g.b - beta * g.a / st->alpha[par ^ 1]);
   }
   if (lead()) {
-    st->rr[par] = rz;
-    if (FIRST) {
+    st->rr[par] = g.c;
+    st->gamma[par] = g.a;
+    st->alpha[par] = alpha;
+    if (first) {
       st->tol = tol;
-      st->rr0 = rr;
+      st->rr0 = g.c;
     }
     if (conv) {
       st->converged = 1;
@@ -293,53 +443,27 @@ __global__ void __launch_bounds__(NT) k_cg32_spmv(Sell<float> A, const float* __
     }
   }
   if (conv) return;
-  const float beta = FIRST ? 0.0f : static_cast<float>(rz / st->rr[par ^ 1]);
+  const float fa = static_cast<float>(alpha), fb = static_cast<float>(beta);
   const Rows rw = block_rows(A.n, A.rpb);
-  double acc = 0;
+  double ag = 0, ad = 0, ar = 0;
   for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const float zi = JAC ? r[i] * dinv[i] : r[i];
-    const float pi = FIRST ? zi : fmaf(beta, pold[i], zi);
-    pnew[i] = pi;
-    const float s = sell_row(A, i, A.diag[i] * pi, [&](uint32_t j) {
-      const float zj = JAC ? r[j] * dinv[j] : r[j];
-      return FIRST ? zj : fmaf(beta, pold[j], zj);
+    const float4 h = gin[i];
+    const float si = fmaf(fb, h.z, h.y);
+    const float pi = fmaf(fb, p[i], h.w * h.x);
+    d[i] = fmaf(fa, pi, d[i]);
+    p[i] = pi;
+    const float ri = fmaf(-fa, si, h.x);
+    const float ui = h.w * ri;
+    const float wi = op_apply<kIterUnroll>(A, i, ui, [&](uint32_t j) {
+      const float4 q = gin[j];
+      return q.w * fmaf(-fa, fmaf(fb, q.z, q.y), q.x);
     });
-    q[i] = s;
-    acc = fma(static_cast<double>(pi), static_cast<double>(s), acc);
+    gout[i] = make_float4(ri, wi, si, h.w);
+    ag = fma(static_cast<double>(ri), static_cast<double>(ui), ag);
+    ad = fma(static_cast<double>(wi), static_cast<double>(ui), ad);
+    ar = fma(static_cast<double>(ri), static_cast<double>(ri), ar);
   }
-  const double sp = block_sum<NT>(acc, lds);
-  if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
-}
-
-template <bool JAC>
-__global__ void __launch_bounds__(NT) k_cg32_update(size_t n, size_t rpb, float* __restrict__ d,
-                                                    float* __restrict__ r, const float* __restrict__ p,
-                                                    const float* __restrict__ q,
-                                                    const float* __restrict__ dinv,
-                                                    const double* __restrict__ part_in, int nparts,
-                                                    double* __restrict__ part_rr,
-                                                    double* __restrict__ part_rz, CgStatus* st,
-                                                    long long it) {
-  __shared__ double lds[NT / 64];
-  if (already_converged(st)) return;
-  const int par = static_cast<int>(it & 1);
-  const double pq = reduce_partials<NT>(part_in, nparts, lds);
-  const float alpha = static_cast<float>(st->rr[par] / pq);
-  const Rows rw = block_rows(n, rpb);
-  double arr = 0, arz = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    d[i] = fmaf(alpha, p[i], d[i]);
-    const float ri = fmaf(-alpha, q[i], r[i]);
-    r[i] = ri;
-    arr = fma(static_cast<double>(ri), static_cast<double>(ri), arr);
-    if (JAC) arz = fma(static_cast<double>(ri), static_cast<double>(ri * dinv[i]), arz);
-  }
-  const double s1 = block_sum<NT>(arr, lds);
-  const double s2 = JAC ? block_sum<NT>(arz, lds) : s1;
-  if (threadIdx.x == 0) {
-    part_rr[blockIdx.x] = s1;
-    part_rz[blockIdx.x] = s2;
-  }
+  write_partials3(part_out, block_sum3(ag, ad, ar, lds));
   if (lead()) st->iterations = it;
 }
 
@@ -350,12 +474,12 @@ __global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict
   if (i < n) x[i] = fma(static_cast<double>(d[i]), norm, x[i]);
 }
 
-__global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const float* __restrict__ diag,
+__global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const double* __restrict__ diag,
                                                     float* __restrict__ dinv) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i < n) {
-    const float d = diag[i];
-    dinv[i] = (d != 0.0f) ? 1.0f / d : 1.0f;
+    const double d = diag[i];
+    dinv[i] = (d != 0.0) ? static_cast<float>(1.0 / d) : 1.0f;
   }
 }
 
@@ -495,28 +619,36 @@ void residual64(Context& c, double& rr, double& bb, bool track) {
 namespace {
 
 // Mixed precision: FP32 CG sweeps on A d = r, FP64 residual and accumulation.
-void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
+// `A` is the FP32 operator (matrix-free or the assembled values rounded),
+// `b_iter` the algorithmic bytes of one inner iteration.
+template <class Op>
+void solve_mixed_op(Context& c, const Op& A, double b_iter, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
-  ppe_make_fp32(c);
   const bool jac = c.solver.jacobi != 0;
   if (jac)
     hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
-                       static_cast<const float*>(c.diag32), c.dinv32);
-  const Sell<float> A = sell<float>(c, c.sval32, c.diag32, L.rpb);
+                       static_cast<const double*>(c.diag), c.dinv32);
+  const float* dinv = jac ? c.dinv32 : nullptr;
   const double inner_tol = c.solver.inner_tol > 0 ? c.solver.inner_tol : 1e-4;
   const long long inner_max = c.solver.inner_max_iter > 0 ? c.solver.inner_max_iter : max_iter;
-  float* pbuf[2] = {c.pa32, c.pb32};
+  float4* gbuf[2] = {c.g32a, c.g32b};
+  double* parts[2] = {c.part3a, c.part3b};
   long long total = 0;
   double rr0 = 0, bb = 0, rr = 0;
-  const double b_spmv = spmv_bytes(c, 4, 4.0 * 4 + (jac ? 4.0 : 0.0));
-  const double b_upd = n * (24.0 + (jac ? 4.0 : 0.0));
   int outer = 0;
   const int max_outer = 64;
+  // an explicit iteration limit caps the FP32 iterations of all sweeps; by
+  // default each sweep may take up to N (CG's finite-termination bound) and
+  // the solve fails when a sweep reduces the FP64 residual norm by less than sqrt(2)
+  const long long total_cap = c.solver.max_iter > 0 ? c.solver.max_iter : LLONG_MAX;
+  double rr_prev = 0;
   bool done = false;
   for (;; outer++) {
     residual64(c, rr, bb, true);
     if (outer == 0) rr0 = rr;
+    const bool stalled = outer > 0 && !(rr < 0.5 * rr_prev);
+    rr_prev = rr;
     double target;  // outer r.r to reach
     if (sr.rule == MPS_STOP_RHS_NORM) {
       target = (sr.tol * sr.tol) * bb;
@@ -525,53 +657,29 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
       target = rr0 * sr.tol * sr.tol;
       done = (outer == 0) ? (target == 0) : (rr < target);
     }
-    if (done || outer >= max_outer || total >= max_iter || !(rr == rr)) break;
+    if (done || stalled || outer >= max_outer || total >= total_cap || !(rr == rr)) break;
     const double norm = std::sqrt(rr);
     // this sweep reduces the residual by inner_tol, or only as far as the
     // outer target needs (with a factor 2 of margin), whichever is less work
     const double sweep_tol = std::max(inner_tol, 0.5 * std::sqrt(target / rr));
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
-    if (jac)
-      hipLaunchKernelGGL(k_ir_prep<true>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb,
-                         static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
-                         c.r32, c.d32, c.partB, c.partC);
-    else
-      hipLaunchKernelGGL(k_ir_prep<false>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb,
-                         static_cast<const double*>(c.r), 1.0 / norm, static_cast<const float*>(c.dinv32),
-                         c.r32, c.d32, c.partB, c.partC);
+    tracked(c, MPS_K_CG_INIT32, b_iter + n * 8.0, [&] {
+      hipLaunchKernelGGL(k_cgf_init<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+                         static_cast<const double*>(c.r), 1.0 / norm, dinv, gbuf[1], c.pa32, c.d32,
+                         parts[1]);
+    });
     check_launch();
     long long it = 1;
     while (true) {
       const long long chunk = next_chunk(it - 1, c.last_inner);
       for (long long m = 0; m < chunk && it <= inner_max; m++, it++) {
-        const float* pold = pbuf[(it + 1) & 1];
-        float* pnew = pbuf[it & 1];
-        tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
-#define MPS_S32(F, J)                                                                                   \
-  hipLaunchKernelGGL((k_cg32_spmv<F, J>), dim3(L.grid), dim3(NT), 0, c.stream, A,                       \
-                     static_cast<const float*>(c.r32), static_cast<const float*>(c.dinv32), pold, pnew, \
-                     c.q32, static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),  \
-                     L.grid, c.partA, c.st32, sweep_tol, it)
-          if (it == 1) {
-            if (jac) MPS_S32(true, true);
-            else MPS_S32(true, false);
-          } else {
-            if (jac) MPS_S32(false, true);
-            else MPS_S32(false, false);
-          }
-#undef MPS_S32
-        }, it);
-        tracked(c, MPS_K_CG_UPDATE32, b_upd, [&] {
-          if (jac)
-            hipLaunchKernelGGL(k_cg32_update<true>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.d32,
-                               c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
-                               static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
-                               L.grid, c.partB, c.partC, c.st32, it);
-          else
-            hipLaunchKernelGGL(k_cg32_update<false>, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.d32,
-                               c.r32, static_cast<const float*>(pnew), static_cast<const float*>(c.q32),
-                               static_cast<const float*>(c.dinv32), static_cast<const double*>(c.partA),
-                               L.grid, c.partB, c.partC, c.st32, it);
+        // iteration it reads buffer it % 2 and writes the other
+        const int in = static_cast<int>(it & 1);
+        tracked(c, MPS_K_CG_ITER32, b_iter, [&] {
+          hipLaunchKernelGGL(k_cgf_iter<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+                             static_cast<const float4*>(gbuf[in]), gbuf[in ^ 1], c.pa32, c.d32,
+                             static_cast<const double*>(parts[in]), L.grid, parts[in ^ 1], c.st32,
+                             sweep_tol, it);
         }, it);
       }
       check_launch();
@@ -580,8 +688,8 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     }
     const long long inner = c.st32_host->converged ? c.st32_host->iterations : inner_max;
     const long long launched = std::min(it - 1, inner_max);
-    uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
-    uncount(c, MPS_K_CG_UPDATE32, launched - inner, b_upd);
+    // the launch that detected convergence did no work: count it out
+    uncount(c, MPS_K_CG_ITER32, launched - inner, b_iter);
     if (outer == 0) c.last_inner = inner;
     total += inner;
     hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
@@ -596,6 +704,31 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   if (!done)
     throw Failure{MPS_ERR_CG_NOT_CONVERGED,
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
+}
+
+void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  const bool jac = c.solver.jacobi != 0;
+  // per row and iteration: the packed (r, w, s, M) read and written, p and d
+  // read and written; + Dinv read by the initialisation only
+  const double vec = 16.0 * 2 + 4.0 * 4;
+  if (c.ppe_from_particles) {
+    ppe_compress_columns(c);
+    const MatFree A{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
+                    static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
+                                       c.qpos_inv_h),
+                    n, L.rpb};
+    // int16 columns of narrow slices, u32 of wide ones; row length, ghost
+    // and position of every row; slice tables
+    const double bytes = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 +
+                         n * (vec + 4 + 4 + 8) + c.nslices * 25.0;
+    solve_mixed_op(c, A, bytes, sr, max_iter);
+  } else {
+    ppe_make_fp32(c);
+    (void)jac;
+    solve_mixed_op(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
+  }
 }
 
 }  // namespace

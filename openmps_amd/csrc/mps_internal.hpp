@@ -25,6 +25,8 @@ constexpr uint8_t kFluid = 0, kWall = 1, kDummy = 2, kDisabled = 3;
 // the same kernel still read the previous one.
 struct CgStatus {
   double rr[2];
+  double gamma[2];    // FP32 inner solver: (r, M r) and alpha by iteration parity
+  double alpha[2];
   double tol;
   double rr0;
   double b2;          // |b|^2 (stop rule on |b|)
@@ -147,6 +149,22 @@ struct Context {
   size_t ppe_alloc = 0;    // capacity of the PPE / CG vectors
   bool have_ppe = false;
   bool have_ppe32 = false;
+  // the system was assembled from the current particle positions, so the
+  // FP32 solver may recompute off-diagonals from them (matrix-free)
+  bool ppe_from_particles = false;
+  float* ghost32 = nullptr;  // SPP ghost share of the diagonal (1 on rows without unknowns)
+  uint2* qpos = nullptr;     // fixed-point positions of the matrix-free operator
+  double qpos_inv_h = 0;     // their scale: q = (x - min) / h
+  // compressed columns of the matrix-free FP32 SpMV: slices whose column
+  // offsets j - i all fit in int16 store them in col16 (offset off16[s]);
+  // the others ("wide") keep the SELL u32 columns
+  uint8_t* slice_wide = nullptr;
+  uint32_t* w16 = nullptr;
+  uint64_t* off16 = nullptr;
+  int16_t* col16 = nullptr;
+  size_t col16_cap = 0;
+  bool have_col16 = false;
+  double mf_nnz_wide = 0;
   unsigned long long* ppe_counters = nullptr;       // [0] off-diagonal nnz
   unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots
   double nnz_offdiag = 0, sell_slots = 0;
@@ -162,8 +180,10 @@ struct Context {
   CgStatus* st32_host = nullptr;
   long long last_inner = 0;
   // mixed-precision work vectors
-  float *r32 = nullptr, *pa32 = nullptr, *pb32 = nullptr, *q32 = nullptr, *d32 = nullptr;
+  float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
   float* dinv32 = nullptr;
+  float4 *g32a = nullptr, *g32b = nullptr;  // packed (r, w, s, M) of the inner CG, by parity
+  double *part3a = nullptr, *part3b = nullptr;  // 3 x kRedBlocks partial sums, by parity
   IrStatus* ir = nullptr;
   IrStatus* ir_host = nullptr;
   long long last_iters = 0;
@@ -274,6 +294,7 @@ void pressure_gradient(Context& c);
 void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
 void ppe_make_fp32(Context& c);
+void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
 void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,
                     const double* val, const double* b, const double* x0);
 void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uint32_t>& col,

@@ -33,7 +33,8 @@ struct NV {
 
 struct PpeConsts {
   double re, re2, rl, rl2, n0, n0l, rho, dt;
-  int warm;  // mps_warm_start
+  int warm;             // mps_warm_start
+  double qx0, qz0, qinv;  // fixed-point positions: q = (x - x0) * qinv mod 2^32
 };
 
 __device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, double sz, double nws_i,
@@ -62,11 +63,16 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
                                                       double* __restrict__ b, double* __restrict__ sol,
                                                       uint32_t* __restrict__ row_len,
                                                       uint32_t* __restrict__ slice_width,
+                                                      uint2* __restrict__ qpos,
                                                       unsigned long long* nnz_total) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   uint32_t cnt = 0;
   if (i < n) {
+    // integer differences of q are exact, so r_ij in the FP32 operator
+    // carries only the quantisation step (|domain| / 2^30)
+    qpos[i] = make_uint2(static_cast<uint32_t>(static_cast<long long>(rint((x[i] - k.qx0) * k.qinv))),
+                         static_cast<uint32_t>(static_cast<long long>(rint((z[i] - k.qz0) * k.qinv))));
     const uint8_t ti = type[i];
     if (ti == kDummy || ti == kDisabled) {
       b[i] = 0;
@@ -103,8 +109,19 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
   }
 }
 
-// HL off-diagonals and diagonal (Computer.hpp:1232-1328).
-template <bool F32>
+// HL off-diagonals and diagonal (Computer.hpp:1232-1328).  ghost32 keeps the
+// SPP ghost's share of the diagonal (diag = ghost - sum_j a_ij) for the
+// matrix-free FP32 operator; rows without unknowns (dummy, disabled) get
+// ghost32 = diag = 1 and no off-diagonals.
+//
+// The diagonal is summed in neighbour-list order (bit-exact with the
+// reference); the off-diagonals are stored with ascending column.  A list is
+// the concatenation of at most 9 ascending runs (one per cell, each cell's
+// members ascending), so a 9-way merge orders it without a sort.  Ascending
+// columns make slot k of neighbouring rows point at neighbouring particles:
+// the solver's gathers of a wavefront touch a few cache lines, not 64.
+constexpr int kRuns = 9;
+
 __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      const uint8_t* __restrict__ type, NV nb,
@@ -112,8 +129,8 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
                                                      const uint64_t* __restrict__ slice_off,
                                                      const uint32_t* __restrict__ slice_width,
                                                      double* __restrict__ val, uint32_t* __restrict__ col,
-                                                     double* __restrict__ diag, float* __restrict__ val32,
-                                                     float* __restrict__ diag32) {
+                                                     double* __restrict__ diag,
+                                                     float* __restrict__ ghost32) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -122,26 +139,36 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   const uint32_t self = static_cast<uint32_t>(i);
   const uint8_t ti = type[i];
   uint32_t kk = 0;
-  double d;
+  double d, g = 0.0;
   if (ti == kDummy || ti == kDisabled) {
     d = 1.0;
+    g = 1.0;
   } else {
     const double xi = x[i], zi = z[i];
     double s = 0.0, sx = 0, sz = 0;
     const uint32_t m = nb.count[i];
+    uint32_t rs[kRuns] = {};  // first list position of each ascending run
+    int nr = 0;
+    bool merge = true;
+    uint32_t prev = 0;
     for (uint32_t q = 0; q < m; q++) {
       const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
+      if (q == 0 || j < prev) {
+        if (nr < kRuns) {
+#pragma unroll
+          for (int r = 0; r < kRuns; r++)
+            if (r == nr) rs[r] = q;
+          nr++;
+        } else {
+          merge = false;  // not a cell-ordered list: store in list order
+        }
+      }
+      prev = j;
       const double dx = x[j] - xi, dz = z[j] - zi;
       const double r2 = dx * dx + dz * dz;
       if (r2 < k.rl2 && type[j] != kDummy) {
         const double r = sqrt(r2);
-        const double a = 3 * k.rl / k.n0l / (r * r * r);
-        const size_t o = base + static_cast<size_t>(kk) * 64;
-        val[o] = a;
-        col[o] = j;
-        if (F32) val32[o] = static_cast<float>(a);
-        kk++;
-        s += -a;
+        s += -(3 * k.rl / k.n0l / (r * r * r));
       }
       if (r2 < k.re2) {
         const double wg = weight(sqrt(r2), k.re);
@@ -155,17 +182,120 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
       const double r = sqrt(rx * rx + rz * rz);
       const double a = 3 * k.rl / k.n0l / (r * r * r);
       s += -a;
+      g = -a;
     }
     d = s;
+    auto emit = [&](uint32_t j) {
+      const double dx = x[j] - xi, dz = z[j] - zi;
+      const double r2 = dx * dx + dz * dz;
+      if (r2 < k.rl2 && type[j] != kDummy) {
+        const double r = sqrt(r2);
+        const size_t o = base + static_cast<size_t>(kk) * 64;
+        val[o] = 3 * k.rl / k.n0l / (r * r * r);
+        col[o] = j;
+        kk++;
+      }
+    };
+    if (merge) {
+      uint32_t cur[kRuns], end[kRuns], hd[kRuns];
+#pragma unroll
+      for (int r = 0; r < kRuns; r++) {
+        cur[r] = (r < nr) ? rs[r] : 0u;
+        end[r] = (r < nr) ? m : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r + 1 < kRuns; r++)
+        if (r + 1 < nr) end[r] = rs[r + 1];
+#pragma unroll
+      for (int r = 0; r < kRuns; r++)
+        hd[r] = (cur[r] < end[r]) ? nb.ids[nbr_slot(self, cur[r], nb.kmax)] : 0xffffffffu;
+      for (uint32_t t = 0; t < m; t++) {
+        uint32_t best = hd[0];
+        int sel = 0;
+#pragma unroll
+        for (int r = 1; r < kRuns; r++)
+          if (hd[r] < best) {
+            best = hd[r];
+            sel = r;
+          }
+        uint32_t nc = 0, ne = 0;
+#pragma unroll
+        for (int r = 0; r < kRuns; r++)
+          if (r == sel) {
+            nc = cur[r] + 1;
+            ne = end[r];
+          }
+        const uint32_t nh = (nc < ne) ? nb.ids[nbr_slot(self, nc, nb.kmax)] : 0xffffffffu;
+#pragma unroll
+        for (int r = 0; r < kRuns; r++)
+          if (r == sel) {
+            cur[r] = nc;
+            hd[r] = nh;
+          }
+        emit(best);
+      }
+    } else {
+      for (uint32_t q = 0; q < m; q++) emit(nb.ids[nbr_slot(self, q, nb.kmax)]);
+    }
   }
   for (; kk < width; kk++) {
     const size_t o = base + static_cast<size_t>(kk) * 64;
     val[o] = 0.0;
     col[o] = self;
-    if (F32) val32[o] = 0.0f;
   }
   diag[i] = d;
-  if (F32) diag32[i] = static_cast<float>(d);
+  ghost32[i] = static_cast<float>(g);
+}
+
+// Per slice: whether every stored column offset j - i fits in int16; the
+// int16 widths of the narrow slices (0 for wide ones) and the wide nnz.
+__global__ void __launch_bounds__(kBlock) k_slice_range(size_t n, const uint64_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ width,
+                                                        const uint32_t* __restrict__ row_len,
+                                                        const uint32_t* __restrict__ col,
+                                                        uint8_t* __restrict__ wide,
+                                                        uint32_t* __restrict__ w16,
+                                                        unsigned long long* nnz_wide) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  int far = 0;
+  uint32_t len = 0;
+  if (i < n) {
+    len = row_len[i];
+    const size_t base = off[i >> 6] + (i & 63);
+    for (uint32_t k = 0; k < len; k++) {
+      const long long d = static_cast<long long>(col[base + static_cast<size_t>(k) * 64]) - static_cast<long long>(i);
+      if (d > 32767 || d < -32768) far = 1;
+    }
+  }
+  const unsigned anyfar = wave_max_u32(static_cast<unsigned>(far));
+  const double wsum = wave_sum(static_cast<double>(len));
+  if ((threadIdx.x & 63) == 0) {
+    const size_t s = i >> 6;
+    if (s * 64 < n) {
+      wide[s] = static_cast<uint8_t>(anyfar);
+      w16[s] = anyfar ? 0u : width[s];
+      if (anyfar) atomicAdd(nnz_wide, static_cast<unsigned long long>(wsum));
+    }
+  }
+}
+
+// int16 column offsets of the narrow slices (padding: offset 0, never read).
+__global__ void __launch_bounds__(kBlock) k_fill16(size_t n, const uint64_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ width,
+                                                   const uint32_t* __restrict__ col,
+                                                   const uint8_t* __restrict__ wide,
+                                                   const uint64_t* __restrict__ off16,
+                                                   int16_t* __restrict__ col16) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const size_t s = i >> 6;
+  if (wide[s]) return;
+  const size_t base = off[s] + (i & 63), b16 = off16[s] + (i & 63);
+  const uint32_t w = width[s];
+  for (uint32_t k = 0; k < w; k++) {
+    const long long d = static_cast<long long>(col[base + static_cast<size_t>(k) * 64]) - static_cast<long long>(i);
+    col16[b16 + static_cast<size_t>(k) * 64] = static_cast<int16_t>(d);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_to_f32(size_t n, const double* __restrict__ a,
@@ -190,8 +320,6 @@ void ensure_sell_capacity(Context& c, size_t nslices, size_t slots) {
   (void)nslices;
 }
 
-static bool want_fp32(const Context& c) { return c.solver.kind == MPS_SOLVER_MIXED_IR; }
-
 void ppe_assemble(Context& c) {
   const size_t n = c.n;
   c.ppe_n = n;
@@ -201,17 +329,18 @@ void ppe_assemble(Context& c) {
   if (n == 0) return;
   // worst case: every row as long as the longest neighbour list
   ensure_sell_capacity(c, c.nslices, c.nslices * 64 * static_cast<size_t>(c.kmax));
-  const bool f32 = want_fp32(c);
-  if (f32 && !c.sval32) c.sval32 = dalloc<float>(c.sell_cap);
+  const double extent = std::max(c.env.max_x - c.env.min_x, c.env.max_z - c.env.min_z);
+  c.qpos_inv_h = (extent > 0) ? 1073741824.0 / extent : 1.0 / c.env.l0;
   PpeConsts k{c.env.r_e, c.env.r_e * c.env.r_e, c.env.r_lap, c.env.r_lap * c.env.r_lap, c.env.n0,
-              c.env.n0_lap, c.env.rho, c.dt, c.solver.warm_start};
+              c.env.n0_lap, c.env.rho, c.dt, c.solver.warm_start, c.env.min_x, c.env.min_z,
+              c.qpos_inv_h};
   const NV nb{c.nbr, c.ncount, c.kmax};
   MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 2 * sizeof(unsigned long long), c.stream));
-  tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 20.0, [&] {
+  tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 28.0, [&] {
     hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
                        c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                        static_cast<const double*>(c.ecs), c.b, c.sol, c.row_len, c.slice_width,
-                       c.ppe_counters);
+                       c.qpos, c.ppe_counters);
   });
   check_launch();
   scan_slices(c, c.slice_width, c.nslices, c.slice_off);
@@ -222,24 +351,46 @@ void ppe_assemble(Context& c) {
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.nnz_offdiag = c.ppe_counters_host[0];
   c.sell_slots = c.ppe_counters_host[1];
-  const double bytes = n * (16.0 + 1 + 8 + 4 + 8) + 4.0 * c.nbr_total +
-                       c.sell_slots * (12.0 + (f32 ? 4.0 : 0.0)) + n * (f32 ? 4.0 : 0.0);
+  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 12.0;
   tracked(c, MPS_K_PPE_FILL, bytes, [&] {
-    if (f32)
-      hipLaunchKernelGGL(k_ppe_fill<true>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
-                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
-                         static_cast<const uint64_t*>(c.slice_off),
-                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.sval32,
-                         c.diag32);
-    else
-      hipLaunchKernelGGL(k_ppe_fill<false>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
-                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
-                         static_cast<const uint64_t*>(c.slice_off),
-                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag,
-                         static_cast<float*>(nullptr), static_cast<float*>(nullptr));
+    hipLaunchKernelGGL(k_ppe_fill, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
+                       static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                       static_cast<const uint64_t*>(c.slice_off),
+                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.ghost32);
   });
   check_launch();
-  c.have_ppe32 = f32;
+  c.ppe_from_particles = true;
+  c.have_col16 = false;
+}
+
+void ppe_compress_columns(Context& c) {
+  const size_t n = c.ppe_n, ns = c.nslices;
+  if (n == 0 || c.have_col16) return;
+  if (c.col16_cap < c.sell_slots) {
+    dfree(c.col16);
+    c.col16_cap = c.sell_cap;
+    c.col16 = dalloc<int16_t>(c.col16_cap);
+  }
+  MPS_HIP(hipMemsetAsync(c.ppe_counters + 1, 0, sizeof(unsigned long long), c.stream));
+  tracked(c, MPS_K_OTHER, n * 4.0 + c.nnz_offdiag * 4.0, [&] {
+    hipLaunchKernelGGL(k_slice_range, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.slice_width),
+                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.scol),
+                       c.slice_wide, c.w16, c.ppe_counters + 1);
+  });
+  scan_slices(c, c.w16, ns, c.off16);
+  tracked(c, MPS_K_OTHER, c.sell_slots * 6.0, [&] {
+    hipLaunchKernelGGL(k_fill16, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.slice_width),
+                       static_cast<const uint32_t*>(c.scol), static_cast<const uint8_t*>(c.slice_wide),
+                       static_cast<const uint64_t*>(c.off16), c.col16);
+  });
+  check_launch();
+  unsigned long long wide = 0;
+  MPS_HIP(hipMemcpyAsync(&wide, c.ppe_counters + 1, sizeof(wide), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.mf_nnz_wide = static_cast<double>(wide);
+  c.have_col16 = true;
 }
 
 // FP32 copies of an already assembled system (the mixed solver on a system
@@ -247,8 +398,10 @@ void ppe_assemble(Context& c) {
 void ppe_make_fp32(Context& c) {
   if (c.have_ppe32 || c.ppe_n == 0) return;
   if (!c.sval32) c.sval32 = dalloc<float>(c.sell_cap);
-  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(c.sell_slots)), dim3(kBlock), 0, c.stream, c.sell_slots,
-                     static_cast<const double*>(c.sval), c.sval32);
+  const size_t slots = static_cast<size_t>(c.sell_slots);
+  if (slots > 0)  // a diagonal system stores no off-diagonal slot
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(slots)), dim3(kBlock), 0, c.stream, slots,
+                       static_cast<const double*>(c.sval), c.sval32);
   hipLaunchKernelGGL(k_to_f32, dim3(grid_for(c.ppe_n)), dim3(kBlock), 0, c.stream, c.ppe_n,
                      static_cast<const double*>(c.diag), c.diag32);
   check_launch();
@@ -304,6 +457,7 @@ void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t
   c.nnz_offdiag = nnz;
   c.have_ppe = true;
   c.have_ppe32 = false;
+  c.ppe_from_particles = false;
 }
 
 // Device SELL-64 -> CSR with ascending columns, diagonal included.

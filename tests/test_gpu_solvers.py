@@ -1,0 +1,150 @@
+"""The mixed-precision pressure solver (FP32 CG sweeps inside FP64 iterative
+refinement) against the FP64 CG on the same assembled systems (GPU).
+
+On a system assembled from particles the FP32 sweeps use the matrix-free
+operator (off-diagonals recomputed from fixed-point positions, int16 column
+offsets or, on slices with far columns, the u32 columns); after the particle
+state is overwritten they fall back to the assembled values rounded to FP32,
+as they do on a system installed with set_ppe.  In every case the returned
+pressure must meet the FP64 stop rule checked here in numpy on the FP64
+matrix, and agree with the FP64 CG's pressure to the accuracy the rule allows.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+import cases
+from openmps_amd import scenes
+from openmps_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+STAGES_A = ("search", "density", "error_correction", "explicit", "density", "save_x",
+            "ppe_assemble")
+TOL = 1e-8
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def reordered(parts, order, seed=3):
+    n = len(parts["x"])
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if order == "natural":
+        return parts
+    if order == "shuffled":  # every slice has far columns
+        perm = rng.permutation(n)
+    else:  # "half": narrow slices in the first half, wide ones in the second
+        perm = np.concatenate([np.arange(n // 2), n // 2 + rng.permutation(n - n // 2)])
+    return {k: v[perm] for k, v in parts.items()}
+
+
+def assembled_system(order, jacobi, rule=L.STOP_RHS_NORM, name="dambreak_125k"):
+    cfg = scenes.config(name)
+    parts = reordered(cfg["particles"], order)
+    g = cases.make("gpu", cfg["env"], cfg["eps"])
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=rule, jacobi=jacobi, tol=TOL)
+    g.add_particles(parts)
+    dt = cfg["dt"]
+    g.forward_time(dt)  # a non-trivial previous pressure for the warm start
+    g.set_time(2 * dt, dt)
+    for st in STAGES_A:
+        g.stage(st)
+    return g, cfg
+
+
+def residual(g, x):
+    rp, col, val, b, _ = g.ppe()
+    n = len(b)
+    A = scipy.sparse.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+def fp64_solution(g, x0, rule=L.STOP_RHS_NORM):
+    g.set_solver(kind=L.SOLVER_CG_FP64, stop_rule=rule, tol=TOL)
+    g.set_aux("ppe_x", x0)
+    g.stage("ppe_solve")
+    return g.aux("ppe_x")
+
+
+@pytest.mark.parametrize("order", ["natural", "half", "shuffled"])
+@pytest.mark.parametrize("jacobi", [True, False])
+def test_mixed_matrix_free_meets_fp64_rule(order, jacobi):
+    g, _ = assembled_system(order, jacobi)
+    x0 = g.aux("ppe_x")
+    say(f"  mixed solve: order {order}, jacobi {jacobi}, {len(x0)} unknowns")
+    g.stage("ppe_solve")
+    s = g.solve_stats()
+    x = g.aux("ppe_x")
+    res = residual(g, x)
+    x64 = fp64_solution(g, x0)
+    s64 = g.solve_stats()
+    diff = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    say(f"    mixed: {s['iterations']} FP32 iterations in {s['outer_iterations']} sweeps, "
+        f"|b-Ax|/|b| {res:.2e} (solver's {s['rel_residual']:.2e}); FP64 CG: "
+        f"{s64['iterations']} iterations; |x - x64| / |x64| {diff:.2e}")
+    assert s["rel_residual"] <= TOL
+    assert res <= TOL * 1.01
+    assert s["outer_iterations"] >= 1
+    # both solutions satisfy the rule: their distance is what the condition allows
+    assert diff < 1e-5
+    # the FP32 sweeps cost about as many iterations as the FP64 CG (not many more)
+    assert s["iterations"] <= 2 * s64["iterations"] + 20
+
+
+def test_mixed_initial_residual_rule():
+    """Stop rule of the reference (r.r < eps^2 r0.r0, Computer.hpp:1386, :1408)
+    measured on the FP64 residual of the refined solution."""
+    g, cfg = assembled_system("natural", True, rule=L.STOP_INITIAL_RESIDUAL)
+    x0 = g.aux("ppe_x")
+    rp, col, val, b, _ = g.ppe()
+    n = len(b)
+    A = scipy.sparse.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    r0 = np.linalg.norm(b - A @ x0)
+    g.stage("ppe_solve")
+    x = g.aux("ppe_x")
+    s = g.solve_stats()
+    r = np.linalg.norm(b - A @ x)
+    say(f"  initial-residual rule: |r|/|r0| = {r / r0:.2e} after {s['iterations']} FP32 iterations")
+    assert r < TOL * r0 * 1.01
+    assert s["rr"] < TOL ** 2 * s["rr0"]
+
+
+def test_mixed_assembled_fallback_after_set_particles():
+    """Overwriting the particle state between assembly and solve: the FP32
+    sweeps use the assembled values (the system no longer matches the
+    positions), and the answer equals the matrix-free one to the rule."""
+    g, _ = assembled_system("natural", True)
+    x0 = g.aux("ppe_x")
+    g.stage("ppe_solve")
+    x_mf = g.aux("ppe_x")
+    it_mf = g.solve_stats()["iterations"]
+    g.set_particles(g.particles())  # same values: only the matrix-free flag drops
+    g.set_aux("ppe_x", x0)
+    g.stage("ppe_solve")
+    x_as = g.aux("ppe_x")
+    s = g.solve_stats()
+    diff = np.linalg.norm(x_as - x_mf) / np.linalg.norm(x_mf)
+    say(f"  assembled FP32: {s['iterations']} iterations (matrix-free {it_mf}), "
+        f"|x_as - x_mf| / |x_mf| {diff:.2e}")
+    assert residual(g, x_as) <= TOL * 1.01
+    assert diff < 1e-5
+    assert abs(s["iterations"] - it_mf) <= 0.25 * it_mf + 10
+
+
+@pytest.mark.parametrize("name", ["identity", "ode1d"])
+def test_mixed_installed_systems(name):
+    """The reference's CG test systems (test_ComputerConjugateGradient.cpp:115-229)
+    installed with set_ppe and solved by the mixed solver."""
+    A, b, want = cases.cg_systems()[name]
+    rp, col, val = cases.csr_from_dense(A)
+    c = cases.make("gpu", cases.CG["env"], 1e-12)
+    c.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, jacobi=True, tol=1e-12)
+    c.set_ppe(rp, col, val, b, np.zeros(len(b)))
+    c.stage("ppe_solve")
+    x = c.aux("ppe_x")
+    assert np.allclose(x, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
