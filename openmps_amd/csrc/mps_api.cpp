@@ -134,7 +134,7 @@ void free_all(Context& c) {
   dfree(c.nbr_total_dev); dfree(c.ppe_counters);
   dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.sval); dfree(c.sval32);
   dfree(c.scol); dfree(c.diag); dfree(c.diag32); dfree(c.b); dfree(c.sol);
-  dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
+  dfree(c.r); dfree(c.pa); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
   dfree(c.feats); dfree(c.ypred);
@@ -209,7 +209,7 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   const size_t m = (n + 63) / 64 * 64 + 64;
   const size_t ns = m / 64 + 1;
   dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.diag); dfree(c.diag32);
-  dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.pb); dfree(c.q);
+  dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.q);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16);
   dfree(c.g32a); dfree(c.g32b);
@@ -222,7 +222,6 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.sol = dalloc<double>(m);
   c.r = dalloc<double>(m);
   c.pa = dalloc<double>(m);
-  c.pb = dalloc<double>(m);
   c.q = dalloc<double>(m);
   c.pa32 = dalloc<float>(m);
   c.d32 = dalloc<float>(m);
@@ -236,7 +235,6 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.off16 = dalloc<uint64_t>(ns + 1);
   MPS_HIP(hipMemsetAsync(c.sol, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa, 0, m * 8, c.stream));
-  MPS_HIP(hipMemsetAsync(c.pb, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa32, 0, m * 4, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.ppe_alloc = n;

@@ -3,9 +3,9 @@
 //
 // CG iteration k is two kernels:
 //   k_cg_spmv   reduces the r.r partials of the previous kernel, applies the
-//               stop rule, forms p_k = r + beta p_{k-1} on the fly (for its own
-//               rows and for every gathered column), q = A p_k, and writes
-//               partial sums of p_k.q;
+//               stop rule, computes A r (r is the only gathered vector) and
+//               p_k = r + beta p_{k-1}, q_k = A r + beta q_{k-1} for its own
+//               rows, and writes partial sums of p_k.q_k;
 //   k_cg_update reduces the p.q partials, alpha = rr / pq, x += alpha p,
 //               r -= alpha q, and writes partial sums of r.r.
 // Partial sums live in fixed slots (one per block of a fixed-size grid); every
@@ -203,15 +203,18 @@ struct StopRule {
 };
 
 // One CG iteration, first half (see file comment).  `it` is the 1-based
-// iteration number; FIRST marks it == 1, where p_1 = r_0.
+// iteration number; FIRST marks it == 1, where p_1 = r_0.  The product is
+// applied to r, the only vector gathered from other rows, and the search
+// direction's product follows by linearity:
+//   p_k = r_k + beta p_{k-1},  A p_k = A r_k + beta A p_{k-1},
+// so p and q = A p are updated in place for the block's own rows.
 template <bool FIRST>
 __global__ void __launch_bounds__(NT, 8) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
-                                                const double* __restrict__ pold,
-                                                double* __restrict__ pnew, double* __restrict__ q,
-                                                const double* __restrict__ part_in,
-                                                const double* __restrict__ part_bb, int nparts,
-                                                double* __restrict__ part_out, CgStatus* st,
-                                                StopRule sr, long long it) {
+                                                   double* __restrict__ p, double* __restrict__ q,
+                                                   const double* __restrict__ part_in,
+                                                   const double* __restrict__ part_bb, int nparts,
+                                                   double* __restrict__ part_out, CgStatus* st,
+                                                   StopRule sr, long long it) {
   __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
@@ -249,13 +252,13 @@ __global__ void __launch_bounds__(NT, 8) k_cg_spmv(Sell<double> A, const double*
   const Rows rw = block_rows(A.n, A.rpb);
   double acc = 0;
   for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const double pi = FIRST ? r[i] : fma(beta, pold[i], r[i]);
-    pnew[i] = pi;
-    const double s = sell_row(A, i, A.diag[i] * pi, [&](uint32_t j) {
-      return FIRST ? r[j] : fma(beta, pold[j], r[j]);
-    });
-    q[i] = s;
-    acc = fma(pi, s, acc);
+    const double ri = r[i];
+    const double s = sell_row(A, i, A.diag[i] * ri, [&](uint32_t j) { return r[j]; });
+    const double pi = FIRST ? ri : fma(beta, p[i], ri);
+    const double qi = FIRST ? s : fma(beta, q[i], s);
+    p[i] = pi;
+    q[i] = qi;
+    acc = fma(pi, qi, acc);
   }
   const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
@@ -535,31 +538,29 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
                        static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
   });
   check_launch();
-  double* pbuf[2] = {c.pa, c.pb};
   long long it = 1;
-  // reads r and p_old, writes p_new and q (the diagonal is in spmv_bytes)
-  const double b_spmv = spmv_bytes(c, 8, 8.0 * 4);
+  // reads r (own row and gathered), reads and writes p and q (the diagonal
+  // is in spmv_bytes)
+  const double b_spmv = spmv_bytes(c, 8, 8.0 * 5);
   const double b_upd = c.ppe_n * 48.0;
   while (true) {
     const long long chunk = next_chunk(it - 1, c.last_iters);
     for (long long m = 0; m < chunk && it <= max_iter; m++, it++) {
-      const double* pold = pbuf[(it + 1) & 1];
-      double* pnew = pbuf[it & 1];
       tracked(c, MPS_K_CG_SPMV, b_spmv, [&] {
         if (it == 1)
           hipLaunchKernelGGL(k_cg_spmv<true>, dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const double*>(c.r), pold, pnew, c.q,
+                             static_cast<const double*>(c.r), c.pa, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
         else
           hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const double*>(c.r), pold, pnew, c.q,
+                             static_cast<const double*>(c.r), c.pa, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
       }, it);
       tracked(c, MPS_K_CG_UPDATE, b_upd, [&] {
         hipLaunchKernelGGL(k_cg_update, dim3(L.grid), dim3(NT), 0, c.stream, n, L.rpb, c.sol, c.r,
-                           static_cast<const double*>(pnew), static_cast<const double*>(c.q),
+                           static_cast<const double*>(c.pa), static_cast<const double*>(c.q),
                            static_cast<const double*>(c.partA), L.grid, c.partB, c.st, it);
       }, it);
     }

@@ -170,7 +170,7 @@ struct Context {
   double nnz_offdiag = 0, sell_slots = 0;
 
   // CG work vectors
-  double *r = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr;
+  double *r = nullptr, *pa = nullptr, *q = nullptr;  // residual, direction, A direction
   double *partA = nullptr, *partB = nullptr, *partC = nullptr;
   double* red_out = nullptr;
   double* red_host = nullptr;  // pinned

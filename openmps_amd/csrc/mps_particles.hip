@@ -360,7 +360,8 @@ __global__ void __launch_bounds__(256) k_max_speed2(size_t n, const double* __re
 
 __global__ void __launch_bounds__(256) k_copy_probe(size_t n4, const float4* __restrict__ a,
                                                     float4* __restrict__ b) {
-  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n4; i += gridDim.x * 256ull) b[i] = a[i];
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i < n4) b[i] = a[i];
 }
 
 __global__ void k_math_check(size_t n, const double* a, const double* b, double* s, double* d) {
@@ -531,6 +532,7 @@ double max_speed_dt(Context& c) {
 
 double device_copy_probe(size_t bytes) {
   const size_t n4 = bytes / 16;
+  if (n4 == 0 || n4 > (1ull << 36)) throw Failure{MPS_ERR_BAD_ARG, "copy probe size out of range"};
   float4* a = dalloc<float4>(n4);
   float4* b = dalloc<float4>(n4);
   MPS_HIP(hipMemset(a, 0, n4 * 16));
@@ -538,10 +540,11 @@ double device_copy_probe(size_t bytes) {
   MPS_HIP(hipEventCreate(&e0));
   MPS_HIP(hipEventCreate(&e1));
   // one untimed launch, then the mean of five
-  hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, 0, n4, static_cast<const float4*>(a), b);
+  const unsigned blocks = grid_for(n4, 256);
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, n4, static_cast<const float4*>(a), b);
   MPS_HIP(hipEventRecord(e0, 0));
   for (int k = 0; k < 5; k++)
-    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, 0, n4, static_cast<const float4*>(a), b);
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, 0, n4, static_cast<const float4*>(a), b);
   MPS_HIP(hipEventRecord(e1, 0));
   MPS_HIP(hipEventSynchronize(e1));
   float ms = 0;
