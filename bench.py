@@ -31,29 +31,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# solver variants: name -> (mixed precision, warm start, Jacobi in the FP32 CG)
-VARIANTS = {
-    "fp64": (False, "previous", False),          # the reference algorithm, previous-step guess
-    "fp64_zero": (False, "zero", False),
-    "fp64_learned": (False, "learned", False),
-    "mixed": (True, "previous", False),
-    "mixed_jacobi": (True, "previous", True),
-    "mixed_zero": (True, "zero", True),
-    "learned_mixed": (True, "learned", True),
-}
-
-
 def make_computer(cfg, solver: str, tol: float, weights: str | None = None):
-    from openmps_amd import Computer, make_environment
-    from openmps_amd import _lib as L
-    mixed, warm, jac = VARIANTS[solver]
+    from openmps_amd import Computer, make_environment, solvers
     c = Computer(cfg["eps"], make_environment(cfg["env"]))
-    ws = {"previous": L.WARM_PREVIOUS, "zero": L.WARM_ZERO, "learned": L.WARM_LEARNED}[warm]
-    c.set_solver(kind=L.SOLVER_MIXED_IR if mixed else L.SOLVER_CG_FP64, stop_rule=L.STOP_RHS_NORM,
-                 warm_start=ws, jacobi=jac, tol=tol, inner_tol=1e-4)
-    if warm == "learned":
-        from openmps_amd import learned
-        learned.attach_model(c, learned.Model.load(weights or learned.DEFAULT_WEIGHTS))
+    solvers.configure(c, solver, tol, weights)
     c.add_particles(cfg["particles"])
     return c
 
@@ -119,9 +100,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="dambreak_1m")
-    ap.add_argument("--solver", default=None, choices=sorted(VARIANTS),
-                    help="headline solver (default: learned_mixed when trained weights exist, "
-                         "else mixed_jacobi)")
+    from openmps_amd import solvers
+    ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
+                    help="headline solver (default: mixed_ml, the FP32 multilevel-preconditioned "
+                         "CG inside FP64 refinement)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
     ap.add_argument("--ablation", action="store_true",
                     help="also time zero / previous / learned guesses for both solvers")
@@ -140,7 +122,7 @@ def main() -> int:
     from openmps_amd import learned as _learned
     have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)
     if args.solver is None:
-        args.solver = "learned_mixed" if have_weights else "mixed_jacobi"
+        args.solver = "mixed_ml"
     log(f"building scene {args.workload} ...")
     cfg = scenes.config(args.workload)
     N = len(cfg["particles"]["x"])
@@ -171,8 +153,8 @@ def main() -> int:
     if not args.no_compare and args.solver != "fp64":
         others.append("fp64")
     if args.ablation:
-        others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_zero", "learned_mixed",
-                               "fp64_learned")
+        others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_ml", "mixed_ml_zero", "learned_ml",
+                               "learned_mixed")
                    if v != args.solver and (have_weights or "learned" not in v)]
     del main_c
     for v in others:
@@ -196,7 +178,8 @@ def main() -> int:
                 "ms_per_launch": d["ms_per_launch"]}
 
     # dominant kernel: the SpMV of the solver that ran the timed steps
-    dom = "cg_iter32" if VARIANTS[args.solver][0] else "cg_spmv"
+    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32"}[
+        solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
     roofline = roof(kt, dom)
     roofline_fp64 = roof(kt64, "cg_spmv")
 
@@ -219,8 +202,8 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("f64" if not VARIANTS[args.solver][0] else "f64+f32") +
-                 ("+bf16" if VARIANTS[args.solver][1] == "learned" else ""),
+        "dtype": ("f64+f32" if solvers.is_mixed(args.solver) else "f64") +
+                 ("+bf16" if solvers.is_learned(args.solver) else ""),
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": N, "fluid_particles": nfluid,
                    "l0": cfg["env"]["l0"], "dt": cfg["dt"], "r_e_by_l0": cfg["env"]["r_e_by_l0"],

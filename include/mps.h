@@ -124,11 +124,21 @@ enum mps_warm_start {
   MPS_WARM_ZERO = 1,
   MPS_WARM_LEARNED = 2   /* x0 from the learned per-particle model */
 };
+/* Preconditioner of the mixed solver's FP32 inner CG (the FP64 CG, the
+ * reference algorithm, is never preconditioned). */
+enum mps_precond {
+  MPS_PRECOND_NONE = 0,
+  MPS_PRECOND_JACOBI = 1,
+  /* Jacobi plus additive piecewise-constant corrections on the neighbour
+   * search cells and on 2x2 and 4x4 blocks of cells; needs a system assembled
+   * from the current particle state and cells, and uses Jacobi otherwise */
+  MPS_PRECOND_MULTILEVEL = 2
+};
 typedef struct mps_solver_opts {
   int kind;          /* mps_solver_kind */
   int stop_rule;     /* mps_stop_rule */
   int warm_start;    /* mps_warm_start */
-  int jacobi;        /* 1: diagonal preconditioner in the mixed solver's inner CG */
+  int precond;       /* mps_precond, mixed solver only */
   double tol;        /* <= 0: the allowable residual given to mps_create */
   int64_t max_iter;  /* <= 0: number of unknowns, as the reference */
   double inner_tol;  /* mixed: relative residual reduction per refinement sweep */
@@ -230,7 +240,7 @@ enum mps_kernel_id {
   MPS_K_DENSITY_ECS, MPS_K_EXPLICIT, MPS_K_UPDATE, MPS_K_DENSITY_SAVE, MPS_K_PPE_COUNT,
   MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_ITER32,
   MPS_K_CG_INIT32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
-  MPS_K_FEATURES, MPS_K_MLP,
+  MPS_K_FEATURES, MPS_K_MLP, MPS_K_ML_SETUP, MPS_K_ML_PREC, MPS_K_CG_SPMV32, MPS_K_CG_UPDATE32,
   MPS_K_COUNT
 };
 typedef struct mps_kernel_stat {

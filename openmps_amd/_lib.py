@@ -32,10 +32,12 @@ WARM_PREVIOUS, WARM_ZERO, WARM_LEARNED = 0, 1, 2
 AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_z": 5,
        "ppe_b": 6, "ppe_x": 7}
 
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL = 0, 1, 2
+
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
            "explicit", "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv",
            "cg_update", "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other",
-           "features", "mlp")
+           "features", "mlp", "ml_setup", "ml_prec", "cg_spmv32", "cg_update32")
 
 
 class MpsEnv(ctypes.Structure):
@@ -46,7 +48,7 @@ class MpsEnv(ctypes.Structure):
 
 
 class SolverOpts(ctypes.Structure):
-    _fields_ = [("kind", _I), ("stop_rule", _I), ("warm_start", _I), ("jacobi", _I), ("tol", _D),
+    _fields_ = [("kind", _I), ("stop_rule", _I), ("warm_start", _I), ("precond", _I), ("tol", _D),
                 ("max_iter", _I64), ("inner_tol", _D), ("inner_max_iter", _I64)]
 
 

@@ -335,8 +335,12 @@ class Computer:
     # -- solver extension and measurement ------------------------------------------
     def set_solver(self, kind: int = L.SOLVER_CG_FP64, stop_rule: int = L.STOP_INITIAL_RESIDUAL,
                    warm_start: int = L.WARM_PREVIOUS, jacobi: bool = False, tol: float = 0.0,
-                   max_iter: int = 0, inner_tol: float = 1e-4, inner_max_iter: int = 0) -> None:
-        o = L.SolverOpts(kind, stop_rule, warm_start, int(jacobi), tol, max_iter, inner_tol,
+                   max_iter: int = 0, inner_tol: float = 1e-4, inner_max_iter: int = 0,
+                   precond: int | None = None) -> None:
+        """`precond` (mps_precond) overrides `jacobi` (PRECOND_JACOBI when true)."""
+        if precond is None:
+            precond = L.PRECOND_JACOBI if jacobi else L.PRECOND_NONE
+        o = L.SolverOpts(kind, stop_rule, warm_start, int(precond), tol, max_iter, inner_tol,
                          inner_max_iter)
         _raise(L.lib().mps_set_solver(self._h, ctypes.byref(o)), self._h)
 

@@ -140,6 +140,7 @@ void free_all(Context& c) {
   dfree(c.feats); dfree(c.ypred);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
+  dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
@@ -200,7 +201,8 @@ void forward(Context& c, double dt) {
 const char* kKernelNames[MPS_K_COUNT] = {
     "cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs", "explicit",
     "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv", "cg_update",
-    "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other", "features", "mlp"};
+    "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other", "features", "mlp",
+    "ml_setup", "ml_prec", "cg_spmv32", "cg_update32"};
 
 }  // namespace
 
@@ -212,7 +214,7 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.q);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16);
-  dfree(c.g32a); dfree(c.g32b);
+  dfree(c.g32a); dfree(c.g32b); dfree(c.r32); dfree(c.z32); dfree(c.q32);
   c.row_len = dalloc<uint32_t>(m);
   c.slice_width = dalloc<uint32_t>(ns);
   c.slice_off = dalloc<uint64_t>(ns + 1);
@@ -229,6 +231,9 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.ghost32 = dalloc<float>(m);
   c.g32a = dalloc<float4>(m);
   c.g32b = dalloc<float4>(m);
+  c.r32 = dalloc<float>(m);
+  c.z32 = dalloc<float>(m);
+  c.q32 = dalloc<float>(m);
   c.qpos = dalloc<uint2>(m);
   c.slice_wide = dalloc<uint8_t>(ns);
   c.w16 = dalloc<uint32_t>(ns);
@@ -514,6 +519,7 @@ int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts) {
   if (opts->kind < 0 || opts->kind > MPS_SOLVER_MIXED_IR) return MPS_ERR_BAD_ARG;
   if (opts->stop_rule < 0 || opts->stop_rule > MPS_STOP_RHS_NORM) return MPS_ERR_BAD_ARG;
   if (opts->warm_start < 0 || opts->warm_start > MPS_WARM_LEARNED) return MPS_ERR_BAD_ARG;
+  if (opts->precond < 0 || opts->precond > MPS_PRECOND_MULTILEVEL) return MPS_ERR_BAD_ARG;
   ctx->c.solver = *opts;
   return MPS_OK;
 }

@@ -470,6 +470,271 @@ __global__ void __launch_bounds__(NT, 8) k_cgf_iter(Op A, const float4* __restri
   if (lead()) st->iterations = it;
 }
 
+// ---------------- multilevel aggregation preconditioner ----------------
+//
+// M r = D^-1 r + sum_{l=0..2} P_l (P_l^T A P_l)^-diag P_l^T r  (additive):
+// P_l is the piecewise-constant prolongation from the aggregates of level l,
+// the neighbour-search cells (l = 0) and 2x2 and 4x4 blocks of cells, and
+// each aggregate k is scaled by 1 / (1^T A_kk 1), the sum of the matrix block
+// of its members.  Dummy and disabled rows belong to no aggregate.  One
+// wavefront owns one 4x4 block of cells: lane l takes cell l % 16 and every
+// fourth particle of it (from l / 16), and the cell, 2x2 and 4x4 sums are
+// butterflies over fixed lane masks, so every sum has a fixed order.
+constexpr int NTP = 256;  // threads per block of the aggregate kernels
+
+struct CellGrid {
+  const uint32_t* __restrict__ start;   // first slot of each cell in `sorted`
+  const uint32_t* __restrict__ count;   // particles per cell
+  const uint32_t* __restrict__ sorted;  // particle ids grouped by cell, ascending in a cell
+  const uint32_t* __restrict__ key;     // cell of each particle
+  const uint8_t* __restrict__ type;
+  uint32_t gnx, gnz;                    // cells
+  uint32_t g1z;                         // 2x2 blocks along z
+  uint32_t bnx, bnz;                    // 4x4 blocks
+  size_t off1, off2;                    // level offsets in the aggregate arrays
+};
+
+struct BlockLane {
+  bool valid;  // the lane's cell lies in the grid
+  uint32_t cell, a1, a2, sub;
+};
+__device__ __forceinline__ BlockLane block_lane(const CellGrid& G, uint32_t b) {
+  const uint32_t lane = threadIdx.x & 63, u = lane & 15;
+  const uint32_t kx = b / G.bnz, kz = b % G.bnz;
+  const uint32_t cx = 4 * kx + (u >> 2), cz = 4 * kz + (u & 3);
+  BlockLane L;
+  L.valid = cx < G.gnx && cz < G.gnz;
+  L.cell = cx * G.gnz + cz;
+  L.a1 = (cx >> 1) * G.g1z + (cz >> 1);
+  L.a2 = b;
+  L.sub = lane >> 4;
+  return L;
+}
+__device__ __forceinline__ bool in_agg(uint8_t t) { return t != kDummy && t != kDisabled; }
+
+// Sums of v over the lanes of one cell, of one 2x2 block and of the 4x4 block.
+template <class T>
+__device__ __forceinline__ void agg_sums(T v, T& y0, T& y1, T& y2) {
+  y0 = v + __shfl_xor(v, 16, 64);
+  y0 = y0 + __shfl_xor(y0, 32, 64);
+  y1 = y0 + __shfl_xor(y0, 1, 64);
+  y1 = y1 + __shfl_xor(y1, 4, 64);
+  y2 = y1 + __shfl_xor(y1, 2, 64);
+  y2 = y2 + __shfl_xor(y2, 8, 64);
+}
+
+// 1 / (1^T A_kk 1) for every aggregate (0 where the sum is not negative: no
+// correction), from the FP64 system; membership of a column by its cell.
+__global__ void __launch_bounds__(NTP) k_ml_setup(CellGrid G, Sell<double> A,
+                                                  const uint32_t* __restrict__ row_len,
+                                                  float* __restrict__ inv) {
+  const uint32_t waves = gridDim.x * (NTP / 64);
+  const uint32_t nb = G.bnx * G.bnz;
+  for (uint32_t b = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6); b < nb; b += waves) {
+    const BlockLane L = block_lane(G, b);
+    double s0 = 0, s1 = 0, s2 = 0;
+    if (L.valid) {
+      const uint32_t cnt = G.count[L.cell], st = G.start[L.cell];
+      const uint32_t cx = L.cell / G.gnz, cz = L.cell % G.gnz;
+      for (uint32_t k = L.sub; k < cnt; k += 4) {
+        const uint32_t i = G.sorted[st + k];
+        if (!in_agg(G.type[i])) continue;
+        const double d = A.diag[i];
+        s0 += d;
+        s1 += d;
+        s2 += d;
+        const size_t base = A.off[i >> 6] + (i & 63);
+        const uint32_t m = row_len[i];
+        for (uint32_t q = 0; q < m; q++) {
+          const size_t o = base + static_cast<size_t>(q) * 64;
+          const uint32_t j = A.col[o];
+          const uint32_t kj = G.key[j];
+          if (kj == kNoCell) continue;
+          const uint32_t jx = kj / G.gnz, jz = kj % G.gnz;
+          const double a = A.val[o];
+          if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
+            s2 += a;
+            if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
+              s1 += a;
+              if (jx == cx && jz == cz) s0 += a;
+            }
+          }
+        }
+      }
+    }
+    double y0, y1, y2, t;
+    agg_sums(s0, y0, t, t);
+    agg_sums(s1, t, y1, t);
+    agg_sums(s2, t, t, y2);
+    const uint32_t lane = threadIdx.x & 63;
+    if (L.valid && L.sub == 0) {
+      inv[L.cell] = (y0 < 0) ? static_cast<float>(1.0 / y0) : 0.0f;
+      if ((lane & 5) == 0) inv[G.off1 + L.a1] = (y1 < 0) ? static_cast<float>(1.0 / y1) : 0.0f;
+      if ((lane & 15) == 0) inv[G.off2 + L.a2] = (y2 < 0) ? static_cast<float>(1.0 / y2) : 0.0f;
+    }
+  }
+}
+
+// Each inner iteration `it` is four kernels:
+//   k_ml_spmv(it):  beta = rho_(it-1) / rho_(it-2); p = z + beta p,
+//                   q = A z + beta q (A p by linearity; z is the only gathered
+//                   vector); partial sums of p.q;
+//   k_ml_alpha:     alpha = rho_(it-1) / p.q (one block);
+//   k_ml_prec(it):  r -= alpha q, d += alpha p; z = M r; partial sums of
+//                   rho_it = r.z and of r.r;
+//   k_ml_rho(it):   rho_it, r.r and the stop test (one block).
+// k_ml_prec visits the particles cell by cell, one wavefront per 4x4 block
+// of cells (every particle of the system that is in a cell exactly once; rows
+// outside every cell are identity rows whose r, d, z stay 0), so the residual
+// update, the aggregate sums and z share one pass; its large grid hides the
+// latency of the id -> value gathers.  Iteration 0 of k_ml_prec starts the
+// sweep: r = r64 / |r64|, d = 0.
+constexpr int kPerLane = 7;  // particles of a cell held in registers per lane (4 lanes per cell)
+
+__global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __restrict__ inv,
+                                                 const double* __restrict__ r64, double inv_norm,
+                                                 float* __restrict__ r, float* __restrict__ d,
+                                                 const float* __restrict__ p, const float* __restrict__ q,
+                                                 const float* __restrict__ dinv, float* __restrict__ z,
+                                                 double* __restrict__ part, size_t part_stride, CgStatus* st,
+                                                 long long it) {
+  __shared__ double lds[NTP / 64];
+  if (already_converged(st)) return;
+  const float alpha = (it > 0) ? static_cast<float>(st->alpha_ml) : 0.0f;
+  const uint32_t b = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  double rho = 0, rr = 0;
+  if (b < G.bnx * G.bnz) {  // wave-uniform
+    const BlockLane L = block_lane(G, b);
+    const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+    uint32_t id[kPerLane];
+    float rv[kPerLane];
+    bool ag[kPerLane];
+#pragma unroll
+    for (int u = 0; u < kPerLane; u++) {
+      const uint32_t k = L.sub + 4u * u;
+      id[u] = (k < cnt) ? G.sorted[st0 + k] : 0u;
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kPerLane; u++) {
+      float ri = 0.0f;
+      ag[u] = false;
+      if (L.sub + 4u * u < cnt) {
+        const uint32_t i = id[u];
+        if (it == 0) {
+          ri = static_cast<float>(r64[i] * inv_norm);
+          d[i] = 0.0f;
+        } else {
+          ri = fmaf(-alpha, q[i], r[i]);
+          d[i] = fmaf(alpha, p[i], d[i]);
+        }
+        r[i] = ri;
+        ag[u] = in_agg(G.type[i]);
+        if (ag[u]) s += ri;
+        rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
+      }
+      rv[u] = ri;
+    }
+    // cells beyond 4 * kPerLane particles (larger cell capacities)
+    for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+      const uint32_t i = G.sorted[st0 + k];
+      float ri;
+      if (it == 0) {
+        ri = static_cast<float>(r64[i] * inv_norm);
+        d[i] = 0.0f;
+      } else {
+        ri = fmaf(-alpha, q[i], r[i]);
+        d[i] = fmaf(alpha, p[i], d[i]);
+      }
+      r[i] = ri;
+      if (in_agg(G.type[i])) s += ri;
+      rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
+    }
+    float y0, y1, y2;
+    agg_sums(s, y0, y1, y2);
+    const float corr = L.valid ? y0 * inv[L.cell] + y1 * inv[G.off1 + L.a1] + y2 * inv[G.off2 + L.a2] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kPerLane; u++) {
+      if (L.sub + 4u * u < cnt) {
+        const uint32_t i = id[u];
+        const float zi = ag[u] ? fmaf(dinv[i], rv[u], corr) : dinv[i] * rv[u];
+        z[i] = zi;
+        rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
+      }
+    }
+    for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+      const uint32_t i = G.sorted[st0 + k];
+      const float ri = r[i];
+      const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
+      z[i] = zi;
+      rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
+    }
+  }
+  const double srho = block_sum<NTP>(rho, lds);
+  const double srr = block_sum<NTP>(rr, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = srho;
+    part[part_stride + blockIdx.x] = srr;
+  }
+}
+
+// rho_it and r_it.r_it from the partials of k_ml_prec(it); stop test on r.r
+// relative to the sweep's r_0.r_0 (single block).
+__global__ void __launch_bounds__(1024) k_ml_rho(const double* __restrict__ part, size_t part_stride,
+                                                 int nparts, CgStatus* st, double tol_rel, long long it) {
+  __shared__ double lds[1024 / 64];
+  if (already_converged(st)) return;
+  const double rho = reduce_partials<1024>(part, nparts, lds);
+  const double rr = reduce_partials<1024>(part + part_stride, nparts, lds);
+  if (threadIdx.x == 0) {
+    const int par = static_cast<int>(it & 1);
+    st->gamma[par] = rho;
+    st->rr[par] = rr;
+    bool conv;
+    if (it == 0) {
+      st->tol = rr * tol_rel * tol_rel;
+      st->rr0 = rr;
+      conv = (st->tol == 0);
+    } else {
+      conv = rr < st->tol;
+    }
+    st->iterations = it;
+    if (conv) st->converged = 1;
+  }
+}
+
+// alpha = rho_(it-1) / p.q (single block).
+__global__ void __launch_bounds__(1024) k_ml_alpha(const double* __restrict__ part, int nparts, CgStatus* st,
+                                                   long long it) {
+  __shared__ double lds[1024 / 64];
+  if (already_converged(st)) return;
+  const double pq = reduce_partials<1024>(part, nparts, lds);
+  if (threadIdx.x == 0) st->alpha_ml = st->gamma[(it - 1) & 1] / pq;
+}
+
+// See above; FIRST is it == 1 (p = z, q = A z).
+template <bool FIRST, class Op>
+__global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict__ z, float* __restrict__ p,
+                                                   float* __restrict__ q, double* __restrict__ part_pq,
+                                                   CgStatus* st, long long it) {
+  __shared__ double lds[NT / 64];
+  if (already_converged(st)) return;
+  const float beta = FIRST ? 0.0f : static_cast<float>(st->gamma[(it - 1) & 1] / st->gamma[it & 1]);
+  const Rows rw = block_rows(A.n, A.rpb);
+  double acc = 0;
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    const float zi = z[i];
+    const float s = op_apply<kIterUnroll + 1>(A, i, zi, [&](uint32_t j) { return z[j]; });
+    const float pi = FIRST ? zi : fmaf(beta, p[i], zi);
+    const float qi = FIRST ? s : fmaf(beta, q[i], s);
+    p[i] = pi;
+    q[i] = qi;
+    acc = fma(static_cast<double>(pi), static_cast<double>(qi), acc);
+  }
+  const double sp = block_sum<NT>(acc, lds);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = sp;
+}
+
 // x += |r| d (FP64 accumulation of the FP32 correction).
 __global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict__ x,
                                                     const float* __restrict__ d, double norm) {
@@ -619,22 +884,14 @@ void residual64(Context& c, double& rr, double& bb, bool track) {
 
 namespace {
 
-// Mixed precision: FP32 CG sweeps on A d = r, FP64 residual and accumulation.
-// `A` is the FP32 operator (matrix-free or the assembled values rounded),
-// `b_iter` the algorithmic bytes of one inner iteration.
-template <class Op>
-void solve_mixed_op(Context& c, const Op& A, double b_iter, const StopRule& sr, long long max_iter) {
+// Mixed precision: FP32 CG sweeps on A d = r / |r| inside FP64 iterative
+// refinement.  `sweep(norm, tol)` runs one FP32 solve to the relative
+// tolerance tol, leaves d in c.d32 and returns its iterations; the refinement
+// adds |r| d to x in FP64 and recomputes r = b - A x in FP64.
+template <class Sweep>
+void refine(Context& c, const StopRule& sr, Sweep sweep) {
   const size_t n = c.ppe_n;
-  const Launch L = cg_grid(n);
-  const bool jac = c.solver.jacobi != 0;
-  if (jac)
-    hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
-                       static_cast<const double*>(c.diag), c.dinv32);
-  const float* dinv = jac ? c.dinv32 : nullptr;
   const double inner_tol = c.solver.inner_tol > 0 ? c.solver.inner_tol : 1e-4;
-  const long long inner_max = c.solver.inner_max_iter > 0 ? c.solver.inner_max_iter : max_iter;
-  float4* gbuf[2] = {c.g32a, c.g32b};
-  double* parts[2] = {c.part3a, c.part3b};
   long long total = 0;
   double rr0 = 0, bb = 0, rr = 0;
   int outer = 0;
@@ -663,34 +920,7 @@ void solve_mixed_op(Context& c, const Op& A, double b_iter, const StopRule& sr, 
     // this sweep reduces the residual by inner_tol, or only as far as the
     // outer target needs (with a factor 2 of margin), whichever is less work
     const double sweep_tol = std::max(inner_tol, 0.5 * std::sqrt(target / rr));
-    MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
-    tracked(c, MPS_K_CG_INIT32, b_iter + n * 8.0, [&] {
-      hipLaunchKernelGGL(k_cgf_init<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
-                         static_cast<const double*>(c.r), 1.0 / norm, dinv, gbuf[1], c.pa32, c.d32,
-                         parts[1]);
-    });
-    check_launch();
-    long long it = 1;
-    while (true) {
-      const long long chunk = next_chunk(it - 1, c.last_inner);
-      for (long long m = 0; m < chunk && it <= inner_max; m++, it++) {
-        // iteration it reads buffer it % 2 and writes the other
-        const int in = static_cast<int>(it & 1);
-        tracked(c, MPS_K_CG_ITER32, b_iter, [&] {
-          hipLaunchKernelGGL(k_cgf_iter<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float4*>(gbuf[in]), gbuf[in ^ 1], c.pa32, c.d32,
-                             static_cast<const double*>(parts[in]), L.grid, parts[in ^ 1], c.st32,
-                             sweep_tol, it);
-        }, it);
-      }
-      check_launch();
-      read_status(c, c.st32, c.st32_host);
-      if (c.st32_host->converged || it > inner_max) break;
-    }
-    const long long inner = c.st32_host->converged ? c.st32_host->iterations : inner_max;
-    const long long launched = std::min(it - 1, inner_max);
-    // the launch that detected convergence did no work: count it out
-    uncount(c, MPS_K_CG_ITER32, launched - inner, b_iter);
+    const long long inner = sweep(norm, sweep_tol);
     if (outer == 0) c.last_inner = inner;
     total += inner;
     hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
@@ -707,12 +937,148 @@ void solve_mixed_op(Context& c, const Op& A, double b_iter, const StopRule& sr, 
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
 }
 
+long long inner_limit(const Context& c, long long max_iter) {
+  return c.solver.inner_max_iter > 0 ? c.solver.inner_max_iter : max_iter;
+}
+
+// Runs iterations 1, 2, ... of an inner solve in chunks until its status
+// reports convergence or the limit; `launch(it)` enqueues iteration it.
+// Returns the iterations done.
+template <class Launch1>
+long long run_inner(Context& c, long long limit, Launch1 launch) {
+  long long it = 1;
+  while (true) {
+    const long long chunk = next_chunk(it - 1, c.last_inner);
+    for (long long m = 0; m < chunk && it <= limit; m++, it++) launch(it);
+    check_launch();
+    read_status(c, c.st32, c.st32_host);
+    if (c.st32_host->converged || it > limit) break;
+  }
+  return c.st32_host->converged ? c.st32_host->iterations : limit;
+}
+
+// Jacobi (or no) preconditioner: the one-kernel Chronopoulos-Gear iteration.
+template <class Op>
+void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  const bool jac = c.solver.precond != MPS_PRECOND_NONE;
+  if (jac)
+    hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                       static_cast<const double*>(c.diag), c.dinv32);
+  const float* dinv = jac ? c.dinv32 : nullptr;
+  const long long limit = inner_limit(c, max_iter);
+  float4* gbuf[2] = {c.g32a, c.g32b};
+  double* parts[2] = {c.part3a, c.part3b};
+  refine(c, sr, [&](double norm, double sweep_tol) {
+    MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
+    tracked(c, MPS_K_CG_INIT32, b_iter + n * 8.0, [&] {
+      hipLaunchKernelGGL(k_cgf_init<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+                         static_cast<const double*>(c.r), 1.0 / norm, dinv, gbuf[1], c.pa32, c.d32,
+                         parts[1]);
+    });
+    long long launched = 0;
+    const long long inner = run_inner(c, limit, [&](long long it) {
+      // iteration it reads buffer it % 2 and writes the other
+      const int in = static_cast<int>(it & 1);
+      tracked(c, MPS_K_CG_ITER32, b_iter, [&] {
+        hipLaunchKernelGGL(k_cgf_iter<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+                           static_cast<const float4*>(gbuf[in]), gbuf[in ^ 1], c.pa32, c.d32,
+                           static_cast<const double*>(parts[in]), L.grid, parts[in ^ 1], c.st32,
+                           sweep_tol, it);
+      }, it);
+      launched = it;
+    });
+    // launches after convergence did no work: count them out
+    uncount(c, MPS_K_CG_ITER32, launched - inner, b_iter);
+    return inner;
+  });
+}
+
+// Multilevel preconditioner: four kernels per iteration (see k_ml_prec).
+void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                     static_cast<const double*>(c.diag), c.dinv32);
+  const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
+  const uint32_t g1x = (gnx + 1) / 2, g1z = (gnz + 1) / 2, bnx = (gnx + 3) / 4, bnz = (gnz + 3) / 4;
+  const size_t off1 = static_cast<size_t>(gnx) * gnz, off2 = off1 + static_cast<size_t>(g1x) * g1z;
+  const size_t total_aggs = off2 + static_cast<size_t>(bnx) * bnz;
+  const size_t nblk = static_cast<size_t>(bnx) * bnz;
+  const size_t pgrid = (nblk + NTP / 64 - 1) / (NTP / 64);  // one wavefront per 4x4 block of cells
+  if (c.agg_cap < total_aggs) {
+    dfree(c.agg_inv);
+    c.agg_inv = dalloc<float>(total_aggs);
+    c.agg_cap = total_aggs;
+  }
+  if (c.part_ml_cap < pgrid) {
+    dfree(c.part_ml);
+    c.part_ml = dalloc<double>(2 * pgrid);
+    c.part_ml_cap = pgrid;
+  }
+  const size_t stride = c.part_ml_cap;
+  const CellGrid G{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz, off1, off2};
+  const int sgrid = static_cast<int>(std::min<size_t>(kRedBlocks, std::max<size_t>(1, pgrid)));
+  // the aggregate sums of A from the FP64 system
+  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 16.0 + n * 24.0 + total_aggs * 4.0, [&] {
+    hipLaunchKernelGGL(k_ml_setup, dim3(sgrid), dim3(NTP), 0, c.stream, G,
+                       sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const uint32_t*>(c.row_len),
+                       c.agg_inv);
+  });
+  // rows outside every cell keep r = d = z = 0
+  MPS_HIP(hipMemsetAsync(c.z32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+  check_launch();
+  double* p_pq = c.part3b;
+  const long long limit = inner_limit(c, max_iter);
+  // per particle: id, r (read, written), q, p, d (read, written), type, M, z; cell tables
+  const double b_prec = n * 41.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  auto prec = [&](double norm, long long it) {
+    hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+                       static_cast<const float*>(c.agg_inv), static_cast<const double*>(c.r), 1.0 / norm,
+                       c.r32, c.d32, static_cast<const float*>(c.pa32), static_cast<const float*>(c.q32),
+                       static_cast<const float*>(c.dinv32), c.z32, c.part_ml, stride, c.st32, it);
+  };
+  refine(c, sr, [&](double norm, double sweep_tol) {
+    MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
+    tracked(c, MPS_K_CG_INIT32, n * 29.0, [&] {
+      prec(norm, 0);
+      hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
+                         stride, static_cast<int>(pgrid), c.st32, sweep_tol, 0LL);
+    });
+    long long launched = 0;
+    const long long inner = run_inner(c, limit, [&](long long it) {
+      tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
+        if (it == 1)
+          hipLaunchKernelGGL((k_ml_spmv<true, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
+                             static_cast<const float*>(c.z32), c.pa32, c.q32, p_pq, c.st32, it);
+        else
+          hipLaunchKernelGGL((k_ml_spmv<false, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
+                             static_cast<const float*>(c.z32), c.pa32, c.q32, p_pq, c.st32, it);
+        hipLaunchKernelGGL(k_ml_alpha, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(p_pq),
+                           L.grid, c.st32, it);
+      }, it);
+      tracked(c, MPS_K_ML_PREC, b_prec, [&] {
+        prec(norm, it);
+        hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
+                           stride, static_cast<int>(pgrid), c.st32, sweep_tol, it);
+      }, it);
+      launched = it;
+    });
+    // launches after convergence did no work: count them out
+    uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
+    uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
+    return inner;
+  });
+}
+
 void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
-  const bool jac = c.solver.jacobi != 0;
-  // per row and iteration: the packed (r, w, s, M) read and written, p and d
-  // read and written; + Dinv read by the initialisation only
+  // per row and iteration of the one-kernel CG: the packed (r, w, s, M) read
+  // and written, p and d read and written
   const double vec = 16.0 * 2 + 4.0 * 4;
   if (c.ppe_from_particles) {
     ppe_compress_columns(c);
@@ -722,13 +1088,16 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
                     n, L.rpb};
     // int16 columns of narrow slices, u32 of wide ones; row length, ghost
     // and position of every row; slice tables
-    const double bytes = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 +
-                         n * (vec + 4 + 4 + 8) + c.nslices * 25.0;
-    solve_mixed_op(c, A, bytes, sr, max_iter);
+    const double cols = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 + c.nslices * 25.0;
+    const double rows = n * (4 + 4 + 8.0);
+    const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
+    if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
+      solve_mixed_ml(c, A, cols + rows + n * (4.0 + 8 + 8), sr, max_iter);  // z; p, q read and written
+    else
+      solve_mixed_cg(c, A, cols + rows + n * vec, sr, max_iter);
   } else {
     ppe_make_fp32(c);
-    (void)jac;
-    solve_mixed_op(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
+    solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
   }
 }
 

@@ -19,6 +19,7 @@ constexpr int kRedBlocks = 1024;   // fixed partial-sum slots of the reduction k
 
 // Particle type codes, Particle.hpp:16-29.
 constexpr uint8_t kFluid = 0, kWall = 1, kDummy = 2, kDisabled = 3;
+constexpr uint32_t kNoCell = 0xffffffffu;  // cell key of a particle outside the grid
 
 // Device-side CG scalars.  rr is double-buffered by iteration parity so that
 // block 0 of a kernel can publish the next value while the other blocks of
@@ -27,6 +28,7 @@ struct CgStatus {
   double rr[2];
   double gamma[2];    // FP32 inner solver: (r, M r) and alpha by iteration parity
   double alpha[2];
+  double alpha_ml;    // multilevel inner solver: alpha of the running iteration
   double tol;
   double rr0;
   double b2;          // |b|^2 (stop rule on |b|)
@@ -131,6 +133,7 @@ struct Context {
   // ((i/64)*kmax + k)*64 + i%64 (coalesced across a wavefront)
   uint32_t *nbr = nullptr, *ncount = nullptr;
   bool have_neighbors = false;
+  long long search_serial = 0;  // neighbour searches run so far (cells of the last one in key / sorted)
   unsigned long long* nbr_total_dev = nullptr;
   unsigned long long* nbr_total_host = nullptr;  // pinned
   double nbr_total = 0;                          // sum of list lengths (byte accounting)
@@ -152,6 +155,7 @@ struct Context {
   // the system was assembled from the current particle positions, so the
   // FP32 solver may recompute off-diagonals from them (matrix-free)
   bool ppe_from_particles = false;
+  long long ppe_search_serial = -1;  // search whose cells the assembled system matches
   float* ghost32 = nullptr;  // SPP ghost share of the diagonal (1 on rows without unknowns)
   uint2* qpos = nullptr;     // fixed-point positions of the matrix-free operator
   double qpos_inv_h = 0;     // their scale: q = (x - min) / h
@@ -181,6 +185,12 @@ struct Context {
   long long last_inner = 0;
   // mixed-precision work vectors
   float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
+  float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr;  // multilevel-preconditioned inner CG
+  float* agg_inv = nullptr;  // 1 / (1^T A_kk 1) of the cells and 2x2, 4x4 cell blocks (0: no correction)
+  double* part_ml = nullptr;  // partial sums of the per-cell-block kernel (2 x part_ml_cap)
+  size_t part_ml_cap = 0;
+  size_t agg_cap = 0;
+  size_t agg_off[4] = {0, 0, 0, 0};  // level l at agg_inv + agg_off[l]
   float* dinv32 = nullptr;
   float4 *g32a = nullptr, *g32b = nullptr;  // packed (r, w, s, M) of the inner CG, by parity
   double *part3a = nullptr, *part3b = nullptr;  // 3 x kRedBlocks partial sums, by parity

@@ -360,6 +360,7 @@ void ppe_assemble(Context& c) {
   });
   check_launch();
   c.ppe_from_particles = true;
+  c.ppe_search_serial = c.search_serial;
   c.have_col16 = false;
 }
 

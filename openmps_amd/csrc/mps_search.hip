@@ -21,7 +21,7 @@
 namespace mps {
 namespace {
 
-constexpr uint32_t kNoKey = 0xffffffffu;
+constexpr uint32_t kNoKey = kNoCell;
 
 struct GridParams {
   double min_x, min_z, nl;
@@ -242,6 +242,7 @@ void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* o
 
 void search(Context& c) {
   const size_t n = c.n;
+  c.search_serial++;
   GridParams g{c.env.min_x, c.env.min_z, c.env.neighbor_length, c.gnx, c.gnz, c.cap};
   MPS_HIP(hipMemsetAsync(c.cell_count, 0, c.ncells * sizeof(uint32_t), c.stream));
   MPS_HIP(hipMemsetAsync(c.cell_fill, 0, c.ncells * sizeof(uint32_t), c.stream));

@@ -1,0 +1,128 @@
+"""Validation runs on the GPU path.
+
+hydrostatic  A still water column (the static-pressure case): after a number
+             of ForwardTime steps, the fluid pressure against rho g (H - z),
+             H the free-surface height (half a spacing above the top layer).
+             Reported: the relative L2 error over the fluid particles deeper
+             than `skin` spacings below the surface, and the largest error of
+             the pressure averaged over horizontal bands 10 spacings thick.
+
+compare      The same scene stepped with two solver presets (e.g. mixed_ml
+             against fp64) for a number of steps; the relative L2 difference of
+             the pressure and the RMS position difference, in spacings.
+
+Usage:
+  python -m openmps_amd.validate hydrostatic --scene still_1m --steps 500 --solver mixed_ml
+  python -m openmps_amd.validate compare --scene dambreak_1m --steps 500 --solvers mixed_ml fp64
+Each run prints one JSON line on stdout (progress on stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _computer(cfg, preset: str, tol: float):
+    from . import Computer, make_environment, solvers
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    solvers.configure(c, preset, tol)
+    c.add_particles(cfg["particles"])
+    return c
+
+
+def _step(c, cfg, steps: int, label: str, every: int = 100):
+    t0 = time.perf_counter()
+    iters = 0
+    for k in range(steps):
+        c.forward_time(cfg["dt"])
+        iters += c.solve_stats()["iterations"]
+        if (k + 1) % every == 0 or k + 1 == steps:
+            _log(f"  [{label}] step {k + 1}/{steps}, {time.perf_counter() - t0:.1f} s")
+    return iters / max(steps, 1)
+
+
+def hydrostatic_error(P: dict, cfg: dict, skin: float = 5.0, band: float = 10.0) -> dict:
+    """Pressure of the fluid particles against rho g (H - z)."""
+    env = cfg["env"]
+    l0, rho, g = env["l0"], env["rho"], env["g"]
+    fluid = P["type"] == 0
+    z0 = cfg["particles"]["z"][cfg["particles"]["type"] == 0]
+    H = z0.max() + 0.5 * l0
+    z, p = P["z"][fluid], P["p"][fluid]
+    deep = z < H - skin * l0
+    want = rho * g * (H - z)
+    rel_l2 = float(np.linalg.norm(p[deep] - want[deep]) / np.linalg.norm(want[deep]))
+    edges = np.arange(0.0, H + band * l0, band * l0)
+    idx = np.digitize(z, edges)
+    worst = 0.0
+    for b in np.unique(idx[deep]):
+        m = (idx == b) & deep
+        if m.sum() < 10:
+            continue
+        worst = max(worst, abs(p[m].mean() - want[m].mean()) / (rho * g * H))
+    return {"rel_l2_deep": rel_l2, "band_mean_max_err_rel_bottom": worst, "particles": int(deep.sum()),
+            "surface_height_m": float(H)}
+
+
+def run_hydrostatic(scene: str, steps: int, preset: str, tol: float) -> dict:
+    from . import scenes
+    cfg = scenes.config(scene)
+    c = _computer(cfg, preset, tol)
+    its = _step(c, cfg, steps, preset)
+    out = hydrostatic_error(c.particles(), cfg)
+    out.update({"check": "hydrostatic", "scene": scene, "steps": steps, "solver": preset,
+                "mean_cg_iterations": its})
+    return out
+
+
+def run_compare(scene: str, steps: int, presets: list[str], tol: float) -> dict:
+    from . import scenes
+    cfg = scenes.config(scene)
+    states = []
+    for pr in presets:
+        c = _computer(cfg, pr, tol)
+        its = _step(c, cfg, steps, pr)
+        P = c.particles()
+        states.append((pr, P, its))
+        del c
+    (a, Pa, ia), (b, Pb, ib) = states[0], states[1]
+    fl = Pb["type"] == 0
+    dp = float(np.linalg.norm(Pa["p"][fl] - Pb["p"][fl]) / np.linalg.norm(Pb["p"][fl]))
+    dx = np.sqrt(((Pa["x"][fl] - Pb["x"][fl]) ** 2 + (Pa["z"][fl] - Pb["z"][fl]) ** 2).mean())
+    return {"check": "compare", "scene": scene, "steps": steps, "solvers": presets,
+            "pressure_rel_l2": dp, "position_rms_in_l0": float(dx / cfg["env"]["l0"]),
+            "mean_cg_iterations": {a: ia, b: ib}}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    h = sub.add_parser("hydrostatic")
+    h.add_argument("--scene", default="still_1m")
+    h.add_argument("--steps", type=int, default=500)
+    h.add_argument("--solver", default="mixed_ml")
+    h.add_argument("--tol", type=float, default=1e-8)
+    c = sub.add_parser("compare")
+    c.add_argument("--scene", default="dambreak_1m")
+    c.add_argument("--steps", type=int, default=500)
+    c.add_argument("--solvers", nargs=2, default=["mixed_ml", "fp64"])
+    c.add_argument("--tol", type=float, default=1e-8)
+    args = ap.parse_args(argv)
+    if args.cmd == "hydrostatic":
+        out = run_hydrostatic(args.scene, args.steps, args.solver, args.tol)
+    else:
+        out = run_compare(args.scene, args.steps, args.solvers, args.tol)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
