@@ -216,10 +216,11 @@ def main() -> int:
         "roofline": roofline,
         "roofline_fp64_spmv": roofline_fp64,
         "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
-                        "launches": v["launches"]} for k, v in kt.items()},
+                        "launches": v["launches"], "bytes_per_launch": round(v["bytes_per_launch"])}
+                    for k, v in kt.items()},
         "kernels_fp64": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
-                             "launches": v["launches"]} for k, v in kt64.items()
-                         if k.startswith("cg_")},
+                             "launches": v["launches"], "bytes_per_launch": round(v["bytes_per_launch"])}
+                         for k, v in kt64.items() if k.startswith("cg_")},
         "copy_kernel": copy,
         "cpu_baseline": cpu,
     }

@@ -55,7 +55,8 @@ class SolverOpts(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", _I64), ("outer_iterations", _I64), ("rr0", _D), ("rr", _D),
                 ("rel_residual", _D), ("initial_rel_residual", _D), ("solve_ms", _D),
-                ("learned_used", _I64), ("prev_rel_residual", _D)]
+                ("learned_used", _I64), ("prev_rel_residual", _D),
+                ("fallback", _I64)]
 
 
 class KernelStat(ctypes.Structure):

@@ -492,7 +492,13 @@ struct CellGrid {
   uint32_t g1z;                         // 2x2 blocks along z
   uint32_t bnx, bnz;                    // 4x4 blocks
   size_t off1, off2;                    // level offsets in the aggregate arrays
+  // renumbered system: `sorted` is null, the rows of a cell are start[cell] + k,
+  // and perm maps a row to its particle (for the FP64 residual)
+  const uint32_t* __restrict__ perm;
 };
+__device__ __forceinline__ uint32_t cell_row(const CellGrid& G, uint32_t st0, uint32_t k) {
+  return G.sorted ? G.sorted[st0 + k] : st0 + k;
+}
 
 struct BlockLane {
   bool valid;  // the lane's cell lies in the grid
@@ -523,55 +529,72 @@ __device__ __forceinline__ void agg_sums(T v, T& y0, T& y1, T& y2) {
   y2 = y2 + __shfl_xor(y2, 8, 64);
 }
 
-// 1 / (1^T A_kk 1) for every aggregate (0 where the sum is not negative: no
-// correction), from the FP64 system; membership of a column by its cell.
-__global__ void __launch_bounds__(NTP) k_ml_setup(CellGrid G, Sell<double> A,
-                                                  const uint32_t* __restrict__ row_len,
-                                                  float* __restrict__ inv) {
-  const uint32_t waves = gridDim.x * (NTP / 64);
-  const uint32_t nb = G.bnx * G.bnz;
-  for (uint32_t b = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6); b < nb; b += waves) {
-    const BlockLane L = block_lane(G, b);
-    double s0 = 0, s1 = 0, s2 = 0;
-    if (L.valid) {
-      const uint32_t cnt = G.count[L.cell], st = G.start[L.cell];
-      const uint32_t cx = L.cell / G.gnz, cz = L.cell % G.gnz;
-      for (uint32_t k = L.sub; k < cnt; k += 4) {
-        const uint32_t i = G.sorted[st + k];
-        if (!in_agg(G.type[i])) continue;
-        const double d = A.diag[i];
-        s0 += d;
-        s1 += d;
-        s2 += d;
-        const size_t base = A.off[i >> 6] + (i & 63);
-        const uint32_t m = row_len[i];
-        for (uint32_t q = 0; q < m; q++) {
-          const size_t o = base + static_cast<size_t>(q) * 64;
-          const uint32_t j = A.col[o];
-          const uint32_t kj = G.key[j];
-          if (kj == kNoCell) continue;
-          const uint32_t jx = kj / G.gnz, jz = kj % G.gnz;
-          const double a = A.val[o];
-          if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
-            s2 += a;
-            if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
-              s1 += a;
-              if (jx == cx && jz == cz) s0 += a;
-            }
+// Per row i (dummy and disabled rows: 0), for the aggregates of levels 0..2
+// containing i: S_l(i) = A_ii + sum_{j in agg_l(i)} A_ij from the FP64 system.
+// Every term of sum_{i in k} S_l(i) = 1^T A_kk 1 is <= 0 (A_ii is minus the
+// sum of the row's off-diagonals and the ghost), so FP32 storage loses nothing
+// to cancellation.
+__global__ void __launch_bounds__(NT, 8) k_ml_rowsum(Sell<double> A, const uint32_t* __restrict__ row_len,
+                                                     const uint32_t* __restrict__ key,
+                                                     const uint8_t* __restrict__ type, uint32_t gnz,
+                                                     float* __restrict__ s0, float* __restrict__ s1,
+                                                     float* __restrict__ s2) {
+  const Rows rw = block_rows(A.n, A.rpb);
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    double a0 = 0, a1 = 0, a2 = 0;
+    const uint32_t ki = key[i];
+    if (in_agg(type[i]) && ki != kNoCell) {
+      const uint32_t cx = ki / gnz, cz = ki % gnz;
+      a0 = a1 = a2 = A.diag[i];
+      const size_t base = A.off[i >> 6] + (i & 63);
+      const uint32_t m = row_len[i];
+      for (uint32_t q = 0; q < m; q++) {
+        const size_t o = base + static_cast<size_t>(q) * 64;
+        const uint32_t kj = key[A.col[o]];
+        if (kj == kNoCell) continue;
+        const uint32_t jx = kj / gnz, jz = kj % gnz;
+        const double a = A.val[o];
+        if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
+          a2 += a;
+          if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
+            a1 += a;
+            if (jx == cx && jz == cz) a0 += a;
           }
         }
       }
     }
-    double y0, y1, y2, t;
-    agg_sums(s0, y0, t, t);
-    agg_sums(s1, t, y1, t);
-    agg_sums(s2, t, t, y2);
-    const uint32_t lane = threadIdx.x & 63;
-    if (L.valid && L.sub == 0) {
-      inv[L.cell] = (y0 < 0) ? static_cast<float>(1.0 / y0) : 0.0f;
-      if ((lane & 5) == 0) inv[G.off1 + L.a1] = (y1 < 0) ? static_cast<float>(1.0 / y1) : 0.0f;
-      if ((lane & 15) == 0) inv[G.off2 + L.a2] = (y2 < 0) ? static_cast<float>(1.0 / y2) : 0.0f;
-    }
+    s0[i] = static_cast<float>(a0);
+    s1[i] = static_cast<float>(a1);
+    s2[i] = static_cast<float>(a2);
+  }
+}
+
+// 1 / (1^T A_kk 1) for every aggregate (0 where the sum is not negative: no
+// correction): the row sums of k_ml_rowsum added over the members, one
+// wavefront per 4x4 block of cells.
+__global__ void __launch_bounds__(NTP) k_ml_setup(CellGrid G, const float* __restrict__ s0,
+                                                  const float* __restrict__ s1, const float* __restrict__ s2,
+                                                  float* __restrict__ inv) {
+  const uint32_t b = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  if (b >= G.bnx * G.bnz) return;  // wave-uniform
+  const BlockLane L = block_lane(G, b);
+  const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+  float t0 = 0, t1 = 0, t2 = 0;
+  for (uint32_t k = L.sub; k < cnt; k += 4) {
+    const uint32_t i = G.sorted[st0 + k];
+    t0 += s0[i];
+    t1 += s1[i];
+    t2 += s2[i];
+  }
+  float y0, y1, y2, t;
+  agg_sums(t0, y0, t, t);
+  agg_sums(t1, t, y1, t);
+  agg_sums(t2, t, t, y2);
+  const uint32_t lane = threadIdx.x & 63;
+  if (L.valid && L.sub == 0) {
+    inv[L.cell] = (y0 < 0) ? 1.0f / y0 : 0.0f;
+    if ((lane & 5) == 0) inv[G.off1 + L.a1] = (y1 < 0) ? 1.0f / y1 : 0.0f;
+    if ((lane & 15) == 0) inv[G.off2 + L.a2] = (y2 < 0) ? 1.0f / y2 : 0.0f;
   }
 }
 
@@ -612,7 +635,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
       const uint32_t k = L.sub + 4u * u;
-      id[u] = (k < cnt) ? G.sorted[st0 + k] : 0u;
+      id[u] = (k < cnt) ? cell_row(G, st0, k) : 0u;
     }
     float s = 0.0f;
 #pragma unroll
@@ -622,7 +645,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
       if (L.sub + 4u * u < cnt) {
         const uint32_t i = id[u];
         if (it == 0) {
-          ri = static_cast<float>(r64[i] * inv_norm);
+          ri = static_cast<float>(r64[G.perm ? G.perm[i] : i] * inv_norm);
           d[i] = 0.0f;
         } else {
           ri = fmaf(-alpha, q[i], r[i]);
@@ -637,10 +660,10 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
     }
     // cells beyond 4 * kPerLane particles (larger cell capacities)
     for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = G.sorted[st0 + k];
+      const uint32_t i = cell_row(G, st0, k);
       float ri;
       if (it == 0) {
-        ri = static_cast<float>(r64[i] * inv_norm);
+        ri = static_cast<float>(r64[G.perm ? G.perm[i] : i] * inv_norm);
         d[i] = 0.0f;
       } else {
         ri = fmaf(-alpha, q[i], r[i]);
@@ -663,7 +686,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
       }
     }
     for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = G.sorted[st0 + k];
+      const uint32_t i = cell_row(G, st0, k);
       const float ri = r[i];
       const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
       z[i] = zi;
@@ -740,6 +763,26 @@ __global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict
                                                     const float* __restrict__ d, double norm) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i < n) x[i] = fma(static_cast<double>(d[i]), norm, x[i]);
+}
+
+// x += |r| d for a correction computed on the renumbered system.
+__global__ void __launch_bounds__(256) k_ir_correct_perm(size_t n, double* __restrict__ x,
+                                                         const float* __restrict__ d,
+                                                         const uint32_t* __restrict__ old2new, double norm) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = old2new[i];
+  if (k != kNoCell) x[i] = fma(static_cast<double>(d[k]), norm, x[i]);
+}
+
+__global__ void __launch_bounds__(256) k_jacobi_inv_perm(size_t m, const uint32_t* __restrict__ new2old,
+                                                         const double* __restrict__ diag,
+                                                         float* __restrict__ dinv) {
+  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (k < m) {
+    const double dg = diag[new2old[k]];
+    dinv[k] = (dg != 0.0) ? static_cast<float>(1.0 / dg) : 1.0f;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const double* __restrict__ diag,
@@ -888,9 +931,10 @@ namespace {
 // refinement.  `sweep(norm, tol)` runs one FP32 solve to the relative
 // tolerance tol, leaves d in c.d32 and returns its iterations; the refinement
 // adds |r| d to x in FP64 and recomputes r = b - A x in FP64.
-template <class Sweep>
-void refine(Context& c, const StopRule& sr, Sweep sweep) {
-  const size_t n = c.ppe_n;
+long long max_iter_fallback(const Context& c) { return static_cast<long long>(c.ppe_n); }
+
+template <class Sweep, class Correct>
+void refine(Context& c, const StopRule& sr, Sweep sweep, Correct correct) {
   const double inner_tol = c.solver.inner_tol > 0 ? c.solver.inner_tol : 1e-4;
   long long total = 0;
   double rr0 = 0, bb = 0, rr = 0;
@@ -915,6 +959,19 @@ void refine(Context& c, const StopRule& sr, Sweep sweep) {
       target = rr0 * sr.tol * sr.tol;
       done = (outer == 0) ? (target == 0) : (rr < target);
     }
+    if (stalled && !done && rr == rr && total < total_cap) {
+      // the FP32 sweeps no longer make progress: the plain FP64 CG (the
+      // reference algorithm) finishes from the current x, to the same target
+      StopRule rest = sr;
+      if (sr.rule == MPS_STOP_INITIAL_RESIDUAL) rest.tol = sr.tol * std::sqrt(rr0 / rr);
+      const long long cap = c.solver.max_iter > 0 ? c.solver.max_iter - total : max_iter_fallback(c);
+      solve_cg64(c, rest, cap);
+      c.stats.iterations += total;
+      c.stats.outer_iterations = outer + 1;
+      c.stats.rr0 = rr0;
+      c.stats.fallback = 1;
+      return;
+    }
     if (done || stalled || outer >= max_outer || total >= total_cap || !(rr == rr)) break;
     const double norm = std::sqrt(rr);
     // this sweep reduces the residual by inner_tol, or only as far as the
@@ -923,8 +980,7 @@ void refine(Context& c, const StopRule& sr, Sweep sweep) {
     const long long inner = sweep(norm, sweep_tol);
     if (outer == 0) c.last_inner = inner;
     total += inner;
-    hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
-                       static_cast<const float*>(c.d32), norm);
+    correct(norm);
     check_launch();
   }
   c.stats.iterations = total;
@@ -947,12 +1003,20 @@ long long inner_limit(const Context& c, long long max_iter) {
 template <class Launch1>
 long long run_inner(Context& c, long long limit, Launch1 launch) {
   long long it = 1;
+  double rr_last = -1;
+  int flat = 0;
   while (true) {
     const long long chunk = next_chunk(it - 1, c.last_inner);
     for (long long m = 0; m < chunk && it <= limit; m++, it++) launch(it);
     check_launch();
     read_status(c, c.st32, c.st32_host);
     if (c.st32_host->converged || it > limit) break;
+    // FP32 stagnation: two chunks in a row that lower r.r by less than 19 %
+    // (|r| by less than 10 %) end the sweep; the FP64 refinement decides
+    const double rr = std::min(c.st32_host->rr[0], c.st32_host->rr[1]);
+    flat = (rr_last > 0 && !(rr < 0.81 * rr_last)) ? flat + 1 : 0;
+    rr_last = rr;
+    if (flat >= 2) return it - 1;
   }
   return c.st32_host->converged ? c.st32_host->iterations : limit;
 }
@@ -992,15 +1056,29 @@ void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, 
     // launches after convergence did no work: count them out
     uncount(c, MPS_K_CG_ITER32, launched - inner, b_iter);
     return inner;
+  }, [&](double norm) {
+    hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
+                       static_cast<const float*>(c.d32), norm);
   });
 }
 
-// Multilevel preconditioner: four kernels per iteration (see k_ml_prec).
-void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule& sr, long long max_iter) {
+// Multilevel preconditioner: four kernels per iteration (see k_ml_prec), on
+// the system renumbered cell by cell (c.pm): the cell-ordered passes then read
+// contiguous rows, and the gathered neighbours of a row are nearby rows.
+void solve_mixed_ml(Context& c, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
-  const Launch L = cg_grid(n);
-  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
-                     static_cast<const double*>(c.diag), c.dinv32);
+  const Launch L0 = cg_grid(n);
+  ppe_permute_cells(c);
+  const Context::Permuted& P = c.pm;
+  const size_t m = P.m;
+  const Launch L = cg_grid(m > 0 ? m : 1);
+  const MatFree A{P.off, P.scol, P.off16, P.col16, P.wide, P.len, P.pos, P.ghost,
+                  static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
+                                     c.qpos_inv_h),
+                  m, L.rpb};
+  if (m > 0)
+    hipLaunchKernelGGL(k_jacobi_inv_perm, dim3(grid_for(m)), dim3(256), 0, c.stream, m,
+                       static_cast<const uint32_t*>(P.new2old), static_cast<const double*>(c.diag), c.dinv32);
   const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
   const uint32_t g1x = (gnx + 1) / 2, g1z = (gnz + 1) / 2, bnx = (gnx + 3) / 4, bnz = (gnz + 3) / 4;
   const size_t off1 = static_cast<size_t>(gnx) * gnz, off2 = off1 + static_cast<size_t>(g1x) * g1z;
@@ -1018,23 +1096,27 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
     c.part_ml_cap = pgrid;
   }
   const size_t stride = c.part_ml_cap;
-  const CellGrid G{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz, off1, off2};
-  const int sgrid = static_cast<int>(std::min<size_t>(kRedBlocks, std::max<size_t>(1, pgrid)));
-  // the aggregate sums of A from the FP64 system
-  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 16.0 + n * 24.0 + total_aggs * 4.0, [&] {
-    hipLaunchKernelGGL(k_ml_setup, dim3(sgrid), dim3(NTP), 0, c.stream, G,
-                       sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const uint32_t*>(c.row_len),
-                       c.agg_inv);
+  // particle order (aggregate sums from the FP64 system) and renumbered order
+  const CellGrid G0{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz,
+                    off1, off2, nullptr};
+  const CellGrid G{P.cell_start, c.cell_count, nullptr, c.key, P.type, gnx, gnz, g1z, bnx, bnz,
+                   off1, off2, P.new2old};
+  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 12.0 + n * 37.0 + total_aggs * 4.0, [&] {
+    hipLaunchKernelGGL(k_ml_rowsum, dim3(L0.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L0.rpb),
+                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
+                       static_cast<const uint8_t*>(c.type), gnz, c.z32, c.r32, c.d32);
+    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G0,
+                       static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
+                       static_cast<const float*>(c.d32), c.agg_inv);
   });
-  // rows outside every cell keep r = d = z = 0
-  MPS_HIP(hipMemsetAsync(c.z32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
-  // per particle: id, r (read, written), q, p, d (read, written), type, M, z; cell tables
-  const double b_prec = n * 41.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  // per row: r (read, written), q, p, d (read, written), type, M, z; cell tables
+  const double b_prec = m * 33.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  // int16 columns of narrow slices, u32 of wide ones; row length, ghost,
+  // position; z; p, q read and written; slice tables
+  const double b_spmv = (P.nnz - P.nnz_wide) * 2.0 + P.nnz_wide * 4.0 + P.nslices * 25.0 + m * (16.0 + 4 + 16);
   auto prec = [&](double norm, long long it) {
     hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
                        static_cast<const float*>(c.agg_inv), static_cast<const double*>(c.r), 1.0 / norm,
@@ -1043,7 +1125,7 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
   };
   refine(c, sr, [&](double norm, double sweep_tol) {
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
-    tracked(c, MPS_K_CG_INIT32, n * 29.0, [&] {
+    tracked(c, MPS_K_CG_INIT32, m * 21.0, [&] {
       prec(norm, 0);
       hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
                          stride, static_cast<int>(pgrid), c.st32, sweep_tol, 0LL);
@@ -1071,6 +1153,9 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
     uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
     uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
     return inner;
+  }, [&](double norm) {
+    hipLaunchKernelGGL(k_ir_correct_perm, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
+                       static_cast<const float*>(c.d32), static_cast<const uint32_t*>(P.old2new), norm);
   });
 }
 
@@ -1080,25 +1165,25 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   // per row and iteration of the one-kernel CG: the packed (r, w, s, M) read
   // and written, p and d read and written
   const double vec = 16.0 * 2 + 4.0 * 4;
-  if (c.ppe_from_particles) {
-    ppe_compress_columns(c);
-    const MatFree A{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
-                    static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
-                                       c.qpos_inv_h),
-                    n, L.rpb};
-    // int16 columns of narrow slices, u32 of wide ones; row length, ghost
-    // and position of every row; slice tables
-    const double cols = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 + c.nslices * 25.0;
-    const double rows = n * (4 + 4 + 8.0);
-    const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
-    if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
-      solve_mixed_ml(c, A, cols + rows + n * (4.0 + 8 + 8), sr, max_iter);  // z; p, q read and written
-    else
-      solve_mixed_cg(c, A, cols + rows + n * vec, sr, max_iter);
-  } else {
+  if (!c.ppe_from_particles) {
     ppe_make_fp32(c);
     solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
+    return;
   }
+  const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
+  if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match) {
+    solve_mixed_ml(c, sr, max_iter);
+    return;
+  }
+  ppe_compress_columns(c);
+  const MatFree A{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
+                  static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
+                                     c.qpos_inv_h),
+                  n, L.rpb};
+  // int16 columns of narrow slices, u32 of wide ones; row length, ghost and
+  // position of every row; slice tables
+  const double cols = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 + c.nslices * 25.0;
+  solve_mixed_cg(c, A, cols + n * (4 + 4 + 8.0 + vec), sr, max_iter);
 }
 
 }  // namespace

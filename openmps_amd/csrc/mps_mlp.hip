@@ -354,25 +354,53 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// Appends (features, (sol - p_prev) / s_d) of a hashed subset of alive rows.
-__global__ void __launch_bounds__(256) k_collect(size_t n, const uint8_t* __restrict__ type,
-                                                 const __bf16* __restrict__ feats,
-                                                 const double* __restrict__ sol,
-                                                 const double* __restrict__ p, double inv_sd,
-                                                 uint64_t seed, uint32_t threshold, size_t cap,
-                                                 unsigned long long* count, __bf16* __restrict__ dx,
-                                                 float* __restrict__ dt) {
+struct CollectArgs {
+  size_t n;
+  const uint8_t* type;
+  const __bf16* feats;
+  const double *sol, *p, *x, *z;  // solution, previous pressure, positions
+  const uint32_t *ids, *count;    // neighbour lists
+  uint32_t kmax;
+  double rl2, inv_sd;
+  uint64_t seed;
+  uint32_t threshold;
+  size_t cap;
+  unsigned long long* used;
+  __bf16* dx;
+  float* dt;
+};
+
+// Appends (features, target) of a hashed subset of alive rows.  The target is
+// the local part of the correction delta = sol - p_prev: delta_i minus the
+// mean of delta over i and its Laplacian neighbours (the matrix pattern),
+// divided by s_d.  The smooth remainder of delta is not a function of a
+// particle's neighbourhood; the local part is, and it carries most of the
+// residual A delta.
+__global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
   const size_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  const uint8_t t = type[i];
+  if (i >= a.n) return;
+  const uint8_t t = a.type[i];
   if (t == kDummy || t == kDisabled) return;
-  if (static_cast<uint32_t>(mix64(seed ^ (i * 0x2545F4914F6CDD1Dull)) >> 32) >= threshold) return;
-  const unsigned long long slot = atomicAdd(count, 1ull);
-  if (slot >= cap) return;
-  const uint4* src = reinterpret_cast<const uint4*>(feats + i * kFeatPad);
-  uint4* dst = reinterpret_cast<uint4*>(dx + slot * kFeatPad);
+  if (static_cast<uint32_t>(mix64(a.seed ^ (i * 0x2545F4914F6CDD1Dull)) >> 32) >= a.threshold) return;
+  const double di = a.sol[i] - a.p[i];
+  double sum = di;
+  int cnt = 1;
+  const double xi = a.x[i], zi = a.z[i];
+  const uint32_t m = a.count[i];
+  for (uint32_t q = 0; q < m; q++) {
+    const uint32_t j = a.ids[nbr_slot(static_cast<uint32_t>(i), q, a.kmax)];
+    if (a.type[j] == kDummy) continue;
+    const double ex = a.x[j] - xi, ez = a.z[j] - zi;
+    if (!(ex * ex + ez * ez < a.rl2)) continue;
+    sum += a.sol[j] - a.p[j];
+    cnt++;
+  }
+  const unsigned long long slot = atomicAdd(a.used, 1ull);
+  if (slot >= a.cap) return;
+  const uint4* src = reinterpret_cast<const uint4*>(a.feats + i * kFeatPad);
+  uint4* dst = reinterpret_cast<uint4*>(a.dx + slot * kFeatPad);
   for (int c = 0; c < kFeatPad / 8; c++) dst[c] = src[c];
-  dt[slot] = static_cast<float>((sol[i] - p[i]) * inv_sd);
+  a.dt[slot] = static_cast<float>((di - sum / cnt) * a.inv_sd);
 }
 
 unsigned blocks(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
@@ -595,6 +623,10 @@ bool learned_features(Context& c, double& s_r, double& rr) {
   reduce2(c, c.partB, c.partC, 256);
   const double pp = c.red_host[0], nalive = c.red_host[1];
   if (nalive <= 0) return false;
+  hipLaunchKernelGGL(k_alive_stats, dim3(256), dim3(256), 0, c.stream, n, static_cast<const uint8_t*>(c.type),
+                     static_cast<const double*>(c.diag), c.partB, c.partC);
+  reduce2(c, c.partB, c.partC, 256);
+  c.learned_drms = std::sqrt(c.red_host[0] / nalive);
   s_r = std::sqrt(rr / nalive);
   const double s_p = std::sqrt(pp / nalive);
   if (!c.feats || c.feats_cap < n) {
@@ -613,7 +645,11 @@ bool learned_features(Context& c, double& s_r, double& rr) {
   return true;
 }
 
-double learned_scale(const Context& c, double s_r) { return s_r * c.env.l0 * c.env.l0 * 0.2; }
+// Scale of the model output: the residual's RMS over the diagonal's RMS, the
+// size of a Jacobi correction.
+double learned_scale(const Context& c, double s_r) {
+  return c.learned_drms > 0 ? s_r / c.learned_drms : 0.0;
+}
 
 // MPS_WARM_LEARNED: x0 = p_prev + s_d MLP(features); keeps p_prev when the
 // prediction does not lower the FP64 residual.
@@ -650,12 +686,13 @@ void collect_before(Context& c) {
 void collect_after(Context& c) {
   if (!c.dataset || !c.collect_ok) return;
   const double sd = learned_scale(c, c.collect_sr);
+  if (!(sd > 0)) return;
   const uint32_t thr = static_cast<uint32_t>(std::min(1.0, std::max(0.0, c.collect_frac)) * 4294967295.0);
-  hipLaunchKernelGGL(k_collect, dim3(blocks(c.n)), dim3(256), 0, c.stream, c.n,
-                     static_cast<const uint8_t*>(c.type), static_cast<const __bf16*>(c.feats),
-                     static_cast<const double*>(c.sol), static_cast<const double*>(c.p), 1.0 / sd,
-                     c.collect_seed ^ (static_cast<uint64_t>(c.collect_step) * 0x9E3779B97F4A7C15ull),
-                     thr, c.dataset->cap, c.dataset->count, c.dataset->x, c.dataset->t);
+  const CollectArgs a{c.n, c.type, c.feats, c.sol, c.p, c.x, c.z, c.nbr, c.ncount, c.kmax,
+                      c.env.r_lap * c.env.r_lap, 1.0 / sd,
+                      c.collect_seed ^ (static_cast<uint64_t>(c.collect_step) * 0x9E3779B97F4A7C15ull), thr,
+                      c.dataset->cap, c.dataset->count, c.dataset->x, c.dataset->t};
+  hipLaunchKernelGGL(k_collect, dim3(blocks(c.n)), dim3(256), 0, c.stream, a);
   check_launch();
   c.collect_step++;
 }

@@ -27,6 +27,9 @@ struct GridParams {
   double min_x, min_z, nl;
   long long gnx, gnz;
   uint32_t cap;
+  // squared-distance bounds around NL^2 (relative 2^-48): below lo a pair is
+  // certainly within NL, above hi certainly not, between them R is computed
+  double lo, hi;
 };
 
 // Grid::Block (Grid.hpp:250-254) with the range test of Grid::Store
@@ -205,8 +208,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
           const uint32_t j = sorted[s + l];
           if (j == i) continue;
           const double rx = xi - x[j], rz = zi - z[j];
-          const double r = sqrt(rx * rx + rz * rz);  // Computer::R, Computer.hpp:568-572
-          if (r < nl) {
+          // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
+          // rounded, so the test only depends on r.r near NL^2: outside the
+          // narrow band [lo, hi] the comparison of r.r decides it exactly.
+          const double q = rx * rx + rz * rz;
+          const bool within = (q < g.lo) || (q <= g.hi && sqrt(q) < nl);
+          if (within) {
             if (cnt < kmax) nbr[nbr_slot(static_cast<uint32_t>(i), cnt, kmax)] = j;
             cnt++;
           }
@@ -236,6 +243,10 @@ static void scan_u32(Context& c, const uint32_t* in, size_t n, TO* out, uint64_t
                      static_cast<const uint64_t*>(c.scan_tmp), out, mult);
 }
 
+void scan_counts(Context& c, const uint32_t* counts, size_t n, uint32_t* offsets, uint64_t* total) {
+  scan_u32<uint32_t>(c, counts, n, offsets, 1, total);
+}
+
 void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* offsets) {
   scan_u32<uint64_t>(c, widths, nslices, offsets, 64, offsets + nslices);
 }
@@ -243,7 +254,9 @@ void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* o
 void search(Context& c) {
   const size_t n = c.n;
   c.search_serial++;
-  GridParams g{c.env.min_x, c.env.min_z, c.env.neighbor_length, c.gnx, c.gnz, c.cap};
+  const double nl2 = c.env.neighbor_length * c.env.neighbor_length;
+  GridParams g{c.env.min_x, c.env.min_z, c.env.neighbor_length, c.gnx, c.gnz, c.cap,
+               nl2 * (1.0 - 0x1p-48), nl2 * (1.0 + 0x1p-48)};
   MPS_HIP(hipMemsetAsync(c.cell_count, 0, c.ncells * sizeof(uint32_t), c.stream));
   MPS_HIP(hipMemsetAsync(c.cell_fill, 0, c.ncells * sizeof(uint32_t), c.stream));
   MPS_HIP(hipMemsetAsync(c.flags, 0, 4 * sizeof(int), c.stream));

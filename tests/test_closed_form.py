@@ -153,3 +153,26 @@ def test_ppe_solve_direct(kind):
     xd = scipy.sparse.linalg.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
     assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b - A @ x0)
+
+
+def test_neighbor_distance_band():
+    """The neighbour test sqrt(r.r) < NL (Computer.hpp:568-572) is evaluated on
+    r.r outside the band NL^2 (1 -/+ 2^-48) and with the square root inside it.
+    Correct rounding of sqrt makes both decide identically; checked here on
+    squared distances within a few ulps of every threshold and on random ones,
+    for the neighbour lengths of the configured scenes."""
+    rng = np.random.Generator(np.random.PCG64(17))
+    for nl in (0.0372, 0.00372, 0.002604, 0.0026039999999999997, 1.0, 3.0e-3 * (1 + 2 ** -40)):
+        nl2 = nl * nl
+        lo, hi = nl2 * (1.0 - 2.0 ** -48), nl2 * (1.0 + 2.0 ** -48)
+        qs = [nl2, lo, hi, (nl * (1 - 2 ** -52)) ** 2, (nl * (1 + 2 ** -52)) ** 2]
+        for q0 in list(qs):
+            for k in range(-64, 65):
+                qs.append(np.nextafter(q0, np.inf) if k > 0 else q0)
+                q0 = np.nextafter(q0, np.inf if k > 0 else -np.inf) if k else q0
+        qs += list(nl2 * (1 + rng.uniform(-1e-12, 1e-12, 20000)))
+        qs += list(nl2 * rng.uniform(0, 2, 20000))
+        for q in qs:
+            want = math.sqrt(q) < nl
+            got = (q < lo) or (q <= hi and math.sqrt(q) < nl)
+            assert got == want, (nl, q)
