@@ -141,7 +141,6 @@ void free_all(Context& c) {
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
-  mps::free_permuted(c);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);

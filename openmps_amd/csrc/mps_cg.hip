@@ -492,13 +492,7 @@ struct CellGrid {
   uint32_t g1z;                         // 2x2 blocks along z
   uint32_t bnx, bnz;                    // 4x4 blocks
   size_t off1, off2;                    // level offsets in the aggregate arrays
-  // renumbered system: `sorted` is null, the rows of a cell are start[cell] + k,
-  // and perm maps a row to its particle (for the FP64 residual)
-  const uint32_t* __restrict__ perm;
 };
-__device__ __forceinline__ uint32_t cell_row(const CellGrid& G, uint32_t st0, uint32_t k) {
-  return G.sorted ? G.sorted[st0 + k] : st0 + k;
-}
 
 struct BlockLane {
   bool valid;  // the lane's cell lies in the grid
@@ -606,12 +600,13 @@ __global__ void __launch_bounds__(NTP) k_ml_setup(CellGrid G, const float* __res
 //   k_ml_prec(it):  r -= alpha q, d += alpha p; z = M r; partial sums of
 //                   rho_it = r.z and of r.r;
 //   k_ml_rho(it):   rho_it, r.r and the stop test (one block).
-// k_ml_prec visits the particles cell by cell, one wavefront per 4x4 block
-// of cells (every particle of the system that is in a cell exactly once; rows
-// outside every cell are identity rows whose r, d, z stay 0), so the residual
-// update, the aggregate sums and z share one pass; its large grid hides the
-// latency of the id -> value gathers.  Iteration 0 of k_ml_prec starts the
-// sweep: r = r64 / |r64|, d = 0.
+// (A grid-wide sum finished by the last block to arrive would save the two
+// one-block launches, but its device-scope release costs an L2 write-back per
+// block on this multi-XCD part, far more than a launch.)  k_ml_prec visits the particles cell by cell,
+// one wavefront per 4x4 block of cells (every particle of the system that is
+// in a cell exactly once; rows outside every cell are identity rows whose r,
+// d, z stay 0), so the residual update, the aggregate sums and z share one
+// pass.  Iteration 0 of k_ml_prec starts the sweep: r = r64 / |r64|, d = 0.
 constexpr int kPerLane = 7;  // particles of a cell held in registers per lane (4 lanes per cell)
 
 __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __restrict__ inv,
@@ -629,23 +624,28 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
   if (b < G.bnx * G.bnz) {  // wave-uniform
     const BlockLane L = block_lane(G, b);
     const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+    // aggregate scales: independent of the particles, loaded with them
+    const float i0 = L.valid ? inv[L.cell] : 0.0f, i1 = L.valid ? inv[G.off1 + L.a1] : 0.0f,
+                i2 = L.valid ? inv[G.off2 + L.a2] : 0.0f;
     uint32_t id[kPerLane];
-    float rv[kPerLane];
+    float rv[kPerLane], dv[kPerLane];
     bool ag[kPerLane];
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
       const uint32_t k = L.sub + 4u * u;
-      id[u] = (k < cnt) ? cell_row(G, st0, k) : 0u;
+      id[u] = (k < cnt) ? G.sorted[st0 + k] : 0u;
     }
     float s = 0.0f;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
       float ri = 0.0f;
       ag[u] = false;
+      dv[u] = 0.0f;
       if (L.sub + 4u * u < cnt) {
         const uint32_t i = id[u];
+        dv[u] = dinv[i];
         if (it == 0) {
-          ri = static_cast<float>(r64[G.perm ? G.perm[i] : i] * inv_norm);
+          ri = static_cast<float>(r64[i] * inv_norm);
           d[i] = 0.0f;
         } else {
           ri = fmaf(-alpha, q[i], r[i]);
@@ -660,10 +660,10 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
     }
     // cells beyond 4 * kPerLane particles (larger cell capacities)
     for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = cell_row(G, st0, k);
+      const uint32_t i = G.sorted[st0 + k];
       float ri;
       if (it == 0) {
-        ri = static_cast<float>(r64[G.perm ? G.perm[i] : i] * inv_norm);
+        ri = static_cast<float>(r64[i] * inv_norm);
         d[i] = 0.0f;
       } else {
         ri = fmaf(-alpha, q[i], r[i]);
@@ -675,18 +675,18 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
     }
     float y0, y1, y2;
     agg_sums(s, y0, y1, y2);
-    const float corr = L.valid ? y0 * inv[L.cell] + y1 * inv[G.off1 + L.a1] + y2 * inv[G.off2 + L.a2] : 0.0f;
+    const float corr = y0 * i0 + y1 * i1 + y2 * i2;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
       if (L.sub + 4u * u < cnt) {
         const uint32_t i = id[u];
-        const float zi = ag[u] ? fmaf(dinv[i], rv[u], corr) : dinv[i] * rv[u];
+        const float zi = ag[u] ? fmaf(dv[u], rv[u], corr) : dv[u] * rv[u];
         z[i] = zi;
         rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
       }
     }
     for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = cell_row(G, st0, k);
+      const uint32_t i = G.sorted[st0 + k];
       const float ri = r[i];
       const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
       z[i] = zi;
@@ -702,7 +702,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
 }
 
 // rho_it and r_it.r_it from the partials of k_ml_prec(it); stop test on r.r
-// relative to the sweep's r_0.r_0 (single block).
+// relative to the sweep's r_0.r_0 (one block).
 __global__ void __launch_bounds__(1024) k_ml_rho(const double* __restrict__ part, size_t part_stride,
                                                  int nparts, CgStatus* st, double tol_rel, long long it) {
   __shared__ double lds[1024 / 64];
@@ -726,7 +726,7 @@ __global__ void __launch_bounds__(1024) k_ml_rho(const double* __restrict__ part
   }
 }
 
-// alpha = rho_(it-1) / p.q (single block).
+// alpha = rho_(it-1) / p.q (one block).
 __global__ void __launch_bounds__(1024) k_ml_alpha(const double* __restrict__ part, int nparts, CgStatus* st,
                                                    long long it) {
   __shared__ double lds[1024 / 64];
@@ -763,26 +763,6 @@ __global__ void __launch_bounds__(256) k_ir_correct(size_t n, double* __restrict
                                                     const float* __restrict__ d, double norm) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i < n) x[i] = fma(static_cast<double>(d[i]), norm, x[i]);
-}
-
-// x += |r| d for a correction computed on the renumbered system.
-__global__ void __launch_bounds__(256) k_ir_correct_perm(size_t n, double* __restrict__ x,
-                                                         const float* __restrict__ d,
-                                                         const uint32_t* __restrict__ old2new, double norm) {
-  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = old2new[i];
-  if (k != kNoCell) x[i] = fma(static_cast<double>(d[k]), norm, x[i]);
-}
-
-__global__ void __launch_bounds__(256) k_jacobi_inv_perm(size_t m, const uint32_t* __restrict__ new2old,
-                                                         const double* __restrict__ diag,
-                                                         float* __restrict__ dinv) {
-  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (k < m) {
-    const double dg = diag[new2old[k]];
-    dinv[k] = (dg != 0.0) ? static_cast<float>(1.0 / dg) : 1.0f;
-  }
 }
 
 __global__ void __launch_bounds__(256) k_jacobi_inv(size_t n, const double* __restrict__ diag,
@@ -1062,23 +1042,12 @@ void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, 
   });
 }
 
-// Multilevel preconditioner: four kernels per iteration (see k_ml_prec), on
-// the system renumbered cell by cell (c.pm): the cell-ordered passes then read
-// contiguous rows, and the gathered neighbours of a row are nearby rows.
-void solve_mixed_ml(Context& c, const StopRule& sr, long long max_iter) {
+// Multilevel preconditioner: four kernels per iteration (see k_ml_prec).
+void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
-  const Launch L0 = cg_grid(n);
-  ppe_permute_cells(c);
-  const Context::Permuted& P = c.pm;
-  const size_t m = P.m;
-  const Launch L = cg_grid(m > 0 ? m : 1);
-  const MatFree A{P.off, P.scol, P.off16, P.col16, P.wide, P.len, P.pos, P.ghost,
-                  static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
-                                     c.qpos_inv_h),
-                  m, L.rpb};
-  if (m > 0)
-    hipLaunchKernelGGL(k_jacobi_inv_perm, dim3(grid_for(m)), dim3(256), 0, c.stream, m,
-                       static_cast<const uint32_t*>(P.new2old), static_cast<const double*>(c.diag), c.dinv32);
+  const Launch L = cg_grid(n);
+  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                     static_cast<const double*>(c.diag), c.dinv32);
   const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
   const uint32_t g1x = (gnx + 1) / 2, g1z = (gnz + 1) / 2, bnx = (gnx + 3) / 4, bnz = (gnz + 3) / 4;
   const size_t off1 = static_cast<size_t>(gnx) * gnz, off2 = off1 + static_cast<size_t>(g1x) * g1z;
@@ -1096,40 +1065,37 @@ void solve_mixed_ml(Context& c, const StopRule& sr, long long max_iter) {
     c.part_ml_cap = pgrid;
   }
   const size_t stride = c.part_ml_cap;
-  // particle order (aggregate sums from the FP64 system) and renumbered order
-  const CellGrid G0{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz,
-                    off1, off2, nullptr};
-  const CellGrid G{P.cell_start, c.cell_count, nullptr, c.key, P.type, gnx, gnz, g1z, bnx, bnz,
-                   off1, off2, P.new2old};
+  const CellGrid G{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz,
+                   off1, off2};
+  // the aggregate sums of A from the FP64 system (row sums in z, r, d as scratch)
   tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 12.0 + n * 37.0 + total_aggs * 4.0, [&] {
-    hipLaunchKernelGGL(k_ml_rowsum, dim3(L0.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L0.rpb),
+    hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
                        static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
                        static_cast<const uint8_t*>(c.type), gnz, c.z32, c.r32, c.d32);
-    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G0,
+    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
                        static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
                        static_cast<const float*>(c.d32), c.agg_inv);
   });
+  // rows outside every cell keep r = d = z = 0
+  MPS_HIP(hipMemsetAsync(c.z32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
-  // per row: r (read, written), q, p, d (read, written), type, M, z; cell tables
-  const double b_prec = m * 33.0 + c.ncells * 8.0 + total_aggs * 4.0;
-  // int16 columns of narrow slices, u32 of wide ones; row length, ghost,
-  // position; z; p, q read and written; slice tables
-  const double b_spmv = (P.nnz - P.nnz_wide) * 2.0 + P.nnz_wide * 4.0 + P.nslices * 25.0 + m * (16.0 + 4 + 16);
-  auto prec = [&](double norm, long long it) {
+  // per particle: id, r (read, written), q, p, d (read, written), type, M, z; cell tables
+  const double b_prec = n * 37.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  auto prec = [&](double norm, double sweep_tol, long long it) {
     hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
                        static_cast<const float*>(c.agg_inv), static_cast<const double*>(c.r), 1.0 / norm,
                        c.r32, c.d32, static_cast<const float*>(c.pa32), static_cast<const float*>(c.q32),
                        static_cast<const float*>(c.dinv32), c.z32, c.part_ml, stride, c.st32, it);
+    hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
+                       stride, static_cast<int>(pgrid), c.st32, sweep_tol, it);
   };
   refine(c, sr, [&](double norm, double sweep_tol) {
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
-    tracked(c, MPS_K_CG_INIT32, m * 21.0, [&] {
-      prec(norm, 0);
-      hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
-                         stride, static_cast<int>(pgrid), c.st32, sweep_tol, 0LL);
-    });
+    tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, sweep_tol, 0); });
     long long launched = 0;
     const long long inner = run_inner(c, limit, [&](long long it) {
       tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
@@ -1142,11 +1108,7 @@ void solve_mixed_ml(Context& c, const StopRule& sr, long long max_iter) {
         hipLaunchKernelGGL(k_ml_alpha, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(p_pq),
                            L.grid, c.st32, it);
       }, it);
-      tracked(c, MPS_K_ML_PREC, b_prec, [&] {
-        prec(norm, it);
-        hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
-                           stride, static_cast<int>(pgrid), c.st32, sweep_tol, it);
-      }, it);
+      tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, sweep_tol, it); }, it);
       launched = it;
     });
     // launches after convergence did no work: count them out
@@ -1154,8 +1116,8 @@ void solve_mixed_ml(Context& c, const StopRule& sr, long long max_iter) {
     uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
     return inner;
   }, [&](double norm) {
-    hipLaunchKernelGGL(k_ir_correct_perm, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
-                       static_cast<const float*>(c.d32), static_cast<const uint32_t*>(P.old2new), norm);
+    hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
+                       static_cast<const float*>(c.d32), norm);
   });
 }
 
@@ -1170,11 +1132,6 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
     return;
   }
-  const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
-  if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match) {
-    solve_mixed_ml(c, sr, max_iter);
-    return;
-  }
   ppe_compress_columns(c);
   const MatFree A{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
                   static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
@@ -1183,7 +1140,12 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   // int16 columns of narrow slices, u32 of wide ones; row length, ghost and
   // position of every row; slice tables
   const double cols = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 + c.nslices * 25.0;
-  solve_mixed_cg(c, A, cols + n * (4 + 4 + 8.0 + vec), sr, max_iter);
+  const double rows = n * (4 + 4 + 8.0);
+  const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
+  if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
+    solve_mixed_ml(c, A, cols + rows + n * (4.0 + 16), sr, max_iter);  // z; p, q read and written
+  else
+    solve_mixed_cg(c, A, cols + rows + n * vec, sr, max_iter);
 }
 
 }  // namespace

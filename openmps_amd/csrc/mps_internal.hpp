@@ -189,21 +189,6 @@ struct Context {
   float* agg_inv = nullptr;  // 1 / (1^T A_kk 1) of the cells and 2x2, 4x4 cell blocks (0: no correction)
   double* part_ml = nullptr;  // partial sums of the per-cell-block kernel (2 x part_ml_cap)
   size_t part_ml_cap = 0;
-  // the system renumbered cell by cell (cells in 4x4 blocks, blocks x-major):
-  // every cell, 2x2 and 4x4 block of cells is a contiguous range of rows
-  struct Permuted {
-    size_t m = 0;                   // rows (particles in a cell)
-    size_t nslices = 0, slots = 0, cap = 0, nalloc = 0, tiles = 0;
-    uint32_t *new2old = nullptr, *old2new = nullptr;
-    uint32_t *tile_count = nullptr, *tile_start = nullptr, *cell_start = nullptr;
-    uint32_t *len = nullptr, *width = nullptr, *w16 = nullptr, *scol = nullptr;
-    uint64_t *off = nullptr, *off16 = nullptr;
-    int16_t* col16 = nullptr;
-    uint8_t *wide = nullptr, *type = nullptr;
-    float* ghost = nullptr;
-    uint2* pos = nullptr;
-    double nnz = 0, nnz_wide = 0;
-  } pm;
   size_t agg_cap = 0;
   size_t agg_off[4] = {0, 0, 0, 0};  // level l at agg_inv + agg_off[l]
   float* dinv32 = nullptr;
@@ -321,8 +306,6 @@ void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
 void ppe_make_fp32(Context& c);
 void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
-void ppe_permute_cells(Context& c);     // mps_ppe.hip: builds c.pm from the cells of the last search
-void free_permuted(Context& c);
 void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,
                     const double* val, const double* b, const double* x0);
 void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uint32_t>& col,

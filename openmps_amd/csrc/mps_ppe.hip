@@ -22,7 +22,6 @@
 namespace mps {
 
 void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* offsets);
-void scan_counts(Context& c, const uint32_t* counts, size_t n, uint32_t* offsets, uint64_t* total);
 
 namespace {
 
@@ -299,122 +298,6 @@ __global__ void __launch_bounds__(kBlock) k_fill16(size_t n, const uint64_t* __r
   }
 }
 
-// ---- the system renumbered cell by cell ----------------------------------------
-// Tile order of the cells: 4x4 blocks of cells x-major, inside a block
-// u = 4 (cx % 4) + cz % 4.  Slot t = 16 b + u.
-struct TileGrid {
-  uint32_t gnx, gnz, bnz;
-};
-__device__ __forceinline__ bool tile_cell(const TileGrid& g, size_t t, uint32_t& cell) {
-  const uint32_t b = static_cast<uint32_t>(t >> 4), u = static_cast<uint32_t>(t & 15);
-  const uint32_t cx = 4 * (b / g.bnz) + (u >> 2), cz = 4 * (b % g.bnz) + (u & 3);
-  cell = cx * g.gnz + cz;
-  return cx < g.gnx && cz < g.gnz;
-}
-
-__global__ void __launch_bounds__(kBlock) k_pm_tile_counts(TileGrid g, size_t tiles,
-                                                           const uint32_t* __restrict__ cell_count,
-                                                           uint32_t* __restrict__ tile_count) {
-  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (t >= tiles) return;
-  uint32_t cell;
-  tile_count[t] = tile_cell(g, t, cell) ? cell_count[cell] : 0u;
-}
-
-// new2old / old2new and the new start of every cell.
-__global__ void __launch_bounds__(kBlock) k_pm_perm(TileGrid g, size_t tiles,
-                                                    const uint32_t* __restrict__ cell_start,
-                                                    const uint32_t* __restrict__ cell_count,
-                                                    const uint32_t* __restrict__ sorted,
-                                                    const uint32_t* __restrict__ tile_start,
-                                                    uint32_t* __restrict__ new_start,
-                                                    uint32_t* __restrict__ new2old,
-                                                    uint32_t* __restrict__ old2new) {
-  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (t >= tiles) return;
-  uint32_t cell;
-  if (!tile_cell(g, t, cell)) return;
-  const uint32_t ns = tile_start[t], os = cell_start[cell], cnt = cell_count[cell];
-  new_start[cell] = ns;
-  for (uint32_t k = 0; k < cnt; k++) {
-    const uint32_t i = sorted[os + k];
-    new2old[ns + k] = i;
-    old2new[i] = ns + k;
-  }
-}
-
-// Per new row: length, ghost share, position, type; per slice: width, and
-// whether every renumbered column offset fits in int16.
-__global__ void __launch_bounds__(kBlock) k_pm_rows(size_t m, const uint32_t* __restrict__ new2old,
-                                                    const uint32_t* __restrict__ old2new,
-                                                    const uint64_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ row_len,
-                                                    const uint32_t* __restrict__ col,
-                                                    const float* __restrict__ ghost, const uint2* __restrict__ pos,
-                                                    const uint8_t* __restrict__ type,
-                                                    uint32_t* __restrict__ len_out, float* __restrict__ ghost_out,
-                                                    uint2* __restrict__ pos_out, uint8_t* __restrict__ type_out,
-                                                    uint32_t* __restrict__ width, uint8_t* __restrict__ wide,
-                                                    uint32_t* __restrict__ w16, unsigned long long* nnz_wide) {
-  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  uint32_t len = 0;
-  int far = 0;
-  if (k < m) {
-    const uint32_t i = new2old[k];
-    len = row_len[i];
-    const size_t base = off[i >> 6] + (i & 63);
-    for (uint32_t q = 0; q < len; q++) {
-      const long long d = static_cast<long long>(old2new[col[base + static_cast<size_t>(q) * 64]]) -
-                          static_cast<long long>(k);
-      if (d > 32767 || d < -32768) far = 1;
-    }
-    len_out[k] = len;
-    ghost_out[k] = ghost[i];
-    pos_out[k] = pos[i];
-    type_out[k] = type[i];
-  }
-  const unsigned wmax = wave_max_u32(len);
-  const unsigned anyfar = wave_max_u32(static_cast<unsigned>(far));
-  const double wsum = wave_sum(static_cast<double>(len));
-  if ((threadIdx.x & 63) == 0) {
-    const size_t sl = k >> 6;
-    if (sl * 64 < m) {
-      width[sl] = wmax;
-      wide[sl] = static_cast<uint8_t>(anyfar);
-      w16[sl] = anyfar ? 0u : wmax;
-      if (anyfar) atomicAdd(nnz_wide, static_cast<unsigned long long>(wsum));
-    }
-  }
-}
-
-// Renumbered columns (u32 for every slice, int16 offsets for narrow slices;
-// padding slots point at the row itself).
-__global__ void __launch_bounds__(kBlock) k_pm_fill(size_t m, const uint32_t* __restrict__ new2old,
-                                                    const uint32_t* __restrict__ old2new,
-                                                    const uint64_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ row_len,
-                                                    const uint32_t* __restrict__ col,
-                                                    const uint64_t* __restrict__ noff,
-                                                    const uint32_t* __restrict__ nwidth,
-                                                    const uint8_t* __restrict__ wide,
-                                                    const uint64_t* __restrict__ noff16,
-                                                    uint32_t* __restrict__ ncol, int16_t* __restrict__ ncol16) {
-  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (k >= m) return;
-  const uint32_t i = new2old[k];
-  const uint32_t len = row_len[i], w = nwidth[k >> 6];
-  const size_t base = off[i >> 6] + (i & 63);
-  const size_t nb = noff[k >> 6] + (k & 63), nb16 = noff16[k >> 6] + (k & 63);
-  const bool narrow = !wide[k >> 6];
-  for (uint32_t q = 0; q < w; q++) {
-    const uint32_t j = (q < len) ? old2new[col[base + static_cast<size_t>(q) * 64]] : static_cast<uint32_t>(k);
-    ncol[nb + static_cast<size_t>(q) * 64] = j;
-    if (narrow)
-      ncol16[nb16 + static_cast<size_t>(q) * 64] =
-          static_cast<int16_t>(static_cast<long long>(j) - static_cast<long long>(k));
-  }
-}
-
 __global__ void __launch_bounds__(kBlock) k_to_f32(size_t n, const double* __restrict__ a,
                                                    float* __restrict__ b) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
@@ -509,92 +392,6 @@ void ppe_compress_columns(Context& c) {
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.mf_nnz_wide = static_cast<double>(wide);
   c.have_col16 = true;
-}
-
-void free_permuted(Context& c) {
-  Context::Permuted& P = c.pm;
-  dfree(P.new2old); dfree(P.old2new); dfree(P.tile_count); dfree(P.tile_start); dfree(P.cell_start);
-  dfree(P.len); dfree(P.width); dfree(P.w16); dfree(P.scol); dfree(P.off); dfree(P.off16);
-  dfree(P.col16); dfree(P.wide); dfree(P.type); dfree(P.ghost); dfree(P.pos);
-  P = Context::Permuted{};
-}
-
-// Renumbers the assembled system cell by cell (cells of the last search, in
-// 4x4 blocks): every particle in a cell gets a row; rows outside every cell
-// (disabled) are left out, their unknowns are identity rows with b = x = 0.
-void ppe_permute_cells(Context& c) {
-  Context::Permuted& P = c.pm;
-  const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
-  const uint32_t bnx = (gnx + 3) / 4, bnz = (gnz + 3) / 4;
-  const size_t tiles = static_cast<size_t>(bnx) * bnz * 16;
-  const size_t n = c.ppe_n;
-  if (P.nalloc < c.ppe_alloc + 64 || P.tiles < tiles || P.cap < c.sell_cap) {
-    free_permuted(c);
-    const size_t m = c.ppe_alloc + 64, ns = m / 64 + 2;
-    P.new2old = dalloc<uint32_t>(m);
-    P.old2new = dalloc<uint32_t>(m);
-    P.len = dalloc<uint32_t>(m);
-    P.ghost = dalloc<float>(m);
-    P.pos = dalloc<uint2>(m);
-    P.type = dalloc<uint8_t>(m);
-    P.width = dalloc<uint32_t>(ns);
-    P.w16 = dalloc<uint32_t>(ns);
-    P.wide = dalloc<uint8_t>(ns);
-    P.off = dalloc<uint64_t>(ns + 1);
-    P.off16 = dalloc<uint64_t>(ns + 1);
-    P.tile_count = dalloc<uint32_t>(tiles);
-    P.tile_start = dalloc<uint32_t>(tiles);
-    P.cell_start = dalloc<uint32_t>(static_cast<size_t>(gnx) * gnz);
-    P.scol = dalloc<uint32_t>(c.sell_cap);
-    P.col16 = dalloc<int16_t>(c.sell_cap);
-    P.nalloc = m;
-    P.tiles = tiles;
-    P.cap = c.sell_cap;
-  }
-  const TileGrid g{gnx, gnz, bnz};
-  MPS_HIP(hipMemsetAsync(P.old2new, 0xff, n * sizeof(uint32_t), c.stream));
-  MPS_HIP(hipMemsetAsync(c.ppe_counters + 1, 0, sizeof(unsigned long long), c.stream));
-  hipLaunchKernelGGL(k_pm_tile_counts, dim3(grid_for(tiles)), dim3(kBlock), 0, c.stream, g, tiles,
-                     static_cast<const uint32_t*>(c.cell_count), P.tile_count);
-  scan_counts(c, P.tile_count, tiles, P.tile_start, reinterpret_cast<uint64_t*>(c.ppe_counters) + 2);
-  hipLaunchKernelGGL(k_pm_perm, dim3(grid_for(tiles)), dim3(kBlock), 0, c.stream, g, tiles,
-                     static_cast<const uint32_t*>(c.cell_start), static_cast<const uint32_t*>(c.cell_count),
-                     static_cast<const uint32_t*>(c.sorted), static_cast<const uint32_t*>(P.tile_start),
-                     P.cell_start, P.new2old, P.old2new);
-  check_launch();
-  unsigned long long m = 0;
-  MPS_HIP(hipMemcpyAsync(&m, c.ppe_counters + 2, sizeof(m), hipMemcpyDeviceToHost, c.stream));
-  MPS_HIP(hipStreamSynchronize(c.stream));
-  if (m > n) throw Failure{MPS_ERR_STATE, "cells hold more particles than the system has rows"};
-  P.m = m;
-  P.nslices = (m + 63) / 64;
-  if (m == 0) return;
-  hipLaunchKernelGGL(k_pm_rows, dim3(grid_for(P.nslices * 64)), dim3(kBlock), 0, c.stream, static_cast<size_t>(m),
-                     static_cast<const uint32_t*>(P.new2old), static_cast<const uint32_t*>(P.old2new),
-                     static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.row_len),
-                     static_cast<const uint32_t*>(c.scol), static_cast<const float*>(c.ghost32),
-                     static_cast<const uint2*>(c.qpos), static_cast<const uint8_t*>(c.type), P.len, P.ghost,
-                     P.pos, P.type, P.width, P.wide, P.w16, c.ppe_counters + 1);
-  scan_slices(c, P.width, P.nslices, P.off);
-  scan_slices(c, P.w16, P.nslices, P.off16);
-  check_launch();
-  unsigned long long host[2] = {0, 0};
-  MPS_HIP(hipMemcpyAsync(&host[0], P.off + P.nslices, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
-  MPS_HIP(hipMemcpyAsync(&host[1], c.ppe_counters + 1, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
-  MPS_HIP(hipStreamSynchronize(c.stream));
-  if (host[0] > P.cap) throw Failure{MPS_ERR_STATE, "renumbered system exceeds its capacity"};
-  P.slots = host[0];
-  P.nnz_wide = static_cast<double>(host[1]);
-  P.nnz = c.nnz_offdiag;
-  tracked(c, MPS_K_OTHER, m * 12.0 + P.slots * 6.0 + c.nnz_offdiag * 8.0, [&] {
-    hipLaunchKernelGGL(k_pm_fill, dim3(grid_for(m)), dim3(kBlock), 0, c.stream, static_cast<size_t>(m),
-                       static_cast<const uint32_t*>(P.new2old), static_cast<const uint32_t*>(P.old2new),
-                       static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.row_len),
-                       static_cast<const uint32_t*>(c.scol), static_cast<const uint64_t*>(P.off),
-                       static_cast<const uint32_t*>(P.width), static_cast<const uint8_t*>(P.wide),
-                       static_cast<const uint64_t*>(P.off16), P.scol, P.col16);
-  });
-  check_launch();
 }
 
 // FP32 copies of an already assembled system (the mixed solver on a system

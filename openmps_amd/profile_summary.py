@@ -63,19 +63,12 @@ def kernel_durations(db: str) -> dict:
 
 
 def pmc_values(db: str, counter: str) -> dict:
-    """Per-dispatch counter values (sum over instances) by kernel name."""
+    """Per-dispatch counter values by kernel name (rocprofv3 `pmc_events` view)."""
     con = sqlite3.connect(db)
-    tables = [r[0] for r in con.execute("select name from sqlite_master where type in ('table','view')")]
     out = defaultdict(list)
-    if "pmc_events" in tables and "kernels" in tables:
-        q = ("select k.name, p.value from pmc_events p join kernels k on p.event_id = k.id "
-             "join pmc_info i on p.pmc_id = i.id where i.name = ?")
-        try:
-            rows = list(con.execute(q, (counter,)))
-        except sqlite3.Error:
-            rows = []
-        for name, v in rows:
-            out[short_name(name)].append(float(v))
+    for name, v in con.execute("select name, counter_value from pmc_events where counter_name = ?",
+                               (counter,)):
+        out[short_name(name)].append(float(v))
     return {k: np.array(v) for k, v in out.items()}
 
 

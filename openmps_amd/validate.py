@@ -38,14 +38,18 @@ def _computer(cfg, preset: str, tol: float):
     return c
 
 
-def _step(c, cfg, steps: int, label: str, every: int = 100):
+def _step(c, cfg, steps: int, label: str, every: int = 20):
+    """`steps` ForwardTime() calls: dt from the CFL rule (Computer.hpp:759-777)
+    capped by the scene's maximum step."""
     t0 = time.perf_counter()
     iters = 0
     for k in range(steps):
-        c.forward_time(cfg["dt"])
-        iters += c.solve_stats()["iterations"]
+        c.forward_time()
+        s = c.solve_stats()
+        iters += s["iterations"]
         if (k + 1) % every == 0 or k + 1 == steps:
-            _log(f"  [{label}] step {k + 1}/{steps}, {time.perf_counter() - t0:.1f} s")
+            _log(f"  [{label}] step {k + 1}/{steps}, {time.perf_counter() - t0:.1f} s, "
+                 f"{s['iterations']} CG iterations, |b-Ax|/|b| {s['rel_residual']:.1e}")
     return iters / max(steps, 1)
 
 
@@ -79,7 +83,7 @@ def run_hydrostatic(scene: str, steps: int, preset: str, tol: float) -> dict:
     its = _step(c, cfg, steps, preset)
     out = hydrostatic_error(c.particles(), cfg)
     out.update({"check": "hydrostatic", "scene": scene, "steps": steps, "solver": preset,
-                "mean_cg_iterations": its})
+                "time_s": c.time()[0], "mean_cg_iterations": its})
     return out
 
 
@@ -91,13 +95,14 @@ def run_compare(scene: str, steps: int, presets: list[str], tol: float) -> dict:
         c = _computer(cfg, pr, tol)
         its = _step(c, cfg, steps, pr)
         P = c.particles()
-        states.append((pr, P, its))
+        states.append((pr, P, its, c.time()[0]))
         del c
-    (a, Pa, ia), (b, Pb, ib) = states[0], states[1]
+    (a, Pa, ia, _), (b, Pb, ib, _) = states[0], states[1]
     fl = Pb["type"] == 0
     dp = float(np.linalg.norm(Pa["p"][fl] - Pb["p"][fl]) / np.linalg.norm(Pb["p"][fl]))
     dx = np.sqrt(((Pa["x"][fl] - Pb["x"][fl]) ** 2 + (Pa["z"][fl] - Pb["z"][fl]) ** 2).mean())
     return {"check": "compare", "scene": scene, "steps": steps, "solvers": presets,
+            "time_s": states[1][3],
             "pressure_rel_l2": dp, "position_rms_in_l0": float(dx / cfg["env"]["l0"]),
             "mean_cg_iterations": {a: ia, b: ib}}
 
