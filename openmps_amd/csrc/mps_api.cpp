@@ -141,10 +141,12 @@ void free_all(Context& c) {
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
+  dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
   if (c.red_host) (void)hipHostFree(c.red_host);
+  if (c.ml_active_host) (void)hipHostFree(c.ml_active_host);
   if (c.st_host) (void)hipHostFree(c.st_host);
   if (c.st32_host) (void)hipHostFree(c.st32_host);
   if (c.ir_host) (void)hipHostFree(c.ir_host);
@@ -336,6 +338,7 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     mps::pinned(c.nbr_total_host, 1);
     mps::pinned(c.ppe_counters_host, 2);
     mps::pinned(c.red_host, 2);
+    mps::pinned(c.ml_active_host, 1);
     mps::pinned(c.st_host, 1);
     mps::pinned(c.st32_host, 1);
     mps::pinned(c.ir_host, 1);

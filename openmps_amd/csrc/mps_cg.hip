@@ -592,71 +592,139 @@ __global__ void __launch_bounds__(NTP) k_ml_setup(CellGrid G, const float* __res
   }
 }
 
-// Each inner iteration `it` is four kernels:
-//   k_ml_spmv(it):  beta = rho_(it-1) / rho_(it-2); p = z + beta p,
-//                   q = A z + beta q (A p by linearity; z is the only gathered
-//                   vector); partial sums of p.q;
-//   k_ml_alpha:     alpha = rho_(it-1) / p.q (one block);
-//   k_ml_prec(it):  r -= alpha q, d += alpha p; z = M r; partial sums of
-//                   rho_it = r.z and of r.r;
-//   k_ml_rho(it):   rho_it, r.r and the stop test (one block).
-// (A grid-wide sum finished by the last block to arrive would save the two
-// one-block launches, but its device-scope release costs an L2 write-back per
-// block on this multi-XCD part, far more than a launch.)  k_ml_prec visits the particles cell by cell,
-// one wavefront per 4x4 block of cells (every particle of the system that is
-// in a cell exactly once; rows outside every cell are identity rows whose r,
-// d, z stay 0), so the residual update, the aggregate sums and z share one
-// pass.  Iteration 0 of k_ml_prec starts the sweep: r = r64 / |r64|, d = 0.
+// Each inner iteration `it` >= 1 is two kernels; every block reduces the
+// partial sums of the kernel before it itself, in one fixed order, so all
+// blocks derive the same scalars without a one-block launch in between:
+//   k_ml_spmv(it):  rho_(it-1) and r.r from the partials of k_ml_prec(it-1)
+//                   and the stop test (block 0 records them); d += alpha_(it-1)
+//                   p_(it-1); unless stopped, beta = rho_(it-1) / rho_(it-2),
+//                   p = z + beta p, q = A z + beta q (A p by linearity; z is
+//                   the only gathered vector), partial sums of p.q;
+//   k_ml_prec(it):  alpha = rho_(it-1) / p.q from the partials of
+//                   k_ml_spmv(it) (block 0 records it); r -= alpha q; z = M r;
+//                   partial sums of rho_it = r.z and of r.r.
+// Iteration 0 is k_ml_prec alone: r = r64 / |r64|, d = 0, z = M r.  The stop
+// at iteration k is found by k_ml_spmv(k + 1), which still adds the last
+// alpha_k p_k to d; a sweep cut short (iteration limit, stagnation) ends with
+// k_ml_finish instead.  (A grid-wide sum finished by the last block to arrive
+// would need a device-scope release per block, an L2 write-back on this
+// multi-XCD part, far more than the redundant sums.)  k_ml_prec visits the
+// particles cell by cell, one wavefront per 4x4 block of cells (every
+// particle of the system that is in a cell exactly once; rows outside every
+// cell are identity rows whose r, d, z stay 0), so the residual update, the
+// aggregate sums and z share one pass.
 constexpr int kPerLane = 7;  // particles of a cell held in registers per lane (4 lanes per cell)
+constexpr int kPqPerThread = kRedBlocks / NTP;  // p.q partials loaded per thread of k_ml_prec
 
-__global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __restrict__ inv,
+// Block-uniform: the inner solve stopped at an iteration below k.
+__device__ __forceinline__ bool stopped_before(const CgStatus* st, long long k) {
+  __shared__ long long s_mark;
+  if (threadIdx.x == 0) s_mark = st->stop_mark;
+  __syncthreads();
+  return s_mark != 0 && s_mark - 1 < k;
+}
+
+// 1 for every 4x4 block of cells that holds a particle (one thread per block).
+__global__ void __launch_bounds__(256) k_ml_active(CellGrid G, uint32_t* __restrict__ flag) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b >= G.bnx * G.bnz) return;
+  const uint32_t kx = b / G.bnz, kz = b % G.bnz;
+  uint32_t any = 0;
+  for (uint32_t u = 0; u < 16; u++) {
+    const uint32_t cx = 4 * kx + (u >> 2), cz = 4 * kz + (u & 3);
+    if (cx < G.gnx && cz < G.gnz) any |= G.count[cx * G.gnz + cz];
+  }
+  flag[b] = any != 0u;
+}
+
+// The flagged blocks in ascending order (off: exclusive scan of the flags).
+__global__ void __launch_bounds__(256) k_ml_compact(uint32_t nblk, const uint32_t* __restrict__ flag,
+                                                    const uint32_t* __restrict__ off,
+                                                    uint32_t* __restrict__ list) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b < nblk && flag[b]) list[off[b]] = b;
+}
+
+// One wavefront per listed 4x4 block of cells.
+__global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __restrict__ list, uint32_t nact,
+                                                 const float* __restrict__ inv,
                                                  const double* __restrict__ r64, double inv_norm,
                                                  float* __restrict__ r, float* __restrict__ d,
-                                                 const float* __restrict__ p, const float* __restrict__ q,
-                                                 const float* __restrict__ dinv, float* __restrict__ z,
-                                                 double* __restrict__ part, size_t part_stride, CgStatus* st,
-                                                 long long it) {
+                                                 const float* __restrict__ q, const float* __restrict__ dinv,
+                                                 float* __restrict__ z, const double* __restrict__ part_pq,
+                                                 int npq, double* __restrict__ part, size_t part_stride,
+                                                 CgStatus* st, long long it) {
   __shared__ double lds[NTP / 64];
-  if (already_converged(st)) return;
-  const float alpha = (it > 0) ? static_cast<float>(st->alpha_ml) : 0.0f;
-  const uint32_t b = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
-  double rho = 0, rr = 0;
-  if (b < G.bnx * G.bnz) {  // wave-uniform
-    const BlockLane L = block_lane(G, b);
-    const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+  if (stopped_before(st, it)) return;
+  // the p.q partials of k_ml_spmv(it) are loaded first and summed once the
+  // particle loads below are in flight (npq <= kRedBlocks)
+  double pq_part[kPqPerThread];
+#pragma unroll
+  for (int u = 0; u < kPqPerThread; u++) {
+    const int k = threadIdx.x + u * NTP;
+    pq_part[u] = (it > 0 && k < npq) ? part_pq[k] : 0.0;
+  }
+  const uint32_t w = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  const bool active = w < nact;  // wave-uniform
+  BlockLane L{false, 0, 0, 0, 0};
+  uint32_t cnt = 0, st0 = 0;
+  float i0 = 0.0f, i1 = 0.0f, i2 = 0.0f;
+  uint32_t id[kPerLane];
+  float rv[kPerLane], qv[kPerLane], dv[kPerLane];
+  bool ag[kPerLane];
+  if (active) {
+    L = block_lane(G, list[w]);
+    cnt = L.valid ? G.count[L.cell] : 0u;
+    st0 = L.valid ? G.start[L.cell] : 0u;
     // aggregate scales: independent of the particles, loaded with them
-    const float i0 = L.valid ? inv[L.cell] : 0.0f, i1 = L.valid ? inv[G.off1 + L.a1] : 0.0f,
-                i2 = L.valid ? inv[G.off2 + L.a2] : 0.0f;
-    uint32_t id[kPerLane];
-    float rv[kPerLane], dv[kPerLane];
-    bool ag[kPerLane];
+    i0 = L.valid ? inv[L.cell] : 0.0f;
+    i1 = L.valid ? inv[G.off1 + L.a1] : 0.0f;
+    i2 = L.valid ? inv[G.off2 + L.a2] : 0.0f;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
       const uint32_t k = L.sub + 4u * u;
       id[u] = (k < cnt) ? G.sorted[st0 + k] : 0u;
     }
-    float s = 0.0f;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
-      float ri = 0.0f;
+      rv[u] = qv[u] = dv[u] = 0.0f;
       ag[u] = false;
-      dv[u] = 0.0f;
       if (L.sub + 4u * u < cnt) {
         const uint32_t i = id[u];
         dv[u] = dinv[i];
-        if (it == 0) {
-          ri = static_cast<float>(r64[i] * inv_norm);
-          d[i] = 0.0f;
-        } else {
-          ri = fmaf(-alpha, q[i], r[i]);
-          d[i] = fmaf(alpha, p[i], d[i]);
-        }
-        r[i] = ri;
         ag[u] = in_agg(G.type[i]);
+        if (it == 0) {
+          rv[u] = static_cast<float>(r64[i] * inv_norm);
+        } else {
+          rv[u] = r[i];
+          qv[u] = q[i];
+        }
+      }
+    }
+  }
+  float alpha = 0.0f;
+  if (it > 0) {
+    double s = 0;
+#pragma unroll
+    for (int u = 0; u < kPqPerThread; u++) s += pq_part[u];
+    const double a = st->gamma[(it - 1) & 1] / block_sum<NTP>(s, lds);
+    if (lead()) st->alpha_ml = a;
+    alpha = static_cast<float>(a);
+  }
+  double rho = 0, rr = 0;
+  if (active) {
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kPerLane; u++) {
+      if (L.sub + 4u * u < cnt) {
+        const uint32_t i = id[u];
+        const float ri = (it == 0) ? rv[u] : fmaf(-alpha, qv[u], rv[u]);
+        rv[u] = ri;
+        r[i] = ri;
+        if (it == 0) d[i] = 0.0f;
         if (ag[u]) s += ri;
         rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
       }
-      rv[u] = ri;
     }
     // cells beyond 4 * kPerLane particles (larger cell capacities)
     for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
@@ -667,7 +735,6 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
         d[i] = 0.0f;
       } else {
         ri = fmaf(-alpha, q[i], r[i]);
-        d[i] = fmaf(alpha, p[i], d[i]);
       }
       r[i] = ri;
       if (in_agg(G.type[i])) s += ri;
@@ -701,61 +768,106 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const float* __rest
   }
 }
 
-// rho_it and r_it.r_it from the partials of k_ml_prec(it); stop test on r.r
-// relative to the sweep's r_0.r_0 (one block).
-__global__ void __launch_bounds__(1024) k_ml_rho(const double* __restrict__ part, size_t part_stride,
-                                                 int nparts, CgStatus* st, double tol_rel, long long it) {
-  __shared__ double lds[1024 / 64];
-  if (already_converged(st)) return;
-  const double rho = reduce_partials<1024>(part, nparts, lds);
-  const double rr = reduce_partials<1024>(part + part_stride, nparts, lds);
-  if (threadIdx.x == 0) {
-    const int par = static_cast<int>(it & 1);
-    st->gamma[par] = rho;
-    st->rr[par] = rr;
-    bool conv;
-    if (it == 0) {
-      st->tol = rr * tol_rel * tol_rel;
-      st->rr0 = rr;
-      conv = (st->tol == 0);
-    } else {
-      conv = rr < st->tol;
-    }
-    st->iterations = it;
-    if (conv) st->converged = 1;
-  }
-}
+// See above; FIRST is it == 1 (p = z, q = A z, tolerance from r_0.r_0).  The
+// partials of k_ml_prec(it - 1) are loaded at the start and summed after the
+// gathers of the first kSpmvRows rows per thread.
+constexpr int kSpmvParts = 3;  // prec partials per thread and array in registers
+constexpr int kSpmvRows = 2;   // rows per thread whose A z waits in registers for beta
 
-// alpha = rho_(it-1) / p.q (one block).
-__global__ void __launch_bounds__(1024) k_ml_alpha(const double* __restrict__ part, int nparts, CgStatus* st,
-                                                   long long it) {
-  __shared__ double lds[1024 / 64];
-  if (already_converged(st)) return;
-  const double pq = reduce_partials<1024>(part, nparts, lds);
-  if (threadIdx.x == 0) st->alpha_ml = st->gamma[(it - 1) & 1] / pq;
-}
-
-// See above; FIRST is it == 1 (p = z, q = A z).
 template <bool FIRST, class Op>
 __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict__ z, float* __restrict__ p,
-                                                   float* __restrict__ q, double* __restrict__ part_pq,
-                                                   CgStatus* st, long long it) {
-  __shared__ double lds[NT / 64];
-  if (already_converged(st)) return;
-  const float beta = FIRST ? 0.0f : static_cast<float>(st->gamma[(it - 1) & 1] / st->gamma[it & 1]);
+                                                   float* __restrict__ q, float* __restrict__ d,
+                                                   const double* __restrict__ part_ml, size_t part_stride,
+                                                   int nparts, double* __restrict__ part_pq, CgStatus* st,
+                                                   double tol_rel, long long it) {
+  __shared__ double lds[2 * (NT / 64)];
+  // a stop at it - 2 or earlier was found (and finished) by an earlier launch
+  if (stopped_before(st, it - 1)) return;
+  double pa[kSpmvParts], pb[kSpmvParts];
+#pragma unroll
+  for (int u = 0; u < kSpmvParts; u++) {
+    const int k = threadIdx.x + u * NT;
+    pa[u] = (k < nparts) ? part_ml[k] : 0.0;
+    pb[u] = (k < nparts) ? part_ml[part_stride + k] : 0.0;
+  }
   const Rows rw = block_rows(A.n, A.rpb);
+  float zc[kSpmvRows], sc[kSpmvRows];
+#pragma unroll
+  for (int u = 0; u < kSpmvRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    zc[u] = sc[u] = 0.0f;
+    if (i < rw.hi) {
+      zc[u] = z[i];
+      sc[u] = op_apply<kIterUnroll + 1>(A, i, zc[u], [&](uint32_t j) { return z[j]; });
+    }
+  }
+  double a = 0, b = 0;
+#pragma unroll
+  for (int u = 0; u < kSpmvParts; u++) {
+    a += pa[u];
+    b += pb[u];
+  }
+  for (int k = threadIdx.x + kSpmvParts * NT; k < nparts; k += NT) {
+    a += part_ml[k];
+    b += part_ml[part_stride + k];
+  }
+  double rho, rr;
+  block_sum2<NT>(a, b, lds, rho, rr);
+  const double tol = FIRST ? rr * tol_rel * tol_rel : st->tol;
+  const bool stop = FIRST ? (tol == 0) : (rr < tol);
+  if (lead()) {
+    const int par = static_cast<int>((it - 1) & 1);
+    st->gamma[par] = rho;
+    st->rr[par] = rr;
+    if (FIRST) {
+      st->tol = tol;
+      st->rr0 = rr;
+    }
+    if (stop) {
+      st->iterations = it - 1;
+      st->converged = 1;
+      st->stop_mark = it;
+    }
+  }
+  const float alpha = FIRST ? 0.0f : static_cast<float>(st->alpha_ml);
+  if (stop) {
+    if (!FIRST)
+      for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) d[i] = fmaf(alpha, p[i], d[i]);
+    return;
+  }
+  const float beta = FIRST ? 0.0f : static_cast<float>(rho / st->gamma[it & 1]);
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const float zi = z[i];
-    const float s = op_apply<kIterUnroll + 1>(A, i, zi, [&](uint32_t j) { return z[j]; });
-    const float pi = FIRST ? zi : fmaf(beta, p[i], zi);
-    const float qi = FIRST ? s : fmaf(beta, q[i], s);
+  auto finish = [&](size_t i, float zi, float s) {
+    float pi = zi, qi = s;
+    if (!FIRST) {
+      const float po = p[i];
+      d[i] = fmaf(alpha, po, d[i]);
+      pi = fmaf(beta, po, zi);
+      qi = fmaf(beta, q[i], s);
+    }
     p[i] = pi;
     q[i] = qi;
     acc = fma(static_cast<double>(pi), static_cast<double>(qi), acc);
+  };
+#pragma unroll
+  for (int u = 0; u < kSpmvRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    if (i < rw.hi) finish(i, zc[u], sc[u]);
+  }
+  for (size_t i = rw.lo + threadIdx.x + static_cast<size_t>(kSpmvRows) * NT; i < rw.hi; i += NT) {
+    const float zi = z[i];
+    finish(i, zi, op_apply<kIterUnroll + 1>(A, i, zi, [&](uint32_t j) { return z[j]; }));
   }
   const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_pq[blockIdx.x] = sp;
+}
+
+// d += alpha p after a sweep that ended without a stop (its last k_ml_prec
+// computed alpha, no k_ml_spmv followed).
+__global__ void __launch_bounds__(256) k_ml_finish(size_t n, const CgStatus* st, const float* __restrict__ p,
+                                                   float* __restrict__ d) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) d[i] = fmaf(static_cast<float>(st->alpha_ml), p[i], d[i]);
 }
 
 // x += |r| d (FP64 accumulation of the FP32 correction).
@@ -1042,7 +1154,7 @@ void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, 
   });
 }
 
-// Multilevel preconditioner: four kernels per iteration (see k_ml_prec).
+// Multilevel preconditioner: two kernels per iteration (see k_ml_prec).
 void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
@@ -1053,29 +1165,47 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
   const size_t off1 = static_cast<size_t>(gnx) * gnz, off2 = off1 + static_cast<size_t>(g1x) * g1z;
   const size_t total_aggs = off2 + static_cast<size_t>(bnx) * bnz;
   const size_t nblk = static_cast<size_t>(bnx) * bnz;
-  const size_t pgrid = (nblk + NTP / 64 - 1) / (NTP / 64);  // one wavefront per 4x4 block of cells
+  const size_t sgrid = (nblk + NTP / 64 - 1) / (NTP / 64);  // one wavefront per 4x4 block of cells
   if (c.agg_cap < total_aggs) {
     dfree(c.agg_inv);
     c.agg_inv = dalloc<float>(total_aggs);
     c.agg_cap = total_aggs;
   }
-  if (c.part_ml_cap < pgrid) {
+  if (c.part_ml_cap < sgrid) {
     dfree(c.part_ml);
-    c.part_ml = dalloc<double>(2 * pgrid);
-    c.part_ml_cap = pgrid;
+    c.part_ml = dalloc<double>(2 * sgrid);
+    c.part_ml_cap = sgrid;
   }
+  if (c.ml_blk_cap < nblk) {
+    dfree(c.ml_flag);
+    dfree(c.ml_off);
+    dfree(c.ml_list);
+    c.ml_flag = dalloc<uint32_t>(nblk);
+    c.ml_off = dalloc<uint32_t>(nblk);
+    c.ml_list = dalloc<uint32_t>(nblk);
+    c.ml_blk_cap = nblk;
+  }
+  if (!c.ml_active) c.ml_active = dalloc<uint64_t>(1);
   const size_t stride = c.part_ml_cap;
   const CellGrid G{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz,
                    off1, off2};
-  // the aggregate sums of A from the FP64 system (row sums in z, r, d as scratch)
-  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 12.0 + n * 37.0 + total_aggs * 4.0, [&] {
+  // the aggregate sums of A from the FP64 system (row sums in z, r, d as
+  // scratch), and the list of the 4x4 cell blocks that hold particles (its
+  // length reaches the host with the first residual below)
+  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 12.0 + n * 37.0 + total_aggs * 4.0 + nblk * 12.0, [&] {
     hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
                        static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
                        static_cast<const uint8_t*>(c.type), gnz, c.z32, c.r32, c.d32);
-    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(sgrid)), dim3(NTP), 0, c.stream, G,
                        static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
                        static_cast<const float*>(c.d32), c.agg_inv);
+    hipLaunchKernelGGL(k_ml_active, dim3(grid_for(nblk)), dim3(256), 0, c.stream, G, c.ml_flag);
+    scan_counts(c, c.ml_flag, nblk, c.ml_off, c.ml_active);
+    hipLaunchKernelGGL(k_ml_compact, dim3(grid_for(nblk)), dim3(256), 0, c.stream, static_cast<uint32_t>(nblk),
+                       static_cast<const uint32_t*>(c.ml_flag), static_cast<const uint32_t*>(c.ml_off),
+                       c.ml_list);
   });
+  MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
   // rows outside every cell keep r = d = z = 0
   MPS_HIP(hipMemsetAsync(c.z32, 0, n * sizeof(float), c.stream));
   MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
@@ -1083,35 +1213,45 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
-  // per particle: id, r (read, written), q, p, d (read, written), type, M, z; cell tables
-  const double b_prec = n * 37.0 + c.ncells * 8.0 + total_aggs * 4.0;
-  auto prec = [&](double norm, double sweep_tol, long long it) {
+  // per particle: id, r (read, written), q, type, M, z; cell tables
+  const double b_prec = n * 25.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  uint32_t nact = 0;
+  size_t pgrid = 1;
+  auto prec = [&](double norm, long long it) {
     hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
-                       static_cast<const float*>(c.agg_inv), static_cast<const double*>(c.r), 1.0 / norm,
-                       c.r32, c.d32, static_cast<const float*>(c.pa32), static_cast<const float*>(c.q32),
-                       static_cast<const float*>(c.dinv32), c.z32, c.part_ml, stride, c.st32, it);
-    hipLaunchKernelGGL(k_ml_rho, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.part_ml),
-                       stride, static_cast<int>(pgrid), c.st32, sweep_tol, it);
+                       static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
+                       static_cast<const double*>(c.r), 1.0 / norm, c.r32, c.d32,
+                       static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), c.z32,
+                       static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
   };
   refine(c, sr, [&](double norm, double sweep_tol) {
+    // refine() has synchronised the stream (first residual): the block count is on the host
+    nact = static_cast<uint32_t>(*c.ml_active_host);
+    pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
-    tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, sweep_tol, 0); });
+    tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, 0); });
     long long launched = 0;
     const long long inner = run_inner(c, limit, [&](long long it) {
       tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
         if (it == 1)
           hipLaunchKernelGGL((k_ml_spmv<true, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(c.z32), c.pa32, c.q32, p_pq, c.st32, it);
+                             static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+                             static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
+                             c.st32, sweep_tol, it);
         else
           hipLaunchKernelGGL((k_ml_spmv<false, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(c.z32), c.pa32, c.q32, p_pq, c.st32, it);
-        hipLaunchKernelGGL(k_ml_alpha, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(p_pq),
-                           L.grid, c.st32, it);
+                             static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+                             static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
+                             c.st32, sweep_tol, it);
       }, it);
-      tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, sweep_tol, it); }, it);
+      tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, it); }, it);
       launched = it;
     });
-    // launches after convergence did no work: count them out
+    if (!c.st32_host->converged && launched > 0)
+      hipLaunchKernelGGL(k_ml_finish, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                         static_cast<const CgStatus*>(c.st32), static_cast<const float*>(c.pa32), c.d32);
+    // launches after the stop did no work: count them out (k_ml_spmv(inner + 1)
+    // only finished d)
     uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
     uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
     return inner;
@@ -1143,7 +1283,7 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const double rows = n * (4 + 4 + 8.0);
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
   if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
-    solve_mixed_ml(c, A, cols + rows + n * (4.0 + 16), sr, max_iter);  // z; p, q read and written
+    solve_mixed_ml(c, A, cols + rows + n * (4.0 + 24), sr, max_iter);  // z; p, q, d read and written
   else
     solve_mixed_cg(c, A, cols + rows + n * vec, sr, max_iter);
 }

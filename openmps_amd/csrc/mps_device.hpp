@@ -70,4 +70,26 @@ __device__ __forceinline__ double reduce_partials(const double* part, int count,
   return block_sum<NT>(s, lds);
 }
 
+// Block sums of two values per thread at once (lds: 2 * NT / 64 entries),
+// in the order of block_sum.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double a, double b, double* lds, double& sa, double& sb) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[wv] = a;
+    lds[NT / 64 + wv] = b;
+  }
+  __syncthreads();
+  sa = lds[0];
+  sb = lds[NT / 64];
+#pragma unroll
+  for (int k = 1; k < NT / 64; k++) {
+    sa += lds[k];
+    sb += lds[NT / 64 + k];
+  }
+  __syncthreads();
+}
+
 }  // namespace mps

@@ -35,6 +35,7 @@ struct CgStatus {
   int converged;      // 1 once the stop rule holds
   int pad;
   long long iterations;
+  long long stop_mark;  // multilevel inner solver: 0 running, k + 1 once it stopped at iteration k
 };
 
 // Mixed-precision refinement scalars.
@@ -189,6 +190,12 @@ struct Context {
   float* agg_inv = nullptr;  // 1 / (1^T A_kk 1) of the cells and 2x2, 4x4 cell blocks (0: no correction)
   double* part_ml = nullptr;  // partial sums of the per-cell-block kernel (2 x part_ml_cap)
   size_t part_ml_cap = 0;
+  uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
+  uint32_t* ml_off = nullptr;
+  uint32_t* ml_list = nullptr;  // ... and their indices, ascending
+  size_t ml_blk_cap = 0;
+  uint64_t* ml_active = nullptr;       // count of such blocks
+  uint64_t* ml_active_host = nullptr;  // pinned
   size_t agg_cap = 0;
   size_t agg_off[4] = {0, 0, 0, 0};  // level l at agg_inv + agg_off[l]
   float* dinv32 = nullptr;
@@ -306,6 +313,8 @@ void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
 void ppe_make_fp32(Context& c);
 void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
+// exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
+void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);
 void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,
                     const double* val, const double* b, const double* x0);
 void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uint32_t>& col,

@@ -243,6 +243,10 @@ static void scan_u32(Context& c, const uint32_t* in, size_t n, TO* out, uint64_t
                      static_cast<const uint64_t*>(c.scan_tmp), out, mult);
 }
 
+void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total) {
+  scan_u32<uint32_t>(c, in, n, out, 1, total);
+}
+
 void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* offsets) {
   scan_u32<uint64_t>(c, widths, nslices, offsets, 64, offsets + nslices);
 }

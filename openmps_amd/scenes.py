@@ -86,6 +86,44 @@ def still_water(width: float, depth: float, l0: float, wall_h: float | None = No
     return dam_break(width, wall_h, width, depth, l0, jitter, seed)
 
 
+def static_pressure(width: int = 50, height: int = 100, l0: float = 1e-3) -> tuple[dict, dict]:
+    """The reference's static-pressure case (Benchmark/StaticPressure/generate.py:
+    56-201): width x height fluid particles at rest, a Wall layer with three
+    Dummy layers below the floor and beside both walls up to row `height`, and
+    the Dummy corner blocks.  Unlike the dam-break tanks the wall tops are
+    ordinary wall rows (Wall inside, Dummy outside)."""
+    xs, zs, ts = [], [], []
+
+    def add(i, j, t):
+        xs.append(i * l0)
+        zs.append(j * l0)
+        ts.append(t)
+
+    for i in range(width):
+        for j in range(height):
+            add(i, j, FLUID)
+    for i in range(-1, width + 1):  # floor
+        add(i, -1, WALL)
+        for j in (-2, -3, -4):
+            add(i, j, DUMMY)
+    for j in range(height + 1):  # left wall
+        add(-1, j, WALL)
+        for i in (-2, -3, -4):
+            add(i, j, DUMMY)
+    for j in range(height + 1):  # right wall
+        add(width, j, WALL)
+        for i in (width + 1, width + 2, width + 3):
+            add(i, j, DUMMY)
+    for i in range(1, 4):  # corners
+        for j in range(-4, 0):
+            add(-1 - i, j, DUMMY)
+            add(width + i, j, DUMMY)
+    parts = _pack(xs, zs, ts)
+    # domain: the particles' bounding box (generate.py:190-195)
+    env = {"min_x": -4 * l0, "min_z": -4 * l0, "max_x": (width + 3) * l0, "max_z": height * l0}
+    return parts, env
+
+
 def lattice(nx: int, nz: int, l0: float, u=None, w=None, p=None) -> dict:
     """nx * nz fluid particles on a square lattice, index i + nx * j (the
     layout of the reference unit tests, e.g. test_ComputerNumberDensity.cpp:82-99)."""
@@ -115,6 +153,8 @@ def config(name: str, seed: int = 1) -> dict:
     dambreak_1m: 2.0 x 1.2 m tank, 0.5 x 1.0 m column, l0 = 0.7 mm, jitter
                  +/-0.02 l0, dt = 1e-4.
     still_1m:    0.5 m wide tank filled 1.0 m deep at l0 = 0.7 mm, dt = 2e-5.
+    static_pressure: the reference's static-pressure case, 50 x 100 fluid
+                 particles at l0 = 1 mm, r_e = 2.4 l0, steps up to 5e-6 s.
     Each accepts a suffix ``_ref`` for the reference's single radius
     (r_lap = r_e = 2.1 l0).
     """
@@ -135,6 +175,14 @@ def config(name: str, seed: int = 1) -> dict:
         # 1e-4 s lets the 1 m column's start-up compression grow unstable
         l0, dt, eps, jit = 0.0007, 2e-5, 1e-8, 0.0
         parts, dom = still_water(0.5, 1.0, l0)
+    elif base == "static_pressure":
+        # Benchmark/StaticPressure/generate.py:8-31: l0 = 1 mm, r_e = 2.4 l0 (one
+        # radius), nu = 1.004e-6, maximum step outputInterval / minStepCountPerOutput
+        # = 0.0005 / 100 (Main.cpp:234), eps = 1e-10
+        parts, dom = static_pressure()
+        env = dict(max_dt=5e-6, courant=COURANT, g=G, rho=RHO, nu=1.004e-6, r_e_by_l0=2.4,
+                   l0=1e-3, r_lap_by_l0=2.4, **dom)
+        return {"name": name, "particles": parts, "env": env, "dt": 5e-6, "eps": 1e-10}
     elif base.startswith("dambreak_l0_"):
         l0 = float(base[len("dambreak_l0_"):])
         dt, eps, jit = 1e-4, 1e-10, 0.02
