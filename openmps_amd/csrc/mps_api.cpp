@@ -336,7 +336,7 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.ir = mps::dalloc<mps::IrStatus>(1);
     mps::pinned(c.flags_host, 4);
     mps::pinned(c.nbr_total_host, 1);
-    mps::pinned(c.ppe_counters_host, 2);
+    mps::pinned(c.ppe_counters_host, 3);
     mps::pinned(c.red_host, 2);
     mps::pinned(c.ml_active_host, 1);
     mps::pinned(c.st_host, 1);

@@ -914,9 +914,13 @@ void read_status(Context& c, CgStatus* dev, CgStatus* host) {
   collect_events(c, host->converged ? host->iterations : 0x7fffffffffffffffLL);
 }
 
+// Iterations to enqueue before the host next reads the status: the predicted
+// count (the same solve's count in the previous step) plus a margin, then
+// short chunks.  Launches past the stop exit at once (a few microseconds);
+// every status read costs a host round trip with the GPU idle.
 long long next_chunk(long long done, long long predicted) {
-  if (done == 0) return std::max<long long>(16, predicted > 0 ? predicted * 15 / 16 : 32);
-  return std::max<long long>(16, std::min<long long>(256, done / 4));
+  if (done == 0) return predicted > 0 ? predicted + 2 : 32;
+  return std::max<long long>(8, std::min<long long>(256, done / 4));
 }
 
 // Algorithmic HBM bytes of one SpMV launch: every stored non-zero (value +
@@ -1069,8 +1073,9 @@ void refine(Context& c, const StopRule& sr, Sweep sweep, Correct correct) {
     // this sweep reduces the residual by inner_tol, or only as far as the
     // outer target needs (with a factor 2 of margin), whichever is less work
     const double sweep_tol = std::max(inner_tol, 0.5 * std::sqrt(target / rr));
+    c.sweep_index = outer;
     const long long inner = sweep(norm, sweep_tol);
-    if (outer == 0) c.last_inner = inner;
+    c.sweep_pred[std::min(outer, Context::kSweepPred - 1)] = inner;
     total += inner;
     correct(norm);
     check_launch();
@@ -1098,7 +1103,7 @@ long long run_inner(Context& c, long long limit, Launch1 launch) {
   double rr_last = -1;
   int flat = 0;
   while (true) {
-    const long long chunk = next_chunk(it - 1, c.last_inner);
+    const long long chunk = next_chunk(it - 1, c.sweep_pred[std::min(c.sweep_index, Context::kSweepPred - 1)]);
     for (long long m = 0; m < chunk && it <= limit; m++, it++) launch(it);
     check_launch();
     read_status(c, c.st32, c.st32_host);
@@ -1114,8 +1119,10 @@ long long run_inner(Context& c, long long limit, Launch1 launch) {
 }
 
 // Jacobi (or no) preconditioner: the one-kernel Chronopoulos-Gear iteration.
+//   b_base: bytes of one iteration without the wide slices' extra column
+//   bytes (wide: their stored nnz, known after the first synchronisation)
 template <class Op>
-void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, long long max_iter) {
+void solve_mixed_cg(Context& c, const Op& A, double b_base, bool wide, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
   const bool jac = c.solver.precond != MPS_PRECOND_NONE;
@@ -1126,7 +1133,10 @@ void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, 
   const long long limit = inner_limit(c, max_iter);
   float4* gbuf[2] = {c.g32a, c.g32b};
   double* parts[2] = {c.part3a, c.part3b};
+  double b_iter = b_base;
   refine(c, sr, [&](double norm, double sweep_tol) {
+    // refine() has synchronised the stream (residual) before every sweep
+    if (wide) b_iter = b_base + 2.0 * mf_nnz_wide(c);
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
     tracked(c, MPS_K_CG_INIT32, b_iter + n * 8.0, [&] {
       hipLaunchKernelGGL(k_cgf_init<Op>, dim3(L.grid), dim3(NT), 0, c.stream, A,
@@ -1155,7 +1165,7 @@ void solve_mixed_cg(Context& c, const Op& A, double b_iter, const StopRule& sr, 
 }
 
 // Multilevel preconditioner: two kernels per iteration (see k_ml_prec).
-void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule& sr, long long max_iter) {
+void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
   hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
@@ -1192,10 +1202,16 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
   // the aggregate sums of A from the FP64 system (row sums in z, r, d as
   // scratch), and the list of the 4x4 cell blocks that hold particles (its
   // length reaches the host with the first residual below)
-  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 12.0 + n * 37.0 + total_aggs * 4.0 + nblk * 12.0, [&] {
-    hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
-                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
-                       static_cast<const uint8_t*>(c.type), gnz, c.z32, c.r32, c.d32);
+  // (the fill of this system already summed the rows unless the solver changed since)
+  const bool sums = c.ml_rowsums_ready;
+  c.ml_rowsums_ready = false;
+  tracked(c, MPS_K_ML_SETUP, (sums ? 0.0 : c.nnz_offdiag * 12.0 + n * 17.0) + n * 20.0 + total_aggs * 4.0 +
+                                 nblk * 12.0, [&] {
+    if (!sums)
+      hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream,
+                         sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const uint32_t*>(c.row_len),
+                         static_cast<const uint32_t*>(c.key), static_cast<const uint8_t*>(c.type), gnz, c.z32,
+                         c.r32, c.d32);
     hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(sgrid)), dim3(NTP), 0, c.stream, G,
                        static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
                        static_cast<const float*>(c.d32), c.agg_inv);
@@ -1224,9 +1240,12 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_spmv, const StopRule&
                        static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), c.z32,
                        static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
   };
+  double b_spmv = b_base;
   refine(c, sr, [&](double norm, double sweep_tol) {
-    // refine() has synchronised the stream (first residual): the block count is on the host
+    // refine() has synchronised the stream (residual) before every sweep: the
+    // block count and the wide nnz are on the host
     nact = static_cast<uint32_t>(*c.ml_active_host);
+    b_spmv = b_base + 2.0 * mf_nnz_wide(c);
     pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
     tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, 0); });
@@ -1269,7 +1288,7 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const double vec = 16.0 * 2 + 4.0 * 4;
   if (!c.ppe_from_particles) {
     ppe_make_fp32(c);
-    solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), sr, max_iter);
+    solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), false, sr, max_iter);
     return;
   }
   ppe_compress_columns(c);
@@ -1277,15 +1296,15 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
                   static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
                                      c.qpos_inv_h),
                   n, L.rpb};
-  // int16 columns of narrow slices, u32 of wide ones; row length, ghost and
-  // position of every row; slice tables
-  const double cols = (c.nnz_offdiag - c.mf_nnz_wide) * 2.0 + c.mf_nnz_wide * 4.0 + c.nslices * 25.0;
+  // int16 columns of narrow slices, u32 of wide ones (2 more bytes each,
+  // added by the solvers); row length, ghost and position of every row; slice tables
+  const double cols = c.nnz_offdiag * 2.0 + c.nslices * 25.0;
   const double rows = n * (4 + 4 + 8.0);
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
   if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
     solve_mixed_ml(c, A, cols + rows + n * (4.0 + 24), sr, max_iter);  // z; p, q, d read and written
   else
-    solve_mixed_cg(c, A, cols + rows + n * vec, sr, max_iter);
+    solve_mixed_cg(c, A, cols + rows + n * vec, true, sr, max_iter);
 }
 
 }  // namespace

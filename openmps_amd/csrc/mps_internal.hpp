@@ -170,8 +170,9 @@ struct Context {
   size_t col16_cap = 0;
   bool have_col16 = false;
   double mf_nnz_wide = 0;
+  bool mf_wide_pending = false;  // mf_nnz_wide still in ppe_counters_host[2]
   unsigned long long* ppe_counters = nullptr;       // [0] off-diagonal nnz
-  unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots
+  unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots, wide nnz
   double nnz_offdiag = 0, sell_slots = 0;
 
   // CG work vectors
@@ -183,13 +184,17 @@ struct Context {
   CgStatus* st_host = nullptr;
   CgStatus* st32 = nullptr;
   CgStatus* st32_host = nullptr;
-  long long last_inner = 0;
+  // iterations of the mixed solver's FP32 sweeps in the last solve, by sweep
+  static constexpr int kSweepPred = 8;
+  long long sweep_pred[kSweepPred] = {};
+  int sweep_index = 0;
   // mixed-precision work vectors
   float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
   float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr;  // multilevel-preconditioned inner CG
   float* agg_inv = nullptr;  // 1 / (1^T A_kk 1) of the cells and 2x2, 4x4 cell blocks (0: no correction)
   double* part_ml = nullptr;  // partial sums of the per-cell-block kernel (2 x part_ml_cap)
   size_t part_ml_cap = 0;
+  bool ml_rowsums_ready = false;  // the last fill left the aggregate row sums in z32, r32, d32
   uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
   uint32_t* ml_off = nullptr;
   uint32_t* ml_list = nullptr;  // ... and their indices, ascending
@@ -313,6 +318,7 @@ void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
 void ppe_make_fp32(Context& c);
 void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
+double mf_nnz_wide(Context& c);         // its wide nnz; valid once the stream was synchronised
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);
 void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,

@@ -122,6 +122,17 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
 // the solver's gathers of a wavefront touch a few cache lines, not 64.
 constexpr int kRuns = 9;
 
+// Optional output of the fill for the multilevel preconditioner: per row i
+// (0 for dummy / disabled rows and rows outside the grid), for the cell, the
+// 2x2 and the 4x4 block of cells of i (cells of the neighbour search, `key`):
+// S_l(i) = A_ii + sum of the stored A_ij with j in the same aggregate, summed
+// in stored (ascending column) order.  s0 == nullptr: not wanted.
+struct AggSums {
+  const uint32_t* __restrict__ key;
+  uint32_t gnz;
+  float *s0, *s1, *s2;
+};
+
 __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      const uint8_t* __restrict__ type, NV nb,
@@ -130,7 +141,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
                                                      const uint32_t* __restrict__ slice_width,
                                                      double* __restrict__ val, uint32_t* __restrict__ col,
                                                      double* __restrict__ diag,
-                                                     float* __restrict__ ghost32) {
+                                                     float* __restrict__ ghost32, AggSums ag) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -140,6 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   const uint8_t ti = type[i];
   uint32_t kk = 0;
   double d, g = 0.0;
+  double a0 = 0, a1 = 0, a2 = 0;  // aggregate row sums (AggSums)
   if (ti == kDummy || ti == kDisabled) {
     d = 1.0;
     g = 1.0;
@@ -185,15 +197,32 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
       g = -a;
     }
     d = s;
+    const uint32_t ki = ag.s0 ? ag.key[i] : kNoCell;
+    const uint32_t cx = ki / ag.gnz, cz = ki % ag.gnz;
+    if (ki != kNoCell) a0 = a1 = a2 = d;
     auto emit = [&](uint32_t j) {
       const double dx = x[j] - xi, dz = z[j] - zi;
       const double r2 = dx * dx + dz * dz;
       if (r2 < k.rl2 && type[j] != kDummy) {
         const double r = sqrt(r2);
         const size_t o = base + static_cast<size_t>(kk) * 64;
-        val[o] = 3 * k.rl / k.n0l / (r * r * r);
+        const double a = 3 * k.rl / k.n0l / (r * r * r);
+        val[o] = a;
         col[o] = j;
         kk++;
+        if (ki != kNoCell) {
+          const uint32_t kj = ag.key[j];
+          if (kj != kNoCell) {
+            const uint32_t jx = kj / ag.gnz, jz = kj % ag.gnz;
+            if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
+              a2 += a;
+              if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
+                a1 += a;
+                if (jx == cx && jz == cz) a0 += a;
+              }
+            }
+          }
+        }
       }
     };
     if (merge) {
@@ -245,6 +274,11 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   }
   diag[i] = d;
   ghost32[i] = static_cast<float>(g);
+  if (ag.s0) {
+    ag.s0[i] = static_cast<float>(a0);
+    ag.s1[i] = static_cast<float>(a1);
+    ag.s2[i] = static_cast<float>(a2);
+  }
 }
 
 // Per slice: whether every stored column offset j - i fits in int16; the
@@ -351,14 +385,19 @@ void ppe_assemble(Context& c) {
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.nnz_offdiag = c.ppe_counters_host[0];
   c.sell_slots = c.ppe_counters_host[1];
-  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 12.0;
+  // row sums of the multilevel preconditioner's aggregates, when that solver will run
+  const bool want_sums = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_MULTILEVEL;
+  const AggSums ag{c.key, static_cast<uint32_t>(c.gnz), want_sums ? c.z32 : nullptr, c.r32, c.d32};
+  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 12.0 +
+                       (want_sums ? n * 12.0 + c.nnz_offdiag * 4.0 : 0.0);
   tracked(c, MPS_K_PPE_FILL, bytes, [&] {
     hipLaunchKernelGGL(k_ppe_fill, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                        static_cast<const uint64_t*>(c.slice_off),
-                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.ghost32);
+                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.ghost32, ag);
   });
   check_launch();
+  c.ml_rowsums_ready = want_sums;
   c.ppe_from_particles = true;
   c.ppe_search_serial = c.search_serial;
   c.have_col16 = false;
@@ -387,11 +426,20 @@ void ppe_compress_columns(Context& c) {
                        static_cast<const uint64_t*>(c.off16), c.col16);
   });
   check_launch();
-  unsigned long long wide = 0;
-  MPS_HIP(hipMemcpyAsync(&wide, c.ppe_counters + 1, sizeof(wide), hipMemcpyDeviceToHost, c.stream));
-  MPS_HIP(hipStreamSynchronize(c.stream));
-  c.mf_nnz_wide = static_cast<double>(wide);
+  // the wide nnz (byte accounting only) reaches the host with the solver's
+  // next synchronisation (mf_nnz_wide)
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 2, c.ppe_counters + 1, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
+  c.mf_wide_pending = true;
   c.have_col16 = true;
+}
+
+double mf_nnz_wide(Context& c) {
+  if (c.mf_wide_pending) {
+    c.mf_nnz_wide = static_cast<double>(c.ppe_counters_host[2]);
+    c.mf_wide_pending = false;
+  }
+  return c.mf_nnz_wide;
 }
 
 // FP32 copies of an already assembled system (the mixed solver on a system
