@@ -25,6 +25,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s @1M; solve ms/step learned+mixed vs plain FP64 @res 1e-8"
 HBM_PEAK_GBS = 8000.0
+REPLAYS = 50
+# HBM bytes per launch of the dambreak_1m SpMVs measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (one run each; FETCH_SIZE scaled by 2, WRITE_SIZE
+# by 1: what the counters read for a copy kernel of known size), recorded with
+# their commands in profiles/dambreak_1m_step_kernels.md
+TRAFFIC = {"cg_spmv32": 109.5e6, "cg_spmv": 389.0e6}
 
 
 def log(*a):
@@ -137,6 +143,13 @@ def main() -> int:
     # the same number of further steps with events on
     elapsed, st = run(main_c, args.steps, f"{args.solver} timed")
     kt = profiled_table(main_c, args.steps, args.solver)
+    # dominant kernel: the SpMV of the solver that ran the timed steps
+    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32"}[
+        solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
+    replay = {}
+    if dom in ("cg_spmv32", "cg_spmv"):
+        replay[dom] = main_c.replay_spmv(dom, REPLAYS)
+        log(f"  {dom}: {replay[dom] * 1e3:.1f} us per launch ({REPLAYS} back-to-back launches)")
     solve_ms = sum(s["solve_ms"] for s in st) / len(st)
     iters = sum(s["iterations"] for s in st) / len(st)
 
@@ -165,21 +178,29 @@ def main() -> int:
         _, stv = run(ref_c, args.steps, f"{v} timed")
         if v == "fp64":
             kt64 = profiled_table(ref_c, args.steps, v)
+            replay["cg_spmv"] = ref_c.replay_spmv("cg_spmv", REPLAYS)
+            log(f"  cg_spmv: {replay['cg_spmv'] * 1e3:.1f} us per launch ({REPLAYS} back-to-back launches)")
         solve[v] = summary(stv)
         del ref_c
 
     def roof(table, name):
+        """Achieved GB/s of a kernel: its algorithmic bytes per launch over the
+        mean launch duration of REPLAYS back-to-back launches between two HIP
+        events (per-launch events inside the steps add the dispatch of every
+        launch; that figure is kept as ms_per_launch_in_steps)."""
         d = table.get(name)
         if not d:
             return None
-        return {"kernel": name, "bound": "hbm", "achieved": round(d["GBps"], 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": d["bytes_per_launch"],
-                "ms_per_launch": d["ms_per_launch"]}
+        ms = replay.get(name, d["ms_per_launch"])
+        gbs = d["bytes_per_launch"] / ms / 1e6
+        return {"kernel": name, "bound": "hbm", "achieved": round(gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "traffic": TRAFFIC.get(name) if args.workload == "dambreak_1m" else None,
+                "bytes_per_launch": d["bytes_per_launch"],
+                "ms_per_launch": ms, "ms_per_launch_in_steps": d["ms_per_launch"],
+                "timing": f"mean of {REPLAYS} back-to-back launches on the solver's operands"
+                          if name in replay else "HIP events around each launch in the steps"}
 
-    # dominant kernel: the SpMV of the solver that ran the timed steps
-    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32"}[
-        solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
     roofline = roof(kt, dom)
     roofline_fp64 = roof(kt64, "cg_spmv")
 

@@ -254,6 +254,14 @@ typedef struct mps_kernel_stat {
 int mps_set_profiling(mps_ctx* ctx, int on);
 int mps_get_kernel_stat(mps_ctx* ctx, int id, mps_kernel_stat* out);
 int mps_reset_kernel_stats(mps_ctx* ctx);
+/* Launches the last solve's SpMV kernel again `reps` times back to back on
+ * the context stream, between two HIP events, with the operands that solve
+ * left (id MPS_K_CG_SPMV32: the multilevel inner CG's, MPS_K_CG_SPMV: the
+ * FP64 CG's), and returns the mean duration of one launch.  A measurement
+ * hook with no counterpart in the reference: the search direction and its
+ * product are overwritten (the next solve rebuilds them); the solution is
+ * not touched.  MPS_ERR_STATE if the last solve did not run that kernel. */
+int mps_replay_spmv(mps_ctx* ctx, int id, int reps, double* ms_per_launch);
 
 /* Copies `bytes` bytes between two device buffers with a plain streaming
  * kernel and returns its duration: the known-size reference for scaling HBM

@@ -111,6 +111,7 @@ SIGNATURES = {
     "mps_set_profiling": (_I, [_P, _I]),
     "mps_get_kernel_stat": (_I, [_P, _I, ctypes.POINTER(KernelStat)]),
     "mps_reset_kernel_stats": (_I, [_P]),
+    "mps_replay_spmv": (_I, [_P, _I, _I, ctypes.POINTER(_D)]),
     "mps_device_math_check": (_I, [_SZ, _P, _P, _P, _P]),
     "mps_device_copy_probe": (_I, [_SZ, ctypes.POINTER(_D)]),
     "mps_model_create": (_I, [_I, _I, ctypes.POINTER(_P)]),

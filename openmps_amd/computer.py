@@ -359,6 +359,14 @@ class Computer:
     def reset_kernel_stats(self) -> None:
         _raise(L.lib().mps_reset_kernel_stats(self._h), self._h)
 
+    def replay_spmv(self, kernel: str, reps: int = 50) -> float:
+        """Mean ms of one launch of the last solve's SpMV ("cg_spmv32" or
+        "cg_spmv"), re-launched `reps` times back to back between two HIP
+        events on the context stream (mps_replay_spmv)."""
+        out = ctypes.c_double()
+        _raise(L.lib().mps_replay_spmv(self._h, L.KERNELS.index(kernel), int(reps), ctypes.byref(out)), self._h)
+        return out.value
+
     def kernel_stats(self) -> dict:
         out = {}
         for k in range(len(L.KERNELS)):

@@ -141,7 +141,7 @@ void free_all(Context& c) {
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
-  dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active);
+  dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active); dfree(c.st_replay);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
@@ -716,6 +716,11 @@ int mps_reset_kernel_stats(mps_ctx* ctx) {
     mps::sync(c);
     for (auto& s : c.kstat) s = mps::KStat{};
   });
+}
+
+int mps_replay_spmv(mps_ctx* ctx, int id, int reps, double* ms_per_launch) {
+  if (!ms_per_launch || reps <= 0 || (id != MPS_K_CG_SPMV32 && id != MPS_K_CG_SPMV)) return MPS_ERR_BAD_ARG;
+  return mps::guard(ctx, [&](Context& c) { *ms_per_launch = mps::replay_spmv(c, id, reps); });
 }
 
 // ---- learned warm start ----------------------------------------------------------

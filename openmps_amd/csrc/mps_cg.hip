@@ -937,6 +937,7 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
   const Launch L = cg_grid(n);
   const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
   MPS_HIP(hipMemsetAsync(c.st, 0, sizeof(CgStatus), c.stream));
+  c.replay_ready = MPS_K_CG_SPMV;
   tracked(c, MPS_K_CG_INIT, spmv_bytes(c, 8, 8.0 + 8 + 8), [&] {
     hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
                        static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
@@ -1241,6 +1242,7 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
                        static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
   };
   double b_spmv = b_base;
+  c.replay_ready = MPS_K_CG_SPMV32;
   refine(c, sr, [&](double norm, double sweep_tol) {
     // refine() has synchronised the stream (residual) before every sweep: the
     // block count and the wide nnz are on the host
@@ -1280,6 +1282,13 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
   });
 }
 
+MatFree matfree(const Context& c, size_t rpb) {
+  return MatFree{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
+                 static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
+                                    c.qpos_inv_h),
+                 c.ppe_n, rpb};
+}
+
 void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
@@ -1292,10 +1301,7 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     return;
   }
   ppe_compress_columns(c);
-  const MatFree A{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
-                  static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
-                                     c.qpos_inv_h),
-                  n, L.rpb};
+  const MatFree A = matfree(c, L.rpb);
   // int16 columns of narrow slices, u32 of wide ones (2 more bytes each,
   // added by the solvers); row length, ghost and position of every row; slice tables
   const double cols = c.nnz_offdiag * 2.0 + c.nslices * 25.0;
@@ -1309,10 +1315,48 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
 
 }  // namespace
 
+// Re-launches the last solve's SpMV on its operands with a status block that
+// never stops (tolerance -1, unit scalars, alpha 0: the correction d is left
+// as it is); only p, q and the partial sums are overwritten.
+double replay_spmv(Context& c, int kid, int reps) {
+  if (c.replay_ready != kid || c.ppe_n == 0)
+    throw Failure{MPS_ERR_STATE, "the last solve did not run this SpMV"};
+  if (!c.st_replay) c.st_replay = dalloc<CgStatus>(1);
+  CgStatus s{};
+  s.rr[0] = s.rr[1] = 1.0;
+  s.gamma[0] = s.gamma[1] = 1.0;
+  s.tol = -1.0;
+  MPS_HIP(hipMemcpyAsync(c.st_replay, &s, sizeof(s), hipMemcpyHostToDevice, c.stream));
+  const Launch L = cg_grid(c.ppe_n);
+  const size_t pgrid = std::max<size_t>(1, (*c.ml_active_host + NTP / 64 - 1) / (NTP / 64));
+  auto launch = [&] {
+    if (kid == MPS_K_CG_SPMV32)
+      hipLaunchKernelGGL((k_ml_spmv<false, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, matfree(c, L.rpb),
+                         static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+                         static_cast<const double*>(c.part_ml), c.part_ml_cap, static_cast<int>(pgrid),
+                         c.part3b, c.st_replay, 1e-8, 2LL);
+    else
+      hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(NT), 0, c.stream,
+                         sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const double*>(c.r), c.pa, c.q,
+                         static_cast<const double*>(c.partB), static_cast<const double*>(c.partC), L.grid,
+                         c.partA, c.st_replay, StopRule{MPS_STOP_RHS_NORM, 1e-8}, 2LL);
+  };
+  launch();  // warm
+  MPS_HIP(hipEventRecord(c.solve_a, c.stream));
+  for (int k = 0; k < reps; k++) launch();
+  MPS_HIP(hipEventRecord(c.solve_b, c.stream));
+  check_launch();
+  MPS_HIP(hipEventSynchronize(c.solve_b));
+  float ms = 0;
+  MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
+  return ms / reps;
+}
+
 void ppe_solve(Context& c) {
   if (!c.have_ppe) throw Failure{MPS_ERR_STATE, "no pressure Poisson system assembled"};
   const size_t n = c.ppe_n;
   c.stats = mps_solve_stats{};
+  c.replay_ready = 0;
   if (n == 0) return;
   StopRule sr;
   sr.rule = c.solver.stop_rule;

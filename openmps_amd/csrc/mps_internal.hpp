@@ -194,6 +194,8 @@ struct Context {
   float* agg_inv = nullptr;  // 1 / (1^T A_kk 1) of the cells and 2x2, 4x4 cell blocks (0: no correction)
   double* part_ml = nullptr;  // partial sums of the per-cell-block kernel (2 x part_ml_cap)
   size_t part_ml_cap = 0;
+  int replay_ready = 0;  // SpMV the last solve ran (MPS_K_CG_SPMV32 / MPS_K_CG_SPMV), 0: none
+  CgStatus* st_replay = nullptr;
   bool ml_rowsums_ready = false;  // the last fill left the aggregate row sums in z32, r32, d32
   uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
   uint32_t* ml_off = nullptr;
@@ -294,6 +296,17 @@ void tracked(Context& c, int kid, double bytes, F&& launch, long long tag = 0) {
   launch();
   MPS_HIP(hipEventRecord(e.b, c.stream));
 }
+// Adds bytes to the last launch of a kernel once they are known on the host
+// (sizes counted by the kernel itself).
+inline void add_bytes(Context& c, int kid, double bytes) {
+  if (c.profiling)
+    for (size_t k = c.events_used; k-- > 0;)
+      if (c.events[k].kid == kid) {
+        c.events[k].bytes += bytes;
+        return;
+      }
+  c.kstat[kid].bytes += bytes;
+}
 // Folds the recorded event pairs into the kernel statistics (stream must be
 // idle); pairs tagged above max_tag are discarded.
 void collect_events(Context& c, long long max_tag = 0x7fffffffffffffffLL);
@@ -319,6 +332,7 @@ void ppe_assemble(Context& c);           // mps_ppe.hip
 void ppe_make_fp32(Context& c);
 void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
 double mf_nnz_wide(Context& c);         // its wide nnz; valid once the stream was synchronised
+double replay_spmv(Context& c, int kid, int reps);  // mps_cg.hip: mean ms of one launch
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);
 void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t* col,

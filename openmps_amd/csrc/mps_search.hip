@@ -282,7 +282,9 @@ void search(Context& c) {
                        c.cap, static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), c.sorted);
   });
-  tracked(c, MPS_K_NEIGHBORS, 25.0 * n, [&] {
+  // x, z, type of every particle, the cell tables, the sorted ids and the
+  // list lengths; the list itself (4 bytes a neighbour) is added below
+  tracked(c, MPS_K_NEIGHBORS, 25.0 * n + 8.0 * c.ncells, [&] {
     hipLaunchKernelGGL(k_neighbors, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.sorted),
@@ -294,6 +296,7 @@ void search(Context& c) {
                          hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.nbr_total = *c.nbr_total_host;
+  add_bytes(c, MPS_K_NEIGHBORS, 4.0 * c.nbr_total);
   c.have_neighbors = true;
   if (c.flags_host[0] || c.flags_host[1])
     throw Failure{MPS_ERR_CELL_OVERFLOW, "Too many particle in a block"};

@@ -176,3 +176,32 @@ def test_mixed_installed_systems(name):
     c.stage("ppe_solve")
     x = c.aux("ppe_x")
     assert np.allclose(x, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("precond", ["multilevel", "fp64"])
+def test_spmv_replay_leaves_the_step_unchanged(precond):
+    """mps_replay_spmv (the bench's launch-duration measurement) re-runs the
+    last solve's SpMV without touching the solution: a run with a replay
+    between two ForwardTime steps matches one without, bit for bit."""
+    cfg = scenes.config("dambreak_125k")
+    runs = []
+    for replay in (False, True):
+        g = cases.make("gpu", cfg["env"], cfg["eps"])
+        if precond == "fp64":
+            g.set_solver(kind=L.SOLVER_CG_FP64, stop_rule=L.STOP_RHS_NORM, tol=TOL)
+            kern = "cg_spmv"
+        else:
+            g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_MULTILEVEL,
+                         tol=TOL)
+            kern = "cg_spmv32"
+        g.add_particles(cfg["particles"])
+        g.forward_time(cfg["dt"])
+        if replay:
+            ms = g.replay_spmv(kern, 5)
+            say(f"  {kern}: {ms * 1e3:.1f} us per launch")
+            assert 0 < ms < 10
+        g.forward_time(cfg["dt"])
+        runs.append(g.particles())
+        del g
+    for k in ("x", "z", "u", "w", "p"):
+        assert np.array_equal(runs[0][k], runs[1][k]), k
