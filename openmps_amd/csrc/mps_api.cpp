@@ -191,8 +191,8 @@ void forward(Context& c, double dt) {
   search(c);
   density(c, /*ecs=*/true, /*savex=*/false, /*dens=*/true);
   explicit_forces(c);
-  density(c, /*ecs=*/false, /*savex=*/true, /*dens=*/true);
-  ppe_assemble(c);
+  // second density pass + SaveX + PPE assembly (its count pass fused with the density pass)
+  density_save_and_assemble(c);
   ppe_solve(c);
   pressure_writeback(c);
   pressure_gradient(c);

@@ -1086,6 +1086,9 @@ void refine(Context& c, const StopRule& sr, Sweep sweep, Correct correct) {
   c.stats.rr0 = rr0;
   c.stats.rr = rr;
   c.last_iters = total;
+  // rr, bb: the FP64 residual of the returned x, computed above
+  c.final_rr = rr;
+  c.final_bb = bb;
   if (!done)
     throw Failure{MPS_ERR_CG_NOT_CONVERGED,
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
@@ -1357,6 +1360,7 @@ void ppe_solve(Context& c) {
   const size_t n = c.ppe_n;
   c.stats = mps_solve_stats{};
   c.replay_ready = 0;
+  c.final_rr = -1;
   if (n == 0) return;
   StopRule sr;
   sr.rule = c.solver.stop_rule;
@@ -1382,9 +1386,10 @@ void ppe_solve(Context& c) {
   float ms = 0;
   MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
   c.stats.solve_ms = ms;
-  // true FP64 residual of the returned x (reported, not part of the solve)
-  double rr = 0, bb = 0;
-  residual64(c, rr, bb, false);
+  // true FP64 residual of the returned x (reported, not part of the solve;
+  // the refinement's last residual is that one already)
+  double rr = c.final_rr, bb = c.final_bb;
+  if (rr < 0) residual64(c, rr, bb, false);
   c.stats.rel_residual = (bb > 0) ? std::sqrt(rr / bb) : std::sqrt(rr);
   c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
   if (rc != MPS_OK) throw Failure{rc, msg};

@@ -73,6 +73,7 @@ struct Model {
   float* m1[6] = {};
   float* m2[6] = {};
   __bf16 *w1b = nullptr, *w2b = nullptr;
+  __bf16* w2p = nullptr;  // w2 in the K order of the fused inference kernel
   float* loss = nullptr;
   long long adam_t = 0;
   size_t cap = 0;  // batch work buffers
@@ -188,6 +189,7 @@ struct Context {
   static constexpr int kSweepPred = 8;
   long long sweep_pred[kSweepPred] = {};
   int sweep_index = 0;
+  double final_rr = -1, final_bb = 0;  // FP64 |b - A x|^2, |b|^2 of the solve's x (-1: not known)
   // mixed-precision work vectors
   float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
   float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr;  // multilevel-preconditioned inner CG
@@ -329,6 +331,9 @@ void pressure_writeback(Context& c);
 void pressure_gradient(Context& c);
 void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
+// density(c, false, true, true) followed by ppe_assemble(c), the density walk
+// and the PPE count walk fused into one kernel (same results)
+void density_save_and_assemble(Context& c);
 void ppe_make_fp32(Context& c);
 void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
 double mf_nnz_wide(Context& c);         // its wide nnz; valid once the stream was synchronised

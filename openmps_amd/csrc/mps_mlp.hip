@@ -287,23 +287,33 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     f[1] = static_cast<float>((a.nd[i] - a.n0) / a.n0);
     f[2] = static_cast<float>(a.p[i] * a.inv_sp);
     f[3] = static_cast<float>(a.r0[i] * a.inv_sr);
-    const double pi = xi, qi = zi;
-    for (int k = 0; k < kNbr; k++) {
-      __bf16 v[4] = {__bf16(0.0f), __bf16(0.0f), __bf16(0.0f), __bf16(0.0f)};
-      if (k < m) {
-        const uint32_t j = sidx[k][t];
-        v[0] = __bf16(static_cast<float>((a.x[j] - pi) * a.inv_l0));
-        v[1] = __bf16(static_cast<float>((a.z[j] - qi) * a.inv_l0));
-        v[2] = __bf16(static_cast<float>(a.p[j] * a.inv_sp));
-        v[3] = __bf16(static_cast<float>(a.r0[j] * a.inv_sr));
-      }
-      for (int e = 0; e < 4; e++) row[4 + 4 * k + e] = v[e];
-    }
-  } else {
-    for (int k = 4; k < kFeat; k++) row[k] = __bf16(0.0f);
   }
-  for (int e = 0; e < 4; e++) row[e] = __bf16(f[e]);
-  for (int k = kFeat; k < kFeatPad; k++) row[k] = __bf16(0.0f);
+  // the row as 16-byte stores: elements 0-3 the particle's own features,
+  // 4 + 4k .. 4 + 4k + 3 neighbour k's (k < m, else 0), zeros from kFeat on
+  const double xi = a.x[i], zi = a.z[i];
+  for (int q = 0; q < kFeatPad / 8; q++) {
+    bf16x8 v;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const int e0 = 8 * q + 4 * half;
+      float u[4] = {0.f, 0.f, 0.f, 0.f};
+      if (e0 == 0) {
+        u[0] = f[0];
+        u[1] = f[1];
+        u[2] = f[2];
+        u[3] = f[3];
+      } else if (e0 < kFeat && (e0 - 4) / 4 < m) {
+        const uint32_t j = sidx[(e0 - 4) / 4][t];
+        u[0] = static_cast<float>((a.x[j] - xi) * a.inv_l0);
+        u[1] = static_cast<float>((a.z[j] - zi) * a.inv_l0);
+        u[2] = static_cast<float>(a.p[j] * a.inv_sp);
+        u[3] = static_cast<float>(a.r0[j] * a.inv_sr);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[4 * half + e] = __bf16(u[e]);
+    }
+    *reinterpret_cast<bf16x8*>(row + 8 * q) = v;
+  }
 }
 
 // sum over alive rows of v^2 and the alive count (block partials).
@@ -403,6 +413,114 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
   a.dt[slot] = static_cast<float>((di - sum / cnt) * a.inv_sd);
 }
 
+
+// ---- fused inference ---------------------------------------------------------
+// y = w3 . gelu(W2 gelu(W1 x + b1) + b2) + b3 in one kernel, nothing but x
+// read and y written: one wavefront per 16 rows, both layers transposed
+// (hidden units as the MFMA M dimension, rows as N), so layer 1's output
+// tiles already sit in the lanes that feed them to layer 2 as its B operand.
+// v_mfma_f32_16x16x32_bf16 lane layout: A row l%16 / B column l%16 with K
+// elements 8(l/16)..8(l/16)+7; C[4(l/16)+v][l%16], v = 0..3.  Layer 1 lane
+// l ends with hidden units 16t + 4(l/16) + v of row l%16, so layer 2's K
+// step ks (hidden 32ks..32ks+31) takes tiles 2ks and 2ks+1: its K slot
+// 8g + e stands for hidden unit w2_hidden(ks, g, e) below, and W2 is stored
+// in that order (k_perm_w2) so that each lane's A operand is one 16-byte load.
+constexpr int kRowGroups = 1;  // groups of 16 rows per wavefront (2 needs ~500 registers)
+constexpr int kRowsPerWave = 16 * kRowGroups;
+
+__host__ __device__ __forceinline__ int w2_hidden(int ks, int g, int e) {
+  return 32 * ks + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
+
+// w2p[o][32 ks + 8 g + e] = bf16(w2[o][w2_hidden(ks, g, e)]).
+__global__ void __launch_bounds__(256) k_perm_w2(const float* __restrict__ w2, __bf16* __restrict__ w2p) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kHid * kHid) return;
+  const int o = i / kHid, k = i % kHid;
+  const int ks = k / 32, g = (k % 32) / 8, e = k % 8;
+  w2p[i] = static_cast<__bf16>(w2[o * kHid + w2_hidden(ks, g, e)]);
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(size_t M, const __bf16* __restrict__ x,
+                                                   const __bf16* __restrict__ w1, const float* __restrict__ b1,
+                                                   const __bf16* __restrict__ w2p, const float* __restrict__ b2,
+                                                   const float* __restrict__ w3, const float* __restrict__ b3,
+                                                   float* __restrict__ y) {
+  const size_t wave = blockIdx.x * static_cast<size_t>(blockDim.x / 64) + (threadIdx.x >> 6);
+  const size_t r0 = wave * kRowsPerWave;
+  if (r0 >= M) return;  // wave-uniform
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  constexpr int K1 = kFeatPad / 32, K2 = kHid / 32;
+  bf16x8 xf[kRowGroups][K1];
+#pragma unroll
+  for (int sg = 0; sg < kRowGroups; sg++) {
+    size_t r = r0 + 16 * sg + c;
+    r = (r < M) ? r : M - 1;
+#pragma unroll
+    for (int ks = 0; ks < K1; ks++)
+      xf[sg][ks] = *reinterpret_cast<const bf16x8*>(x + r * kFeatPad + 32 * ks + 8 * g);
+  }
+  bf16x8 h1[kRowGroups][K2];
+#pragma unroll
+  for (int t2 = 0; t2 < K2; t2++) {
+    // keeps the next pair's weight loads from being hoisted (register pressure)
+    asm volatile("" ::: "memory");
+    f32x4 acc[2][kRowGroups];
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const int h0 = 16 * (2 * t2 + half);
+#pragma unroll
+      for (int sg = 0; sg < kRowGroups; sg++) acc[half][sg] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < K1; ks++) {
+        const bf16x8 af = *reinterpret_cast<const bf16x8*>(w1 + (h0 + c) * kFeatPad + 32 * ks + 8 * g);
+#pragma unroll
+        for (int sg = 0; sg < kRowGroups; sg++)
+          acc[half][sg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, xf[sg][ks], acc[half][sg], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int half = 0; half < 2; half++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const float bias = b1[16 * (2 * t2 + half) + 4 * g + v];
+#pragma unroll
+        for (int sg = 0; sg < kRowGroups; sg++) h1[sg][t2][4 * half + v] = static_cast<__bf16>(gelu_f(acc[half][sg][v] + bias));
+      }
+  }
+  float part[kRowGroups];
+#pragma unroll
+  for (int sg = 0; sg < kRowGroups; sg++) part[sg] = 0.f;
+  for (int ot = 0; ot < kHid / 16; ot++) {
+    asm volatile("" ::: "memory");
+    const int o0 = 16 * ot;
+    f32x4 acc2[kRowGroups];
+#pragma unroll
+    for (int sg = 0; sg < kRowGroups; sg++) acc2[sg] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K2; ks++) {
+      const bf16x8 af = *reinterpret_cast<const bf16x8*>(w2p + (o0 + c) * kHid + 32 * ks + 8 * g);
+#pragma unroll
+      for (int sg = 0; sg < kRowGroups; sg++)
+        acc2[sg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, h1[sg][ks], acc2[sg], 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int o = o0 + 4 * g + v;
+      const float bias = b2[o], wo = w3[o];
+#pragma unroll
+      for (int sg = 0; sg < kRowGroups; sg++) part[sg] = fmaf(wo, gelu_f(acc2[sg][v] + bias), part[sg]);
+    }
+  }
+#pragma unroll
+  for (int sg = 0; sg < kRowGroups; sg++) {
+    part[sg] += __shfl_xor(part[sg], 16, 64);
+    part[sg] += __shfl_xor(part[sg], 32, 64);
+    const size_t r = r0 + 16 * sg + c;
+    if (g == 0 && r < M) y[r] = part[sg] + b3[0];
+  }
+}
+
 unsigned blocks(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 void gemm(hipStream_t s, int M, int N, int K, const __bf16* A, long long sam, long long sak,
@@ -458,6 +576,7 @@ void Model::init(int seed) {
   }
   w1b = dalloc<__bf16>(param_count(0));
   w2b = dalloc<__bf16>(param_count(2));
+  w2p = dalloc<__bf16>(param_count(2));
   loss = dalloc<float>(1);
   // U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases (seeded)
   std::mt19937_64 rng(static_cast<uint64_t>(seed));
@@ -480,6 +599,8 @@ void Model::refresh_bf16() {
                      static_cast<const float*>(w[0]), w1b);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks(param_count(2))), dim3(256), 0, stream, param_count(2),
                      static_cast<const float*>(w[2]), w2b);
+  hipLaunchKernelGGL(k_perm_w2, dim3(blocks(param_count(2))), dim3(256), 0, stream,
+                     static_cast<const float*>(w[2]), w2p);
   check_launch();
   MPS_HIP(hipStreamSynchronize(stream));
 }
@@ -488,7 +609,7 @@ void Model::release() {
   for (int k = 0; k < 6; k++) {
     dfree(w[k]); dfree(g[k]); dfree(m1[k]); dfree(m2[k]);
   }
-  dfree(w1b); dfree(w2b); dfree(loss);
+  dfree(w1b); dfree(w2b); dfree(w2p); dfree(loss);
   dfree(xb); dfree(tb); dfree(z1); dfree(z2); dfree(a2); dfree(h1b); dfree(y); dfree(dy);
   dfree(da2b); dfree(dh1); dfree(da1b); dfree(idx);
   if (stream) (void)hipStreamDestroy(stream);
@@ -498,6 +619,16 @@ void Model::release() {
 // Forward pass of m rows already in xb (BF16, 160 wide) -> y; with `train`
 // the pre-activations stay in z1 (+b1) and a2 for the backward pass.
 void Model::forward(hipStream_t s, size_t m, const __bf16* x, float* out, bool train) {
+  if (m == 0) return;
+  if (!train) {  // inference: the fused kernel, no intermediate in memory
+    const size_t waves = (m + kRowsPerWave - 1) / kRowsPerWave;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, s, m, x,
+                       static_cast<const __bf16*>(w1b), static_cast<const float*>(w[1]),
+                       static_cast<const __bf16*>(w2p), static_cast<const float*>(w[3]),
+                       static_cast<const float*>(w[4]), static_cast<const float*>(w[5]), out);
+    check_launch();
+    return;
+  }
   ensure_batch(m);
   const int M = static_cast<int>(m);
   gemm(s, M, kHid, kFeatPad, x, kFeatPad, 1, w1b, 1, kFeatPad, z1, kHid, 1, 1);
@@ -542,6 +673,8 @@ void Model::adam(hipStream_t s, float lr) {
                      static_cast<const float*>(w[0]), w1b);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks(param_count(2))), dim3(256), 0, s, param_count(2),
                      static_cast<const float*>(w[2]), w2b);
+  hipLaunchKernelGGL(k_perm_w2, dim3(blocks(param_count(2))), dim3(256), 0, s, static_cast<const float*>(w[2]),
+                     w2p);
   check_launch();
 }
 

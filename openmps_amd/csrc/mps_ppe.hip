@@ -109,6 +109,75 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
   }
 }
 
+// The second ComputeNeighborDensities pass with SaveX (Computer.hpp:780-833,
+// :1025-1039) and k_ppe_count in one walk of the neighbour list: nothing
+// moves between the two stages, so the RHS reads the positions, velocities
+// and nWithoutSpp the density pass has in hand.  Every value is computed as
+// the two separate kernels compute it (same expressions, same list order).
+__global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const double* __restrict__ x,
+                                                              const double* __restrict__ z,
+                                                              const double* __restrict__ u,
+                                                              const double* __restrict__ w,
+                                                              const double* __restrict__ p,
+                                                              const uint8_t* __restrict__ type, NV nb,
+                                                              PpeConsts k, double* __restrict__ nd,
+                                                              double* __restrict__ nws,
+                                                              const double* __restrict__ ecs,
+                                                              double* __restrict__ ox, double* __restrict__ oz,
+                                                              double* __restrict__ b, double* __restrict__ sol,
+                                                              uint32_t* __restrict__ row_len,
+                                                              uint32_t* __restrict__ slice_width,
+                                                              uint2* __restrict__ qpos,
+                                                              unsigned long long* nnz_total) {
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  uint32_t cnt = 0;
+  if (i < n) {
+    const double xi = x[i], zi = z[i];
+    qpos[i] = make_uint2(static_cast<uint32_t>(static_cast<long long>(rint((xi - k.qx0) * k.qinv))),
+                         static_cast<uint32_t>(static_cast<long long>(rint((zi - k.qz0) * k.qinv))));
+    const uint8_t ti = type[i];
+    if (ti == kDummy || ti == kDisabled) {
+      nws[i] = k.n0;
+      b[i] = 0;
+      sol[i] = 0;
+    } else {
+      ox[i] = xi;
+      oz[i] = zi;
+      const double ui = u[i], wi = w[i];
+      double s = 0.0, h = 0.0;
+      const uint32_t m = nb.count[i];
+      for (uint32_t q = 0; q < m; q++) {
+        const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
+        const double dx = x[j] - xi, dz = z[j] - zi;
+        const double r2 = dx * dx + dz * dz;
+        if (r2 < k.re2) {
+          const double r = sqrt(r2);
+          s += weight(r, k.re);
+          const double du = u[j] - ui, dw = w[j] - wi;
+          h += (dx * du + dz * dw) / (r * r * r);
+        }
+        if (r2 < k.rl2 && type[j] != kDummy) cnt++;
+      }
+      nws[i] = s;
+      nd[i] = (s < k.n0) ? k.n0 : s;
+      const double speed = (s < k.n0) ? 0.0 : -k.re * h;  // HS, :838-872
+      b[i] = -k.rho / (k.n0 * k.dt) * (speed + ecs[i]);
+      sol[i] = (k.warm == MPS_WARM_ZERO) ? 0.0 : p[i];
+    }
+    row_len[i] = cnt;
+  }
+  const unsigned wmax = wave_max_u32(cnt);
+  const unsigned long long wsum = static_cast<unsigned long long>(wave_sum(static_cast<double>(cnt)));
+  if ((threadIdx.x & 63) == 0) {
+    const size_t sl = (blk * static_cast<size_t>(blockDim.x) + threadIdx.x) >> 6;
+    if (sl * 64 < n) {
+      slice_width[sl] = wmax;
+      atomicAdd(nnz_total, wsum);
+    }
+  }
+}
+
 // HL off-diagonals and diagonal (Computer.hpp:1232-1328).  ghost32 keeps the
 // SPP ghost's share of the diagonal (diag = ghost - sum_j a_ij) for the
 // matrix-free FP32 operator; rows without unknowns (dummy, disabled) get
@@ -354,29 +423,30 @@ void ensure_sell_capacity(Context& c, size_t nslices, size_t slots) {
   (void)nslices;
 }
 
-void ppe_assemble(Context& c) {
+namespace {
+
+// Shapes, capacity and constants of the system about to be assembled.
+PpeConsts ppe_prepare(Context& c) {
   const size_t n = c.n;
   c.ppe_n = n;
   c.nslices = (n + 63) / 64;
   c.have_ppe = true;
   c.have_ppe32 = false;
-  if (n == 0) return;
   // worst case: every row as long as the longest neighbour list
-  ensure_sell_capacity(c, c.nslices, c.nslices * 64 * static_cast<size_t>(c.kmax));
+  if (n > 0) ensure_sell_capacity(c, c.nslices, c.nslices * 64 * static_cast<size_t>(c.kmax));
   const double extent = std::max(c.env.max_x - c.env.min_x, c.env.max_z - c.env.min_z);
   c.qpos_inv_h = (extent > 0) ? 1073741824.0 / extent : 1.0 / c.env.l0;
   PpeConsts k{c.env.r_e, c.env.r_e * c.env.r_e, c.env.r_lap, c.env.r_lap * c.env.r_lap, c.env.n0,
               c.env.n0_lap, c.env.rho, c.dt, c.solver.warm_start, c.env.min_x, c.env.min_z,
               c.qpos_inv_h};
+  if (n > 0) MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 2 * sizeof(unsigned long long), c.stream));
+  return k;
+}
+
+// Slice offsets from the counted row lengths, then the fill.
+void ppe_finish(Context& c, const PpeConsts& k) {
+  const size_t n = c.n;
   const NV nb{c.nbr, c.ncount, c.kmax};
-  MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 2 * sizeof(unsigned long long), c.stream));
-  tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 28.0, [&] {
-    hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
-                       c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
-                       static_cast<const double*>(c.ecs), c.b, c.sol, c.row_len, c.slice_width,
-                       c.qpos, c.ppe_counters);
-  });
-  check_launch();
   scan_slices(c, c.slice_width, c.nslices, c.slice_off);
   MPS_HIP(hipMemcpyAsync(c.ppe_counters_host, c.ppe_counters, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, c.stream));
@@ -401,6 +471,40 @@ void ppe_assemble(Context& c) {
   c.ppe_from_particles = true;
   c.ppe_search_serial = c.search_serial;
   c.have_col16 = false;
+}
+
+}  // namespace
+
+void ppe_assemble(Context& c) {
+  const PpeConsts k = ppe_prepare(c);
+  const size_t n = c.n;
+  if (n == 0) return;
+  const NV nb{c.nbr, c.ncount, c.kmax};
+  tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 28.0, [&] {
+    hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
+                       c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                       static_cast<const double*>(c.ecs), c.b, c.sol, c.row_len, c.slice_width,
+                       c.qpos, c.ppe_counters);
+  });
+  check_launch();
+  ppe_finish(c, k);
+}
+
+void density_save_and_assemble(Context& c) {
+  const PpeConsts k = ppe_prepare(c);
+  const size_t n = c.n;
+  if (n == 0) return;
+  const NV nb{c.nbr, c.ncount, c.kmax};
+  // the second density pass's bytes and the count pass's (x, z, u, w, p, type,
+  // ecs read; n, nWithoutSpp, X0, b, x0, row length, position written)
+  tracked(c, MPS_K_DENSITY_SAVE, n * (40.0 + 1 + 8 + 4 + 16 + 16 + 16 + 4 + 8) + 4.0 * c.nbr_total, [&] {
+    hipLaunchKernelGGL(k_density_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u,
+                       c.w, c.p, static_cast<const uint8_t*>(c.type), nb, k, c.nd, c.nws,
+                       static_cast<const double*>(c.ecs), c.ox, c.oz, c.b, c.sol, c.row_len, c.slice_width,
+                       c.qpos, c.ppe_counters);
+  });
+  check_launch();
+  ppe_finish(c, k);
 }
 
 void ppe_compress_columns(Context& c) {

@@ -9,6 +9,13 @@ hydrostatic  A still water column (the static-pressure case): the fluid
              pressure averaged over horizontal bands 10 spacings thick
              (relative to the bottom pressure rho g H).
 
+dambreak     The collapsing water column (column width L, height 2L, tank
+             4L: the Koshizuka & Oka 1996 geometry): the surge front (largest
+             x of the fluid particles, as Benchmark/DamBreak/
+             koshizukaoka1996_edge.py:15-20 takes it) over t sqrt(2g/L),
+             against the experiments and computations shipped in
+             openmps_amd/data/dambreak (front / L at their sample times).
+
 compare      The same scene stepped with two solver presets (e.g. mixed_ml
              against fp64) for a number of steps; the relative L2 difference of
              the pressure and the RMS position difference, in spacings.
@@ -17,12 +24,14 @@ Usage:
   python -m openmps_amd.validate hydrostatic --scene static_pressure --end-time 0.5 --solver fp64
   python -m openmps_amd.validate hydrostatic --scene still_1m --steps 500 --solver mixed_ml
   python -m openmps_amd.validate compare --scene dambreak_1m --steps 500 --solvers mixed_ml fp64
+  python -m openmps_amd.validate dambreak --scene dambreak_125k --t-star 3.0 --solver mixed_ml
 Each run prints one JSON line on stdout (progress on stderr).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -124,6 +133,60 @@ def run_hydrostatic(scene: str, steps: int, preset: str, tol: float, end_time: f
     return out
 
 
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "dambreak")
+
+
+def front_references() -> dict:
+    """name -> (t*, Z/L) arrays of the shipped surge-front data."""
+    out = {}
+    for f in sorted(os.listdir(DATA)):
+        if f.endswith(".csv"):
+            a = np.loadtxt(os.path.join(DATA, f), delimiter=",", skiprows=1, ndmin=2)
+            out[f[:-4]] = (a[:, 0], a[:, 1])
+    return out
+
+
+def run_dambreak(scene: str, t_star: float, preset: str, tol: float, every: float = 0.005) -> dict:
+    """Front position Z/L against t sqrt(2g/L) up to t_star, sampled every
+    `every` seconds, compared with every reference curve at its own sample
+    times inside the simulated range."""
+    from . import scenes
+    cfg = scenes.config(scene)
+    P0 = cfg["particles"]
+    fluid0 = P0["type"] == 0
+    l0, g = cfg["env"]["l0"], cfg["env"]["g"]
+    L = P0["x"][fluid0].max() - P0["x"][fluid0].min() + l0  # column width
+    scale = np.sqrt(2 * g / L)
+    c = _computer(cfg, preset, tol)
+    ts, zs = [0.0], [P0["x"][fluid0].max() / L]
+    end = t_star / scale
+    nxt = [every]
+
+    def on_step(cc):
+        t = cc.time()[0]
+        if t >= nxt[0] - 1e-12:
+            nxt[0] += every
+            P = cc.particles()
+            ts.append(t * scale)
+            zs.append(P["x"][P["type"] == 0].max() / L)
+
+    k, its = _step(c, cfg, 0, preset, every=500, end_time=end, on_step=on_step)
+    ts, zs = np.array(ts), np.array(zs)
+    cmp = {}
+    for name, (rt, rz) in front_references().items():
+        m = (rt >= ts[0]) & (rt <= ts[-1])
+        if m.sum() == 0:
+            continue
+        ours = np.interp(rt[m], ts, zs)
+        d = ours - rz[m]
+        cmp[name] = {"points": int(m.sum()), "mean_diff": float(d.mean()), "max_abs_diff": float(np.abs(d).max()),
+                     "rms_rel": float(np.sqrt(np.mean((d / rz[m]) ** 2)))}
+    return {"check": "dambreak", "scene": scene, "solver": preset, "steps": k, "mean_cg_iterations": its,
+            "column_width_m": float(L), "t_star_end": float(ts[-1]),
+            "front": {"t_star": [round(float(v), 4) for v in ts], "z_over_L": [round(float(v), 4) for v in zs]},
+            "versus": cmp}
+
+
 def run_compare(scene: str, steps: int, presets: list[str], tol: float) -> dict:
     from . import scenes
     cfg = scenes.config(scene)
@@ -160,11 +223,22 @@ def main(argv=None) -> int:
     c.add_argument("--steps", type=int, default=500)
     c.add_argument("--solvers", nargs=2, default=["mixed_ml", "fp64"])
     c.add_argument("--tol", type=float, default=1e-8)
+    d = sub.add_parser("dambreak")
+    d.add_argument("--scene", default="dambreak_125k")
+    d.add_argument("--t-star", type=float, default=3.0)
+    d.add_argument("--solver", default="mixed_ml")
+    d.add_argument("--tol", type=float, default=1e-8)
     args = ap.parse_args(argv)
-    if args.cmd == "hydrostatic":
+    if args.cmd == "dambreak":
+        out = run_dambreak(args.scene, args.t_star, args.solver, args.tol)
+    elif args.cmd == "hydrostatic":
         out = run_hydrostatic(args.scene, args.steps, args.solver, args.tol, args.end_time, args.average_from)
     else:
         out = run_compare(args.scene, args.steps, args.solvers, args.tol)
+    if args.cmd == "dambreak":
+        front = out["front"]
+        _log("  t*   Z/L: " + ", ".join(f"{a:.2f} {b:.3f}" for a, b in
+                                       list(zip(front["t_star"], front["z_over_L"]))[::10]))
     print(json.dumps(out), flush=True)
     return 0
 
