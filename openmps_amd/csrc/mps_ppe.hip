@@ -186,9 +186,12 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
 // The diagonal is summed in neighbour-list order (bit-exact with the
 // reference); the off-diagonals are stored with ascending column.  A list is
 // the concatenation of at most 9 ascending runs (one per cell, each cell's
-// members ascending), so a 9-way merge orders it without a sort.  Ascending
-// columns make slot k of neighbouring rows point at neighbouring particles:
-// the solver's gathers of a wavefront touch a few cache lines, not 64.
+// members ascending), so a 9-way merge orders it without a sort; the list
+// walk stages the emitted columns in LDS, so the merge's chain of dependent
+// reads stays in LDS.  Ascending columns make slot k of neighbouring rows
+// point at neighbouring particles: the solver's gathers of a wavefront touch
+// a few cache lines, not 64.  A row with more than kStage columns or more
+// than 9 runs is stored in list order.
 constexpr int kRuns = 9;
 
 // Optional output of the fill for the multilevel preconditioner: per row i
@@ -202,6 +205,8 @@ struct AggSums {
   float *s0, *s1, *s2;
 };
 
+constexpr int kStage = 40;  // emitted columns per row staged in LDS for the merge
+
 __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      const uint8_t* __restrict__ type, NV nb,
@@ -211,9 +216,12 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
                                                      double* __restrict__ val, uint32_t* __restrict__ col,
                                                      double* __restrict__ diag,
                                                      float* __restrict__ ghost32, AggSums ag) {
+  // each thread's emitted columns in list order, [slot][thread] (conflict-free)
+  __shared__ uint32_t stage[kStage][kBlock];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  const int t = threadIdx.x;
   const size_t base = slice_off[i >> 6] + (i & 63);
   const uint32_t width = slice_width[i >> 6];
   const uint32_t self = static_cast<uint32_t>(i);
@@ -228,28 +236,33 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
     const double xi = x[i], zi = z[i];
     double s = 0.0, sx = 0, sz = 0;
     const uint32_t m = nb.count[i];
-    uint32_t rs[kRuns] = {};  // first list position of each ascending run
+    // the emitted columns (r < r_lap, not dummy) form at most 9 ascending
+    // runs, one per cell: their starts, for the merge below
+    uint32_t er[kRuns] = {};
     int nr = 0;
-    bool merge = true;
-    uint32_t prev = 0;
+    uint32_t ne = 0, prev = 0;
+    bool staged = true;
     for (uint32_t q = 0; q < m; q++) {
       const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
-      if (q == 0 || j < prev) {
-        if (nr < kRuns) {
-#pragma unroll
-          for (int r = 0; r < kRuns; r++)
-            if (r == nr) rs[r] = q;
-          nr++;
-        } else {
-          merge = false;  // not a cell-ordered list: store in list order
-        }
-      }
-      prev = j;
       const double dx = x[j] - xi, dz = z[j] - zi;
       const double r2 = dx * dx + dz * dz;
       if (r2 < k.rl2 && type[j] != kDummy) {
         const double r = sqrt(r2);
         s += -(3 * k.rl / k.n0l / (r * r * r));
+        if (ne == 0 || j < prev) {
+          if (nr < kRuns) {
+#pragma unroll
+            for (int u = 0; u < kRuns; u++)
+              if (u == nr) er[u] = ne;
+            nr++;
+          } else {
+            staged = false;  // not a cell-ordered list: store in list order
+          }
+        }
+        prev = j;
+        if (ne < kStage) stage[ne][t] = j;
+        else staged = false;
+        ne++;
       }
       if (r2 < k.re2) {
         const double wg = weight(sqrt(r2), k.re);
@@ -269,71 +282,76 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
     const uint32_t ki = ag.s0 ? ag.key[i] : kNoCell;
     const uint32_t cx = ki / ag.gnz, cz = ki % ag.gnz;
     if (ki != kNoCell) a0 = a1 = a2 = d;
+    // value of an emitted column j in slot kk (and its share of the aggregate row sums)
     auto emit = [&](uint32_t j) {
       const double dx = x[j] - xi, dz = z[j] - zi;
-      const double r2 = dx * dx + dz * dz;
-      if (r2 < k.rl2 && type[j] != kDummy) {
-        const double r = sqrt(r2);
-        const size_t o = base + static_cast<size_t>(kk) * 64;
-        const double a = 3 * k.rl / k.n0l / (r * r * r);
-        val[o] = a;
-        col[o] = j;
-        kk++;
-        if (ki != kNoCell) {
-          const uint32_t kj = ag.key[j];
-          if (kj != kNoCell) {
-            const uint32_t jx = kj / ag.gnz, jz = kj % ag.gnz;
-            if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
-              a2 += a;
-              if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
-                a1 += a;
-                if (jx == cx && jz == cz) a0 += a;
-              }
+      const double r = sqrt(dx * dx + dz * dz);
+      const size_t o = base + static_cast<size_t>(kk) * 64;
+      const double a = 3 * k.rl / k.n0l / (r * r * r);
+      val[o] = a;
+      col[o] = j;
+      kk++;
+      if (ki != kNoCell) {
+        const uint32_t kj = ag.key[j];
+        if (kj != kNoCell) {
+          const uint32_t jx = kj / ag.gnz, jz = kj % ag.gnz;
+          if ((jx >> 2) == (cx >> 2) && (jz >> 2) == (cz >> 2)) {
+            a2 += a;
+            if ((jx >> 1) == (cx >> 1) && (jz >> 1) == (cz >> 1)) {
+              a1 += a;
+              if (jx == cx && jz == cz) a0 += a;
             }
           }
         }
       }
     };
-    if (merge) {
+    if (staged) {
+      // 9-way merge of the staged runs (LDS only on its dependency chain),
+      // the ascending columns written first, the values computed after
       uint32_t cur[kRuns], end[kRuns], hd[kRuns];
 #pragma unroll
-      for (int r = 0; r < kRuns; r++) {
-        cur[r] = (r < nr) ? rs[r] : 0u;
-        end[r] = (r < nr) ? m : 0u;
+      for (int u = 0; u < kRuns; u++) {
+        cur[u] = (u < nr) ? er[u] : 0u;
+        end[u] = (u < nr) ? ne : 0u;
       }
 #pragma unroll
-      for (int r = 0; r + 1 < kRuns; r++)
-        if (r + 1 < nr) end[r] = rs[r + 1];
+      for (int u = 0; u + 1 < kRuns; u++)
+        if (u + 1 < nr) end[u] = er[u + 1];
 #pragma unroll
-      for (int r = 0; r < kRuns; r++)
-        hd[r] = (cur[r] < end[r]) ? nb.ids[nbr_slot(self, cur[r], nb.kmax)] : 0xffffffffu;
-      for (uint32_t t = 0; t < m; t++) {
+      for (int u = 0; u < kRuns; u++) hd[u] = (cur[u] < end[u]) ? stage[cur[u]][t] : 0xffffffffu;
+      for (uint32_t v = 0; v < ne; v++) {
         uint32_t best = hd[0];
         int sel = 0;
 #pragma unroll
-        for (int r = 1; r < kRuns; r++)
-          if (hd[r] < best) {
-            best = hd[r];
-            sel = r;
+        for (int u = 1; u < kRuns; u++)
+          if (hd[u] < best) {
+            best = hd[u];
+            sel = u;
           }
-        uint32_t nc = 0, ne = 0;
+        uint32_t nc = 0, nend = 0;
 #pragma unroll
-        for (int r = 0; r < kRuns; r++)
-          if (r == sel) {
-            nc = cur[r] + 1;
-            ne = end[r];
+        for (int u = 0; u < kRuns; u++)
+          if (u == sel) {
+            nc = cur[u] + 1;
+            nend = end[u];
           }
-        const uint32_t nh = (nc < ne) ? nb.ids[nbr_slot(self, nc, nb.kmax)] : 0xffffffffu;
+        const uint32_t nh = (nc < nend) ? stage[nc][t] : 0xffffffffu;
 #pragma unroll
-        for (int r = 0; r < kRuns; r++)
-          if (r == sel) {
-            cur[r] = nc;
-            hd[r] = nh;
+        for (int u = 0; u < kRuns; u++)
+          if (u == sel) {
+            cur[u] = nc;
+            hd[u] = nh;
           }
-        emit(best);
+        col[base + static_cast<size_t>(v) * 64] = best;
       }
+#pragma unroll 4
+      for (uint32_t v = 0; v < ne; v++) emit(col[base + static_cast<size_t>(v) * 64]);
     } else {
-      for (uint32_t q = 0; q < m; q++) emit(nb.ids[nbr_slot(self, q, nb.kmax)]);
+      for (uint32_t q = 0; q < m; q++) {
+        const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
+        const double dx = x[j] - xi, dz = z[j] - zi;
+        if (dx * dx + dz * dz < k.rl2 && type[j] != kDummy) emit(j);
+      }
     }
   }
   for (; kk < width; kk++) {
