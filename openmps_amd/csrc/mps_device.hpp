@@ -22,6 +22,31 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t i, uint32_t k, uint32_t kmax
   return (static_cast<size_t>(i >> 6) * kmax + k) * 64 + (i & 63u);
 }
 
+// Walks particle i's m-entry neighbour list in list order, U neighbours per
+// round trip to memory: the U ids, then load(j) for all U of them (the
+// gathered fields of neighbour j, returned as a T), then use(j, T) in list
+// order.  A walk that issues each neighbour's loads only after the previous
+// one's arithmetic pays a full memory latency per neighbour.
+template <int U, class T, class Load, class Use>
+__device__ __forceinline__ void walk_list(uint32_t i, uint32_t m, const uint32_t* __restrict__ ids, uint32_t kmax,
+                                          Load load, Use use) {
+  uint32_t q = 0;
+  for (; q + U <= m; q += U) {
+    uint32_t jj[U];
+#pragma unroll
+    for (int v = 0; v < U; v++) jj[v] = ids[nbr_slot(i, q + v, kmax)];
+    T f[U];
+#pragma unroll
+    for (int v = 0; v < U; v++) f[v] = load(jj[v]);
+#pragma unroll
+    for (int v = 0; v < U; v++) use(jj[v], f[v]);
+  }
+  for (; q < m; q++) {
+    const uint32_t j = ids[nbr_slot(i, q, kmax)];
+    use(j, load(j));
+  }
+}
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // round-robin; remapping makes each XCD walk one contiguous range of
 // particles, so gathered neighbour data stays in that XCD's L2.

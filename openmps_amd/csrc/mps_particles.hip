@@ -25,6 +25,8 @@ struct Consts {
   double re, re2, rl, rl2, n0, n0l, nu, rho, dt, gx, gz, d2;
 };
 
+constexpr int kWalk = 4;  // neighbours whose loads are issued together (walk_list)
+
 Consts consts(const Context& c) {
   Consts k;
   k.re = c.env.r_e;
@@ -96,20 +98,30 @@ __global__ void __launch_bounds__(kBlock) k_density(size_t n, const double* __re
     wi = w[i];
   }
   double s = 0.0, h = 0.0;
-  const uint32_t cnt = nb.count[i];
-  for (uint32_t q = 0; q < cnt; q++) {
-    const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-    const double dx = x[j] - xi, dz = z[j] - zi;
-    const double r2 = dx * dx + dz * dz;
-    if (r2 < k.re2) {
-      const double r = sqrt(r2);
-      if (DENS) s += weight(r, k.re);
-      if (ECS) {
-        const double du = u[j] - ui, dw = w[j] - wi;
-        h += (dx * du + dz * dw) / (r * r * r);
-      }
-    }
-  }
+  struct F {
+    double x, z, u, w;
+  };
+  walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                      [&](uint32_t j) {
+                        F f{x[j], z[j], 0.0, 0.0};
+                        if (ECS) {
+                          f.u = u[j];
+                          f.w = w[j];
+                        }
+                        return f;
+                      },
+                      [&](uint32_t, const F& f) {
+                        const double dx = f.x - xi, dz = f.z - zi;
+                        const double r2 = dx * dx + dz * dz;
+                        if (r2 < k.re2) {
+                          const double r = sqrt(r2);
+                          if (DENS) s += weight(r, k.re);
+                          if (ECS) {
+                            const double du = f.u - ui, dw = f.w - wi;
+                            h += (dx * du + dz * dw) / (r * r * r);
+                          }
+                        }
+                      });
   double thisNws, thisN;
   if (DENS) {
     thisNws = s;
@@ -166,23 +178,27 @@ __global__ void __launch_bounds__(kBlock) k_explicit(size_t n, const double* __r
   if (type[i] != kFluid) return;
   const double xi = x[i], zi = z[i], ui = u[i], wi = w[i];
   double vx = 0, vz = 0, sx = 0, sz = 0;
-  const uint32_t cnt = nb.count[i];
-  for (uint32_t q = 0; q < cnt; q++) {
-    const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-    const double dx = x[j] - xi, dz = z[j] - zi;
-    const double r2 = dx * dx + dz * dz;
-    if (r2 < k.rl2 && type[j] != kDummy) {
-      const double r = sqrt(r2);
-      const double c = k.nu * 3.0 * k.rl / k.n0l / (r * r * r);
-      vx += c * (u[j] - ui);
-      vz += c * (w[j] - wi);
-    }
-    if (r2 < k.re2) {
-      const double wg = weight(sqrt(r2), k.re);
-      sx += wg * dx;
-      sz += wg * dz;
-    }
-  }
+  struct F {
+    double x, z, u, w;
+    uint8_t t;
+  };
+  walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                      [&](uint32_t j) { return F{x[j], z[j], u[j], w[j], type[j]}; },
+                      [&](uint32_t, const F& f) {
+                        const double dx = f.x - xi, dz = f.z - zi;
+                        const double r2 = dx * dx + dz * dz;
+                        if (r2 < k.rl2 && f.t != kDummy) {
+                          const double r = sqrt(r2);
+                          const double c = k.nu * 3.0 * k.rl / k.n0l / (r * r * r);
+                          vx += c * (f.u - ui);
+                          vz += c * (f.w - wi);
+                        }
+                        if (r2 < k.re2) {
+                          const double wg = weight(sqrt(r2), k.re);
+                          sx += wg * dx;
+                          sz += wg * dz;
+                        }
+                      });
   double gxp, gzp;
   if (ghost_pos(xi, zi, sx, sz, nws[i], k.n0, k.re, gxp, gzp)) {
     const double rx = xi - gxp, rz = zi - gzp;
@@ -246,22 +262,26 @@ __global__ void __launch_bounds__(kBlock) k_gradient(size_t n, const double* __r
   if (type[i] != kFluid) return;
   const double xi = x[i], zi = z[i], pi = p[i];
   double fx = 0, fz = 0, sx = 0, sz = 0;
-  const uint32_t cnt = nb.count[i];
-  for (uint32_t q = 0; q < cnt; q++) {
-    const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-    const double dx = x[j] - xi, dz = z[j] - zi;
-    const double r2 = dx * dx + dz * dz;
-    if (r2 < k.re2) {
-      const double wg = weight(sqrt(r2), k.re);
-      sx += wg * dx;
-      sz += wg * dz;
-      if (type[j] != kDummy) {
-        const double s = (p[j] + pi) / r2 * wg;
-        fx += s * dx;
-        fz += s * dz;
-      }
-    }
-  }
+  struct F {
+    double x, z, p;
+    uint8_t t;
+  };
+  walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                      [&](uint32_t j) { return F{x[j], z[j], p[j], type[j]}; },
+                      [&](uint32_t, const F& f) {
+                        const double dx = f.x - xi, dz = f.z - zi;
+                        const double r2 = dx * dx + dz * dz;
+                        if (r2 < k.re2) {
+                          const double wg = weight(sqrt(r2), k.re);
+                          sx += wg * dx;
+                          sz += wg * dz;
+                          if (f.t != kDummy) {
+                            const double s = (f.p + pi) / r2 * wg;
+                            fx += s * dx;
+                            fz += s * dz;
+                          }
+                        }
+                      });
   double gxp, gzp;
   if (ghost_pos(xi, zi, sx, sz, nws[i], k.n0, k.re, gxp, gzp)) {
     const double dx = gxp - xi, dz = gzp - zi;
@@ -287,12 +307,17 @@ __global__ void __launch_bounds__(kBlock) k_ds(size_t n, const double* __restric
   if (type[i] != kFluid) return;
   const double xi = x[i], zi = z[i], oxi = ox[i], ozi = oz[i];
   double fx = 0, fz = 0;
-  const uint32_t cnt = nb.count[i];
-  for (uint32_t q = 0; q < cnt; q++) {
-    const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-    const double dx = x[j] - xi, dz = z[j] - zi;
+  // X0 is read only for the few neighbours closer than d = (1 - C) l0
+  struct F {
+    double x, z;
+    uint8_t t;
+  };
+  walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                      [&](uint32_t j) { return F{x[j], z[j], type[j]}; },
+                      [&](uint32_t j, const F& f) {
+    const double dx = f.x - xi, dz = f.z - zi;
     const double r2 = dx * dx + dz * dz;
-    if (r2 < k.re2 && type[j] != kDummy && r2 < k.d2) {
+    if (r2 < k.re2 && f.t != kDummy && r2 < k.d2) {
       const double e0x = ox[j] - oxi, e0z = oz[j] - ozi;
       const double nrm = sqrt(e0x * e0x + e0z * e0z);
       const double ex = e0x / nrm, ez = e0z / nrm;
@@ -303,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_ds(size_t n, const double* __restric
       fx += m * ex;
       fz += m * ez;
     }
-  }
+                      });
   const double coef = -1.0 / (2 * k.dt * k.n0);
   dux[i] = coef * fx;
   duz[i] = coef * fz;

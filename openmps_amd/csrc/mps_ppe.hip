@@ -51,6 +51,8 @@ __device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, doubl
   return true;
 }
 
+constexpr int kWalk = 4;  // neighbours whose loads are issued together (walk_list)
+
 // b and x0 (Computer.hpp:1185-1221), row lengths and slice widths.
 __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z,
@@ -80,18 +82,22 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
     } else {
       const double xi = x[i], zi = z[i], ui = u[i], wi = w[i];
       double h = 0.0;
-      const uint32_t m = nb.count[i];
-      for (uint32_t q = 0; q < m; q++) {
-        const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-        const double dx = x[j] - xi, dz = z[j] - zi;
-        const double r2 = dx * dx + dz * dz;
-        if (r2 < k.re2) {
-          const double r = sqrt(r2);
-          const double du = u[j] - ui, dw = w[j] - wi;
-          h += (dx * du + dz * dw) / (r * r * r);
-        }
-        if (r2 < k.rl2 && type[j] != kDummy) cnt++;
-      }
+      struct F {
+        double x, z, u, w;
+        uint8_t t;
+      };
+      walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                          [&](uint32_t j) { return F{x[j], z[j], u[j], w[j], type[j]}; },
+                          [&](uint32_t, const F& f) {
+                            const double dx = f.x - xi, dz = f.z - zi;
+                            const double r2 = dx * dx + dz * dz;
+                            if (r2 < k.re2) {
+                              const double r = sqrt(r2);
+                              const double du = f.u - ui, dw = f.w - wi;
+                              h += (dx * du + dz * dw) / (r * r * r);
+                            }
+                            if (r2 < k.rl2 && f.t != kDummy) cnt++;
+                          });
       const double speed = (nws[i] < k.n0) ? 0.0 : -k.re * h;  // HS, :838-872
       b[i] = -k.rho / (k.n0 * k.dt) * (speed + ecs[i]);
       sol[i] = (k.warm == MPS_WARM_ZERO) ? 0.0 : p[i];
@@ -147,17 +153,40 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
       const double ui = u[i], wi = w[i];
       double s = 0.0, h = 0.0;
       const uint32_t m = nb.count[i];
-      for (uint32_t q = 0; q < m; q++) {
-        const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-        const double dx = x[j] - xi, dz = z[j] - zi;
+      auto use = [&](double xj, double zj, double uj, double wj, uint8_t tj) {
+        const double dx = xj - xi, dz = zj - zi;
         const double r2 = dx * dx + dz * dz;
         if (r2 < k.re2) {
           const double r = sqrt(r2);
           s += weight(r, k.re);
-          const double du = u[j] - ui, dw = w[j] - wi;
+          const double du = uj - ui, dw = wj - wi;
           h += (dx * du + dz * dw) / (r * r * r);
         }
-        if (r2 < k.rl2 && type[j] != kDummy) cnt++;
+        if (r2 < k.rl2 && tj != kDummy) cnt++;
+      };
+      // kWalk neighbours per round trip: their ids, then all their fields,
+      // loaded before any is used; the sums still run in list order
+      uint32_t q = 0;
+      for (; q + kWalk <= m; q += kWalk) {
+        uint32_t jj[kWalk];
+#pragma unroll
+        for (int v = 0; v < kWalk; v++) jj[v] = nb.ids[nbr_slot(static_cast<uint32_t>(i), q + v, nb.kmax)];
+        double xj[kWalk], zj[kWalk], uj[kWalk], wj[kWalk];
+        uint8_t tj[kWalk];
+#pragma unroll
+        for (int v = 0; v < kWalk; v++) {
+          xj[v] = x[jj[v]];
+          zj[v] = z[jj[v]];
+          uj[v] = u[jj[v]];
+          wj[v] = w[jj[v]];
+          tj[v] = type[jj[v]];
+        }
+#pragma unroll
+        for (int v = 0; v < kWalk; v++) use(xj[v], zj[v], uj[v], wj[v], tj[v]);
+      }
+      for (; q < m; q++) {
+        const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
+        use(x[j], z[j], u[j], w[j], type[j]);
       }
       nws[i] = s;
       nd[i] = (s < k.n0) ? k.n0 : s;
@@ -242,11 +271,15 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
     int nr = 0;
     uint32_t ne = 0, prev = 0;
     bool staged = true;
-    for (uint32_t q = 0; q < m; q++) {
-      const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
-      const double dx = x[j] - xi, dz = z[j] - zi;
+    struct F {
+      double x, z;
+      uint8_t t;
+    };
+    walk_list<kWalk, F>(self, m, nb.ids, nb.kmax, [&](uint32_t j) { return F{x[j], z[j], type[j]}; },
+                        [&](uint32_t j, const F& f) {
+      const double dx = f.x - xi, dz = f.z - zi;
       const double r2 = dx * dx + dz * dz;
-      if (r2 < k.rl2 && type[j] != kDummy) {
+      if (r2 < k.rl2 && f.t != kDummy) {
         const double r = sqrt(r2);
         s += -(3 * k.rl / k.n0l / (r * r * r));
         if (ne == 0 || j < prev) {
@@ -269,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
         sx += wg * dx;
         sz += wg * dz;
       }
-    }
+                        });
     double gxp, gzp;
     if (ghost_pos(xi, zi, sx, sz, nws[i], k.n0, k.re, gxp, gzp)) {
       const double rx = xi - gxp, rz = zi - gzp;

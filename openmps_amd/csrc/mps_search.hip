@@ -204,10 +204,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
         if (cz < 0 || cz >= g.gnz) continue;
         const size_t c = static_cast<size_t>(cx * g.gnz + cz);
         const uint32_t s = cell_start[c], m = cell_count[c];
-        for (uint32_t l = 0; l < m; l++) {
-          const uint32_t j = sorted[s + l];
-          if (j == i) continue;
-          const double rx = xi - x[j], rz = zi - z[j];
+        auto test = [&](uint32_t j, double xj, double zj) {
+          if (j == i) return;
+          const double rx = xi - xj, rz = zi - zj;
           // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
           // rounded, so the test only depends on r.r near NL^2: outside the
           // narrow band [lo, hi] the comparison of r.r decides it exactly.
@@ -217,6 +216,25 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
             if (cnt < kmax) nbr[nbr_slot(static_cast<uint32_t>(i), cnt, kmax)] = j;
             cnt++;
           }
+        };
+        // four candidates per round trip to memory (ids, then positions)
+        uint32_t l = 0;
+        for (; l + 4 <= m; l += 4) {
+          uint32_t jj[4];
+          double xj[4], zj[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) jj[v] = sorted[s + l + v];
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            xj[v] = x[jj[v]];
+            zj[v] = z[jj[v]];
+          }
+#pragma unroll
+          for (int v = 0; v < 4; v++) test(jj[v], xj[v], zj[v]);
+        }
+        for (; l < m; l++) {
+          const uint32_t j = sorted[s + l];
+          test(j, x[j], z[j]);
         }
       }
     }
