@@ -645,9 +645,47 @@ __global__ void __launch_bounds__(256) k_ml_compact(uint32_t nblk, const uint32_
   if (b < nblk && flag[b]) list[off[b]] = b;
 }
 
+// The preconditioner's per-solve constants in the order its wavefronts use
+// them, so that an iteration reads them with coalesced loads instead of the
+// chain list -> cell table -> sorted ids -> per-particle data: for listed
+// block w, slot u < kPerLane and lane l, at (w * kPerLane + u) * 64 + l,
+//   pid: the particle id (top bit set: not in an aggregate, a dummy or
+//        disabled row; kNoSlot: empty slot), dv: its Jacobi factor 1 / A_ii;
+// per (w, l): the lane's cell, 2x2 and 4x4 aggregate scales and its cell's
+// particle count (float4; counts above 4 kPerLane take the cell-table path).
+constexpr uint32_t kNoSlot = 0xffffffffu;
+constexpr uint32_t kOutAgg = 0x80000000u;
+
+__global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __restrict__ list, uint32_t nact,
+                                                 const float* __restrict__ inv, const float* __restrict__ dinv,
+                                                 uint32_t* __restrict__ pid, float* __restrict__ dv,
+                                                 float4* __restrict__ sc) {
+  const uint32_t w = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  if (w >= nact) return;  // wave-uniform
+  const uint32_t lane = threadIdx.x & 63;
+  const BlockLane L = block_lane(G, list[w]);
+  const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+  sc[static_cast<size_t>(w) * 64 + lane] =
+      make_float4(L.valid ? inv[L.cell] : 0.0f, L.valid ? inv[G.off1 + L.a1] : 0.0f,
+                  L.valid ? inv[G.off2 + L.a2] : 0.0f, static_cast<float>(cnt));
+  for (int u = 0; u < kPerLane; u++) {
+    const uint32_t k = L.sub + 4u * u;
+    const size_t o = (static_cast<size_t>(w) * kPerLane + u) * 64 + lane;
+    if (k < cnt) {
+      const uint32_t i = G.sorted[st0 + k];
+      pid[o] = i | (in_agg(G.type[i]) ? 0u : kOutAgg);
+      dv[o] = dinv[i];
+    } else {
+      pid[o] = kNoSlot;
+      dv[o] = 0.0f;
+    }
+  }
+}
+
 // One wavefront per listed 4x4 block of cells.
 __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __restrict__ list, uint32_t nact,
-                                                 const float* __restrict__ inv,
+                                                 const uint32_t* __restrict__ pid, const float* __restrict__ dvp,
+                                                 const float4* __restrict__ sc,
                                                  const double* __restrict__ r64, double inv_norm,
                                                  float* __restrict__ r, float* __restrict__ d,
                                                  const float* __restrict__ q, const float* __restrict__ dinv,
@@ -665,34 +703,24 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
     pq_part[u] = (it > 0 && k < npq) ? part_pq[k] : 0.0;
   }
   const uint32_t w = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
   const bool active = w < nact;  // wave-uniform
-  BlockLane L{false, 0, 0, 0, 0};
-  uint32_t cnt = 0, st0 = 0;
-  float i0 = 0.0f, i1 = 0.0f, i2 = 0.0f;
+  float4 scl = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t id[kPerLane];
   float rv[kPerLane], qv[kPerLane], dv[kPerLane];
-  bool ag[kPerLane];
   if (active) {
-    L = block_lane(G, list[w]);
-    cnt = L.valid ? G.count[L.cell] : 0u;
-    st0 = L.valid ? G.start[L.cell] : 0u;
-    // aggregate scales: independent of the particles, loaded with them
-    i0 = L.valid ? inv[L.cell] : 0.0f;
-    i1 = L.valid ? inv[G.off1 + L.a1] : 0.0f;
-    i2 = L.valid ? inv[G.off2 + L.a2] : 0.0f;
+    scl = sc[static_cast<size_t>(w) * 64 + lane];
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
-      const uint32_t k = L.sub + 4u * u;
-      id[u] = (k < cnt) ? G.sorted[st0 + k] : 0u;
+      const size_t o = (static_cast<size_t>(w) * kPerLane + u) * 64 + lane;
+      id[u] = pid[o];
+      dv[u] = dvp[o];
     }
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
-      rv[u] = qv[u] = dv[u] = 0.0f;
-      ag[u] = false;
-      if (L.sub + 4u * u < cnt) {
-        const uint32_t i = id[u];
-        dv[u] = dinv[i];
-        ag[u] = in_agg(G.type[i]);
+      rv[u] = qv[u] = 0.0f;
+      if (id[u] != kNoSlot) {
+        const uint32_t i = id[u] & ~kOutAgg;
         if (it == 0) {
           rv[u] = static_cast<float>(r64[i] * inv_norm);
         } else {
@@ -713,52 +741,63 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
   }
   double rho = 0, rr = 0;
   if (active) {
+    const uint32_t cnt = static_cast<uint32_t>(scl.w);
+    // a cell beyond 4 kPerLane particles (larger cell capacities): its
+    // remaining members through the cell table
+    BlockLane L{false, 0, 0, 0, 0};
+    uint32_t st0 = 0;
+    const bool over = cnt > 4u * kPerLane;
+    if (over) {
+      L = block_lane(G, list[w]);
+      st0 = G.start[L.cell];
+    }
     float s = 0.0f;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
-      if (L.sub + 4u * u < cnt) {
-        const uint32_t i = id[u];
+      if (id[u] != kNoSlot) {
+        const uint32_t i = id[u] & ~kOutAgg;
         const float ri = (it == 0) ? rv[u] : fmaf(-alpha, qv[u], rv[u]);
         rv[u] = ri;
         r[i] = ri;
         if (it == 0) d[i] = 0.0f;
-        if (ag[u]) s += ri;
+        if (!(id[u] & kOutAgg)) s += ri;
         rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
       }
     }
-    // cells beyond 4 * kPerLane particles (larger cell capacities)
-    for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = G.sorted[st0 + k];
-      float ri;
-      if (it == 0) {
-        ri = static_cast<float>(r64[i] * inv_norm);
-        d[i] = 0.0f;
-      } else {
-        ri = fmaf(-alpha, q[i], r[i]);
+    if (over)
+      for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+        const uint32_t i = G.sorted[st0 + k];
+        float ri;
+        if (it == 0) {
+          ri = static_cast<float>(r64[i] * inv_norm);
+          d[i] = 0.0f;
+        } else {
+          ri = fmaf(-alpha, q[i], r[i]);
+        }
+        r[i] = ri;
+        if (in_agg(G.type[i])) s += ri;
+        rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
       }
-      r[i] = ri;
-      if (in_agg(G.type[i])) s += ri;
-      rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
-    }
     float y0, y1, y2;
     agg_sums(s, y0, y1, y2);
-    const float corr = y0 * i0 + y1 * i1 + y2 * i2;
+    const float corr = y0 * scl.x + y1 * scl.y + y2 * scl.z;
 #pragma unroll
     for (int u = 0; u < kPerLane; u++) {
-      if (L.sub + 4u * u < cnt) {
-        const uint32_t i = id[u];
-        const float zi = ag[u] ? fmaf(dv[u], rv[u], corr) : dv[u] * rv[u];
+      if (id[u] != kNoSlot) {
+        const uint32_t i = id[u] & ~kOutAgg;
+        const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(dv[u], rv[u], corr);
         z[i] = zi;
         rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
       }
     }
-    for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-      const uint32_t i = G.sorted[st0 + k];
-      const float ri = r[i];
-      const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
-      z[i] = zi;
-      rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
-    }
+    if (over)
+      for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+        const uint32_t i = G.sorted[st0 + k];
+        const float ri = r[i];
+        const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
+        z[i] = zi;
+        rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
+      }
   }
   const double srho = block_sum<NTP>(rho, lds);
   const double srr = block_sum<NTP>(rr, lds);
@@ -1233,13 +1272,16 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
-  // per particle: id, r (read, written), q, type, M, z; cell tables
-  const double b_prec = n * 25.0 + c.ncells * 8.0 + total_aggs * 4.0;
+  // per particle: r (read, written), q, z; per slot of the packed layout: id
+  // and Jacobi factor; per lane: the aggregate scales (set once nact is known)
+  double b_prec = n * 16.0;
   uint32_t nact = 0;
   size_t pgrid = 1;
+  bool packed = false;
   auto prec = [&](double norm, long long it) {
     hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
-                       static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
+                       static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const uint32_t*>(c.ml_pid),
+                       static_cast<const float*>(c.ml_dv), static_cast<const float4*>(c.ml_sc),
                        static_cast<const double*>(c.r), 1.0 / norm, c.r32, c.d32,
                        static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), c.z32,
                        static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
@@ -1252,6 +1294,25 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
     nact = static_cast<uint32_t>(*c.ml_active_host);
     b_spmv = b_base + 2.0 * mf_nnz_wide(c);
     pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
+    b_prec = n * 16.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 16);
+    if (!packed) {
+      const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
+      if (c.ml_pack_cap < slots) {
+        dfree(c.ml_pid);
+        dfree(c.ml_dv);
+        dfree(c.ml_sc);
+        c.ml_pid = dalloc<uint32_t>(slots * kPerLane);
+        c.ml_dv = dalloc<float>(slots * kPerLane);
+        c.ml_sc = dalloc<float4>(slots);
+        c.ml_pack_cap = slots;
+      }
+      tracked(c, MPS_K_ML_SETUP, n * 13.0 + slots * (8.0 * kPerLane + 16), [&] {
+        hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+                           static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
+                           static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc);
+      });
+      packed = true;
+    }
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
     tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, 0); });
     long long launched = 0;

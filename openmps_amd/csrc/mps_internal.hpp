@@ -203,6 +203,10 @@ struct Context {
   uint32_t* ml_off = nullptr;
   uint32_t* ml_list = nullptr;  // ... and their indices, ascending
   size_t ml_blk_cap = 0;
+  uint32_t* ml_pid = nullptr;  // packed per-slot particle ids / Jacobi factors, per-lane scales
+  float* ml_dv = nullptr;
+  float4* ml_sc = nullptr;
+  size_t ml_pack_cap = 0;      // lanes (64 per listed block)
   uint64_t* ml_active = nullptr;       // count of such blocks
   uint64_t* ml_active_host = nullptr;  // pinned
   size_t agg_cap = 0;
