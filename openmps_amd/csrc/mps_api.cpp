@@ -142,7 +142,7 @@ void free_all(Context& c) {
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
   dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active); dfree(c.st_replay);
-  dfree(c.ml_pid); dfree(c.ml_dv); dfree(c.ml_sc);
+  dfree(c.ml_pid); dfree(c.ml_dv); dfree(c.ml_sc); dfree(c.zrec);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "mps_device.hpp"
 #include "mps_internal.hpp"
@@ -172,6 +173,81 @@ __device__ __forceinline__ float op_apply(const MatFree& A, size_t i, float pi, 
   return mf_sum<UNROLL>(A, qi, m, pi, s,
                         [&](uint32_t k) { return static_cast<uint32_t>(ii + cp[static_cast<size_t>(k) * 64]); },
                         v);
+}
+
+// The matrix-free operator reading each row's fixed-point position and the
+// vector value it is applied to as one 16-byte record {qx, qz, v, 0}: one
+// gather per stored column instead of two (the multilevel solver keeps its
+// z in the records; the positions are set once per solve).
+struct MatFreeRec {
+  const uint64_t* __restrict__ off;
+  const uint32_t* __restrict__ col;
+  const uint64_t* __restrict__ off16;
+  const int16_t* __restrict__ col16;
+  const uint8_t* __restrict__ wide;
+  const uint32_t* __restrict__ len;
+  const uint4* __restrict__ rec;
+  const float* __restrict__ ghost;
+  float c0;
+  size_t n;
+  size_t rpb;
+};
+
+template <int UNROLL, class J>
+__device__ __forceinline__ float rec_sum(const MatFreeRec& A, uint2 qi, uint32_t m, float pi, float s, J colk) {
+  uint32_t k = 0;
+  for (; k + UNROLL <= m; k += UNROLL) {
+    uint32_t j[UNROLL];
+    uint4 rj[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) j[u] = colk(k + u);
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) rj[u] = A.rec[j[u]];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      const float fx = static_cast<float>(static_cast<int>(rj[u].x - qi.x));
+      const float fz = static_cast<float>(static_cast<int>(rj[u].y - qi.y));
+      const float t = __builtin_amdgcn_rsqf(fmaf(fx, fx, fz * fz));
+      s = fmaf(A.c0 * (t * t * t), __uint_as_float(rj[u].z) - pi, s);
+    }
+  }
+  for (; k < m; k++) {
+    const uint4 rj = A.rec[colk(k)];
+    const float fx = static_cast<float>(static_cast<int>(rj.x - qi.x));
+    const float fz = static_cast<float>(static_cast<int>(rj.y - qi.y));
+    const float t = __builtin_amdgcn_rsqf(fmaf(fx, fx, fz * fz));
+    s = fmaf(A.c0 * (t * t * t), __uint_as_float(rj.z) - pi, s);
+  }
+  return s;
+}
+
+// (A v)_i with v in the records; vi returns v_i.
+template <int UNROLL = 4>
+__device__ __forceinline__ float rec_apply(const MatFreeRec& A, size_t i, float& vi) {
+  const size_t sl = i >> 6;
+  const uint32_t m = A.len[i];
+  const uint4 ri = A.rec[i];
+  const uint2 qi = make_uint2(ri.x, ri.y);
+  const float pi = __uint_as_float(ri.z);
+  vi = pi;
+  const float s = A.ghost[i] * pi;
+  if (A.wide[sl]) {
+    const uint32_t* __restrict__ cp = A.col + A.off[sl] + (i & 63);
+    return rec_sum<UNROLL>(A, qi, m, pi, s, [&](uint32_t k) { return cp[static_cast<size_t>(k) * 64]; });
+  }
+  const int16_t* __restrict__ cp = A.col16 + A.off16[sl] + (i & 63);
+  const int ii = static_cast<int>(i);
+  return rec_sum<UNROLL>(A, qi, m, pi, s,
+                         [&](uint32_t k) { return static_cast<uint32_t>(ii + cp[static_cast<size_t>(k) * 64]); });
+}
+
+// {qx, qz, 0, 0} for every row (rows outside every cell keep z = 0).
+__global__ void __launch_bounds__(256) k_rec_init(size_t n, const uint2* __restrict__ qpos, uint4* __restrict__ rec) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    const uint2 q = qpos[i];
+    rec[i] = make_uint4(q.x, q.y, 0u, 0u);
+  }
 }
 
 // r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
@@ -689,11 +765,13 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
                                                  const double* __restrict__ r64, double inv_norm,
                                                  float* __restrict__ r, float* __restrict__ d,
                                                  const float* __restrict__ q, const float* __restrict__ dinv,
-                                                 float* __restrict__ z, const double* __restrict__ part_pq,
+                                                 float* __restrict__ z, unsigned zs,
+                                                 const double* __restrict__ part_pq,
                                                  int npq, double* __restrict__ part, size_t part_stride,
                                                  CgStatus* st, long long it) {
   __shared__ double lds[NTP / 64];
   if (stopped_before(st, it)) return;
+  // z_i is written at z[zs * i] (the value field of the SpMV's records)
   // the p.q partials of k_ml_spmv(it) are loaded first and summed once the
   // particle loads below are in flight (npq <= kRedBlocks)
   double pq_part[kPqPerThread];
@@ -786,7 +864,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
       if (id[u] != kNoSlot) {
         const uint32_t i = id[u] & ~kOutAgg;
         const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(dv[u], rv[u], corr);
-        z[i] = zi;
+        z[static_cast<size_t>(i) * zs] = zi;
         rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
       }
     }
@@ -795,7 +873,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
         const uint32_t i = G.sorted[st0 + k];
         const float ri = r[i];
         const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
-        z[i] = zi;
+        z[static_cast<size_t>(i) * zs] = zi;
         rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
       }
   }
@@ -810,6 +888,17 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
 // See above; FIRST is it == 1 (p = z, q = A z, tolerance from r_0.r_0).  The
 // partials of k_ml_prec(it - 1) are loaded at the start and summed after the
 // gathers of the first kSpmvRows rows per thread.
+// (A z)_i and z_i: z from the records (MatFreeRec) or the array z.
+template <class Op>
+__device__ __forceinline__ float spmv_row(const Op& A, const float* __restrict__ z, size_t i, float& zi) {
+  if constexpr (std::is_same<Op, MatFreeRec>::value) {
+    return rec_apply<kIterUnroll + 1>(A, i, zi);
+  } else {
+    zi = z[i];
+    return op_apply<kIterUnroll + 1>(A, i, zi, [&](uint32_t j) { return z[j]; });
+  }
+}
+
 constexpr int kSpmvParts = 3;  // prec partials per thread and array in registers
 constexpr int kSpmvRows = 2;   // rows per thread whose A z waits in registers for beta
 
@@ -835,10 +924,7 @@ __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict
   for (int u = 0; u < kSpmvRows; u++) {
     const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
     zc[u] = sc[u] = 0.0f;
-    if (i < rw.hi) {
-      zc[u] = z[i];
-      sc[u] = op_apply<kIterUnroll + 1>(A, i, zc[u], [&](uint32_t j) { return z[j]; });
-    }
+    if (i < rw.hi) sc[u] = spmv_row(A, z, i, zc[u]);
   }
   double a = 0, b = 0;
 #pragma unroll
@@ -894,8 +980,9 @@ __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict
     if (i < rw.hi) finish(i, zc[u], sc[u]);
   }
   for (size_t i = rw.lo + threadIdx.x + static_cast<size_t>(kSpmvRows) * NT; i < rw.hi; i += NT) {
-    const float zi = z[i];
-    finish(i, zi, op_apply<kIterUnroll + 1>(A, i, zi, [&](uint32_t j) { return z[j]; }));
+    float zi;
+    const float si = spmv_row(A, z, i, zi);
+    finish(i, zi, si);
   }
   const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_pq[blockIdx.x] = sp;
@@ -1208,9 +1295,23 @@ void solve_mixed_cg(Context& c, const Op& A, double b_base, bool wide, const Sto
 }
 
 // Multilevel preconditioner: two kernels per iteration (see k_ml_prec).
-void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule& sr, long long max_iter) {
+MatFreeRec matfree_rec(const Context& c, size_t rpb) {
+  return MatFreeRec{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.zrec, c.ghost32,
+                    static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
+                                       c.qpos_inv_h),
+                    c.ppe_n, rpb};
+}
+
+void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
+  if (c.zrec_cap < n) {
+    dfree(c.zrec);
+    c.zrec = dalloc<uint4>(n);
+    c.zrec_cap = n;
+  }
+  const MatFreeRec A = matfree_rec(c, L.rpb);
+  float* const zval = reinterpret_cast<float*>(c.zrec) + 2;  // z_i at zval[4 i]
   hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
                      static_cast<const double*>(c.diag), c.dinv32);
   const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
@@ -1265,8 +1366,9 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
                        c.ml_list);
   });
   MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
-  // rows outside every cell keep r = d = z = 0
-  MPS_HIP(hipMemsetAsync(c.z32, 0, n * sizeof(float), c.stream));
+  // rows outside every cell keep r = d = z = 0; the records take the positions
+  hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
+                     c.zrec);
   MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
   MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
   check_launch();
@@ -1283,7 +1385,7 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
                        static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const uint32_t*>(c.ml_pid),
                        static_cast<const float*>(c.ml_dv), static_cast<const float4*>(c.ml_sc),
                        static_cast<const double*>(c.r), 1.0 / norm, c.r32, c.d32,
-                       static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), c.z32,
+                       static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), zval, 4u,
                        static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
   };
   double b_spmv = b_base;
@@ -1319,13 +1421,13 @@ void solve_mixed_ml(Context& c, const MatFree& A, double b_base, const StopRule&
     const long long inner = run_inner(c, limit, [&](long long it) {
       tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
         if (it == 1)
-          hipLaunchKernelGGL((k_ml_spmv<true, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+          hipLaunchKernelGGL((k_ml_spmv<true, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, A,
+                             static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
                              static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
                              c.st32, sweep_tol, it);
         else
-          hipLaunchKernelGGL((k_ml_spmv<false, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+          hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, A,
+                             static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
                              static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
                              c.st32, sweep_tol, it);
       }, it);
@@ -1372,7 +1474,8 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const double rows = n * (4 + 4 + 8.0);
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
   if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
-    solve_mixed_ml(c, A, cols + rows + n * (4.0 + 24), sr, max_iter);  // z; p, q, d read and written
+    // row length and ghost; the own record {position, z}; p, q, d read and written
+    solve_mixed_ml(c, cols + n * (4 + 4 + 16 + 24.0), sr, max_iter);
   else
     solve_mixed_cg(c, A, cols + rows + n * vec, true, sr, max_iter);
 }
@@ -1395,8 +1498,8 @@ double replay_spmv(Context& c, int kid, int reps) {
   const size_t pgrid = std::max<size_t>(1, (*c.ml_active_host + NTP / 64 - 1) / (NTP / 64));
   auto launch = [&] {
     if (kid == MPS_K_CG_SPMV32)
-      hipLaunchKernelGGL((k_ml_spmv<false, MatFree>), dim3(L.grid), dim3(NT), 0, c.stream, matfree(c, L.rpb),
-                         static_cast<const float*>(c.z32), c.pa32, c.q32, c.d32,
+      hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream,
+                         matfree_rec(c, L.rpb), static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
                          static_cast<const double*>(c.part_ml), c.part_ml_cap, static_cast<int>(pgrid),
                          c.part3b, c.st_replay, 1e-8, 2LL);
     else

@@ -203,6 +203,8 @@ struct Context {
   uint32_t* ml_off = nullptr;
   uint32_t* ml_list = nullptr;  // ... and their indices, ascending
   size_t ml_blk_cap = 0;
+  uint4* zrec = nullptr;  // multilevel SpMV records {qx, qz, z, 0} per row
+  size_t zrec_cap = 0;
   uint32_t* ml_pid = nullptr;  // packed per-slot particle ids / Jacobi factors, per-lane scales
   float* ml_dv = nullptr;
   float4* ml_sc = nullptr;
