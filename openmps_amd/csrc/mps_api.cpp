@@ -190,8 +190,8 @@ void forward(Context& c, double dt) {
   c.dt = dt;
   c.t += dt;
   search(c);
-  density(c, /*ecs=*/true, /*savex=*/false, /*dens=*/true);
-  explicit_forces(c);
+  // density + HS/ECS and the explicit accelerations in one list walk, then the update
+  density_and_explicit(c);
   // second density pass + SaveX + PPE assembly (its count pass fused with the density pass)
   density_save_and_assemble(c);
   ppe_solve(c);

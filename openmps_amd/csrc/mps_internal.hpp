@@ -332,6 +332,9 @@ void search(Context& c);                 // mps_search.hip
 void density(Context& c, bool ecs, bool savex, bool dens);  // mps_particles.hip
 void density_variation(Context& c, double* out_dev);
 void explicit_forces(Context& c);
+// density(c, true, false, true) followed by explicit_forces(c), the two list
+// walks fused into one kernel (same results)
+void density_and_explicit(Context& c);
 void wall_targets_from_callback(Context& c);
 void pressure_writeback(Context& c);
 void pressure_gradient(Context& c);

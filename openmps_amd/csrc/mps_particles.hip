@@ -211,6 +211,79 @@ __global__ void __launch_bounds__(kBlock) k_explicit(size_t n, const double* __r
   az[i] = vz + k.gz;
 }
 
+// The first ComputeNeighborDensities pass with HS + ECS (Computer.hpp:780-910)
+// and the acceleration pass of ComputeExplicitForces (:914-990) in one walk of
+// the neighbour list: the accelerations read the unmoved positions and
+// velocities and only the particle's own nWithoutSpp (for the SPP ghost),
+// which this walk has just summed.  Every value is computed as k_density and
+// k_explicit compute it (same expressions, same list order).
+__global__ void __launch_bounds__(kBlock) k_density_explicit(size_t n, const double* __restrict__ x,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ u,
+                                                             const double* __restrict__ w,
+                                                             const uint8_t* __restrict__ type, NV nb,
+                                                             Consts k, double* __restrict__ nd,
+                                                             double* __restrict__ nws, double* __restrict__ ecs,
+                                                             double* __restrict__ ax, double* __restrict__ az) {
+  const size_t i = thread_particle();
+  if (i >= n) return;
+  const uint8_t ti = type[i];
+  if (ti == kDummy || ti == kDisabled) {
+    nws[i] = k.n0;
+    return;
+  }
+  const bool fluid = (ti == kFluid);
+  const double xi = x[i], zi = z[i], ui = u[i], wi = w[i];
+  double s = 0.0, h = 0.0;                     // density, HS
+  double vx = 0, vz = 0, sx = 0, sz = 0;       // viscosity, ghost direction (fluid)
+  struct F {
+    double x, z, u, w;
+    uint8_t t;
+  };
+  walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
+                      [&](uint32_t j) { return F{x[j], z[j], u[j], w[j], type[j]}; },
+                      [&](uint32_t, const F& f) {
+                        const double dx = f.x - xi, dz = f.z - zi;
+                        const double r2 = dx * dx + dz * dz;
+                        if (r2 < k.re2) {
+                          const double r = sqrt(r2);
+                          s += weight(r, k.re);
+                          const double du = f.u - ui, dw = f.w - wi;
+                          h += (dx * du + dz * dw) / (r * r * r);
+                        }
+                        if (fluid) {
+                          if (r2 < k.rl2 && f.t != kDummy) {
+                            const double r = sqrt(r2);
+                            const double c = k.nu * 3.0 * k.rl / k.n0l / (r * r * r);
+                            vx += c * (f.u - ui);
+                            vz += c * (f.w - wi);
+                          }
+                          if (r2 < k.re2) {
+                            const double wg = weight(sqrt(r2), k.re);
+                            sx += wg * dx;
+                            sz += wg * dz;
+                          }
+                        }
+                      });
+  const double thisN = (s < k.n0) ? k.n0 : s;
+  nws[i] = s;
+  nd[i] = thisN;
+  const double speed = (s < k.n0) ? 0.0 : -k.re * h;
+  const double e = (thisN - k.n0) / k.n0;
+  ecs[i] = fabs(e) * speed + fabs(speed) * e;
+  if (!fluid) return;
+  double gxp, gzp;
+  if (ghost_pos(xi, zi, sx, sz, s, k.n0, k.re, gxp, gzp)) {
+    const double rx = xi - gxp, rz = zi - gzp;
+    const double r = sqrt(rx * rx + rz * rz);
+    const double c = k.nu * 3.0 * k.rl / k.n0l / (r * r * r);
+    vx += c * (ui - ui);
+    vz += c * (wi - wi);
+  }
+  ax[i] = vx + k.gx;
+  az[i] = vz + k.gz;
+}
+
 // ComputeExplicitForces second pass (Computer.hpp:996-1020).
 __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __restrict__ x,
                                                             double* __restrict__ z, double* __restrict__ u,
@@ -432,16 +505,11 @@ void density_variation(Context& c, double* out_dev) {
   check_launch();
 }
 
-void explicit_forces(Context& c) {
+namespace {
+
+// Wall targets and the integration of ComputeExplicitForces (accelerations in dux, duz).
+void explicit_update(Context& c) {
   const size_t n = c.n;
-  if (n == 0) return;
-  const Consts k = consts(c);
-  tracked(c, MPS_K_EXPLICIT, n * (32.0 + 1 + 4 + 8 + 16) + 4.0 * c.nbr_total, [&] {
-    hipLaunchKernelGGL(k_explicit, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
-                       static_cast<const uint8_t*>(c.type), nview(c), k,
-                       static_cast<const double*>(c.nws), c.dux, c.duz);
-  });
-  check_launch();
   // positionWallPre(t, dt) then positionWall(i, t, dt) for every non-fluid i
   // (Computer.hpp:993, :1015), after the accelerations and before the update.
   if (c.wall_fn || c.wall_pre) wall_targets_from_callback(c);
@@ -453,6 +521,34 @@ void explicit_forces(Context& c) {
                        c.dt);
   });
   check_launch();
+}
+
+}  // namespace
+
+void explicit_forces(Context& c) {
+  const size_t n = c.n;
+  if (n == 0) return;
+  const Consts k = consts(c);
+  tracked(c, MPS_K_EXPLICIT, n * (32.0 + 1 + 4 + 8 + 16) + 4.0 * c.nbr_total, [&] {
+    hipLaunchKernelGGL(k_explicit, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
+                       static_cast<const uint8_t*>(c.type), nview(c), k,
+                       static_cast<const double*>(c.nws), c.dux, c.duz);
+  });
+  check_launch();
+  explicit_update(c);
+}
+
+void density_and_explicit(Context& c) {
+  const size_t n = c.n;
+  if (n == 0) return;
+  // x, z, u, w, type read; n, nWithoutSpp, ECS, accelerations written
+  tracked(c, MPS_K_DENSITY_ECS, n * (32.0 + 1 + 4 + 24 + 16) + 4.0 * c.nbr_total, [&] {
+    hipLaunchKernelGGL(k_density_explicit, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u,
+                       c.w, static_cast<const uint8_t*>(c.type), nview(c), consts(c), c.nd, c.nws, c.ecs,
+                       c.dux, c.duz);
+  });
+  check_launch();
+  explicit_update(c);
 }
 
 void wall_targets_from_callback(Context& c) {
