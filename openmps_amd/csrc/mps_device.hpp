@@ -22,6 +22,23 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t i, uint32_t k, uint32_t kmax
   return (static_cast<size_t>(i >> 6) * kmax + k) * 64 + (i & 63u);
 }
 
+// SPP ghost particle, Computer.hpp:657-693.  (sx, sz) is sum W(|dx|) dx over
+// the r_e-filtered neighbours; the ghost exists when nWithoutSpp_i < n0 and
+// the centroid offset exceeds DBL_EPSILON.
+__device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, double sz, double nws_i,
+                                          double n0, double re, double& gx, double& gz) {
+  sx /= n0;
+  sz /= n0;
+  if (!(nws_i < n0)) return false;
+  const double rg = sqrt(sx * sx + sz * sz);
+  if (!(rg > DBL_EPSILON)) return false;
+  const double wspp = n0 - nws_i;
+  const double rspp = re / (wspp + 1);
+  gx = xi - (rspp / rg) * sx;
+  gz = zi - (rspp / rg) * sz;
+  return true;
+}
+
 // Walks particle i's m-entry neighbour list in list order, U neighbours per
 // round trip to memory: the U ids, then load(j) for all U of them (the
 // gathered fields of neighbour j, returned as a T), then use(j, T) in list

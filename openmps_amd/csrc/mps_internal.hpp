@@ -335,6 +335,7 @@ void explicit_forces(Context& c);
 // density(c, true, false, true) followed by explicit_forces(c), the two list
 // walks fused into one kernel (same results)
 void density_and_explicit(Context& c);
+void explicit_update(Context& c);  // wall targets and the integration of the explicit stage
 void wall_targets_from_callback(Context& c);
 void pressure_writeback(Context& c);
 void pressure_gradient(Context& c);

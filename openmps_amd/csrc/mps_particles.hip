@@ -50,23 +50,6 @@ __device__ __forceinline__ size_t thread_particle() {
   return blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
 }
 
-// SPP ghost particle, Computer.hpp:657-693.  (sx, sz) is sum W(|dx|) dx over
-// the r_e-filtered neighbours; the ghost exists when nWithoutSpp_i < n0 and
-// the centroid offset exceeds DBL_EPSILON.
-__device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, double sz, double nws_i,
-                                          double n0, double re, double& gx, double& gz) {
-  sx /= n0;
-  sz /= n0;
-  if (!(nws_i < n0)) return false;
-  const double rg = sqrt(sx * sx + sz * sz);
-  if (!(rg > DBL_EPSILON)) return false;
-  const double wspp = n0 - nws_i;
-  const double rspp = re / (wspp + 1);
-  gx = xi - (rspp / rg) * sx;
-  gz = zi - (rspp / rg) * sz;
-  return true;
-}
-
 // ComputeNeighborDensities (Computer.hpp:780-833), fused with
 // NeighborDensityVariationSpeed + ComputeErrorCorrection (:838-910) and with
 // SaveX (:1025-1039) as the template flags ask.
@@ -505,11 +488,10 @@ void density_variation(Context& c, double* out_dev) {
   check_launch();
 }
 
-namespace {
-
 // Wall targets and the integration of ComputeExplicitForces (accelerations in dux, duz).
 void explicit_update(Context& c) {
   const size_t n = c.n;
+  if (n == 0) return;
   // positionWallPre(t, dt) then positionWall(i, t, dt) for every non-fluid i
   // (Computer.hpp:993, :1015), after the accelerations and before the update.
   if (c.wall_fn || c.wall_pre) wall_targets_from_callback(c);
@@ -522,8 +504,6 @@ void explicit_update(Context& c) {
   });
   check_launch();
 }
-
-}  // namespace
 
 void explicit_forces(Context& c) {
   const size_t n = c.n;

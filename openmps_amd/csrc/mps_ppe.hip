@@ -37,20 +37,6 @@ struct PpeConsts {
   double qx0, qz0, qinv;  // fixed-point positions: q = (x - x0) * qinv mod 2^32
 };
 
-__device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, double sz, double nws_i,
-                                          double n0, double re, double& gx, double& gz) {
-  sx /= n0;
-  sz /= n0;
-  if (!(nws_i < n0)) return false;
-  const double rg = sqrt(sx * sx + sz * sz);
-  if (!(rg > DBL_EPSILON)) return false;
-  const double wspp = n0 - nws_i;
-  const double rspp = re / (wspp + 1);
-  gx = xi - (rspp / rg) * sx;
-  gz = zi - (rspp / rg) * sz;
-  return true;
-}
-
 constexpr int kWalk = 4;  // neighbours whose loads are issued together (walk_list)
 
 // b and x0 (Computer.hpp:1185-1221), row lengths and slice widths.
