@@ -138,7 +138,7 @@ void free_all(Context& c) {
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
   dfree(c.feats); dfree(c.ypred);
-  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16); dfree(c.col16);
+  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
   dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active); dfree(c.st_replay);
@@ -216,7 +216,7 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   dfree(c.row_len); dfree(c.slice_width); dfree(c.slice_off); dfree(c.diag); dfree(c.diag32);
   dfree(c.b); dfree(c.sol); dfree(c.r); dfree(c.pa); dfree(c.q);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
-  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.w16); dfree(c.off16);
+  dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide);
   dfree(c.g32a); dfree(c.g32b); dfree(c.r32); dfree(c.z32); dfree(c.q32);
   c.row_len = dalloc<uint32_t>(m);
   c.slice_width = dalloc<uint32_t>(ns);
@@ -239,8 +239,6 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.q32 = dalloc<float>(m);
   c.qpos = dalloc<uint2>(m);
   c.slice_wide = dalloc<uint8_t>(ns);
-  c.w16 = dalloc<uint32_t>(ns);
-  c.off16 = dalloc<uint64_t>(ns + 1);
   MPS_HIP(hipMemsetAsync(c.sol, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa, 0, m * 8, c.stream));
   MPS_HIP(hipMemsetAsync(c.pa32, 0, m * 4, c.stream));
@@ -249,7 +247,6 @@ void ensure_ppe_vectors(Context& c, size_t n) {
   c.have_ppe = false;
   c.have_ppe32 = false;
   c.ppe_from_particles = false;
-  c.have_col16 = false;
 }
 
 }  // namespace mps

@@ -106,7 +106,7 @@ __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
 struct MatFree {
   const uint64_t* __restrict__ off;
   const uint32_t* __restrict__ col;
-  const uint64_t* __restrict__ off16;
+  const uint64_t* __restrict__ off16;  // slice offsets of col16 (the SELL ones: same slots)
   const int16_t* __restrict__ col16;
   const uint8_t* __restrict__ wide;
   const uint32_t* __restrict__ len;
@@ -1296,7 +1296,7 @@ void solve_mixed_cg(Context& c, const Op& A, double b_base, bool wide, const Sto
 
 // Multilevel preconditioner: two kernels per iteration (see k_ml_prec).
 MatFreeRec matfree_rec(const Context& c, size_t rpb) {
-  return MatFreeRec{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.zrec, c.ghost32,
+  return MatFreeRec{c.slice_off, c.scol, c.slice_off, c.col16, c.slice_wide, c.row_len, c.zrec, c.ghost32,
                     static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
                                        c.qpos_inv_h),
                     c.ppe_n, rpb};
@@ -1449,7 +1449,7 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
 }
 
 MatFree matfree(const Context& c, size_t rpb) {
-  return MatFree{c.slice_off, c.scol, c.off16, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
+  return MatFree{c.slice_off, c.scol, c.slice_off, c.col16, c.slice_wide, c.row_len, c.qpos, c.ghost32,
                  static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
                                     c.qpos_inv_h),
                  c.ppe_n, rpb};
@@ -1466,7 +1466,6 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), false, sr, max_iter);
     return;
   }
-  ppe_compress_columns(c);
   const MatFree A = matfree(c, L.rpb);
   // int16 columns of narrow slices, u32 of wide ones (2 more bytes each,
   // added by the solvers); row length, ghost and position of every row; slice tables

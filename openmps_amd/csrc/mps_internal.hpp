@@ -161,15 +161,11 @@ struct Context {
   float* ghost32 = nullptr;  // SPP ghost share of the diagonal (1 on rows without unknowns)
   uint2* qpos = nullptr;     // fixed-point positions of the matrix-free operator
   double qpos_inv_h = 0;     // their scale: q = (x - min) / h
-  // compressed columns of the matrix-free FP32 SpMV: slices whose column
-  // offsets j - i all fit in int16 store them in col16 (offset off16[s]);
-  // the others ("wide") keep the SELL u32 columns
+  // compressed columns of the matrix-free FP32 SpMV: the fill stores every
+  // column also as the int16 offset j - i in the same SELL slot; a slice with
+  // an offset outside int16 ("wide") is read through the u32 columns
   uint8_t* slice_wide = nullptr;
-  uint32_t* w16 = nullptr;
-  uint64_t* off16 = nullptr;
-  int16_t* col16 = nullptr;
-  size_t col16_cap = 0;
-  bool have_col16 = false;
+  int16_t* col16 = nullptr;  // sell_cap entries, beside scol
   double mf_nnz_wide = 0;
   bool mf_wide_pending = false;  // mf_nnz_wide still in ppe_counters_host[2]
   unsigned long long* ppe_counters = nullptr;       // [0] off-diagonal nnz
@@ -345,8 +341,7 @@ void ppe_assemble(Context& c);           // mps_ppe.hip
 // and the PPE count walk fused into one kernel (same results)
 void density_save_and_assemble(Context& c);
 void ppe_make_fp32(Context& c);
-void ppe_compress_columns(Context& c);  // mps_ppe.hip: builds slice_wide / off16 / col16
-double mf_nnz_wide(Context& c);         // its wide nnz; valid once the stream was synchronised
+double mf_nnz_wide(Context& c);  // mps_ppe.hip: nnz of the wide slices; valid once the stream was synchronised
 double replay_spmv(Context& c, int kid, int reps);  // mps_cg.hip: mean ms of one launch
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);

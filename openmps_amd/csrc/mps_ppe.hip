@@ -7,9 +7,10 @@
 //                off-diagonal count of row i and, per slice, the wavefront
 //                maximum of those counts;
 //   scan         slice widths -> slice offsets;
-//   k_ppe_fill   HL off-diagonals 3 r_e / (n0 r^3) in neighbour-list order,
-//                zero padding, and the diagonal -sum(a_ij) including the SPP
-//                ghost (Computer.hpp:1290-1292).
+//   k_ppe_fill   HL off-diagonals 3 r_e / (n0 r^3) with ascending columns
+//                (u32, and int16 offsets j - i for the FP32 operator), zero
+//                padding, and the diagonal -sum(a_ij) including the SPP ghost
+//                (Computer.hpp:1290-1292).
 //
 // The reference builds per-row vectors in parallel and scatters them serially
 // into a uBLAS compressed matrix before copying to ViennaCL CSR
@@ -222,26 +223,22 @@ struct AggSums {
 
 constexpr int kStage = 40;  // emitted columns per row staged in LDS for the merge
 
-__global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
-                                                     const double* __restrict__ z,
-                                                     const uint8_t* __restrict__ type, NV nb,
-                                                     PpeConsts k, const double* __restrict__ nws,
-                                                     const uint64_t* __restrict__ slice_off,
-                                                     const uint32_t* __restrict__ slice_width,
-                                                     double* __restrict__ val, uint32_t* __restrict__ col,
-                                                     double* __restrict__ diag,
-                                                     float* __restrict__ ghost32, AggSums ag) {
-  // each thread's emitted columns in list order, [slot][thread] (conflict-free)
-  __shared__ uint32_t stage[kStage][kBlock];
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
+// One row of k_ppe_fill; kk returns its stored non-zeros.
+__device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __restrict__ x,
+                                         const double* __restrict__ z, const uint8_t* __restrict__ type, NV nb,
+                                         const PpeConsts& k, const double* __restrict__ nws,
+                                         const uint64_t* __restrict__ slice_off,
+                                         const uint32_t* __restrict__ slice_width, double* __restrict__ val,
+                                         uint32_t* __restrict__ col, int16_t* __restrict__ col16,
+                                         double* __restrict__ diag, float* __restrict__ ghost32, const AggSums& ag,
+                                         uint32_t (*stage)[kBlock], bool& fits, uint32_t& kk) {
+  (void)n;
   const int t = threadIdx.x;
   const size_t base = slice_off[i >> 6] + (i & 63);
   const uint32_t width = slice_width[i >> 6];
   const uint32_t self = static_cast<uint32_t>(i);
   const uint8_t ti = type[i];
-  uint32_t kk = 0;
+  kk = 0;
   double d, g = 0.0;
   double a0 = 0, a1 = 0, a2 = 0;  // aggregate row sums (AggSums)
   if (ti == kDummy || ti == kDisabled) {
@@ -309,6 +306,9 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
       const double a = 3 * k.rl / k.n0l / (r * r * r);
       val[o] = a;
       col[o] = j;
+      const long long off = static_cast<long long>(j) - static_cast<long long>(self);
+      fits = fits && off >= -32768 && off <= 32767;
+      col16[o] = static_cast<int16_t>(off);
       kk++;
       if (ki != kNoCell) {
         const uint32_t kj = ag.key[j];
@@ -373,11 +373,14 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
       }
     }
   }
-  for (; kk < width; kk++) {
-    const size_t o = base + static_cast<size_t>(kk) * 64;
+  const uint32_t stored = kk;
+  for (uint32_t pad = kk; pad < width; pad++) {
+    const size_t o = base + static_cast<size_t>(pad) * 64;
     val[o] = 0.0;
     col[o] = self;
+    col16[o] = 0;
   }
+  kk = stored;
   diag[i] = d;
   ghost32[i] = static_cast<float>(g);
   if (ag.s0) {
@@ -387,54 +390,34 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   }
 }
 
-// Per slice: whether every stored column offset j - i fits in int16; the
-// int16 widths of the narrow slices (0 for wide ones) and the wide nnz.
-__global__ void __launch_bounds__(kBlock) k_slice_range(size_t n, const uint64_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ width,
-                                                        const uint32_t* __restrict__ row_len,
-                                                        const uint32_t* __restrict__ col,
-                                                        uint8_t* __restrict__ wide,
-                                                        uint32_t* __restrict__ w16,
-                                                        unsigned long long* nnz_wide) {
-  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  int far = 0;
-  uint32_t len = 0;
-  if (i < n) {
-    len = row_len[i];
-    const size_t base = off[i >> 6] + (i & 63);
-    for (uint32_t k = 0; k < len; k++) {
-      const long long d = static_cast<long long>(col[base + static_cast<size_t>(k) * 64]) - static_cast<long long>(i);
-      if (d > 32767 || d < -32768) far = 1;
-    }
-  }
-  const unsigned anyfar = wave_max_u32(static_cast<unsigned>(far));
-  const double wsum = wave_sum(static_cast<double>(len));
+__global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ z,
+                                                     const uint8_t* __restrict__ type, NV nb,
+                                                     PpeConsts k, const double* __restrict__ nws,
+                                                     const uint64_t* __restrict__ slice_off,
+                                                     const uint32_t* __restrict__ slice_width,
+                                                     double* __restrict__ val, uint32_t* __restrict__ col,
+                                                     int16_t* __restrict__ col16, uint8_t* __restrict__ slice_wide,
+                                                     unsigned long long* nnz_wide, double* __restrict__ diag,
+                                                     float* __restrict__ ghost32, AggSums ag) {
+  // each thread's emitted columns in list order, [slot][thread] (conflict-free)
+  __shared__ uint32_t stage[kStage][kBlock];
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  // int16 column offsets j - i beside the u32 columns (same slots): a slice
+  // (one wavefront of rows) is "wide" when any of its offsets does not fit
+  bool fits = true;
+  uint32_t kk = 0;
+  if (i < n) fill_row(n, i, x, z, type, nb, k, nws, slice_off, slice_width, val, col, col16, diag, ghost32, ag,
+                      stage, fits, kk);
+  const unsigned long long far = __ballot(!fits);
+  const double wsum = wave_sum(static_cast<double>(kk));
   if ((threadIdx.x & 63) == 0) {
-    const size_t s = i >> 6;
-    if (s * 64 < n) {
-      wide[s] = static_cast<uint8_t>(anyfar);
-      w16[s] = anyfar ? 0u : width[s];
-      if (anyfar) atomicAdd(nnz_wide, static_cast<unsigned long long>(wsum));
+    const size_t sl = i >> 6;
+    if (sl * 64 < n) {
+      slice_wide[sl] = static_cast<uint8_t>(far != 0);
+      if (far) atomicAdd(nnz_wide, static_cast<unsigned long long>(wsum));
     }
-  }
-}
-
-// int16 column offsets of the narrow slices (padding: offset 0, never read).
-__global__ void __launch_bounds__(kBlock) k_fill16(size_t n, const uint64_t* __restrict__ off,
-                                                   const uint32_t* __restrict__ width,
-                                                   const uint32_t* __restrict__ col,
-                                                   const uint8_t* __restrict__ wide,
-                                                   const uint64_t* __restrict__ off16,
-                                                   int16_t* __restrict__ col16) {
-  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const size_t s = i >> 6;
-  if (wide[s]) return;
-  const size_t base = off[s] + (i & 63), b16 = off16[s] + (i & 63);
-  const uint32_t w = width[s];
-  for (uint32_t k = 0; k < w; k++) {
-    const long long d = static_cast<long long>(col[base + static_cast<size_t>(k) * 64]) - static_cast<long long>(i);
-    col16[b16 + static_cast<size_t>(k) * 64] = static_cast<int16_t>(d);
   }
 }
 
@@ -451,9 +434,11 @@ void ensure_sell_capacity(Context& c, size_t nslices, size_t slots) {
     dfree(c.sval);
     dfree(c.scol);
     dfree(c.sval32);
+    dfree(c.col16);
     const size_t cap = slots + slots / 8 + 64;
     c.sval = dalloc<double>(cap);
     c.scol = dalloc<uint32_t>(cap);
+    c.col16 = dalloc<int16_t>(cap);
     c.sell_cap = cap;
     c.have_ppe32 = false;
   }
@@ -495,19 +480,24 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   // row sums of the multilevel preconditioner's aggregates, when that solver will run
   const bool want_sums = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_MULTILEVEL;
   const AggSums ag{c.key, static_cast<uint32_t>(c.gnz), want_sums ? c.z32 : nullptr, c.r32, c.d32};
-  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 12.0 +
+  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 14.0 +
                        (want_sums ? n * 12.0 + c.nnz_offdiag * 4.0 : 0.0);
   tracked(c, MPS_K_PPE_FILL, bytes, [&] {
     hipLaunchKernelGGL(k_ppe_fill, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                        static_cast<const uint64_t*>(c.slice_off),
-                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.diag, c.ghost32, ag);
+                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
+                       c.ppe_counters + 1, c.diag, c.ghost32, ag);
   });
   check_launch();
+  // the wide nnz (byte accounting only) reaches the host with the solver's
+  // next synchronisation (mf_nnz_wide)
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 2, c.ppe_counters + 1, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
+  c.mf_wide_pending = true;
   c.ml_rowsums_ready = want_sums;
   c.ppe_from_particles = true;
   c.ppe_search_serial = c.search_serial;
-  c.have_col16 = false;
 }
 
 }  // namespace
@@ -542,37 +532,6 @@ void density_save_and_assemble(Context& c) {
   });
   check_launch();
   ppe_finish(c, k);
-}
-
-void ppe_compress_columns(Context& c) {
-  const size_t n = c.ppe_n, ns = c.nslices;
-  if (n == 0 || c.have_col16) return;
-  if (c.col16_cap < c.sell_slots) {
-    dfree(c.col16);
-    c.col16_cap = c.sell_cap;
-    c.col16 = dalloc<int16_t>(c.col16_cap);
-  }
-  MPS_HIP(hipMemsetAsync(c.ppe_counters + 1, 0, sizeof(unsigned long long), c.stream));
-  tracked(c, MPS_K_OTHER, n * 4.0 + c.nnz_offdiag * 4.0, [&] {
-    hipLaunchKernelGGL(k_slice_range, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
-                       static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.slice_width),
-                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.scol),
-                       c.slice_wide, c.w16, c.ppe_counters + 1);
-  });
-  scan_slices(c, c.w16, ns, c.off16);
-  tracked(c, MPS_K_OTHER, c.sell_slots * 6.0, [&] {
-    hipLaunchKernelGGL(k_fill16, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
-                       static_cast<const uint64_t*>(c.slice_off), static_cast<const uint32_t*>(c.slice_width),
-                       static_cast<const uint32_t*>(c.scol), static_cast<const uint8_t*>(c.slice_wide),
-                       static_cast<const uint64_t*>(c.off16), c.col16);
-  });
-  check_launch();
-  // the wide nnz (byte accounting only) reaches the host with the solver's
-  // next synchronisation (mf_nnz_wide)
-  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 2, c.ppe_counters + 1, sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, c.stream));
-  c.mf_wide_pending = true;
-  c.have_col16 = true;
 }
 
 double mf_nnz_wide(Context& c) {
