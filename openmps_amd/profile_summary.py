@@ -35,6 +35,9 @@ KERNEL_IDS = {
     "k_gradient": "gradient", "k_ds": "ds",
     "k_ml_spmv<false, mps::(anonymous namespace)::MatFree>": "cg_spmv32",
     "k_ml_spmv<true, mps::(anonymous namespace)::MatFree>": "cg_spmv32",
+    "k_ml_spmv<false, mps::(anonymous namespace)::MatFreeRec>": "cg_spmv32",
+    "k_ml_spmv<true, mps::(anonymous namespace)::MatFreeRec>": "cg_spmv32",
+    "k_density_explicit": "density_ecs", "k_density_ppe_count": "density_save",
     "k_ml_prec": "ml_prec",
     "k_cgf_iter<mps::(anonymous namespace)::MatFree>": "cg_iter32",
 }
