@@ -103,6 +103,22 @@ typedef void (*mps_wall_fn)(void* user, double t, double dt, size_t count, const
                             double* x_out, double* z_out);
 int mps_set_wall_callbacks(mps_ctx* ctx, mps_wall_fn wall, mps_wall_pre_fn pre, void* user);
 
+/* Device-side wall kinematics, in place of positionWall callbacks
+ * (Computer.hpp:1013-1019) for prescribed motions; no counterpart in the
+ * reference, whose walls move only through the callbacks.  Each step the
+ * non-fluid particle i of group g = group[i] >= 1 is moved to its position
+ * at mps_add_particles plus
+ *   (vx t + ax sin(omega t + phase), vz t + az sin(omega t + phase))
+ * of motions[g - 1], t the clock after the step's advance (the t
+ * positionWall receives), and its velocity becomes the displacement over dt,
+ * as in the reference; group 0 (and particles added later) stays put.
+ * Clears the wall callbacks; n_motions = 0 switches the motion off.  group
+ * has one entry per particle present (mps_size). */
+typedef struct mps_wall_motion {
+  double vx, vz, ax, az, omega, phase;
+} mps_wall_motion;
+int mps_set_wall_motion(mps_ctx* ctx, size_t n_motions, const mps_wall_motion* motions, const uint8_t* group);
+
 /* Computer::ForwardTime(dt), Computer.hpp:1700-1742 */
 int mps_forward_time(mps_ctx* ctx, double dt);
 /* Computer::ForwardTime() with Computer::DetermineDt, Computer.hpp:1745-1751, :759-777 */

@@ -200,6 +200,11 @@ class Computer final {
 
   // Extensions: pressure solver selection and statistics of the last solve.
   void SetSolver(const mps_solver_opts& o) { check(mps_set_solver(s_->ctx, &o), "SetSolver"); }
+  // Device-side wall kinematics in place of the positionWall callbacks
+  // (mps_set_wall_motion): group has one entry per particle.
+  void SetWallMotion(const std::vector<mps_wall_motion>& motions, const std::vector<uint8_t>& group) {
+    check(mps_set_wall_motion(s_->ctx, motions.size(), motions.data(), group.data()), "SetWallMotion");
+  }
   mps_solve_stats SolveStats() const {
     mps_solve_stats st{};
     mps_get_solve_stats(s_->ctx, &st);

@@ -59,6 +59,10 @@ class SolveStats(ctypes.Structure):
                 ("fallback", _I64)]
 
 
+class WallMotion(ctypes.Structure):
+    _fields_ = [("vx", _D), ("vz", _D), ("ax", _D), ("az", _D), ("omega", _D), ("phase", _D)]
+
+
 class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("launches", _I64), ("total_ms", _D),
                 ("total_bytes", _D)]
@@ -111,6 +115,7 @@ SIGNATURES = {
     "mps_set_profiling": (_I, [_P, _I]),
     "mps_get_kernel_stat": (_I, [_P, _I, ctypes.POINTER(KernelStat)]),
     "mps_reset_kernel_stats": (_I, [_P]),
+    "mps_set_wall_motion": (_I, [_P, _SZ, ctypes.POINTER(WallMotion), ctypes.POINTER(ctypes.c_uint8)]),
     "mps_replay_spmv": (_I, [_P, _I, _I, ctypes.POINTER(_D)]),
     "mps_device_math_check": (_I, [_SZ, _P, _P, _P, _P]),
     "mps_device_copy_probe": (_I, [_SZ, ctypes.POINTER(_D)]),

@@ -164,6 +164,19 @@ class Computer:
         self._wall_cbs = (fw, fp)  # keep the thunks alive
         _raise(L.lib().mps_set_wall_callbacks(self._h, fw, fp, None), self._h)
 
+    def set_wall_motion(self, motions, group) -> None:
+        """Device-side wall kinematics (mps_set_wall_motion): non-fluid
+        particle i of group g = group[i] >= 1 moves to its initial position plus
+        (vx t + ax sin(omega t + phase), vz t + az sin(omega t + phase)) of
+        motions[g - 1] (dicts with those keys; missing keys are 0).  An empty
+        `motions` switches the motion off."""
+        m = (L.WallMotion * max(len(motions), 1))()
+        for k, d in enumerate(motions):
+            m[k] = L.WallMotion(*(float(d.get(f, 0.0)) for f, _ in L.WallMotion._fields_))
+        g = np.ascontiguousarray(group, dtype=np.uint8)
+        _raise(L.lib().mps_set_wall_motion(self._h, len(motions), m,
+                                           g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), self._h)
+
     # -- reference interface ----------------------------------------------------
     def AddParticles(self, particles) -> None:
         a = _as_arrays(particles)

@@ -128,7 +128,7 @@ void require_neighbors(const Context& c) {
 void free_all(Context& c) {
   dfree(c.x); dfree(c.z); dfree(c.u); dfree(c.w); dfree(c.p); dfree(c.nd); dfree(c.type);
   dfree(c.nws); dfree(c.ecs); dfree(c.ox); dfree(c.oz); dfree(c.dux); dfree(c.duz);
-  dfree(c.wall_x); dfree(c.wall_z); dfree(c.hs_out);
+  dfree(c.wall_x); dfree(c.wall_z); dfree(c.hs_out); dfree(c.wall_group); dfree(c.wall_motion);
   dfree(c.key); dfree(c.sorted); dfree(c.nbr); dfree(c.ncount);
   dfree(c.cell_count); dfree(c.cell_start); dfree(c.cell_fill); dfree(c.scan_tmp); dfree(c.flags);
   dfree(c.nbr_total_dev); dfree(c.ppe_counters);
@@ -405,6 +405,7 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
       mps::grow(c.duz, old, m, c.stream);
       mps::grow(c.wall_x, old, m, c.stream);
       mps::grow(c.wall_z, old, m, c.stream);
+      if (c.wall_group) mps::grow(c.wall_group, old, m, c.stream);
       mps::grow(c.hs_out, 0, m, c.stream);
       mps::grow(c.key, 0, m, c.stream);
       mps::grow(c.sorted, 0, m, c.stream);
@@ -425,6 +426,8 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
     MPS_HIP(hipMemcpyAsync(c.type + old, type, count, h2d, c.stream));
     MPS_HIP(hipMemcpyAsync(c.wall_x + old, x, count * 8, h2d, c.stream));
     MPS_HIP(hipMemcpyAsync(c.wall_z + old, z, count * 8, h2d, c.stream));
+    // particles added while device kinematics are on stay put (group 0)
+    if (c.wall_group) MPS_HIP(hipMemsetAsync(c.wall_group + old, 0, count, c.stream));
     MPS_HIP(hipStreamSynchronize(c.stream));
     c.n = nn;
     c.have_ppe = false;
@@ -485,11 +488,35 @@ int mps_set_time(mps_ctx* ctx, double t, double dt) {
   return MPS_OK;
 }
 
+int mps_set_wall_motion(mps_ctx* ctx, size_t n_motions, const mps_wall_motion* motions, const uint8_t* group) {
+  if (n_motions > 255 || (n_motions > 0 && (!motions || !group))) return MPS_ERR_BAD_ARG;
+  return mps::guard(ctx, [&](Context& c) {
+    mps::sync(c);
+    c.wall_motions = 0;
+    if (n_motions == 0) return;
+    for (size_t i = 0; i < c.n; i++)
+      if (group[i] > n_motions) throw Failure{MPS_ERR_BAD_ARG, "wall group without a motion"};
+    if (!c.wall_group) {
+      c.wall_group = mps::dalloc<uint8_t>(std::max<size_t>(c.alloc_n, 1));
+      MPS_HIP(hipMemset(c.wall_group, 0, std::max<size_t>(c.alloc_n, 1)));
+    }
+    mps::dfree(c.wall_motion);
+    c.wall_motion = mps::dalloc<mps_wall_motion>(n_motions);
+    MPS_HIP(hipMemcpy(c.wall_motion, motions, n_motions * sizeof(mps_wall_motion), hipMemcpyHostToDevice));
+    if (c.n) MPS_HIP(hipMemcpy(c.wall_group, group, c.n, hipMemcpyHostToDevice));
+    c.wall_motions = n_motions;
+    c.wall_fn = nullptr;
+    c.wall_pre = nullptr;
+    c.wall_user = nullptr;
+  });
+}
+
 int mps_set_wall_callbacks(mps_ctx* ctx, mps_wall_fn wall, mps_wall_pre_fn pre, void* user) {
   if (!ctx) return MPS_ERR_BAD_ARG;
   ctx->c.wall_fn = wall;
   ctx->c.wall_pre = pre;
   ctx->c.wall_user = user;
+  ctx->c.wall_motions = 0;  // the callbacks replace any device kinematics
   return MPS_OK;
 }
 

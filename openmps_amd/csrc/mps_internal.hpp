@@ -235,6 +235,9 @@ struct Context {
   mps_wall_fn wall_fn = nullptr;
   mps_wall_pre_fn wall_pre = nullptr;
   void* wall_user = nullptr;
+  uint8_t* wall_group = nullptr;        // device wall kinematics: group of each particle (alloc_n)
+  mps_wall_motion* wall_motion = nullptr;  // its motions (device)
+  size_t wall_motions = 0;
 
   // measurement
   bool profiling = false;

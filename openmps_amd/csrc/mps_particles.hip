@@ -268,6 +268,9 @@ __global__ void __launch_bounds__(kBlock) k_density_explicit(size_t n, const dou
 }
 
 // ComputeExplicitForces second pass (Computer.hpp:996-1020).
+// Non-fluid targets: wall_x / wall_z (positions at AddParticles, or the
+// callbacks' targets), displaced by the group's prescribed motion at time t
+// when device kinematics are on (group != nullptr).
 __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __restrict__ x,
                                                             double* __restrict__ z, double* __restrict__ u,
                                                             double* __restrict__ w,
@@ -275,7 +278,10 @@ __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __
                                                             const double* __restrict__ ax,
                                                             const double* __restrict__ az,
                                                             const double* __restrict__ wall_x,
-                                                            const double* __restrict__ wall_z, double dt) {
+                                                            const double* __restrict__ wall_z, double dt,
+                                                            const uint8_t* __restrict__ group,
+                                                            const mps_wall_motion* __restrict__ motion,
+                                                            unsigned n_motions, double t) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   if (type[i] == kFluid) {
@@ -285,7 +291,14 @@ __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __
     x[i] = x[i] + un * dt;
     z[i] = z[i] + wn * dt;
   } else {
-    const double nx = wall_x[i], nz = wall_z[i];
+    double nx = wall_x[i], nz = wall_z[i];
+    const unsigned g = group ? group[i] : 0u;
+    if (g >= 1 && g <= n_motions) {
+      const mps_wall_motion m = motion[g - 1];
+      const double sn = sin(m.omega * t + m.phase);
+      nx = nx + (m.vx * t + m.ax * sn);
+      nz = nz + (m.vz * t + m.az * sn);
+    }
     u[i] = (nx - x[i]) / dt;
     w[i] = (nz - z[i]) / dt;
     x[i] = nx;
@@ -500,7 +513,9 @@ void explicit_update(Context& c) {
                        c.u, c.w, static_cast<const uint8_t*>(c.type),
                        static_cast<const double*>(c.dux), static_cast<const double*>(c.duz),
                        static_cast<const double*>(c.wall_x), static_cast<const double*>(c.wall_z),
-                       c.dt);
+                       c.dt, static_cast<const uint8_t*>(c.wall_motions ? c.wall_group : nullptr),
+                       static_cast<const mps_wall_motion*>(c.wall_motion), static_cast<unsigned>(c.wall_motions),
+                       c.t);
   });
   check_launch();
 }
