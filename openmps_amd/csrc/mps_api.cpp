@@ -315,6 +315,7 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
       throw Failure{MPS_ERR_BAD_ARG, "grid shape out of range"};
     c.cap = static_cast<uint32_t>(c1 * c1);
     c.kmax = c.cap * 9;
+    c.kstride = (c.kmax + 3u) & ~3u;
     c.ncells = static_cast<size_t>(c.gnx * c.gnz);
     c.cell_count = mps::dalloc<uint32_t>(c.ncells);
     c.cell_start = mps::dalloc<uint32_t>(c.ncells);
@@ -411,7 +412,7 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
       mps::grow(c.sorted, 0, m, c.stream);
       mps::grow(c.ncount, old, m, c.stream);
       // neighbour rows of existing particles are kept (boost::multi_array::resize)
-      mps::grow(c.nbr, (mo / 64) * 64 * static_cast<size_t>(c.kmax), m * static_cast<size_t>(c.kmax),
+      mps::grow(c.nbr, (mo / 64) * 64 * static_cast<size_t>(c.kstride), m * static_cast<size_t>(c.kstride),
                 c.stream);
       c.alloc_n = m;
     }
@@ -657,7 +658,7 @@ int mps_get_neighbors(mps_ctx* ctx, uint32_t* ids) {
     const size_t N = c.n;
     if (N == 0) return;
     std::vector<uint32_t> cnt(N);
-    const size_t slots = (N + 63) / 64 * 64 * static_cast<size_t>(c.kmax);
+    const size_t slots = (N + 63) / 64 * 64 * static_cast<size_t>(c.kstride);
     std::vector<uint32_t> all(slots);
     MPS_HIP(hipMemcpyAsync(cnt.data(), c.ncount, N * 4, hipMemcpyDeviceToHost, c.stream));
     MPS_HIP(hipMemcpyAsync(all.data(), c.nbr, slots * 4, hipMemcpyDeviceToHost, c.stream));
@@ -665,7 +666,8 @@ int mps_get_neighbors(mps_ctx* ctx, uint32_t* ids) {
     size_t o = 0;
     for (size_t i = 0; i < N; i++)
       for (uint32_t k = 0; k < cnt[i]; k++)
-        ids[o++] = all[((i / 64) * c.kmax + k) * 64 + (i % 64)];
+        // slot k of particle i (the device layout, mps_device.hpp nbr_slot)
+        ids[o++] = all[(((i >> 6) * (c.kstride >> 2) + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3)];
   });
 }
 

@@ -17,9 +17,12 @@ __device__ __forceinline__ double weight(double r, double re) {
   return ((0 < r) && (r < re)) ? (re / r - 1) : 0;
 }
 
-// Slot k of particle i in the sliced neighbour layout.
-__device__ __forceinline__ size_t nbr_slot(uint32_t i, uint32_t k, uint32_t kmax) {
-  return (static_cast<size_t>(i >> 6) * kmax + k) * 64 + (i & 63u);
+// Slot k of particle i in the neighbour layout: per slice of 64 particles,
+// groups of four consecutive slots of one particle (16 bytes), the 64
+// particles' groups side by side; kstride (slots per particle) is a
+// multiple of 4.  The search stores and the walks load a whole group at once.
+__host__ __device__ __forceinline__ size_t nbr_slot(uint32_t i, uint32_t k, uint32_t kstride) {
+  return ((static_cast<size_t>(i >> 6) * (kstride >> 2) + (k >> 2)) * 64 + (i & 63u)) * 4 + (k & 3u);
 }
 
 // SPP ghost particle, Computer.hpp:657-693.  (sx, sz) is sum W(|dx|) dx over
@@ -47,11 +50,18 @@ __device__ __forceinline__ bool ghost_pos(double xi, double zi, double sx, doubl
 template <int U, class T, class Load, class Use>
 __device__ __forceinline__ void walk_list(uint32_t i, uint32_t m, const uint32_t* __restrict__ ids, uint32_t kmax,
                                           Load load, Use use) {
+  static_assert(U % 4 == 0, "whole slot groups");
   uint32_t q = 0;
   for (; q + U <= m; q += U) {
     uint32_t jj[U];
 #pragma unroll
-    for (int v = 0; v < U; v++) jj[v] = ids[nbr_slot(i, q + v, kmax)];
+    for (int v = 0; v < U; v += 4) {
+      const uint4 g = *reinterpret_cast<const uint4*>(ids + nbr_slot(i, q + v, kmax));
+      jj[v] = g.x;
+      jj[v + 1] = g.y;
+      jj[v + 2] = g.z;
+      jj[v + 3] = g.w;
+    }
     T f[U];
 #pragma unroll
     for (int v = 0; v < U; v++) f[v] = load(jj[v]);

@@ -123,7 +123,8 @@ struct Context {
 
   // uniform grid of cells of side NeighborLength (Grid.hpp:137-150)
   long long gnx = 0, gnz = 0;
-  uint32_t cap = 0, kmax = 0;
+  uint32_t cap = 0, kmax = 0;  // cell capacity; list bound 9 cap (Computer.hpp:1767-1770)
+  uint32_t kstride = 0;        // list slots per particle: kmax rounded up to a multiple of 4
   size_t ncells = 0;
   uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
   uint32_t *key = nullptr, *sorted = nullptr;
@@ -131,8 +132,8 @@ struct Context {
   int* flags = nullptr;           // [0] cell overflow, [1] neighbour overflow
   int* flags_host = nullptr;      // pinned mirror
 
-  // neighbour lists: sliced layout, slot k of particle i at
-  // ((i/64)*kmax + k)*64 + i%64 (coalesced across a wavefront)
+  // neighbour lists: slot k of particle i at nbr_slot(i, k, kstride) (groups
+  // of four slots per particle, a slice's 64 particles side by side)
   uint32_t *nbr = nullptr, *ncount = nullptr;
   bool have_neighbors = false;
   long long search_serial = 0;  // neighbour searches run so far (cells of the last one in key / sorted)

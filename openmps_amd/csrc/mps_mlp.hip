@@ -769,7 +769,7 @@ bool learned_features(Context& c, double& s_r, double& rr) {
     c.ypred = dalloc<float>(n);
     c.feats_cap = n;
   }
-  FeatArgs a{n, c.x, c.z, c.p, c.r, c.nd, c.type, c.nbr, c.ncount, c.kmax, c.env.r_lap * c.env.r_lap,
+  FeatArgs a{n, c.x, c.z, c.p, c.r, c.nd, c.type, c.nbr, c.ncount, c.kstride, c.env.r_lap * c.env.r_lap,
              1.0 / c.env.l0, c.env.n0, s_p > 0 ? 1.0 / s_p : 0.0, s_r > 0 ? 1.0 / s_r : 0.0, c.feats};
   tracked(c, MPS_K_FEATURES, n * (8.0 * 5 + 1 + 4 + kFeatPad * 2) + 4.0 * c.nbr_total, [&] {
     hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, c.stream, a);
@@ -821,7 +821,7 @@ void collect_after(Context& c) {
   const double sd = learned_scale(c, c.collect_sr);
   if (!(sd > 0)) return;
   const uint32_t thr = static_cast<uint32_t>(std::min(1.0, std::max(0.0, c.collect_frac)) * 4294967295.0);
-  const CollectArgs a{c.n, c.type, c.feats, c.sol, c.p, c.x, c.z, c.nbr, c.ncount, c.kmax,
+  const CollectArgs a{c.n, c.type, c.feats, c.sol, c.p, c.x, c.z, c.nbr, c.ncount, c.kstride,
                       c.env.r_lap * c.env.r_lap, 1.0 / sd,
                       c.collect_seed ^ (static_cast<uint64_t>(c.collect_step) * 0x9E3779B97F4A7C15ull), thr,
                       c.dataset->cap, c.dataset->count, c.dataset->x, c.dataset->t};

@@ -465,7 +465,7 @@ __global__ void k_math_check(size_t n, const double* a, const double* b, double*
   d[i] = a[i] / b[i];
 }
 
-NV nview(const Context& c) { return NV{c.nbr, c.ncount, c.kmax}; }
+NV nview(const Context& c) { return NV{c.nbr, c.ncount, c.kstride}; }
 
 }  // namespace
 

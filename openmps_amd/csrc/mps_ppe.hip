@@ -468,7 +468,7 @@ PpeConsts ppe_prepare(Context& c) {
 // Slice offsets from the counted row lengths, then the fill.
 void ppe_finish(Context& c, const PpeConsts& k) {
   const size_t n = c.n;
-  const NV nb{c.nbr, c.ncount, c.kmax};
+  const NV nb{c.nbr, c.ncount, c.kstride};
   scan_slices(c, c.slice_width, c.nslices, c.slice_off);
   MPS_HIP(hipMemcpyAsync(c.ppe_counters_host, c.ppe_counters, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, c.stream));
@@ -506,7 +506,7 @@ void ppe_assemble(Context& c) {
   const PpeConsts k = ppe_prepare(c);
   const size_t n = c.n;
   if (n == 0) return;
-  const NV nb{c.nbr, c.ncount, c.kmax};
+  const NV nb{c.nbr, c.ncount, c.kstride};
   tracked(c, MPS_K_PPE_COUNT, n * (64.0 + 1 + 4) + 4.0 * c.nbr_total + n * 28.0, [&] {
     hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
                        c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
@@ -521,7 +521,7 @@ void density_save_and_assemble(Context& c) {
   const PpeConsts k = ppe_prepare(c);
   const size_t n = c.n;
   if (n == 0) return;
-  const NV nb{c.nbr, c.ncount, c.kmax};
+  const NV nb{c.nbr, c.ncount, c.kstride};
   // the second density pass's bytes and the count pass's (x, z, u, w, p, type,
   // ecs read; n, nWithoutSpp, X0, b, x0, row length, position written)
   tracked(c, MPS_K_DENSITY_SAVE, n * (40.0 + 1 + 8 + 4 + 16 + 16 + 16 + 4 + 8) + 4.0 * c.nbr_total, [&] {

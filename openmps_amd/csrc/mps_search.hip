@@ -14,7 +14,7 @@
 //                      restores exactly the serial insertion order;
 //   5. k_neighbors     one thread per particle walks the 9 cells in the
 //                      reference order and writes its list into the sliced
-//                      (wavefront-coalesced) neighbour layout.
+//                      neighbour layout, four slots (16 bytes) per store.
 #include "mps_device.hpp"
 #include "mps_internal.hpp"
 
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
                                                       const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ cell_count,
                                                       const uint32_t* __restrict__ sorted, uint32_t kmax,
-                                                      uint32_t* __restrict__ nbr,
+                                                      uint32_t kstride, uint32_t* __restrict__ nbr,
                                                       uint32_t* __restrict__ ncount, int* flags,
                                                       unsigned long long* total) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -193,6 +193,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
   uint32_t cnt = 0;
   if (i < n && type[i] != kDisabled) {
     const double xi = x[i], zi = z[i];
+    // found neighbours are stored a group of four (16 bytes) at a time
+    uint4 grp = make_uint4(0u, 0u, 0u, 0u);
+    uint4* const out = reinterpret_cast<uint4*>(nbr);
     long long bx = 0, bz = 0;
     cell_of(g, xi, zi, bx, bz);
     const double nl = g.nl;
@@ -213,7 +216,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
           const double q = rx * rx + rz * rz;
           const bool within = (q < g.lo) || (q <= g.hi && sqrt(q) < nl);
           if (within) {
-            if (cnt < kmax) nbr[nbr_slot(static_cast<uint32_t>(i), cnt, kmax)] = j;
+            const uint32_t lane = cnt & 3u;
+            grp.x = (lane == 0) ? j : grp.x;
+            grp.y = (lane == 1) ? j : grp.y;
+            grp.z = (lane == 2) ? j : grp.z;
+            grp.w = (lane == 3) ? j : grp.w;
+            if (lane == 3 && cnt < kmax) out[nbr_slot(static_cast<uint32_t>(i), cnt - 3, kstride) >> 2] = grp;
             cnt++;
           }
         };
@@ -238,6 +246,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
         }
       }
     }
+    // the last, partial group (slots past the count are never read)
+    if ((cnt & 3u) && cnt <= kmax) out[nbr_slot(static_cast<uint32_t>(i), cnt & ~3u, kstride) >> 2] = grp;
     if (cnt > kmax) {
       flags[1] = 1;
       cnt = kmax;
@@ -306,7 +316,7 @@ void search(Context& c) {
     hipLaunchKernelGGL(k_neighbors, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.sorted),
-                       c.kmax, c.nbr, c.ncount, c.flags, c.nbr_total_dev);
+                       c.kmax, c.kstride, c.nbr, c.ncount, c.flags, c.nbr_total_dev);
   });
   check_launch();
   MPS_HIP(hipMemcpyAsync(c.flags_host, c.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
