@@ -137,7 +137,7 @@ void free_all(Context& c) {
   dfree(c.r); dfree(c.pa); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
-  dfree(c.feats); dfree(c.ypred);
+  dfree(c.feats); dfree(c.ypred); dfree(c.feat_rec);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
@@ -329,14 +329,14 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.partC = mps::dalloc<double>(mps::kRedBlocks);
     c.part3a = mps::dalloc<double>(3 * mps::kRedBlocks);
     c.part3b = mps::dalloc<double>(3 * mps::kRedBlocks);
-    c.red_out = mps::dalloc<double>(2);
+    c.red_out = mps::dalloc<double>(mps::kRedOut);
     c.st = mps::dalloc<mps::CgStatus>(1);
     c.st32 = mps::dalloc<mps::CgStatus>(1);
     c.ir = mps::dalloc<mps::IrStatus>(1);
     mps::pinned(c.flags_host, 4);
     mps::pinned(c.nbr_total_host, 1);
     mps::pinned(c.ppe_counters_host, 3);
-    mps::pinned(c.red_host, 2);
+    mps::pinned(c.red_host, mps::kRedOut);
     mps::pinned(c.ml_active_host, 1);
     mps::pinned(c.st_host, 1);
     mps::pinned(c.st32_host, 1);
@@ -932,13 +932,11 @@ int mps_attach_dataset(mps_ctx* ctx, mps_dataset* d, double fraction, uint64_t s
 int mps_get_features(mps_ctx* ctx, float* out) {
   return mps::guard(ctx, [&](Context& c) {
     if (!c.have_ppe || c.ppe_n != c.n) throw Failure{MPS_ERR_STATE, "assemble the PPE first"};
-    double s_r = 0, rr = 0;
     const size_t n = c.n;
     std::vector<__bf16> h(n * mps::kFeatPad);
-    if (mps::learned_features(c, s_r, rr)) {
-      MPS_HIP(hipMemcpyAsync(h.data(), c.feats, h.size() * 2, hipMemcpyDeviceToHost, c.stream));
-      mps::sync(c);
-    }
+    mps::learned_features(c);
+    MPS_HIP(hipMemcpyAsync(h.data(), c.feats, h.size() * 2, hipMemcpyDeviceToHost, c.stream));
+    mps::sync(c);
     for (size_t r = 0; r < n; r++)
       for (int q = 0; q < mps::kFeat; q++) out[r * mps::kFeat + q] = static_cast<float>(h[r * mps::kFeatPad + q]);
   });

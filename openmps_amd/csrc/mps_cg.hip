@@ -1130,15 +1130,17 @@ void reduce2(Context& c, const double* a, const double* b, int nparts) {
 }
 
 // FP64 residual |b - A x|^2 and |b|^2 (reuses the init kernel on scratch r).
-void residual64(Context& c, double& rr, double& bb, bool track) {
+void residual64_launch(Context& c) {
   const Launch L = cg_grid(c.ppe_n);
   const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
-  auto launch = [&] {
-    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
-                       static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
-    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(NT), 0, c.stream, static_cast<const double*>(c.partB),
-                       static_cast<const double*>(c.partC), L.grid, c.red_out);
-  };
+  hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
+                     static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(NT), 0, c.stream, static_cast<const double*>(c.partB),
+                     static_cast<const double*>(c.partC), L.grid, c.red_out);
+}
+
+void residual64(Context& c, double& rr, double& bb, bool track) {
+  auto launch = [&] { residual64_launch(c); };
   if (track) tracked(c, MPS_K_IR_RESIDUAL, spmv_bytes(c, 8, 8.0 + 8 + 8), launch);
   else launch();
   check_launch();
@@ -1170,7 +1172,8 @@ void refine(Context& c, const StopRule& sr, Sweep sweep, Correct correct) {
   double rr_prev = 0;
   bool done = false;
   for (;; outer++) {
-    residual64(c, rr, bb, true);
+    // the learned warm start may have left r = b - A x and its norms
+    if (!(outer == 0 && c.learned_pending && learned_resolve(c, rr, bb))) residual64(c, rr, bb, true);
     if (outer == 0) rr0 = rr;
     const bool stalled = outer > 0 && !(rr < 0.5 * rr_prev);
     rr_prev = rr;
@@ -1524,6 +1527,7 @@ void ppe_solve(Context& c) {
   c.stats = mps_solve_stats{};
   c.replay_ready = 0;
   c.final_rr = -1;
+  c.learned_pending = false;
   if (n == 0) return;
   StopRule sr;
   sr.rule = c.solver.stop_rule;
@@ -1539,7 +1543,9 @@ void ppe_solve(Context& c) {
     if (c.solver.warm_start == MPS_WARM_LEARNED && on_particles) learned_warm_start(c);
     if (c.solver.kind == MPS_SOLVER_MIXED_IR) solve_mixed(c, sr, max_iter);
     else solve_cg64(c, sr, max_iter);
+    c.learned_pending = false;
   } catch (Failure& f) {
+    c.learned_pending = false;
     if (f.code != MPS_ERR_CG_NOT_CONVERGED) throw;
     rc = f.code;
     msg = f.msg;

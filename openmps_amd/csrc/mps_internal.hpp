@@ -16,6 +16,7 @@ namespace mps {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // threads per block of the particle passes
 constexpr int kRedBlocks = 1024;   // fixed partial-sum slots of the reduction kernels
+constexpr int kRedOut = 16;        // reduced scalars (red_out, red_host)
 
 // Particle type codes, Particle.hpp:16-29.
 constexpr uint8_t kFluid = 0, kWall = 1, kDummy = 2, kDisabled = 3;
@@ -76,6 +77,7 @@ struct Model {
   __bf16* w2p = nullptr;  // w2 in the K order of the fused inference kernel
   float* loss = nullptr;
   long long adam_t = 0;
+  int cus = 0;     // compute units of the device (persistent inference grid)
   size_t cap = 0;  // batch work buffers
   __bf16 *xb = nullptr, *h1b = nullptr, *da2b = nullptr, *da1b = nullptr;
   float *tb = nullptr, *z1 = nullptr, *z2 = nullptr, *a2 = nullptr, *y = nullptr, *dy = nullptr,
@@ -177,7 +179,7 @@ struct Context {
   double *r = nullptr, *pa = nullptr, *q = nullptr;  // residual, direction, A direction
   double *partA = nullptr, *partB = nullptr, *partC = nullptr;
   double* red_out = nullptr;
-  double* red_host = nullptr;  // pinned
+  double* red_host = nullptr;  // pinned, kRedOut entries (as red_out)
   CgStatus* st = nullptr;
   CgStatus* st_host = nullptr;
   CgStatus* st32 = nullptr;
@@ -187,6 +189,9 @@ struct Context {
   long long sweep_pred[kSweepPred] = {};
   int sweep_index = 0;
   double final_rr = -1, final_bb = 0;  // FP64 |b - A x|^2, |b|^2 of the solve's x (-1: not known)
+  // the learned warm start's residual norms are on their way to red_host
+  // (learned_resolve reads them at the solver's first synchronisation)
+  bool learned_pending = false;
   // mixed-precision work vectors
   float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
   float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr;  // multilevel-preconditioned inner CG
@@ -223,12 +228,13 @@ struct Context {
   // learned warm start and snapshot collection
   Model* model = nullptr;
   Dataset* dataset = nullptr;
-  double collect_frac = 0, collect_sr = 0;
-  double learned_drms = 0;  // RMS of the alive diagonal at the last feature pass
+  double collect_frac = 0;
+  double collect_sd = 0;  // output scale of the snapshot's feature pass
   uint64_t collect_seed = 0;
   long long collect_step = 0;
   bool collect_ok = false;
   __bf16* feats = nullptr;  // n x kFeatPad of the last feature pass
+  void* feat_rec = nullptr;  // n 32-byte neighbour records of the feature pass
   float* ypred = nullptr;
   size_t feats_cap = 0;
 
@@ -358,7 +364,15 @@ void ensure_ppe_vectors(Context& c, size_t n);  // mps_api.cpp
 void ppe_solve(Context& c);              // mps_cg.hip
 void residual64(Context& c, double& rr, double& bb, bool track);
 void reduce2(Context& c, const double* a, const double* b, int nparts);  // -> c.red_host[0..1]
-bool learned_features(Context& c, double& s_r, double& rr);             // mps_mlp.hip
+void residual64_launch(Context& c);  // r = b - A x; |r|^2, |b|^2 -> c.red_out[0..1] (no sync)
+// mps_mlp.hip.  learned_features enqueues (no synchronisation) the residual
+// of x = p_prev, the scales and the feature rows; red_out then holds |r|^2,
+// |b|^2 [0, 1], the alive sums of p^2, diag^2 and count [2..4], and
+// [6..9] = |r_prev|^2, 1/s_p, 1/s_r and the output scale s_d.
+void learned_features(Context& c);
+// after learned_warm_start on a mixed solve: synchronises, records the
+// statistics and returns whether r, rr, bb hold the residual of the solver's x
+bool learned_resolve(Context& c, double& rr, double& bb);
 void learned_warm_start(Context& c);
 void collect_before(Context& c);
 void collect_after(Context& c);

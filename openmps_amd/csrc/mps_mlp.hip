@@ -26,17 +26,26 @@ namespace mps {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr float kG0 = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kG1 = 0.044715f;
 
-__device__ __forceinline__ float gelu_f(float z) {
-  return 0.5f * z * (1.0f + tanhf(kG0 * (z + kG1 * z * z * z)));
+// With u = kG0 (z + kG1 z^3): 0.5 (1 + tanh u) = sigmoid(2u) = 1 / (1 + 2^(-2u log2 e)),
+// one hardware exp2 and one reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp
+// each) where tanhf costs tens of instructions; the MLP's GELUs outnumber its
+// MFMA instructions 2.5 : 1 per row.
+constexpr float kGE0 = -2.0f * kG0 * 1.4426950408889634f;  // -2 kG0 log2(e)
+constexpr float kGE1 = kGE0 * kG1;
+__device__ __forceinline__ float gelu_sig(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(kGE1, z * z, kGE0)));
 }
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
+// d/dz z s(2u) = s + 2 z s (1 - s) u'(z)
 __device__ __forceinline__ float gelu_grad_f(float z) {
-  const float th = tanhf(kG0 * (z + kG1 * z * z * z));
-  return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * kG0 * (1.0f + 3.0f * kG1 * z * z);
+  const float s = gelu_sig(z);
+  return s + 2.0f * z * s * (1.0f - s) * kG0 * (1.0f + 3.0f * kG1 * z * z);
 }
 
 struct GemmArgs {
@@ -233,86 +242,257 @@ __global__ void __launch_bounds__(256) k_gather_rows(size_t m, const uint32_t* _
   if (c == 0) tb[r] = src_t[s];
 }
 
+// What the feature pass gathers of a neighbour, in one 32-byte record (one
+// cache line access where four arrays cost four): the position (x = NaN for
+// dummy particles, which no r^2 < r_lap^2 test accepts) and the scaled
+// pressure and residual.
+struct __attribute__((aligned(32))) FeatRec {
+  double x, z;
+  float ps, rs;
+  uint32_t pad[2];
+};
+
+// The feature scales from the reduced sums (one thread): out[6..9] =
+// |r_prev|^2, 1/s_p, 1/s_r, s_d = s_r / RMS(diag) over the alive rows, with
+// s_v = RMS(v); zero where a scale is zero or no row is alive.
+__global__ void k_feat_scales(double* __restrict__ out) {
+  const double rr = out[0], pp = out[2], dd = out[3], na = out[4];
+  double inv_sp = 0, inv_sr = 0, sd = 0;
+  if (na > 0) {
+    const double s_r = sqrt(rr / na), s_p = sqrt(pp / na), drms = sqrt(dd / na);
+    inv_sp = s_p > 0 ? 1.0 / s_p : 0.0;
+    inv_sr = s_r > 0 ? 1.0 / s_r : 0.0;
+    sd = (drms > 0 && s_r > 0) ? s_r / drms : 0.0;
+  }
+  out[6] = rr;
+  out[7] = inv_sp;
+  out[8] = inv_sr;
+  out[9] = sd;
+}
+
+__global__ void __launch_bounds__(256) k_feat_rec(size_t n, const uint8_t* __restrict__ type,
+                                                  const double* __restrict__ x, const double* __restrict__ z,
+                                                  const double* __restrict__ p, const double* __restrict__ r0,
+                                                  const double* __restrict__ scales, FeatRec* __restrict__ rec) {
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i >= n) return;
+  const double inv_sp = scales[7], inv_sr = scales[8];
+  FeatRec o;
+  o.x = (type[i] == kDummy) ? __builtin_nan("") : x[i];
+  o.z = z[i];
+  o.ps = static_cast<float>(p[i] * inv_sp);
+  o.rs = static_cast<float>(r0[i] * inv_sr);
+  o.pad[0] = o.pad[1] = 0u;
+  rec[i] = o;
+}
+
+// sums over alive rows of p^2 and diag^2, and the alive count (block partials
+// at part[0..G), part[G..2G), part[2G..3G) for G = gridDim.x).
+__global__ void __launch_bounds__(256) k_alive_stats3(size_t n, const uint8_t* __restrict__ type,
+                                                      const double* __restrict__ p,
+                                                      const double* __restrict__ diag,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[4];
+  double sp = 0, sd = 0, cnt = 0;
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) {
+    const uint8_t t = type[i];
+    if (t != kDummy && t != kDisabled) {
+      sp += p[i] * p[i];
+      sd += diag[i] * diag[i];
+      cnt += 1.0;
+    }
+  }
+  const double s1 = block_sum<256>(sp, lds);
+  const double s2 = block_sum<256>(sd, lds);
+  const double s3 = block_sum<256>(cnt, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[gridDim.x + blockIdx.x] = s2;
+    part[2 * gridDim.x + blockIdx.x] = s3;
+  }
+}
+
+// out[k] = sum of part[256 k .. 256 k + 255], k = 0, 1, 2 (one block).
+__global__ void __launch_bounds__(256) k_reduce3(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double lds[4];
+  const double s0 = block_sum<256>(part[threadIdx.x], lds);
+  const double s1 = block_sum<256>(part[256 + threadIdx.x], lds);
+  const double s2 = block_sum<256>(part[512 + threadIdx.x], lds);
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
+}
+
 struct FeatArgs {
   size_t n;
-  const double *x, *z, *p, *r0, *nd;
+  const double *x, *z, *nd;
   const uint8_t* type;
+  const FeatRec* rec;
   const uint32_t *ids, *count;
-  uint32_t kmax;
-  double rl2, inv_l0, n0, inv_sp, inv_sr;
+  uint32_t kstride;
+  double rl2, inv_l0, n0;
   __bf16* out;
 };
 
+struct FeatNbr {
+  double x, z;
+};
+
 // Features of every particle (zero rows for dummy / disabled particles).
+// One thread per particle; its kept neighbours are entries of LDS arrays:
+// the key float(r^2) and the list position Q (8 or 16 bits), 160-192 bytes
+// per thread.  The walk issues eight neighbours' loads at once and appends
+// the first kNbr accepted neighbours as entries.  A row then places entry e
+// at its rank among the entries (keys ascending, equal keys in list order,
+// as a stable sort orders them), counted over the keys in LDS: independent
+// loads, where sorting on every append waits on one LDS load per shifted
+// entry.  A particle with more than kNbr accepted neighbours walks its list
+// a second time with an insertion sort that keeps the kNbr nearest.
+constexpr int kFeatWalk = 8;    // neighbours whose loads the walk issues together
+constexpr int kFeatGather = 8;  // ranks whose neighbours' state is gathered together
+static_assert(kNbr % kFeatGather == 0, "whole gather groups");
+
+template <class Q>
 __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
-  __shared__ float skey[kNbr][256];
-  __shared__ uint32_t sidx[kNbr][256];
+  __shared__ uint32_t skey[kNbr][256];  // float bits: non-negative, so ordered as unsigned
+  __shared__ Q sq[kNbr][256];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * 256ull + threadIdx.x;
   if (i >= a.n) return;
   const int t = threadIdx.x;
+  const uint32_t ii = static_cast<uint32_t>(i);
   __bf16* row = a.out + i * kFeatPad;
   const uint8_t ti = a.type[i];
+  const double xi = a.x[i], zi = a.z[i];
   float f[4] = {0, 0, 0, 0};
   int m = 0;
+  bool sorted = false;
   if (ti != kDummy && ti != kDisabled) {
-    const double xi = a.x[i], zi = a.z[i];
+    const uint32_t count = a.count[i];
     int cnt = 0;
-    const uint32_t c = a.count[i];
-    for (uint32_t q = 0; q < c; q++) {
-      const uint32_t j = a.ids[nbr_slot(static_cast<uint32_t>(i), q, a.kmax)];
-      if (a.type[j] == kDummy) continue;
-      const double dx = a.x[j] - xi, dz = a.z[j] - zi;
-      const double r2 = dx * dx + dz * dz;
-      if (!(r2 < a.rl2)) continue;
-      cnt++;
-      const float kf = static_cast<float>(r2);
-      int pos;
-      if (m < kNbr) {
-        pos = m++;
-      } else if (kf < skey[kNbr - 1][t]) {
-        pos = kNbr - 1;
-      } else {
-        continue;
+    uint32_t q = 0;
+    walk_list<kFeatWalk, FeatNbr>(
+        ii, count, a.ids, a.kstride,
+        [&](uint32_t j) {
+          const double2 xz = *reinterpret_cast<const double2*>(&a.rec[j]);
+          return FeatNbr{xz.x, xz.y};
+        },
+        [&](uint32_t, const FeatNbr& nb) {
+          const uint32_t qj = q++;
+          const double dx = nb.x - xi, dz = nb.z - zi;
+          const double r2 = dx * dx + dz * dz;
+          if (!(r2 < a.rl2)) return;  // also a dummy's NaN
+          cnt++;
+          if (m < kNbr) {
+            skey[m][t] = __float_as_uint(static_cast<float>(r2));
+            sq[m][t] = static_cast<Q>(qj);
+            m++;
+          }
+        });
+    if (cnt > kNbr) {  // keep the kNbr nearest: stable insertion sort over the whole list
+      m = 0;
+      for (uint32_t qq = 0; qq < count; qq++) {
+        const uint32_t j = a.ids[nbr_slot(ii, qq, a.kstride)];
+        const double dx = a.rec[j].x - xi, dz = a.rec[j].z - zi;
+        const double r2 = dx * dx + dz * dz;
+        if (!(r2 < a.rl2)) continue;  // also a dummy's NaN
+        const uint32_t kf = __float_as_uint(static_cast<float>(r2));
+        int pos;
+        if (m < kNbr) pos = m++;
+        else if (kf < skey[kNbr - 1][t]) pos = kNbr - 1;  // an equal key listed earlier ranks first
+        else continue;
+        while (pos > 0 && skey[pos - 1][t] > kf) {
+          skey[pos][t] = skey[pos - 1][t];
+          sq[pos][t] = sq[pos - 1][t];
+          pos--;
+        }
+        skey[pos][t] = kf;
+        sq[pos][t] = static_cast<Q>(qq);
       }
-      while (pos > 0 && skey[pos - 1][t] > kf) {
-        skey[pos][t] = skey[pos - 1][t];
-        sidx[pos][t] = sidx[pos - 1][t];
-        pos--;
-      }
-      skey[pos][t] = kf;
-      sidx[pos][t] = j;
+      sorted = true;
     }
     f[0] = cnt / 32.0f;
     f[1] = static_cast<float>((a.nd[i] - a.n0) / a.n0);
-    f[2] = static_cast<float>(a.p[i] * a.inv_sp);
-    f[3] = static_cast<float>(a.r0[i] * a.inv_sr);
+    f[2] = a.rec[i].ps;
+    f[3] = a.rec[i].rs;
   }
-  // the row as 16-byte stores: elements 0-3 the particle's own features,
-  // 4 + 4k .. 4 + 4k + 3 neighbour k's (k < m, else 0), zeros from kFeat on
-  const double xi = a.x[i], zi = a.z[i];
-  for (int q = 0; q < kFeatPad / 8; q++) {
-    bf16x8 v;
+  // rank of every entry (keys ascending, equal keys in list order), then
+  // the neighbours' indices in rank order over the keys (their loads are in
+  // flight while the ranks are counted).  (b - k) >> 31 over unsigned keys
+  // below 2^31 is 1 exactly when b < k; entry b precedes entry e if its key
+  // is smaller, or equal with b < e.  The ranks of entries 4g .. 4g + 3 are
+  // the bytes of rk[g].
+  if (!sorted) {
+    uint32_t je[kNbr], rk[kNbr / 4];
 #pragma unroll
-    for (int half = 0; half < 2; half++) {
-      const int e0 = 8 * q + 4 * half;
-      float u[4] = {0.f, 0.f, 0.f, 0.f};
-      if (e0 == 0) {
-        u[0] = f[0];
-        u[1] = f[1];
-        u[2] = f[2];
-        u[3] = f[3];
-      } else if (e0 < kFeat && (e0 - 4) / 4 < m) {
-        const uint32_t j = sidx[(e0 - 4) / 4][t];
-        u[0] = static_cast<float>((a.x[j] - xi) * a.inv_l0);
-        u[1] = static_cast<float>((a.z[j] - zi) * a.inv_l0);
-        u[2] = static_cast<float>(a.p[j] * a.inv_sp);
-        u[3] = static_cast<float>(a.r0[j] * a.inv_sr);
+    for (int e = 0; e < kNbr; e++) je[e] = (e < m) ? a.ids[nbr_slot(ii, sq[e][t], a.kstride)] : 0u;
+#pragma unroll
+    for (int e0 = 0; e0 < kNbr; e0 += 4) {
+      uint32_t r = 0;
+      if (e0 < m) {
+        uint32_t ke[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) ke[u] = skey[e0 + u][t];  // entries past m are not used
+        for (int b = 0; b < e0; b++) {  // b < e: smaller or equal
+          const uint32_t kb = skey[b][t];
+#pragma unroll
+          for (int u = 0; u < 4; u++) r += ((kb - ke[u] - 1u) >> 31) << (8 * u);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const uint32_t kb = (e0 + v < m) ? skey[e0 + v][t] : 0x7f800000u;
+#pragma unroll
+          for (int u = 0; u < 4; u++) r += ((kb - ke[u] - (v < u ? 1u : 0u)) >> 31) << (8 * u);
+        }
+        for (int b = e0 + 4; b < m; b++) {  // b > e: smaller
+          const uint32_t kb = skey[b][t];
+#pragma unroll
+          for (int u = 0; u < 4; u++) r += ((kb - ke[u]) >> 31) << (8 * u);
+        }
       }
-#pragma unroll
-      for (int e = 0; e < 4; e++) v[4 * half + e] = __bf16(u[e]);
+      rk[e0 / 4] = r;
     }
-    *reinterpret_cast<bf16x8*>(row + 8 * q) = v;
+    // every key has been read: skey[s] becomes the neighbour of rank s
+#pragma unroll
+    for (int e = 0; e < kNbr; e++)
+      if (e < m) skey[(rk[e / 4] >> (8 * (e % 4))) & 0xffu][t] = je[e];
+  } else {
+    for (int e = 0; e < m; e++) skey[e][t] = a.ids[nbr_slot(ii, sq[e][t], a.kstride)];
+  }
+  // the row in registers (elements 0-3: the particle's own features;
+  // 4 + 4s .. 7 + 4s: the neighbour of rank s, zero for s >= m; zeros from
+  // kFeat on), gathered eight neighbours at a time, then stored as 16-byte
+  // vectors after the last load
+  bf16x4 val[kNbr + 1];
+  val[0] = bf16x4{__bf16(f[0]), __bf16(f[1]), __bf16(f[2]), __bf16(f[3])};
+  const bf16x4 zero4 = bf16x4{__bf16(0.f), __bf16(0.f), __bf16(0.f), __bf16(0.f)};
+#pragma unroll
+  for (int s0 = 0; s0 < kNbr; s0 += kFeatGather) {
+#pragma unroll
+    for (int u = 0; u < kFeatGather; u++) val[1 + s0 + u] = zero4;
+    if (s0 < m) {
+      uint32_t jj[kFeatGather];
+#pragma unroll
+      for (int u = 0; u < kFeatGather; u++) jj[u] = (s0 + u < m) ? skey[s0 + u][t] : ii;
+#pragma unroll
+      for (int u = 0; u < kFeatGather; u++) {
+        const FeatRec rj = a.rec[jj[u]];
+        const bf16x4 v = bf16x4{__bf16(static_cast<float>((rj.x - xi) * a.inv_l0)),
+                                __bf16(static_cast<float>((rj.z - zi) * a.inv_l0)), __bf16(rj.ps),
+                                __bf16(rj.rs)};
+        val[1 + s0 + u] = (s0 + u < m) ? v : zero4;
+      }
+    }
+  }
+  static_assert(kFeat == 4 + 4 * kNbr && (kNbr + 1) % 2 == 1 && kFeatPad % 8 == 0, "row layout");
+#pragma unroll
+  for (int vv = 0; vv < kFeatPad / 8; vv++) {
+    const bf16x4 lo = (2 * vv < kNbr + 1) ? val[2 * vv] : zero4;
+    const bf16x4 hi = (2 * vv + 1 < kNbr + 1) ? val[2 * vv + 1] : zero4;
+    *reinterpret_cast<bf16x8*>(row + 8 * vv) =
+        bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
   }
 }
 
@@ -339,20 +519,25 @@ __global__ void __launch_bounds__(256) k_alive_stats(size_t n, const uint8_t* __
 }
 
 // sol = p_prev + s_d y on alive rows.
+// sol = p_prev + s_d y on alive rows (s_d = scales[9]).
 __global__ void __launch_bounds__(256) k_apply_pred(size_t n, const uint8_t* __restrict__ type,
                                                     const double* __restrict__ p,
-                                                    const float* __restrict__ y, double sd,
+                                                    const float* __restrict__ y, const double* __restrict__ scales,
                                                     double* __restrict__ sol) {
   const size_t i = blockIdx.x * 256ull + threadIdx.x;
   if (i >= n) return;
   const uint8_t t = type[i];
-  if (t != kDummy && t != kDisabled) sol[i] = p[i] + sd * static_cast<double>(y[i]);
+  if (t != kDummy && t != kDisabled) sol[i] = p[i] + scales[9] * static_cast<double>(y[i]);
 }
 
-__global__ void __launch_bounds__(256) k_revert(size_t n, const uint8_t* __restrict__ type,
-                                                const double* __restrict__ p, double* __restrict__ sol) {
+// sol = p_prev on alive rows unless the prediction lowered the FP64 residual
+// (red[0] = |b - A sol|^2 < red[6] = |b - A p_prev|^2).
+__global__ void __launch_bounds__(256) k_revert_unless_better(size_t n, const uint8_t* __restrict__ type,
+                                                              const double* __restrict__ p,
+                                                              const double* __restrict__ red,
+                                                              double* __restrict__ sol) {
   const size_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || red[0] < red[6]) return;
   const uint8_t t = type[i];
   if (t != kDummy && t != kDisabled) sol[i] = p[i];
 }
@@ -370,7 +555,7 @@ struct CollectArgs {
   const __bf16* feats;
   const double *sol, *p, *x, *z;  // solution, previous pressure, positions
   const uint32_t *ids, *count;    // neighbour lists
-  uint32_t kmax;
+  uint32_t kstride;
   double rl2, inv_sd;
   uint64_t seed;
   uint32_t threshold;
@@ -398,7 +583,7 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
   const double xi = a.x[i], zi = a.z[i];
   const uint32_t m = a.count[i];
   for (uint32_t q = 0; q < m; q++) {
-    const uint32_t j = a.ids[nbr_slot(static_cast<uint32_t>(i), q, a.kmax)];
+    const uint32_t j = a.ids[nbr_slot(static_cast<uint32_t>(i), q, a.kstride)];
     if (a.type[j] == kDummy) continue;
     const double ex = a.x[j] - xi, ez = a.z[j] - zi;
     if (!(ex * ex + ez * ez < a.rl2)) continue;
@@ -416,23 +601,30 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
 
 // ---- fused inference ---------------------------------------------------------
 // y = w3 . gelu(W2 gelu(W1 x + b1) + b2) + b3 in one kernel, nothing but x
-// read and y written: one wavefront per 16 rows, both layers transposed
-// (hidden units as the MFMA M dimension, rows as N), so layer 1's output
-// tiles already sit in the lanes that feed them to layer 2 as its B operand.
-// v_mfma_f32_16x16x32_bf16 lane layout: A row l%16 / B column l%16 with K
-// elements 8(l/16)..8(l/16)+7; C[4(l/16)+v][l%16], v = 0..3.  Layer 1 lane
-// l ends with hidden units 16t + 4(l/16) + v of row l%16, so layer 2's K
-// step ks (hidden 32ks..32ks+31) takes tiles 2ks and 2ks+1: its K slot
-// 8g + e stands for hidden unit w2_hidden(ks, g, e) below, and W2 is stored
-// in that order (k_perm_w2) so that each lane's A operand is one 16-byte load.
-constexpr int kRowGroups = 1;  // groups of 16 rows per wavefront (2 needs ~500 registers)
-constexpr int kRowsPerWave = 16 * kRowGroups;
+// read and y written.  Both layers are computed transposed (hidden units as
+// the MFMA M dimension, rows as N) with v_mfma_f32_16x16x32_bf16, whose lane
+// layout is: A row l%16 / B column l%16 with K elements 8(l/16)..8(l/16)+7;
+// C[4(l/16)+v][l%16], v = 0..3.
+//
+// Weight-stationary and persistent: one block of kMlpWaves wavefronts per CU;
+// wavefront w holds, in registers for the block's lifetime, the A operands of
+// hidden units 32w..32w+31 (W1) and of outputs 32w..32w+31 (W2), and the
+// block walks batches of kBatch rows.  Per batch the rows go to LDS (loaded
+// into registers during the previous batch), every wavefront computes its 32
+// hidden units of every row into an LDS tile h1, then its 32 outputs, and the
+// head's per-wavefront partial sums meet in LDS.  Weights are read from
+// memory once per block; the per-row LDS traffic is the B operands (x, h1).
+//
+// Layer 1 leaves lane (g, c) of wavefront w with hidden units
+// 32w + 16mt + 4g + v (tile mt = 0, 1) of row c: one 16-byte vector whose
+// element 4mt + v is layer 2's K slot 8g + 4mt + v of K step w.  That K slot
+// stands for hidden unit w2_hidden(w, g, 4mt + v) below, and W2 is stored in
+// that order (k_perm_w2) so that h1 is written and read as 16-byte vectors.
 
 __host__ __device__ __forceinline__ int w2_hidden(int ks, int g, int e) {
   return 32 * ks + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
 }
 
-// w2p[o][32 ks + 8 g + e] = bf16(w2[o][w2_hidden(ks, g, e)]).
 __global__ void __launch_bounds__(256) k_perm_w2(const float* __restrict__ w2, __bf16* __restrict__ w2p) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kHid * kHid) return;
@@ -441,83 +633,123 @@ __global__ void __launch_bounds__(256) k_perm_w2(const float* __restrict__ w2, _
   w2p[i] = static_cast<__bf16>(w2[o * kHid + w2_hidden(ks, g, e)]);
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(size_t M, const __bf16* __restrict__ x,
-                                                   const __bf16* __restrict__ w1, const float* __restrict__ b1,
-                                                   const __bf16* __restrict__ w2p, const float* __restrict__ b2,
-                                                   const float* __restrict__ w3, const float* __restrict__ b3,
-                                                   float* __restrict__ y) {
-  const size_t wave = blockIdx.x * static_cast<size_t>(blockDim.x / 64) + (threadIdx.x >> 6);
-  const size_t r0 = wave * kRowsPerWave;
-  if (r0 >= M) return;  // wave-uniform
-  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+constexpr int kMlpWaves = 8;
+constexpr int kMlpThreads = 64 * kMlpWaves;
+constexpr int kBatch = 64;             // rows per block iteration: 4 MFMA column tiles
+constexpr int kXPitch = kFeatPad + 8;  // LDS row pitches (bf16), padded against bank conflicts
+constexpr int kHPitch = kHid + 8;
+constexpr int kXVec = kBatch * (kFeatPad / 8);  // 16-byte vectors of one batch of rows
+constexpr int kXPer = (kXVec + kMlpThreads - 1) / kMlpThreads;
+static_assert(kHid == 32 * kMlpWaves, "one 32-unit slice of each layer per wavefront");
+
+__global__ void __launch_bounds__(kMlpThreads) k_mlp_infer(size_t M, const __bf16* __restrict__ x,
+                                                           const __bf16* __restrict__ w1,
+                                                           const float* __restrict__ b1,
+                                                           const __bf16* __restrict__ w2p,
+                                                           const float* __restrict__ b2,
+                                                           const float* __restrict__ w3,
+                                                           const float* __restrict__ b3, float* __restrict__ y) {
+  __shared__ __attribute__((aligned(16))) __bf16 xs[kBatch * kXPitch];
+  __shared__ __attribute__((aligned(16))) __bf16 hs[kBatch * kHPitch];
+  __shared__ float red[kMlpWaves][kBatch];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, c = lane & 15;
   constexpr int K1 = kFeatPad / 32, K2 = kHid / 32;
-  bf16x8 xf[kRowGroups][K1];
+  const size_t nbatch = (M + kBatch - 1) / kBatch;
+  if (blockIdx.x >= nbatch) return;  // block-uniform, before any barrier
+  // this wavefront's weights, biases and head coefficients
+  bf16x8 a1[2][K1], a2[2][K2];
+  float bias1[2][4], bias2[2][4], head[2][4];
 #pragma unroll
-  for (int sg = 0; sg < kRowGroups; sg++) {
-    size_t r = r0 + 16 * sg + c;
-    r = (r < M) ? r : M - 1;
+  for (int mt = 0; mt < 2; mt++) {
+    const int h = 32 * w + 16 * mt + c;
 #pragma unroll
-    for (int ks = 0; ks < K1; ks++)
-      xf[sg][ks] = *reinterpret_cast<const bf16x8*>(x + r * kFeatPad + 32 * ks + 8 * g);
-  }
-  bf16x8 h1[kRowGroups][K2];
+    for (int ks = 0; ks < K1; ks++) a1[mt][ks] = *reinterpret_cast<const bf16x8*>(w1 + h * kFeatPad + 32 * ks + 8 * g);
 #pragma unroll
-  for (int t2 = 0; t2 < K2; t2++) {
-    // keeps the next pair's weight loads from being hoisted (register pressure)
-    asm volatile("" ::: "memory");
-    f32x4 acc[2][kRowGroups];
-#pragma unroll
-    for (int half = 0; half < 2; half++) {
-      const int h0 = 16 * (2 * t2 + half);
-#pragma unroll
-      for (int sg = 0; sg < kRowGroups; sg++) acc[half][sg] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < K1; ks++) {
-        const bf16x8 af = *reinterpret_cast<const bf16x8*>(w1 + (h0 + c) * kFeatPad + 32 * ks + 8 * g);
-#pragma unroll
-        for (int sg = 0; sg < kRowGroups; sg++)
-          acc[half][sg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, xf[sg][ks], acc[half][sg], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int half = 0; half < 2; half++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const float bias = b1[16 * (2 * t2 + half) + 4 * g + v];
-#pragma unroll
-        for (int sg = 0; sg < kRowGroups; sg++) h1[sg][t2][4 * half + v] = static_cast<__bf16>(gelu_f(acc[half][sg][v] + bias));
-      }
-  }
-  float part[kRowGroups];
-#pragma unroll
-  for (int sg = 0; sg < kRowGroups; sg++) part[sg] = 0.f;
-  for (int ot = 0; ot < kHid / 16; ot++) {
-    asm volatile("" ::: "memory");
-    const int o0 = 16 * ot;
-    f32x4 acc2[kRowGroups];
-#pragma unroll
-    for (int sg = 0; sg < kRowGroups; sg++) acc2[sg] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < K2; ks++) {
-      const bf16x8 af = *reinterpret_cast<const bf16x8*>(w2p + (o0 + c) * kHid + 32 * ks + 8 * g);
-#pragma unroll
-      for (int sg = 0; sg < kRowGroups; sg++)
-        acc2[sg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, h1[sg][ks], acc2[sg], 0, 0, 0);
-    }
+    for (int ks = 0; ks < K2; ks++) a2[mt][ks] = *reinterpret_cast<const bf16x8*>(w2p + h * kHid + 32 * ks + 8 * g);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int o = o0 + 4 * g + v;
-      const float bias = b2[o], wo = w3[o];
-#pragma unroll
-      for (int sg = 0; sg < kRowGroups; sg++) part[sg] = fmaf(wo, gelu_f(acc2[sg][v] + bias), part[sg]);
+      const int o = 32 * w + 16 * mt + 4 * g + v;
+      bias1[mt][v] = b1[o];
+      bias2[mt][v] = b2[o];
+      head[mt][v] = w3[o];
     }
   }
+  const float bias3 = b3[0];
+  // the next batch's rows, global -> registers -> LDS (rows past M read as zero)
+  uint4 xr[kXPer];
+  auto fetch = [&](size_t bt) {
 #pragma unroll
-  for (int sg = 0; sg < kRowGroups; sg++) {
-    part[sg] += __shfl_xor(part[sg], 16, 64);
-    part[sg] += __shfl_xor(part[sg], 32, 64);
-    const size_t r = r0 + 16 * sg + c;
-    if (g == 0 && r < M) y[r] = part[sg] + b3[0];
+    for (int v = 0; v < kXPer; v++) {
+      const int e = tid + v * kMlpThreads;
+      const size_t r = bt * kBatch + e / (kFeatPad / 8);
+      xr[v] = (e < kXVec && r < M)
+                  ? reinterpret_cast<const uint4*>(x + r * kFeatPad)[e % (kFeatPad / 8)]
+                  : make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  auto put = [&] {
+#pragma unroll
+    for (int v = 0; v < kXPer; v++) {
+      const int e = tid + v * kMlpThreads;
+      if (e < kXVec)
+        *reinterpret_cast<uint4*>(&xs[(e / (kFeatPad / 8)) * kXPitch + (e % (kFeatPad / 8)) * 8]) = xr[v];
+    }
+  };
+  fetch(blockIdx.x);
+  for (size_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    // xs was last read before the previous batch's second barrier, red
+    // before this barrier by wavefront 0
+    put();
+    __syncthreads();
+    if (bt + gridDim.x < nbatch) fetch(bt + gridDim.x);
+#pragma unroll
+    for (int nt = 0; nt < kBatch / 16; nt++) {
+      const __bf16* xrow = xs + (16 * nt + c) * kXPitch + 8 * g;
+      f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int ks = 0; ks < K1; ks++) {
+        const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(xrow + 32 * ks);
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[0][ks], bfr, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[1][ks], bfr, acc[1], 0, 0, 0);
+      }
+      bf16x8 hv;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        hv[v] = static_cast<__bf16>(gelu_f(acc[0][v] + bias1[0][v]));
+        hv[4 + v] = static_cast<__bf16>(gelu_f(acc[1][v] + bias1[1][v]));
+      }
+      *reinterpret_cast<bf16x8*>(&hs[(16 * nt + c) * kHPitch + 32 * w + 8 * g]) = hv;
+    }
+    __syncthreads();  // h1 complete
+#pragma unroll
+    for (int nt = 0; nt < kBatch / 16; nt++) {
+      const __bf16* hrow = hs + (16 * nt + c) * kHPitch + 8 * g;
+      f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int ks = 0; ks < K2; ks++) {
+        const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(hrow + 32 * ks);
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][ks], bfr, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][ks], bfr, acc[1], 0, 0, 0);
+      }
+      float part = 0.f;
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) part = fmaf(head[mt][v], gelu_f(acc[mt][v] + bias2[mt][v]), part);
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      if (g == 0) red[w][16 * nt + c] = part;
+    }
+    __syncthreads();  // partial sums complete; hs free
+    if (tid < kBatch) {
+      const size_t r = bt * kBatch + tid;
+      if (r < M) {
+        float s = bias3;
+#pragma unroll
+        for (int k = 0; k < kMlpWaves; k++) s += red[k][tid];
+        y[r] = s;
+      }
+    }
   }
 }
 
@@ -565,6 +797,9 @@ void Model::ensure_batch(size_t m) {
 
 void Model::init(int seed) {
   MPS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  int dev = 0;
+  MPS_HIP(hipGetDevice(&dev));
+  MPS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   for (int k = 0; k < 6; k++) {
     const size_t n = param_count(k);
     w[k] = dalloc<float>(n);
@@ -621,8 +856,9 @@ void Model::release() {
 void Model::forward(hipStream_t s, size_t m, const __bf16* x, float* out, bool train) {
   if (m == 0) return;
   if (!train) {  // inference: the fused kernel, no intermediate in memory
-    const size_t waves = (m + kRowsPerWave - 1) / kRowsPerWave;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, s, m, x,
+    const size_t nbatch = (m + kBatch - 1) / kBatch;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>(nbatch, static_cast<size_t>(cus > 0 ? cus : 256)));
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kMlpThreads), 0, s, m, x,
                        static_cast<const __bf16*>(w1b), static_cast<const float*>(w[1]),
                        static_cast<const __bf16*>(w2p), static_cast<const float*>(w[3]),
                        static_cast<const float*>(w[4]), static_cast<const float*>(w[5]), out);
@@ -746,80 +982,99 @@ void Dataset::release() {
 // --------------------------- use inside a step -------------------------------------
 
 // Residual of the current x (= p_prev), the scales, and the features of every
-// particle into c.feats.  Returns false if there are no alive rows.
-bool learned_features(Context& c, double& s_r, double& rr) {
+// particle, all on the stream without a synchronisation (the scales stay on
+// the device, in red_out).
+void learned_features(Context& c) {
   const size_t n = c.n;
-  double bb = 0;
-  residual64(c, rr, bb, false);
-  hipLaunchKernelGGL(k_alive_stats, dim3(256), dim3(256), 0, c.stream, n, static_cast<const uint8_t*>(c.type),
-                     static_cast<const double*>(c.p), c.partB, c.partC);
-  reduce2(c, c.partB, c.partC, 256);
-  const double pp = c.red_host[0], nalive = c.red_host[1];
-  if (nalive <= 0) return false;
-  hipLaunchKernelGGL(k_alive_stats, dim3(256), dim3(256), 0, c.stream, n, static_cast<const uint8_t*>(c.type),
-                     static_cast<const double*>(c.diag), c.partB, c.partC);
-  reduce2(c, c.partB, c.partC, 256);
-  c.learned_drms = std::sqrt(c.red_host[0] / nalive);
-  s_r = std::sqrt(rr / nalive);
-  const double s_p = std::sqrt(pp / nalive);
   if (!c.feats || c.feats_cap < n) {
     dfree(c.feats);
     dfree(c.ypred);
+    dfree(c.feat_rec);
     c.feats = dalloc<__bf16>(n * kFeatPad);
     c.ypred = dalloc<float>(n);
+    c.feat_rec = dalloc<FeatRec>(n);
     c.feats_cap = n;
   }
-  FeatArgs a{n, c.x, c.z, c.p, c.r, c.nd, c.type, c.nbr, c.ncount, c.kstride, c.env.r_lap * c.env.r_lap,
-             1.0 / c.env.l0, c.env.n0, s_p > 0 ? 1.0 / s_p : 0.0, s_r > 0 ? 1.0 / s_r : 0.0, c.feats};
-  tracked(c, MPS_K_FEATURES, n * (8.0 * 5 + 1 + 4 + kFeatPad * 2) + 4.0 * c.nbr_total, [&] {
-    hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, c.stream, a);
-  });
+  if (c.kmax > 65536u) throw Failure{MPS_ERR_BAD_ARG, "neighbour list bound exceeds 16-bit positions"};
+  residual64_launch(c);
+  hipLaunchKernelGGL(k_alive_stats3, dim3(256), dim3(256), 0, c.stream, n, static_cast<const uint8_t*>(c.type),
+                     static_cast<const double*>(c.p), static_cast<const double*>(c.diag), c.part3a);
+  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.part3a),
+                     c.red_out + 2);
+  hipLaunchKernelGGL(k_feat_scales, dim3(1), dim3(1), 0, c.stream, c.red_out);
   check_launch();
-  return true;
+  FeatRec* rec = static_cast<FeatRec*>(c.feat_rec);
+  const FeatArgs a{n, c.x, c.z, c.nd, c.type, rec, c.nbr, c.ncount, c.kstride, c.env.r_lap * c.env.r_lap,
+                   1.0 / c.env.l0, c.env.n0, c.feats};
+  // records: type, x, z, p, r0 read, 32 bytes written; features: the
+  // particle's own state, its list and one record per listed neighbour
+  // (counted once), the row written
+  tracked(c, MPS_K_FEATURES, n * (33.0 + 32) + n * (8.0 * 3 + 1 + 4 + 32 + kFeatPad * 2) + 4.0 * c.nbr_total,
+          [&] {
+            hipLaunchKernelGGL(k_feat_rec, dim3(blocks(n)), dim3(256), 0, c.stream, n,
+                               static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.x),
+                               static_cast<const double*>(c.z), static_cast<const double*>(c.p),
+                               static_cast<const double*>(c.r), static_cast<const double*>(c.red_out), rec);
+            if (c.kmax <= 256u)  // list positions fit 8 bits
+              hipLaunchKernelGGL(k_features<uint8_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
+            else
+              hipLaunchKernelGGL(k_features<uint16_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
+          });
+  check_launch();
 }
 
-// Scale of the model output: the residual's RMS over the diagonal's RMS, the
-// size of a Jacobi correction.
-double learned_scale(const Context& c, double s_r) {
-  return c.learned_drms > 0 ? s_r / c.learned_drms : 0.0;
-}
-
-// MPS_WARM_LEARNED: x0 = p_prev + s_d MLP(features); keeps p_prev when the
-// prediction does not lower the FP64 residual.
+// MPS_WARM_LEARNED: x0 = p_prev + s_d MLP(features), kept only when it lowers
+// the FP64 residual (decided on the device).  Nothing waits here: a mixed
+// solve reads the outcome at its first synchronisation (learned_resolve),
+// which finds r = b - A x0 and its norms already computed; the FP64 CG reads
+// it at once.
 void learned_warm_start(Context& c) {
   c.stats.learned_used = 0;
   if (!c.model) return;
-  double s_r = 0, rr_prev = 0;
-  if (!learned_features(c, s_r, rr_prev) || s_r == 0) return;
   const size_t n = c.n;
+  learned_features(c);
   const double flops = 2.0 * n * (static_cast<double>(kFeatPad) * kHid + kHid * kHid + kHid);
   tracked(c, MPS_K_MLP, flops, [&] { c.model->forward(c.stream, n, c.feats, c.ypred, false); });
   hipLaunchKernelGGL(k_apply_pred, dim3(blocks(n)), dim3(256), 0, c.stream, n,
                      static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.p),
-                     static_cast<const float*>(c.ypred), learned_scale(c, s_r), c.sol);
+                     static_cast<const float*>(c.ypred), static_cast<const double*>(c.red_out), c.sol);
+  residual64_launch(c);
+  hipLaunchKernelGGL(k_revert_unless_better, dim3(blocks(n)), dim3(256), 0, c.stream, n,
+                     static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.p),
+                     static_cast<const double*>(c.red_out), c.sol);
   check_launch();
-  double rr_new = 0, bb = 0;
-  residual64(c, rr_new, bb, false);
-  c.stats.prev_rel_residual = bb > 0 ? std::sqrt(rr_prev / bb) : 0;
-  if (rr_new < rr_prev) {
-    c.stats.learned_used = 1;
-  } else {
-    hipLaunchKernelGGL(k_revert, dim3(blocks(n)), dim3(256), 0, c.stream, n,
-                       static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.p), c.sol);
-    check_launch();
+  MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, kRedOut * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  c.learned_pending = true;
+  if (c.solver.kind != MPS_SOLVER_MIXED_IR) {
+    double rr = 0, bb = 0;
+    learned_resolve(c, rr, bb);
   }
+}
+
+bool learned_resolve(Context& c, double& rr, double& bb) {
+  c.learned_pending = false;
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  const double rr_new = c.red_host[0], bbv = c.red_host[1], rr_prev = c.red_host[6];
+  c.stats.prev_rel_residual = bbv > 0 ? std::sqrt(rr_prev / bbv) : 0;
+  if (!(rr_new < rr_prev)) return false;  // x went back to p_prev: r is not its residual
+  c.stats.learned_used = 1;
+  rr = rr_new;
+  bb = bbv;
+  return true;
 }
 
 void collect_before(Context& c) {
   if (!c.dataset) return;
-  double rr = 0;
-  c.collect_ok = learned_features(c, c.collect_sr, rr) && c.collect_sr > 0;
+  learned_features(c);
+  MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, kRedOut * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  c.collect_sd = c.red_host[9];
+  c.collect_ok = c.collect_sd > 0;
 }
 
 void collect_after(Context& c) {
   if (!c.dataset || !c.collect_ok) return;
-  const double sd = learned_scale(c, c.collect_sr);
-  if (!(sd > 0)) return;
+  const double sd = c.collect_sd;
   const uint32_t thr = static_cast<uint32_t>(std::min(1.0, std::max(0.0, c.collect_frac)) * 4294967295.0);
   const CollectArgs a{c.n, c.type, c.feats, c.sol, c.p, c.x, c.z, c.nbr, c.ncount, c.kstride,
                       c.env.r_lap * c.env.r_lap, 1.0 / sd,
