@@ -83,6 +83,24 @@ def test_gpu_forward_matches_oracle():
 
 
 @pytest.mark.gpu
+def test_gpu_forward_row_counts():
+    """The inference kernel walks 64-row batches, several per block when the
+    rows outnumber 64 x the compute units: tails, a single row and a
+    multi-pass count all match the rounding-point emulation."""
+    from openmps_amd.learned import Model
+    p = random_params(11)
+    m = Model(0)
+    m.set_params(p)
+    rng = np.random.Generator(np.random.PCG64(12))
+    for rows in (1, 63, 65, 40_003):
+        x = rng.normal(size=(rows, FEAT)).astype(np.float32).astype(np.float64)
+        y = m.forward(x).astype(np.float64)
+        ye = emulate_gpu_forward(p, x)
+        assert y.shape == (rows,)
+        assert np.abs(y - ye).max() <= 2e-3 * (np.abs(ye).max() + 1e-3), rows
+
+
+@pytest.mark.gpu
 def test_gpu_gradients_match_oracle():
     from openmps_amd.learned import Model
     p = random_params(5)
@@ -111,8 +129,6 @@ def _small_scene(seed=5):
 def test_features_definition():
     """The feature rows equal the definition evaluated in numpy from the
     particle state, the assembled system and r0 = b - A p_prev."""
-    import scipy.sparse
-    from openmps_amd import learned
     parts, env = _small_scene()
     c = cases.make("gpu", env, 1e-10)
     c.add_particles(parts)
@@ -122,6 +138,29 @@ def test_features_definition():
     for st in ("search", "density", "error_correction", "explicit", "density", "save_x",
                "ppe_assemble"):
         c.stage(st)
+    _check_features(c)
+
+
+@pytest.mark.gpu
+def test_features_keep_the_32_nearest():
+    """A lattice 1.2 times denser than l0 gives about 43 neighbours within
+    r_lap: the rows keep the 32 nearest (the second walk of the feature
+    kernel), and their count feature exceeds 1."""
+    parts, env = _small_scene()
+    env = dict(env, l0=0.0012)
+    c = cases.make("gpu", env, 1e-10)
+    c.add_particles(parts)
+    c.set_time(1e-4, 1e-4)
+    for st in ("search", "density", "error_correction", "explicit", "density", "save_x",
+               "ppe_assemble"):
+        c.stage(st)
+    F = _check_features(c)
+    assert (F[:, 0] > 1.0).sum() >= 500  # rows with more than 32 accepted neighbours
+
+
+def _check_features(c):
+    import scipy.sparse
+    from openmps_amd import learned
     F = learned.features(c)
     rp, col, val, b, _ = c.ppe()
     P = c.particles()
@@ -131,6 +170,8 @@ def test_features_definition():
     alive = (P["type"] != 2) & (P["type"] != 3)
     s_r = np.sqrt((r0[alive] ** 2).sum() / alive.sum())
     s_p = np.sqrt((P["p"][alive] ** 2).sum() / alive.sum())
+    isr = 1.0 / s_r if s_r > 0 else 0.0  # a zero scale zeroes its features
+    isp = 1.0 / s_p if s_p > 0 else 0.0
     cnt, ids = c.neighbors()
     off = np.concatenate([[0], np.cumsum(cnt.astype(np.int64))])
     e = c.env()
@@ -147,15 +188,16 @@ def test_features_definition():
         want = np.zeros(FEAT)
         want[0] = len(js) / 32.0
         want[1] = (P["n"][i] - e["n0"]) / e["n0"]
-        want[2] = P["p"][i] / s_p
-        want[3] = r0[i] / s_r
+        want[2] = P["p"][i] * isp
+        want[3] = r0[i] * isr
         for k, q in enumerate(order):
-            want[4 + 4 * k:8 + 4 * k] = [dx[q] / e["l0"], dz[q] / e["l0"], P["p"][js[q]] / s_p,
-                                         r0[js[q]] / s_r]
+            want[4 + 4 * k:8 + 4 * k] = [dx[q] / e["l0"], dz[q] / e["l0"], P["p"][js[q]] * isp,
+                                         r0[js[q]] * isr]
         if not np.allclose(F[i], omlp.to_bf16(want), rtol=1e-2, atol=2e-2):
             bad += 1
     assert bad <= 3  # ties in float distance may swap two equal-distance neighbours
     assert np.all(F[~alive] == 0)
+    return F
 
 
 @pytest.mark.gpu
