@@ -172,7 +172,8 @@ typedef struct mps_solve_stats {
   double initial_rel_residual; /* |b - A x0| / |b| */
   double solve_ms;          /* device time of the solve (HIP events), warm start included */
   int64_t learned_used;     /* 1: the learned guess replaced the previous-step guess */
-  double prev_rel_residual; /* |b - A p_prev| / |b| when the learned guess was evaluated */
+  double prev_rel_residual; /* |b - A p_prev| / |b| when the learned guess was evaluated (A p_prev in
+                               FP32 for systems assembled from particles) */
   int64_t fallback;         /* mixed: 1 when a refinement sweep stalled and the FP64 CG finished */
 } mps_solve_stats;
 int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out);

@@ -1119,6 +1119,57 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
 }
 
+// r0 = b - A p with the FP32 matrix-free operator (the learned warm start's
+// feature input, for which FP32 is ample), written into the feature records
+// with the positions and p, and block partials of |r0|^2, |b|^2 and, over
+// alive rows, of p^2, diag^2 and their count: sum k at pa[k G + block] for
+// k < 3, pb[(k - 3) G + block] after, G = gridDim.x.
+__global__ void __launch_bounds__(256) k_learn_resid32(MatFree A, const double* __restrict__ b,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ diag,
+                                                       const uint8_t* __restrict__ type,
+                                                       const double* __restrict__ x, const double* __restrict__ z,
+                                                       FeatRec* __restrict__ rec, double* __restrict__ pa,
+                                                       double* __restrict__ pb) {
+  __shared__ double lds[4];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < A.n; i += gridDim.x * 256ull) {
+    const double pv = p[i], bv = b[i];
+    const float ap = op_apply<4>(A, i, static_cast<float>(pv), [&](uint32_t j) { return static_cast<float>(p[j]); });
+    const double rv = bv - static_cast<double>(ap);
+    const uint8_t t = type[i];
+    rec[i] = FeatRec{(t == kDummy) ? __builtin_nan("") : x[i], z[i], pv, rv};
+    s[0] += rv * rv;
+    s[1] += bv * bv;
+    if (t != kDummy && t != kDisabled) {
+      s[2] += pv * pv;
+      s[3] += diag[i] * diag[i];
+      s[4] += 1.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double tot = block_sum<256>(s[k], lds);
+    if (threadIdx.x == 0) (k < 3 ? pa : pb)[(k % 3) * gridDim.x + blockIdx.x] = tot;
+  }
+}
+
+// out[k] = the sum of partials k of k_learn_resid32 (one block), then the
+// feature scales from them.
+__global__ void __launch_bounds__(256) k_reduce5(const double* __restrict__ pa, const double* __restrict__ pb, int G,
+                                                 double* __restrict__ out) {
+  __shared__ double lds[4];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double* part = (k < 3 ? pa : pb) + (k % 3) * G;
+    double v = 0;
+    for (int j = threadIdx.x; j < G; j += 256) v += part[j];
+    const double tot = block_sum<256>(v, lds);
+    if (threadIdx.x == 0) out[k] = tot;
+  }
+  if (threadIdx.x == 0) feat_scales(out);
+}
+
 }  // namespace
 
 // Single-block reduction of two partial arrays into c.red_host[0..1] (synchronous).
@@ -1563,6 +1614,27 @@ void ppe_solve(Context& c) {
   c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
   if (rc != MPS_OK) throw Failure{rc, msg};
   if (c.dataset && on_particles) collect_after(c);
+}
+
+
+// The learned warm start's residual r0 of x = p, its feature records and
+// scales, from particles' systems only (the matrix-free operator):
+// red_out[0..4] = |r0|^2, |b|^2, sum p^2, sum diag^2, alive count and
+// red_out[6..9] the scales (feat_scales).  Returns false (nothing enqueued)
+// for a system given as a matrix.
+bool learned_residual32(Context& c, FeatRec* rec) {
+  if (!c.ppe_from_particles || c.ppe_n != c.n) return false;
+  const Launch L = cg_grid(c.ppe_n);
+  // four wavefronts per SIMD for the gathers; the partials fit part3a, part3b
+  const int G = static_cast<int>(std::min<size_t>((c.ppe_n + 255) / 256, kRedBlocks));
+  hipLaunchKernelGGL(k_learn_resid32, dim3(G), dim3(256), 0, c.stream, matfree(c, L.rpb),
+                     static_cast<const double*>(c.b), static_cast<const double*>(c.p),
+                     static_cast<const double*>(c.diag), static_cast<const uint8_t*>(c.type),
+                     static_cast<const double*>(c.x), static_cast<const double*>(c.z), rec, c.part3a, c.part3b);
+  hipLaunchKernelGGL(k_reduce5, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.part3a),
+                     static_cast<const double*>(c.part3b), G, c.red_out);
+  check_launch();
+  return true;
 }
 
 }  // namespace mps

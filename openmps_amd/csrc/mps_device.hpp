@@ -144,4 +144,31 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* lds, doub
   __syncthreads();
 }
 
+// What the learned warm start's feature pass gathers of a neighbour, one
+// 32-byte record (one cache line access where four arrays cost four): the
+// position (x = NaN for dummy particles, which no r^2 < r_lap^2 test
+// accepts), the pressure p_prev and the residual r0 = b - A p_prev.
+struct __attribute__((aligned(32))) FeatRec {
+  double x, z, p, r0;
+};
+
+// The feature scales from the reduced sums out[0] = |r0|^2, out[2] = sum p^2,
+// out[3] = sum diag^2, out[4] = alive count (one thread): out[6..9] =
+// |r0|^2, 1/s_p, 1/s_r and s_d = s_r / RMS(diag) over the alive rows, with
+// s_v = RMS(v); zero where a scale is zero or no row is alive.
+__device__ __forceinline__ void feat_scales(double* out) {
+  const double rr = out[0], pp = out[2], dd = out[3], na = out[4];
+  double inv_sp = 0, inv_sr = 0, sd = 0;
+  if (na > 0) {
+    const double s_r = sqrt(rr / na), s_p = sqrt(pp / na), drms = sqrt(dd / na);
+    inv_sp = s_p > 0 ? 1.0 / s_p : 0.0;
+    inv_sr = s_r > 0 ? 1.0 / s_r : 0.0;
+    sd = (drms > 0 && s_r > 0) ? s_r / drms : 0.0;
+  }
+  out[6] = rr;
+  out[7] = inv_sp;
+  out[8] = inv_sr;
+  out[9] = sd;
+}
+
 }  // namespace mps

@@ -365,6 +365,8 @@ void ppe_solve(Context& c);              // mps_cg.hip
 void residual64(Context& c, double& rr, double& bb, bool track);
 void reduce2(Context& c, const double* a, const double* b, int nparts);  // -> c.red_host[0..1]
 void residual64_launch(Context& c);  // r = b - A x; |r|^2, |b|^2 -> c.red_out[0..1] (no sync)
+struct FeatRec;
+bool learned_residual32(Context& c, FeatRec* rec);  // mps_cg.hip: see there
 // mps_mlp.hip.  learned_features enqueues (no synchronisation) the residual
 // of x = p_prev, the scales and the feature rows; red_out then holds |r|^2,
 // |b|^2 [0, 1], the alive sums of p^2, diag^2 and count [2..4], and

@@ -242,117 +242,44 @@ __global__ void __launch_bounds__(256) k_gather_rows(size_t m, const uint32_t* _
   if (c == 0) tb[r] = src_t[s];
 }
 
-// What the feature pass gathers of a neighbour, in one 32-byte record (one
-// cache line access where four arrays cost four): the position (x = NaN for
-// dummy particles, which no r^2 < r_lap^2 test accepts) and the scaled
-// pressure and residual.
-struct __attribute__((aligned(32))) FeatRec {
-  double x, z;
-  float ps, rs;
-  uint32_t pad[2];
-};
-
-// The feature scales from the reduced sums (one thread): out[6..9] =
-// |r_prev|^2, 1/s_p, 1/s_r, s_d = s_r / RMS(diag) over the alive rows, with
-// s_v = RMS(v); zero where a scale is zero or no row is alive.
-__global__ void k_feat_scales(double* __restrict__ out) {
-  const double rr = out[0], pp = out[2], dd = out[3], na = out[4];
-  double inv_sp = 0, inv_sr = 0, sd = 0;
-  if (na > 0) {
-    const double s_r = sqrt(rr / na), s_p = sqrt(pp / na), drms = sqrt(dd / na);
-    inv_sp = s_p > 0 ? 1.0 / s_p : 0.0;
-    inv_sr = s_r > 0 ? 1.0 / s_r : 0.0;
-    sd = (drms > 0 && s_r > 0) ? s_r / drms : 0.0;
-  }
-  out[6] = rr;
-  out[7] = inv_sp;
-  out[8] = inv_sr;
-  out[9] = sd;
-}
-
-__global__ void __launch_bounds__(256) k_feat_rec(size_t n, const uint8_t* __restrict__ type,
-                                                  const double* __restrict__ x, const double* __restrict__ z,
-                                                  const double* __restrict__ p, const double* __restrict__ r0,
-                                                  const double* __restrict__ scales, FeatRec* __restrict__ rec) {
-  const size_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  const double inv_sp = scales[7], inv_sr = scales[8];
-  FeatRec o;
-  o.x = (type[i] == kDummy) ? __builtin_nan("") : x[i];
-  o.z = z[i];
-  o.ps = static_cast<float>(p[i] * inv_sp);
-  o.rs = static_cast<float>(r0[i] * inv_sr);
-  o.pad[0] = o.pad[1] = 0u;
-  rec[i] = o;
-}
-
-// sums over alive rows of p^2 and diag^2, and the alive count (block partials
-// at part[0..G), part[G..2G), part[2G..3G) for G = gridDim.x).
-__global__ void __launch_bounds__(256) k_alive_stats3(size_t n, const uint8_t* __restrict__ type,
-                                                      const double* __restrict__ p,
-                                                      const double* __restrict__ diag,
-                                                      double* __restrict__ part) {
-  __shared__ double lds[4];
-  double sp = 0, sd = 0, cnt = 0;
-  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) {
-    const uint8_t t = type[i];
-    if (t != kDummy && t != kDisabled) {
-      sp += p[i] * p[i];
-      sd += diag[i] * diag[i];
-      cnt += 1.0;
-    }
-  }
-  const double s1 = block_sum<256>(sp, lds);
-  const double s2 = block_sum<256>(sd, lds);
-  const double s3 = block_sum<256>(cnt, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = s1;
-    part[gridDim.x + blockIdx.x] = s2;
-    part[2 * gridDim.x + blockIdx.x] = s3;
-  }
-}
-
-// out[k] = sum of part[256 k .. 256 k + 255], k = 0, 1, 2 (one block).
-__global__ void __launch_bounds__(256) k_reduce3(const double* __restrict__ part, double* __restrict__ out) {
-  __shared__ double lds[4];
-  const double s0 = block_sum<256>(part[threadIdx.x], lds);
-  const double s1 = block_sum<256>(part[256 + threadIdx.x], lds);
-  const double s2 = block_sum<256>(part[512 + threadIdx.x], lds);
-  if (threadIdx.x == 0) {
-    out[0] = s0;
-    out[1] = s1;
-    out[2] = s2;
-  }
-}
-
 struct FeatArgs {
   size_t n;
   const double *x, *z, *nd;
   const uint8_t* type;
   const FeatRec* rec;
-  const uint32_t *ids, *count;
-  uint32_t kstride;
-  double rl2, inv_l0, n0;
+  const double* scales;  // [7] = 1/s_p, [8] = 1/s_r
+  // the assembled system's rows: their off-diagonal columns are the accepted
+  // neighbours (non-dummy, r^2 < r_lap^2), ascending (SELL-64 slices: int16
+  // offsets from the row, or u32 columns in wide slices)
+  const uint64_t* off;
+  const uint32_t* col;
+  const int16_t* col16;
+  const uint8_t* wide;
+  const uint32_t* len;
+  double inv_l0, n0;
   __bf16* out;
 };
 
-struct FeatNbr {
-  double x, z;
-};
+// Column k of row i of the system.
+__device__ __forceinline__ uint32_t feat_col(const FeatArgs& a, size_t i, bool wide, uint64_t base, uint32_t k) {
+  const size_t at = base + static_cast<size_t>(k) * 64 + (i & 63);
+  return wide ? a.col[at] : static_cast<uint32_t>(static_cast<int>(i) + a.col16[at]);
+}
 
 // Features of every particle (zero rows for dummy / disabled particles).
-// One thread per particle; its kept neighbours are entries of LDS arrays:
-// the key float(r^2) and the list position Q (8 or 16 bits), 160-192 bytes
-// per thread.  The walk issues eight neighbours' loads at once and appends
-// the first kNbr accepted neighbours as entries.  A row then places entry e
-// at its rank among the entries (keys ascending, equal keys in list order,
-// as a stable sort orders them), counted over the keys in LDS: independent
-// loads, where sorting on every append waits on one LDS load per shifted
-// entry.  A particle with more than kNbr accepted neighbours walks its list
-// a second time with an insertion sort that keeps the kNbr nearest.
-constexpr int kFeatWalk = 8;    // neighbours whose loads the walk issues together
-constexpr int kFeatGather = 8;  // ranks whose neighbours' state is gathered together
-static_assert(kNbr % kFeatGather == 0, "whole gather groups");
+// One thread per particle.  Its accepted neighbours are the off-diagonal
+// columns of its row of the assembled system (the same test, r^2 < r_lap^2
+// and not dummy, picked them), read as coalesced SELL slots; their
+// positions come from the records.  Kept neighbours are entries of LDS
+// arrays: the key float(r^2) and the column position Q (8 or 16 bits),
+// 160-192 bytes per thread.  The first kNbr columns become entries, and a
+// row places entry e at its rank among them (keys ascending, equal keys by
+// ascending neighbour index), counted over the keys in LDS.  A row with more
+// than kNbr columns goes through an insertion sort (stable, in column
+// order) that keeps the kNbr nearest.
+constexpr int kFeatWalk = 8;    // columns whose loads are issued together
+constexpr int kFeatGather = 4;  // ranks whose neighbours' state is gathered together
+static_assert(kNbr % kFeatGather == 0 && kNbr % kFeatWalk == 0, "whole groups");
 
 template <class Q>
 __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
@@ -360,48 +287,45 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
   __shared__ Q sq[kNbr][256];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * 256ull + threadIdx.x;
-  if (i >= a.n) return;
+  const bool live = i < a.n;  // every thread reaches the barriers of the row output
   const int t = threadIdx.x;
-  const uint32_t ii = static_cast<uint32_t>(i);
-  __bf16* row = a.out + i * kFeatPad;
-  const uint8_t ti = a.type[i];
-  const double xi = a.x[i], zi = a.z[i];
+  const uint8_t ti = live ? a.type[i] : kDummy;
+  const double xi = live ? a.x[i] : 0.0, zi = live ? a.z[i] : 0.0;
+  const double inv_sp = a.scales[7], inv_sr = a.scales[8];
+  const size_t sl = i >> 6;
+  const bool wide = live && a.wide[sl] != 0;
+  const uint64_t base = live ? a.off[sl] : 0;
   float f[4] = {0, 0, 0, 0};
   int m = 0;
   bool sorted = false;
   if (ti != kDummy && ti != kDisabled) {
-    const uint32_t count = a.count[i];
-    int cnt = 0;
-    uint32_t q = 0;
-    walk_list<kFeatWalk, FeatNbr>(
-        ii, count, a.ids, a.kstride,
-        [&](uint32_t j) {
-          const double2 xz = *reinterpret_cast<const double2*>(&a.rec[j]);
-          return FeatNbr{xz.x, xz.y};
-        },
-        [&](uint32_t, const FeatNbr& nb) {
-          const uint32_t qj = q++;
-          const double dx = nb.x - xi, dz = nb.z - zi;
-          const double r2 = dx * dx + dz * dz;
-          if (!(r2 < a.rl2)) return;  // also a dummy's NaN
-          cnt++;
-          if (m < kNbr) {
-            skey[m][t] = __float_as_uint(static_cast<float>(r2));
-            sq[m][t] = static_cast<Q>(qj);
-            m++;
+    const uint32_t cnt = a.len[i];
+    auto key = [&](uint32_t j) {
+      const double2 xz = *reinterpret_cast<const double2*>(&a.rec[j]);
+      const double dx = xz.x - xi, dz = xz.y - zi;
+      return __float_as_uint(static_cast<float>(dx * dx + dz * dz));
+    };
+    if (cnt <= static_cast<uint32_t>(kNbr)) {
+      m = static_cast<int>(cnt);
+      for (int k0 = 0; k0 < m; k0 += kFeatWalk) {
+        uint32_t jj[kFeatWalk];
+#pragma unroll
+        for (int u = 0; u < kFeatWalk; u++) jj[u] = (k0 + u < m) ? feat_col(a, i, wide, base, k0 + u) : i;
+#pragma unroll
+        for (int u = 0; u < kFeatWalk; u++) {
+          const uint32_t kf = key(jj[u]);
+          if (k0 + u < m) {
+            skey[k0 + u][t] = kf;
+            sq[k0 + u][t] = static_cast<Q>(k0 + u);
           }
-        });
-    if (cnt > kNbr) {  // keep the kNbr nearest: stable insertion sort over the whole list
-      m = 0;
-      for (uint32_t qq = 0; qq < count; qq++) {
-        const uint32_t j = a.ids[nbr_slot(ii, qq, a.kstride)];
-        const double dx = a.rec[j].x - xi, dz = a.rec[j].z - zi;
-        const double r2 = dx * dx + dz * dz;
-        if (!(r2 < a.rl2)) continue;  // also a dummy's NaN
-        const uint32_t kf = __float_as_uint(static_cast<float>(r2));
+        }
+      }
+    } else {  // keep the kNbr nearest: stable insertion sort in column order
+      for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t kf = key(feat_col(a, i, wide, base, k));
         int pos;
         if (m < kNbr) pos = m++;
-        else if (kf < skey[kNbr - 1][t]) pos = kNbr - 1;  // an equal key listed earlier ranks first
+        else if (kf < skey[kNbr - 1][t]) pos = kNbr - 1;  // an equal key with a lower index ranks first
         else continue;
         while (pos > 0 && skey[pos - 1][t] > kf) {
           skey[pos][t] = skey[pos - 1][t];
@@ -409,17 +333,17 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
           pos--;
         }
         skey[pos][t] = kf;
-        sq[pos][t] = static_cast<Q>(qq);
+        sq[pos][t] = static_cast<Q>(k);
       }
       sorted = true;
     }
-    f[0] = cnt / 32.0f;
+    f[0] = static_cast<float>(cnt) / 32.0f;
     f[1] = static_cast<float>((a.nd[i] - a.n0) / a.n0);
-    f[2] = a.rec[i].ps;
-    f[3] = a.rec[i].rs;
+    f[2] = static_cast<float>(a.rec[i].p * inv_sp);
+    f[3] = static_cast<float>(a.rec[i].r0 * inv_sr);
   }
-  // rank of every entry (keys ascending, equal keys in list order), then
-  // the neighbours' indices in rank order over the keys (their loads are in
+  // rank of every entry (keys ascending, equal keys by column), then the
+  // neighbours' indices in rank order over the keys (their loads are in
   // flight while the ranks are counted).  (b - k) >> 31 over unsigned keys
   // below 2^31 is 1 exactly when b < k; entry b precedes entry e if its key
   // is smaller, or equal with b < e.  The ranks of entries 4g .. 4g + 3 are
@@ -427,7 +351,7 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
   if (!sorted) {
     uint32_t je[kNbr], rk[kNbr / 4];
 #pragma unroll
-    for (int e = 0; e < kNbr; e++) je[e] = (e < m) ? a.ids[nbr_slot(ii, sq[e][t], a.kstride)] : 0u;
+    for (int e = 0; e < kNbr; e++) je[e] = (e < m) ? feat_col(a, i, wide, base, sq[e][t]) : 0u;
 #pragma unroll
     for (int e0 = 0; e0 < kNbr; e0 += 4) {
       uint32_t r = 0;
@@ -459,12 +383,11 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     for (int e = 0; e < kNbr; e++)
       if (e < m) skey[(rk[e / 4] >> (8 * (e % 4))) & 0xffu][t] = je[e];
   } else {
-    for (int e = 0; e < m; e++) skey[e][t] = a.ids[nbr_slot(ii, sq[e][t], a.kstride)];
+    for (int e = 0; e < m; e++) skey[e][t] = feat_col(a, i, wide, base, sq[e][t]);
   }
   // the row in registers (elements 0-3: the particle's own features;
   // 4 + 4s .. 7 + 4s: the neighbour of rank s, zero for s >= m; zeros from
-  // kFeat on), gathered eight neighbours at a time, then stored as 16-byte
-  // vectors after the last load
+  // kFeat on), gathered four neighbours at a time
   bf16x4 val[kNbr + 1];
   val[0] = bf16x4{__bf16(f[0]), __bf16(f[1]), __bf16(f[2]), __bf16(f[3])};
   const bf16x4 zero4 = bf16x4{__bf16(0.f), __bf16(0.f), __bf16(0.f), __bf16(0.f)};
@@ -475,50 +398,54 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     if (s0 < m) {
       uint32_t jj[kFeatGather];
 #pragma unroll
-      for (int u = 0; u < kFeatGather; u++) jj[u] = (s0 + u < m) ? skey[s0 + u][t] : ii;
+      for (int u = 0; u < kFeatGather; u++) jj[u] = (s0 + u < m) ? skey[s0 + u][t] : static_cast<uint32_t>(i);
 #pragma unroll
       for (int u = 0; u < kFeatGather; u++) {
         const FeatRec rj = a.rec[jj[u]];
         const bf16x4 v = bf16x4{__bf16(static_cast<float>((rj.x - xi) * a.inv_l0)),
-                                __bf16(static_cast<float>((rj.z - zi) * a.inv_l0)), __bf16(rj.ps),
-                                __bf16(rj.rs)};
+                                __bf16(static_cast<float>((rj.z - zi) * a.inv_l0)),
+                                __bf16(static_cast<float>(rj.p * inv_sp)), __bf16(static_cast<float>(rj.r0 * inv_sr))};
         val[1 + s0 + u] = (s0 + u < m) ? v : zero4;
       }
     }
   }
   static_assert(kFeat == 4 + 4 * kNbr && (kNbr + 1) % 2 == 1 && kFeatPad % 8 == 0, "row layout");
+  // The rows leave through LDS (the key array, free once every thread has
+  // gathered), 16 rows of a wavefront at a time, as contiguous 16-byte
+  // vectors: stored straight from the registers, each wavefront store would
+  // touch 64 rows 320 bytes apart.
+  constexpr int kStageRows = 16;
+  constexpr int kStagePitch = kFeatPad * 2 + 16;  // bytes; the pad spreads a row's vectors over the banks
+  constexpr int kRowVec = kFeatPad / 8;
+  static_assert(4 * kStageRows * kStagePitch <= static_cast<int>(sizeof(skey)), "staging fits the keys");
+  char* const stage = reinterpret_cast<char*>(&skey[0][0]) + (t >> 6) * (kStageRows * kStagePitch);
+  const int lane = t & 63;
+  const size_t wrow0 = blk * 256ull + (t & ~63);  // the wavefront's first row
+  __syncthreads();
+#pragma unroll 1
+  for (int h = 0; h < 64 / kStageRows; h++) {
+    if ((lane / kStageRows) == h) {
+      char* const dst = stage + (lane % kStageRows) * kStagePitch;
 #pragma unroll
-  for (int vv = 0; vv < kFeatPad / 8; vv++) {
-    const bf16x4 lo = (2 * vv < kNbr + 1) ? val[2 * vv] : zero4;
-    const bf16x4 hi = (2 * vv + 1 < kNbr + 1) ? val[2 * vv + 1] : zero4;
-    *reinterpret_cast<bf16x8*>(row + 8 * vv) =
-        bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-  }
-}
-
-// sum over alive rows of v^2 and the alive count (block partials).
-__global__ void __launch_bounds__(256) k_alive_stats(size_t n, const uint8_t* __restrict__ type,
-                                                     const double* __restrict__ v,
-                                                     double* __restrict__ part_vv,
-                                                     double* __restrict__ part_n) {
-  __shared__ double lds[4];
-  double s = 0, c = 0;
-  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) {
-    const uint8_t t = type[i];
-    if (t != kDummy && t != kDisabled) {
-      s += v[i] * v[i];
-      c += 1.0;
+      for (int vv = 0; vv < kRowVec; vv++) {
+        const bf16x4 lo = (2 * vv < kNbr + 1) ? val[2 * vv] : zero4;
+        const bf16x4 hi = (2 * vv + 1 < kNbr + 1) ? val[2 * vv + 1] : zero4;
+        *reinterpret_cast<bf16x8*>(dst + 16 * vv) = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      }
     }
-  }
-  const double s1 = block_sum<256>(s, lds);
-  const double s2 = block_sum<256>(c, lds);
-  if (threadIdx.x == 0) {
-    part_vv[blockIdx.x] = s1;
-    part_n[blockIdx.x] = s2;
+    __syncthreads();
+    const size_t r0 = wrow0 + h * kStageRows;
+#pragma unroll
+    for (int k = 0; k < (kStageRows * kRowVec + 63) / 64; k++) {
+      const int v = lane + 64 * k, rr = v / kRowVec, cc = v % kRowVec;
+      if (v < kStageRows * kRowVec && r0 + rr < a.n)
+        *reinterpret_cast<uint4*>(a.out + (r0 + rr) * kFeatPad + 8 * cc) =
+            *reinterpret_cast<const uint4*>(stage + rr * kStagePitch + 16 * cc);
+    }
+    __syncthreads();
   }
 }
 
-// sol = p_prev + s_d y on alive rows.
 // sol = p_prev + s_d y on alive rows (s_d = scales[9]).
 __global__ void __launch_bounds__(256) k_apply_pred(size_t n, const uint8_t* __restrict__ type,
                                                     const double* __restrict__ p,
@@ -995,31 +922,22 @@ void learned_features(Context& c) {
     c.feat_rec = dalloc<FeatRec>(n);
     c.feats_cap = n;
   }
-  if (c.kmax > 65536u) throw Failure{MPS_ERR_BAD_ARG, "neighbour list bound exceeds 16-bit positions"};
-  residual64_launch(c);
-  hipLaunchKernelGGL(k_alive_stats3, dim3(256), dim3(256), 0, c.stream, n, static_cast<const uint8_t*>(c.type),
-                     static_cast<const double*>(c.p), static_cast<const double*>(c.diag), c.part3a);
-  hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.part3a),
-                     c.red_out + 2);
-  hipLaunchKernelGGL(k_feat_scales, dim3(1), dim3(1), 0, c.stream, c.red_out);
-  check_launch();
+  if (c.kmax > 65536u) throw Failure{MPS_ERR_BAD_ARG, "row length bound exceeds 16-bit positions"};
   FeatRec* rec = static_cast<FeatRec*>(c.feat_rec);
-  const FeatArgs a{n, c.x, c.z, c.nd, c.type, rec, c.nbr, c.ncount, c.kstride, c.env.r_lap * c.env.r_lap,
-                   1.0 / c.env.l0, c.env.n0, c.feats};
-  // records: type, x, z, p, r0 read, 32 bytes written; features: the
-  // particle's own state, its list and one record per listed neighbour
-  // (counted once), the row written
-  tracked(c, MPS_K_FEATURES, n * (33.0 + 32) + n * (8.0 * 3 + 1 + 4 + 32 + kFeatPad * 2) + 4.0 * c.nbr_total,
-          [&] {
-            hipLaunchKernelGGL(k_feat_rec, dim3(blocks(n)), dim3(256), 0, c.stream, n,
-                               static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.x),
-                               static_cast<const double*>(c.z), static_cast<const double*>(c.p),
-                               static_cast<const double*>(c.r), static_cast<const double*>(c.red_out), rec);
-            if (c.kmax <= 256u)  // list positions fit 8 bits
-              hipLaunchKernelGGL(k_features<uint8_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
-            else
-              hipLaunchKernelGGL(k_features<uint16_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
-          });
+  // the residual r0 of x = p_prev (a feature input; also the revert test's
+  // reference), the records and the scales, in FP32 in one pass
+  if (!learned_residual32(c, rec))
+    throw Failure{MPS_ERR_STATE, "learned features need the pressure system assembled from the particles"};
+  const FeatArgs a{n, c.x, c.z, c.nd, c.type, rec, c.red_out, c.slice_off, c.scol, c.col16, c.slice_wide,
+                   c.row_len, 1.0 / c.env.l0, c.env.n0, c.feats};
+  // per particle: x, z, n, type, row length, the records (each once) and the
+  // row written; the system's column offsets
+  tracked(c, MPS_K_FEATURES, n * (8.0 * 3 + 1 + 4 + 32 + kFeatPad * 2) + 2.0 * c.nnz_offdiag, [&] {
+    if (c.kmax <= 256u)  // column positions fit 8 bits
+      hipLaunchKernelGGL(k_features<uint8_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
+    else
+      hipLaunchKernelGGL(k_features<uint16_t>, dim3(blocks(n)), dim3(256), 0, c.stream, a);
+  });
   check_launch();
 }
 
@@ -1030,7 +948,7 @@ void learned_features(Context& c) {
 // it at once.
 void learned_warm_start(Context& c) {
   c.stats.learned_used = 0;
-  if (!c.model) return;
+  if (!c.model || !c.ppe_from_particles) return;
   const size_t n = c.n;
   learned_features(c);
   const double flops = 2.0 * n * (static_cast<double>(kFeatPad) * kHid + kHid * kHid + kHid);
@@ -1064,7 +982,8 @@ bool learned_resolve(Context& c, double& rr, double& bb) {
 }
 
 void collect_before(Context& c) {
-  if (!c.dataset) return;
+  c.collect_ok = false;
+  if (!c.dataset || !c.ppe_from_particles) return;
   learned_features(c);
   MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, kRedOut * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));

@@ -40,6 +40,7 @@ KERNEL_IDS = {
     "k_density_explicit": "density_ecs", "k_density_ppe_count": "density_save",
     "k_ml_prec": "ml_prec",
     "k_cgf_iter<mps::(anonymous namespace)::MatFree>": "cg_iter32",
+    "k_features<unsigned char>": "features", "k_features<unsigned short>": "features",
 }
 
 

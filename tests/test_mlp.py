@@ -184,7 +184,7 @@ def _check_features(c):
         r2 = dx * dx + dz * dz
         keep = (P["type"][js] != 2) & (r2 < rl2)
         js, dx, dz, r2 = js[keep], dx[keep], dz[keep], r2[keep]
-        order = np.argsort(r2.astype(np.float32), kind="stable")[:32]
+        order = np.lexsort((js, r2.astype(np.float32)))[:32]  # float r^2, then index
         want = np.zeros(FEAT)
         want[0] = len(js) / 32.0
         want[1] = (P["n"][i] - e["n0"]) / e["n0"]
@@ -195,7 +195,7 @@ def _check_features(c):
                                          r0[js[q]] * isr]
         if not np.allclose(F[i], omlp.to_bf16(want), rtol=1e-2, atol=2e-2):
             bad += 1
-    assert bad <= 3  # ties in float distance may swap two equal-distance neighbours
+    assert bad <= 3  # bf16 rounding of a feature on either side of a rounding boundary
     assert np.all(F[~alive] == 0)
     return F
 
@@ -239,7 +239,8 @@ def test_learned_warm_start_is_judged_by_the_solver():
         s = c.solve_stats()
         assert s["rel_residual"] <= 1e-8 * 1.5
         if not s["learned_used"]:
-            assert s["initial_rel_residual"] == pytest.approx(s["prev_rel_residual"], rel=1e-9)
+            # the previous guess's residual was evaluated with A p_prev in FP32
+            assert s["initial_rel_residual"] == pytest.approx(s["prev_rel_residual"], rel=1e-5)
 
 
 @pytest.mark.gpu
