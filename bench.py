@@ -216,10 +216,12 @@ def main() -> int:
     others = []
     if not args.no_compare and args.solver != "fp64":
         others.append("fp64")
+        if have_weights and not solvers.is_learned(args.solver):
+            others.append("learned_ml")  # the learned warm start in front of the same mixed solver
     if args.ablation:
         others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_ml", "mixed_ml_zero", "learned_ml",
                                "learned_mixed")
-                   if v != args.solver and (have_weights or "learned" not in v)]
+                   if v != args.solver and v not in others and (have_weights or "learned" not in v)]
     del main_c
     for v in others:
         ref_c = make_computer(cfg, v, args.tol, args.weights)
@@ -285,6 +287,8 @@ def main() -> int:
         "solve": solve,
         "solve_speedup_vs_fp64": (solve["fp64"]["ms_per_step"] / solve_ms
                                   if "fp64" in solve and args.solver != "fp64" else None),
+        "solve_speedup_learned_ml_vs_fp64": (solve["fp64"]["ms_per_step"] / solve["learned_ml"]["ms_per_step"]
+                                             if "fp64" in solve and "learned_ml" in solve else None),
         "roofline": roofline,
         "roofline_fp64_spmv": roofline_fp64,
         "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),

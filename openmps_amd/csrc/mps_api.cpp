@@ -137,7 +137,7 @@ void free_all(Context& c) {
   dfree(c.r); dfree(c.pa); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
-  dfree(c.feats); dfree(c.ypred); dfree(c.feat_rec);
+  dfree(c.feats); dfree(c.ypred); dfree(c.feat_rec); dfree(c.learn_part);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);

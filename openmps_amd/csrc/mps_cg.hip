@@ -1119,23 +1119,33 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
                   "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
 }
 
-// r0 = b - A p with the FP32 matrix-free operator (the learned warm start's
-// feature input, for which FP32 is ample), written into the feature records
+// r0 = b - A p with the FP32 matrix-free operator on records holding p (the
+// learned warm start's feature input, for which FP32 is ample), written into the feature records
 // with the positions and p, and block partials of |r0|^2, |b|^2 and, over
-// alive rows, of p^2, diag^2 and their count: sum k at pa[k G + block] for
-// k < 3, pb[(k - 3) G + block] after, G = gridDim.x.
-__global__ void __launch_bounds__(256) k_learn_resid32(MatFree A, const double* __restrict__ b,
+// alive rows, of p^2, diag^2 and their count: sum k at part[k G + block],
+// G = gridDim.x.
+// {qx, qz, float(p), 0} for every row: the operator's records holding p.
+__global__ void __launch_bounds__(256) k_rec_p(size_t n, const uint2* __restrict__ qpos, const double* __restrict__ p,
+                                               uint4* __restrict__ rec) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    const uint2 q = qpos[i];
+    rec[i] = make_uint4(q.x, q.y, __float_as_uint(static_cast<float>(p[i])), 0u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_learn_resid32(MatFreeRec A, const double* __restrict__ b,
                                                        const double* __restrict__ p,
                                                        const double* __restrict__ diag,
                                                        const uint8_t* __restrict__ type,
                                                        const double* __restrict__ x, const double* __restrict__ z,
-                                                       FeatRec* __restrict__ rec, double* __restrict__ pa,
-                                                       double* __restrict__ pb) {
+                                                       FeatRec* __restrict__ rec, double* __restrict__ part) {
   __shared__ double lds[4];
   double s[5] = {0, 0, 0, 0, 0};
   for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < A.n; i += gridDim.x * 256ull) {
     const double pv = p[i], bv = b[i];
-    const float ap = op_apply<4>(A, i, static_cast<float>(pv), [&](uint32_t j) { return static_cast<float>(p[j]); });
+    float pf;
+    const float ap = rec_apply<4>(A, i, pf);
     const double rv = bv - static_cast<double>(ap);
     const uint8_t t = type[i];
     rec[i] = FeatRec{(t == kDummy) ? __builtin_nan("") : x[i], z[i], pv, rv};
@@ -1150,21 +1160,25 @@ __global__ void __launch_bounds__(256) k_learn_resid32(MatFree A, const double* 
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     const double tot = block_sum<256>(s[k], lds);
-    if (threadIdx.x == 0) (k < 3 ? pa : pb)[(k % 3) * gridDim.x + blockIdx.x] = tot;
+    if (threadIdx.x == 0) part[k * static_cast<size_t>(gridDim.x) + blockIdx.x] = tot;
   }
 }
 
-// out[k] = the sum of partials k of k_learn_resid32 (one block), then the
-// feature scales from them.
-__global__ void __launch_bounds__(256) k_reduce5(const double* __restrict__ pa, const double* __restrict__ pb, int G,
-                                                 double* __restrict__ out) {
-  __shared__ double lds[4];
+// out[k] = the sum of partials k of k_learn_resid32 (one block; the five
+// sums' loads are all issued before any is added), then the feature scales.
+__global__ void __launch_bounds__(1024) k_reduce5(const double* __restrict__ parts, int G, double* __restrict__ out) {
+  __shared__ double lds[16];
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int j = threadIdx.x; j < G; j += 1024) {
+    double x[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) x[k] = parts[static_cast<size_t>(k) * G + j];
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] += x[k];
+  }
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    const double* part = (k < 3 ? pa : pb) + (k % 3) * G;
-    double v = 0;
-    for (int j = threadIdx.x; j < G; j += 256) v += part[j];
-    const double tot = block_sum<256>(v, lds);
+    const double tot = block_sum<1024>(v[k], lds);
     if (threadIdx.x == 0) out[k] = tot;
   }
   if (threadIdx.x == 0) feat_scales(out);
@@ -1349,6 +1363,14 @@ void solve_mixed_cg(Context& c, const Op& A, double b_base, bool wide, const Sto
 }
 
 // Multilevel preconditioner: two kernels per iteration (see k_ml_prec).
+void ensure_zrec(Context& c, size_t n) {
+  if (c.zrec_cap < n) {
+    dfree(c.zrec);
+    c.zrec = dalloc<uint4>(n);
+    c.zrec_cap = n;
+  }
+}
+
 MatFreeRec matfree_rec(const Context& c, size_t rpb) {
   return MatFreeRec{c.slice_off, c.scol, c.slice_off, c.col16, c.slice_wide, c.row_len, c.zrec, c.ghost32,
                     static_cast<float>(3.0 * c.env.r_lap / c.env.n0_lap * c.qpos_inv_h * c.qpos_inv_h *
@@ -1359,11 +1381,7 @@ MatFreeRec matfree_rec(const Context& c, size_t rpb) {
 void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
-  if (c.zrec_cap < n) {
-    dfree(c.zrec);
-    c.zrec = dalloc<uint4>(n);
-    c.zrec_cap = n;
-  }
+  ensure_zrec(c, n);
   const MatFreeRec A = matfree_rec(c, L.rpb);
   float* const zval = reinterpret_cast<float*>(c.zrec) + 2;  // z_i at zval[4 i]
   hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
@@ -1625,14 +1643,23 @@ void ppe_solve(Context& c) {
 bool learned_residual32(Context& c, FeatRec* rec) {
   if (!c.ppe_from_particles || c.ppe_n != c.n) return false;
   const Launch L = cg_grid(c.ppe_n);
-  // four wavefronts per SIMD for the gathers; the partials fit part3a, part3b
-  const int G = static_cast<int>(std::min<size_t>((c.ppe_n + 255) / 256, kRedBlocks));
-  hipLaunchKernelGGL(k_learn_resid32, dim3(G), dim3(256), 0, c.stream, matfree(c, L.rpb),
+  // one row per thread (the gathers want every wavefront the CU can hold);
+  // five partial sums per block
+  const int G = static_cast<int>((c.ppe_n + 255) / 256);
+  if (c.learn_part_cap < static_cast<size_t>(G)) {
+    dfree(c.learn_part);
+    c.learn_part = dalloc<double>(5 * static_cast<size_t>(G));
+    c.learn_part_cap = G;
+  }
+  ensure_zrec(c, c.ppe_n);
+  hipLaunchKernelGGL(k_rec_p, dim3(grid_for(c.ppe_n)), dim3(256), 0, c.stream, c.ppe_n,
+                     static_cast<const uint2*>(c.qpos), static_cast<const double*>(c.p), c.zrec);
+  hipLaunchKernelGGL(k_learn_resid32, dim3(G), dim3(256), 0, c.stream, matfree_rec(c, L.rpb),
                      static_cast<const double*>(c.b), static_cast<const double*>(c.p),
                      static_cast<const double*>(c.diag), static_cast<const uint8_t*>(c.type),
-                     static_cast<const double*>(c.x), static_cast<const double*>(c.z), rec, c.part3a, c.part3b);
-  hipLaunchKernelGGL(k_reduce5, dim3(1), dim3(256), 0, c.stream, static_cast<const double*>(c.part3a),
-                     static_cast<const double*>(c.part3b), G, c.red_out);
+                     static_cast<const double*>(c.x), static_cast<const double*>(c.z), rec, c.learn_part);
+  hipLaunchKernelGGL(k_reduce5, dim3(1), dim3(1024), 0, c.stream, static_cast<const double*>(c.learn_part), G,
+                     c.red_out);
   check_launch();
   return true;
 }

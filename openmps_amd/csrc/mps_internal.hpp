@@ -235,6 +235,8 @@ struct Context {
   bool collect_ok = false;
   __bf16* feats = nullptr;  // n x kFeatPad of the last feature pass
   void* feat_rec = nullptr;  // n 32-byte neighbour records of the feature pass
+  double* learn_part = nullptr;  // 5 partial sums per block of the feature pass's residual
+  size_t learn_part_cap = 0;
   float* ypred = nullptr;
   size_t feats_cap = 0;
 

@@ -271,12 +271,12 @@ __device__ __forceinline__ uint32_t feat_col(const FeatArgs& a, size_t i, bool w
 // columns of its row of the assembled system (the same test, r^2 < r_lap^2
 // and not dummy, picked them), read as coalesced SELL slots; their
 // positions come from the records.  Kept neighbours are entries of LDS
-// arrays: the key float(r^2) and the column position Q (8 or 16 bits),
-// 160-192 bytes per thread.  The first kNbr columns become entries, and a
-// row places entry e at its rank among them (keys ascending, equal keys by
-// ascending neighbour index), counted over the keys in LDS.  A row with more
-// than kNbr columns goes through an insertion sort (stable, in column
-// order) that keeps the kNbr nearest.
+// arrays: the key float(r^2) and (for rows longer than kNbr) the column
+// position Q (8 or 16 bits), 160-192 bytes per thread.  A row of at most
+// kNbr columns makes them its entries, in order, and places entry e at its
+// rank among them (keys ascending, equal keys by ascending neighbour index),
+// counted over the keys in LDS.  A longer row goes through an insertion
+// sort (stable, in column order) that keeps the kNbr nearest.
 constexpr int kFeatWalk = 8;    // columns whose loads are issued together
 constexpr int kFeatGather = 4;  // ranks whose neighbours' state is gathered together
 static_assert(kNbr % kFeatGather == 0 && kNbr % kFeatWalk == 0, "whole groups");
@@ -298,6 +298,9 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
   float f[4] = {0, 0, 0, 0};
   int m = 0;
   bool sorted = false;
+  uint32_t je[kNbr];  // the neighbour of entry e (while entries are in column order)
+#pragma unroll
+  for (int e = 0; e < kNbr; e++) je[e] = 0u;
   if (ti != kDummy && ti != kDisabled) {
     const uint32_t cnt = a.len[i];
     auto key = [&](uint32_t j) {
@@ -305,18 +308,18 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
       const double dx = xz.x - xi, dz = xz.y - zi;
       return __float_as_uint(static_cast<float>(dx * dx + dz * dz));
     };
-    if (cnt <= static_cast<uint32_t>(kNbr)) {
+    if (cnt <= static_cast<uint32_t>(kNbr)) {  // entry e is column e
       m = static_cast<int>(cnt);
-      for (int k0 = 0; k0 < m; k0 += kFeatWalk) {
-        uint32_t jj[kFeatWalk];
 #pragma unroll
-        for (int u = 0; u < kFeatWalk; u++) jj[u] = (k0 + u < m) ? feat_col(a, i, wide, base, k0 + u) : i;
+      for (int k0 = 0; k0 < kNbr; k0 += kFeatWalk) {
+        if (k0 < m) {
 #pragma unroll
-        for (int u = 0; u < kFeatWalk; u++) {
-          const uint32_t kf = key(jj[u]);
-          if (k0 + u < m) {
-            skey[k0 + u][t] = kf;
-            sq[k0 + u][t] = static_cast<Q>(k0 + u);
+          for (int u = 0; u < kFeatWalk; u++)
+            je[k0 + u] = (k0 + u < m) ? feat_col(a, i, wide, base, k0 + u) : static_cast<uint32_t>(i);
+#pragma unroll
+          for (int u = 0; u < kFeatWalk; u++) {
+            const uint32_t kf = key(je[k0 + u]);
+            if (k0 + u < m) skey[k0 + u][t] = kf;
           }
         }
       }
@@ -343,15 +346,12 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     f[3] = static_cast<float>(a.rec[i].r0 * inv_sr);
   }
   // rank of every entry (keys ascending, equal keys by column), then the
-  // neighbours' indices in rank order over the keys (their loads are in
-  // flight while the ranks are counted).  (b - k) >> 31 over unsigned keys
+  // neighbours' indices in rank order over the keys.  (b - k) >> 31 over unsigned keys
   // below 2^31 is 1 exactly when b < k; entry b precedes entry e if its key
   // is smaller, or equal with b < e.  The ranks of entries 4g .. 4g + 3 are
   // the bytes of rk[g].
   if (!sorted) {
-    uint32_t je[kNbr], rk[kNbr / 4];
-#pragma unroll
-    for (int e = 0; e < kNbr; e++) je[e] = (e < m) ? feat_col(a, i, wide, base, sq[e][t]) : 0u;
+    uint32_t rk[kNbr / 4];
 #pragma unroll
     for (int e0 = 0; e0 < kNbr; e0 += 4) {
       uint32_t r = 0;
