@@ -249,7 +249,8 @@ int mps_attach_model(mps_ctx* ctx, mps_model* m);
 /* At every pressure solve, append (features, (p - p_prev) / s_d) of a
  * `fraction` of the alive rows to the dataset (NULL detaches). */
 int mps_attach_dataset(mps_ctx* ctx, mps_dataset* d, double fraction, uint64_t seed);
-/* Features of the current state (computes them): out [n][132]. */
+/* Features of the current state (computes them): out [n][132].  Needs the pressure system
+ * assembled from the particles (MPS_ERR_STATE otherwise). */
 int mps_get_features(mps_ctx* ctx, float* out);
 
 /* ---- measurement (no reference counterpart) ---- */
