@@ -244,6 +244,35 @@ def test_learned_warm_start_is_judged_by_the_solver():
 
 
 @pytest.mark.gpu
+def test_learned_start_needs_a_particle_system():
+    """A system installed with set_ppe over the particles (rows that are not
+    the particles' stencils): the solve starts from the given guess with the
+    model attached, and the feature query fails cleanly."""
+    from openmps_amd import learned
+    from openmps_amd import _lib as L
+    from openmps_amd.computer import MpsError
+    parts, env = _small_scene(3)
+    c = cases.make("gpu", env, 1e-10)
+    c.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, warm_start=L.WARM_LEARNED, tol=1e-10)
+    learned.attach_model(c, learned.Model(1))
+    c.add_particles(parts)
+    n = c.size()
+    # tridiagonal, diagonally dominant, with a known solution
+    import scipy.sparse
+    T = scipy.sparse.diags([-np.ones(n - 1), 4.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csr")
+    T.sort_indices()
+    xs = np.linspace(-1.0, 1.0, n)
+    c.set_ppe(T.indptr.astype(np.uint64), T.indices.astype(np.uint32), T.data.astype(np.float64), T @ xs,
+              np.zeros(n))
+    c.stage("ppe_solve")
+    s = c.solve_stats()
+    assert s["learned_used"] == 0 and s["rel_residual"] <= 1e-10
+    assert np.allclose(c.aux("ppe_x"), xs, atol=1e-8)
+    with pytest.raises(MpsError):
+        learned.features(c)
+
+
+@pytest.mark.gpu
 def test_trained_model_gains_over_previous_guess():
     """What the trained network gains: a model trained on the FP64 solver's own
     snapshots of one scene, applied to another seed, against the previous-step
