@@ -44,6 +44,44 @@ KERNEL_IDS = {
 }
 
 
+# algorithmic bytes per launch of the kernels the library does not time on
+# their own, from the workload's sizes (N rows, Z stored off-diagonals)
+FORMULAS = {
+    "k_apply_du": lambda N, Z: 81 * N,  # u, w, x, z read and written; type
+    "k_explicit_update": lambda N, Z: 97 * N,
+    "k_writeback": lambda N, Z: 17 * N,  # type, sol read; p written
+    "k_ir_correct": lambda N, Z: 20 * N,  # x read and written, d read
+    "k_jacobi_inv": lambda N, Z: 12 * N,
+    "k_rec_init": lambda N, Z: 24 * N,
+    "k_rec_p": lambda N, Z: 32 * N,
+    "k_apply_pred": lambda N, Z: 21 * N,
+    "k_revert_unless_better": lambda N, Z: 17 * N,
+    # b, p, diag, type, x, z; the records written; length, ghost and own
+    # record of the operator's row; 2-byte column offsets
+    "k_learn_resid32": lambda N, Z: 97 * N + 2 * Z,
+    "k_cg_init": lambda N, Z: 12 * Z + 32 * N,
+    # x, z, n, type, row length, the records and the row written; column offsets
+    "k_features<unsigned char>": lambda N, Z: 381 * N + 2 * Z,
+    "k_features<unsigned short>": lambda N, Z: 381 * N + 2 * Z,
+}
+MLP_FLOP_PER_ROW = 2 * (160 * 256 + 256 * 256 + 256)
+DENSE_BF16_PEAK_TFLOPS = 2500.0
+
+
+def sizes(bench: dict):
+    """N and Z of the profiled workload: N from the config, Z from a kernel
+    whose accounted bytes it determines (FP64 SpMV 12 Z + 48 N, or the
+    feature pass 2 Z + 381 N)."""
+    N = bench.get("config", {}).get("particles")
+    Z = None
+    k = {**bench.get("kernels", {}), **bench.get("kernels_fp64", {})}
+    if N and "cg_spmv" in k:
+        Z = (k["cg_spmv"]["bytes_per_launch"] - 48 * N) / 12
+    elif N and "features" in k:
+        Z = (k["features"]["bytes_per_launch"] - 381 * N) / 2
+    return N, Z
+
+
 def short_name(n: str) -> str:
     n = n.replace("void ", "").replace("mps::(anonymous namespace)::", "", 1)
     depth, out = 0, []
@@ -108,19 +146,33 @@ def main(argv=None) -> int:
     lines += ["Durations: rocprofv3 kernel dispatch records.  'working' = launches longer than 10 % of",
               "the kernel's longest (solver launches after convergence exit at once).  Bytes: the",
               "algorithmic bytes per launch the library accounts for the kernel (each array once per",
-              "launch), from the bench JSON of the same command; GB/s = bytes / mean working duration.",
+              "launch), from the bench JSON of the same command, or for kernels the library does not",
+              "time on their own from the workload's sizes (N rows, Z off-diagonals; see FORMULAS in",
+              "openmps_amd/profile_summary.py); GB/s = bytes / mean working duration.  The MLP",
+              "inference is FLOP-bound: TFLOP/s against the 2.5 PFLOP/s dense BF16 peak.",
               "", "| kernel | calls | total ms | % time | mean working us | MB / launch | GB/s | % of 8 TB/s |",
               "|---|---|---|---|---|---|---|---|"]
+    N, Z = sizes(bench)
     for k in sorted(dur, key=lambda k: -dur[k].sum()):
         a = dur[k]
         w = working(a)
         kid = KERNEL_IDS.get(k)
         mb = gbs = frac = "—"
+        bpl = None
         if kid and kid in table and "bytes_per_launch" in table[kid]:
             bpl = table[kid]["bytes_per_launch"]
+        elif k in FORMULAS and N and Z is not None:
+            bpl = FORMULAS[k](N, Z)
+        elif k == "k_copy_probe" and "copy_kernel" in bench:
+            bpl = bench["copy_kernel"]["bytes_moved"]
+        if bpl is not None:
             mb = f"{bpl / 1e6:.1f}"
             g = bpl / (w.mean() * 1e3)
             gbs, frac = f"{g:.0f}", f"{100 * g / HBM_PEAK_GBS:.1f}"
+        elif k == "k_mlp_infer" and N:
+            tf = MLP_FLOP_PER_ROW * N / (w.mean() * 1e-6) / 1e12
+            mb = "(FLOP)"
+            gbs, frac = f"{tf:.0f} TFLOP/s", f"{100 * tf / DENSE_BF16_PEAK_TFLOPS:.1f} of BF16"
         lines.append(f"| `{k}` | {len(a)} | {a.sum() / 1e3:.3f} | {100 * a.sum() / total:.1f} | "
                      f"{w.mean():.2f} | {mb} | {gbs} | {frac} |")
     if args.fetch and args.write:
@@ -143,7 +195,11 @@ def main(argv=None) -> int:
             fm = np.median(f[k]) * 1024 * (cal[0] if cal else 1) / 1e6
             wm = np.median(wr[k]) * 1024 * (cal[1] if cal else 1) / 1e6
             kid = KERNEL_IDS.get(k)
-            alg = f"{table[kid]['bytes_per_launch'] / 1e6:.1f}" if kid in table else "—"
+            alg = "—"
+            if kid in table:
+                alg = f"{table[kid]['bytes_per_launch'] / 1e6:.1f}"
+            elif k in FORMULAS and N and Z is not None:
+                alg = f"{FORMULAS[k](N, Z) / 1e6:.1f}"
             lines.append(f"| `{k}` | {len(f[k])} | {fm:.1f} | {wm:.1f} | {fm + wm:.1f} | {alg} |")
     open(args.out, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
