@@ -158,6 +158,9 @@ void free_all(Context& c) {
   c.events.clear();
   if (c.solve_a) (void)hipEventDestroy(c.solve_a);
   if (c.solve_b) (void)hipEventDestroy(c.solve_b);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  if (c.stream_aux) (void)hipStreamDestroy(c.stream_aux);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }
@@ -306,6 +309,9 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.solver.warm_start = MPS_WARM_PREVIOUS;
     c.solver.inner_tol = 1e-4;
     MPS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    MPS_HIP(hipStreamCreateWithFlags(&c.stream_aux, hipStreamNonBlocking));
+    MPS_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    MPS_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     // Grid.hpp:137-150
     const double NL = c.env.neighbor_length;
     c.gnx = static_cast<long long>(std::ceil((c.env.max_x - c.env.min_x) / NL)) + 2;

@@ -1378,66 +1378,107 @@ MatFreeRec matfree_rec(const Context& c, size_t rpb) {
                     c.ppe_n, rpb};
 }
 
+// After ml_setup_ahead: the main stream waits for the auxiliary stream's
+// set-up; false if none is pending.
+bool join_ml_setup(Context& c) {
+  if (!c.ml_setup_pending) return false;
+  c.ml_setup_pending = false;
+  MPS_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+  return true;
+}
+
+// The multilevel preconditioner's grid of aggregates: cells, 2x2 and 4x4
+// blocks of cells.
+struct MlGeom {
+  uint32_t gnx, gnz, g1x, g1z, bnx, bnz;
+  size_t off1, off2, total_aggs, nblk, sgrid;
+};
+
+MlGeom ml_geom(const Context& c) {
+  MlGeom g;
+  g.gnx = static_cast<uint32_t>(c.gnx);
+  g.gnz = static_cast<uint32_t>(c.gnz);
+  g.g1x = (g.gnx + 1) / 2;
+  g.g1z = (g.gnz + 1) / 2;
+  g.bnx = (g.gnx + 3) / 4;
+  g.bnz = (g.gnz + 3) / 4;
+  g.off1 = static_cast<size_t>(g.gnx) * g.gnz;
+  g.off2 = g.off1 + static_cast<size_t>(g.g1x) * g.g1z;
+  g.total_aggs = g.off2 + static_cast<size_t>(g.bnx) * g.bnz;
+  g.nblk = static_cast<size_t>(g.bnx) * g.bnz;
+  g.sgrid = (g.nblk + NTP / 64 - 1) / (NTP / 64);  // one wavefront per 4x4 block of cells
+  return g;
+}
+
+CellGrid ml_grid(const Context& c, const MlGeom& g) {
+  return CellGrid{c.cell_start, c.cell_count, c.sorted, c.key, c.type, g.gnx, g.gnz, g.g1z, g.bnx, g.bnz,
+                  g.off1, g.off2};
+}
+
+// The multilevel solver's set-up on c.stream, which does not depend on the
+// initial guess: Jacobi factors, the aggregate sums of A from the FP64 system
+// (row sums in z, r, d as scratch, unless the fill of this system summed the
+// rows already), and the list of the 4x4 cell blocks that hold particles (its
+// length reaches the host with the solver's first synchronisation).
+void ml_setup(Context& c) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                     static_cast<const double*>(c.diag), c.dinv32);
+  const MlGeom g = ml_geom(c);
+  if (c.agg_cap < g.total_aggs) {
+    dfree(c.agg_inv);
+    c.agg_inv = dalloc<float>(g.total_aggs);
+    c.agg_cap = g.total_aggs;
+  }
+  if (c.part_ml_cap < g.sgrid) {
+    dfree(c.part_ml);
+    c.part_ml = dalloc<double>(2 * g.sgrid);
+    c.part_ml_cap = g.sgrid;
+  }
+  if (c.ml_blk_cap < g.nblk) {
+    dfree(c.ml_flag);
+    dfree(c.ml_off);
+    dfree(c.ml_list);
+    c.ml_flag = dalloc<uint32_t>(g.nblk);
+    c.ml_off = dalloc<uint32_t>(g.nblk);
+    c.ml_list = dalloc<uint32_t>(g.nblk);
+    c.ml_blk_cap = g.nblk;
+  }
+  if (!c.ml_active) c.ml_active = dalloc<uint64_t>(1);
+  const CellGrid G = ml_grid(c, g);
+  const bool sums = c.ml_rowsums_ready;
+  c.ml_rowsums_ready = false;
+  tracked(c, MPS_K_ML_SETUP, (sums ? 0.0 : c.nnz_offdiag * 12.0 + n * 17.0) + n * 20.0 + g.total_aggs * 4.0 +
+                                 g.nblk * 12.0, [&] {
+    if (!sums)
+      hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream,
+                         sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const uint32_t*>(c.row_len),
+                         static_cast<const uint32_t*>(c.key), static_cast<const uint8_t*>(c.type), g.gnz, c.z32,
+                         c.r32, c.d32);
+    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(g.sgrid)), dim3(NTP), 0, c.stream, G,
+                       static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
+                       static_cast<const float*>(c.d32), c.agg_inv);
+    hipLaunchKernelGGL(k_ml_active, dim3(grid_for(g.nblk)), dim3(256), 0, c.stream, G, c.ml_flag);
+    scan_counts(c, c.ml_flag, g.nblk, c.ml_off, c.ml_active);
+    hipLaunchKernelGGL(k_ml_compact, dim3(grid_for(g.nblk)), dim3(256), 0, c.stream, static_cast<uint32_t>(g.nblk),
+                       static_cast<const uint32_t*>(c.ml_flag), static_cast<const uint32_t*>(c.ml_off),
+                       c.ml_list);
+  });
+  MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+  check_launch();
+}
+
 void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max_iter) {
   const size_t n = c.ppe_n;
   const Launch L = cg_grid(n);
   ensure_zrec(c, n);
   const MatFreeRec A = matfree_rec(c, L.rpb);
   float* const zval = reinterpret_cast<float*>(c.zrec) + 2;  // z_i at zval[4 i]
-  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
-                     static_cast<const double*>(c.diag), c.dinv32);
-  const uint32_t gnx = static_cast<uint32_t>(c.gnx), gnz = static_cast<uint32_t>(c.gnz);
-  const uint32_t g1x = (gnx + 1) / 2, g1z = (gnz + 1) / 2, bnx = (gnx + 3) / 4, bnz = (gnz + 3) / 4;
-  const size_t off1 = static_cast<size_t>(gnx) * gnz, off2 = off1 + static_cast<size_t>(g1x) * g1z;
-  const size_t total_aggs = off2 + static_cast<size_t>(bnx) * bnz;
-  const size_t nblk = static_cast<size_t>(bnx) * bnz;
-  const size_t sgrid = (nblk + NTP / 64 - 1) / (NTP / 64);  // one wavefront per 4x4 block of cells
-  if (c.agg_cap < total_aggs) {
-    dfree(c.agg_inv);
-    c.agg_inv = dalloc<float>(total_aggs);
-    c.agg_cap = total_aggs;
-  }
-  if (c.part_ml_cap < sgrid) {
-    dfree(c.part_ml);
-    c.part_ml = dalloc<double>(2 * sgrid);
-    c.part_ml_cap = sgrid;
-  }
-  if (c.ml_blk_cap < nblk) {
-    dfree(c.ml_flag);
-    dfree(c.ml_off);
-    dfree(c.ml_list);
-    c.ml_flag = dalloc<uint32_t>(nblk);
-    c.ml_off = dalloc<uint32_t>(nblk);
-    c.ml_list = dalloc<uint32_t>(nblk);
-    c.ml_blk_cap = nblk;
-  }
-  if (!c.ml_active) c.ml_active = dalloc<uint64_t>(1);
+  if (!join_ml_setup(c)) ml_setup(c);  // enqueued ahead on the auxiliary stream, or now
+  const MlGeom g = ml_geom(c);
   const size_t stride = c.part_ml_cap;
-  const CellGrid G{c.cell_start, c.cell_count, c.sorted, c.key, c.type, gnx, gnz, g1z, bnx, bnz,
-                   off1, off2};
-  // the aggregate sums of A from the FP64 system (row sums in z, r, d as
-  // scratch), and the list of the 4x4 cell blocks that hold particles (its
-  // length reaches the host with the first residual below)
-  // (the fill of this system already summed the rows unless the solver changed since)
-  const bool sums = c.ml_rowsums_ready;
-  c.ml_rowsums_ready = false;
-  tracked(c, MPS_K_ML_SETUP, (sums ? 0.0 : c.nnz_offdiag * 12.0 + n * 17.0) + n * 20.0 + total_aggs * 4.0 +
-                                 nblk * 12.0, [&] {
-    if (!sums)
-      hipLaunchKernelGGL(k_ml_rowsum, dim3(L.grid), dim3(NT), 0, c.stream,
-                         sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const uint32_t*>(c.row_len),
-                         static_cast<const uint32_t*>(c.key), static_cast<const uint8_t*>(c.type), gnz, c.z32,
-                         c.r32, c.d32);
-    hipLaunchKernelGGL(k_ml_setup, dim3(static_cast<unsigned>(sgrid)), dim3(NTP), 0, c.stream, G,
-                       static_cast<const float*>(c.z32), static_cast<const float*>(c.r32),
-                       static_cast<const float*>(c.d32), c.agg_inv);
-    hipLaunchKernelGGL(k_ml_active, dim3(grid_for(nblk)), dim3(256), 0, c.stream, G, c.ml_flag);
-    scan_counts(c, c.ml_flag, nblk, c.ml_off, c.ml_active);
-    hipLaunchKernelGGL(k_ml_compact, dim3(grid_for(nblk)), dim3(256), 0, c.stream, static_cast<uint32_t>(nblk),
-                       static_cast<const uint32_t*>(c.ml_flag), static_cast<const uint32_t*>(c.ml_off),
-                       c.ml_list);
-  });
-  MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+  const CellGrid G = ml_grid(c, g);
   // rows outside every cell keep r = d = z = 0; the records take the positions
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
                      c.zrec);
@@ -1609,12 +1650,17 @@ void ppe_solve(Context& c) {
   int rc = MPS_OK;
   std::string msg;
   try {
-    if (c.solver.warm_start == MPS_WARM_LEARNED && on_particles) learned_warm_start(c);
+    if (c.solver.warm_start == MPS_WARM_LEARNED && on_particles && c.model && c.ppe_from_particles) {
+      ml_setup_ahead(c);
+      learned_warm_start(c);
+    }
     if (c.solver.kind == MPS_SOLVER_MIXED_IR) solve_mixed(c, sr, max_iter);
     else solve_cg64(c, sr, max_iter);
     c.learned_pending = false;
+    join_aux(c);
   } catch (Failure& f) {
     c.learned_pending = false;
+    join_aux(c);
     if (f.code != MPS_ERR_CG_NOT_CONVERGED) throw;
     rc = f.code;
     msg = f.msg;
@@ -1663,5 +1709,32 @@ bool learned_residual32(Context& c, FeatRec* rec) {
   check_launch();
   return true;
 }
+
+
+// With the learned warm start in front of the multilevel mixed solver: the
+// solver's set-up, which does not depend on the initial guess, goes to the
+// auxiliary stream after the assembly and runs beside the warm start's
+// kernels; solve_mixed_ml joins it.
+void ml_setup_ahead(Context& c) {
+  const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
+  if (c.solver.kind != MPS_SOLVER_MIXED_IR || !c.ppe_from_particles || c.solver.precond != MPS_PRECOND_MULTILEVEL ||
+      !cells_match || c.ml_setup_pending)
+    return;
+  MPS_HIP(hipEventRecord(c.ev_fork, c.stream));
+  MPS_HIP(hipStreamWaitEvent(c.stream_aux, c.ev_fork, 0));
+  std::swap(c.stream, c.stream_aux);  // everything below enqueues on c.stream
+  try {
+    ml_setup(c);
+  } catch (...) {
+    std::swap(c.stream, c.stream_aux);
+    throw;
+  }
+  std::swap(c.stream, c.stream_aux);
+  MPS_HIP(hipEventRecord(c.ev_join, c.stream_aux));
+  c.ml_setup_pending = true;
+}
+
+// Joins set-up work left on the auxiliary stream (a solve that ended early).
+void join_aux(Context& c) { join_ml_setup(c); }
 
 }  // namespace mps

@@ -192,6 +192,12 @@ struct Context {
   // the learned warm start's residual norms are on their way to red_host
   // (learned_resolve reads them at the solver's first synchronisation)
   bool learned_pending = false;
+  // work enqueued ahead on the auxiliary stream: the multilevel set-up, which
+  // runs beside the learned warm start (fork after the assembly, join before
+  // the solve)
+  hipStream_t stream_aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool ml_setup_pending = false;
   // mixed-precision work vectors
   float *pa32 = nullptr, *d32 = nullptr;  // inner CG direction and correction
   float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr;  // multilevel-preconditioned inner CG
@@ -369,6 +375,8 @@ void reduce2(Context& c, const double* a, const double* b, int nparts);  // -> c
 void residual64_launch(Context& c);  // r = b - A x; |r|^2, |b|^2 -> c.red_out[0..1] (no sync)
 struct FeatRec;
 bool learned_residual32(Context& c, FeatRec* rec);  // mps_cg.hip: see there
+void ml_setup_ahead(Context& c);  // mps_cg.hip: the multilevel set-up on the auxiliary stream
+void join_aux(Context& c);        // the main stream waits for it (if pending)
 // mps_mlp.hip.  learned_features enqueues (no synchronisation) the residual
 // of x = p_prev, the scales and the feature rows; red_out then holds |r|^2,
 // |b|^2 [0, 1], the alive sums of p^2, diag^2 and count [2..4], and
