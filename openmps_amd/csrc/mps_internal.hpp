@@ -130,6 +130,8 @@ struct Context {
   size_t ncells = 0;
   uint32_t *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
   uint32_t *key = nullptr, *sorted = nullptr;
+  double2* pos_sorted = nullptr;  // (x, z) in cell order (the search's candidate ranges)
+  uint64_t* in_cells = nullptr;    // particles in cells (device scalar, from the cell scan)
   uint64_t* scan_tmp = nullptr;   // block sums of the scans
   int* flags = nullptr;           // [0] cell overflow, [1] neighbour overflow
   int* flags_host = nullptr;      // pinned mirror

@@ -11,9 +11,11 @@
 //   3. k_cell_scatter  counting-sort scatter (a one-digit radix sort whose digit
 //                      is the whole cell key);
 //   4. k_cell_sort     orders each cell's few entries by particle index, which
-//                      restores exactly the serial insertion order;
+//                      restores exactly the serial insertion order; k_cell_pos
+//                      stores the positions in that order;
 //   5. k_neighbors     one thread per particle walks the 9 cells in the
-//                      reference order and writes its list into the sliced
+//                      reference order (each a contiguous range of ids and
+//                      positions) and writes its list into the sliced
 //                      neighbour layout, four slots (16 bytes) per store.
 #include "mps_device.hpp"
 #include "mps_internal.hpp"
@@ -176,6 +178,19 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t ca
   }
 }
 
+// The positions in cell order (slots below the number of particles in
+// cells), so that the neighbour walk reads a cell's candidates as one
+// contiguous range instead of gathering them by index.
+__global__ void __launch_bounds__(kBlock) k_cell_pos(size_t n, const uint64_t* __restrict__ in_cells,
+                                                     const uint32_t* __restrict__ sorted,
+                                                     const double* __restrict__ x, const double* __restrict__ z,
+                                                     double2* __restrict__ pos) {
+  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (k >= n || k >= in_cells[0]) return;
+  const uint32_t j = sorted[k];
+  pos[k] = make_double2(x[j], z[j]);
+}
+
 // Computer::SearchNeighbor's per-particle walk, Computer.hpp:720-755: the 9
 // cells in Grid.hpp:393-401 order, j != i, R(x_i, x_j) < NeighborLength.
 // Disabled particles keep their previous list, as in the reference.
@@ -184,7 +199,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
                                                       const uint8_t* __restrict__ type,
                                                       const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ cell_count,
-                                                      const uint32_t* __restrict__ sorted, uint32_t kmax,
+                                                      const uint32_t* __restrict__ sorted,
+                                                      const double2* __restrict__ pos, uint32_t kmax,
                                                       uint32_t kstride, uint32_t* __restrict__ nbr,
                                                       uint32_t* __restrict__ ncount, int* flags,
                                                       unsigned long long* total) {
@@ -225,24 +241,23 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
             cnt++;
           }
         };
-        // four candidates per round trip to memory (ids, then positions)
+        // four candidates per round trip to memory: ids and positions of the
+        // cell's contiguous range, loaded together
         uint32_t l = 0;
         for (; l + 4 <= m; l += 4) {
           uint32_t jj[4];
-          double xj[4], zj[4];
-#pragma unroll
-          for (int v = 0; v < 4; v++) jj[v] = sorted[s + l + v];
+          double2 pj[4];
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            xj[v] = x[jj[v]];
-            zj[v] = z[jj[v]];
+            jj[v] = sorted[s + l + v];
+            pj[v] = pos[s + l + v];
           }
 #pragma unroll
-          for (int v = 0; v < 4; v++) test(jj[v], xj[v], zj[v]);
+          for (int v = 0; v < 4; v++) test(jj[v], pj[v].x, pj[v].y);
         }
         for (; l < m; l++) {
-          const uint32_t j = sorted[s + l];
-          test(j, x[j], z[j]);
+          const double2 pj = pos[s + l];
+          test(sorted[s + l], pj.x, pj.y);
         }
       }
     }
@@ -298,7 +313,7 @@ void search(Context& c) {
                        c.type, c.key, c.cell_count, c.flags);
   });
   tracked(c, MPS_K_CELL_SCAN, 8.0 * c.ncells, [&] {
-    scan_u32<uint32_t>(c, c.cell_count, c.ncells, c.cell_start, 1, nullptr);
+    scan_u32<uint32_t>(c, c.cell_count, c.ncells, c.cell_start, 1, c.in_cells);
   });
   tracked(c, MPS_K_CELL_SCATTER, 16.0 * n, [&] {
     hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
@@ -310,13 +325,21 @@ void search(Context& c) {
                        c.cap, static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), c.sorted);
   });
-  // x, z, type of every particle, the cell tables, the sorted ids and the
-  // list lengths; the list itself (4 bytes a neighbour) is added below
-  tracked(c, MPS_K_NEIGHBORS, 25.0 * n + 8.0 * c.ncells, [&] {
+  // the sorted ids and x, z read, the positions in cell order written
+  tracked(c, MPS_K_CELL_SORT, 36.0 * n, [&] {
+    hipLaunchKernelGGL(k_cell_pos, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       static_cast<const uint64_t*>(c.in_cells), static_cast<const uint32_t*>(c.sorted),
+                       static_cast<const double*>(c.x), static_cast<const double*>(c.z), c.pos_sorted);
+  });
+  // x, z, type of every particle, the cell tables, the sorted ids and
+  // positions and the list lengths; the list itself (4 bytes a neighbour) is
+  // added below
+  tracked(c, MPS_K_NEIGHBORS, 41.0 * n + 8.0 * c.ncells, [&] {
     hipLaunchKernelGGL(k_neighbors, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.sorted),
-                       c.kmax, c.kstride, c.nbr, c.ncount, c.flags, c.nbr_total_dev);
+                       static_cast<const double2*>(c.pos_sorted), c.kmax, c.kstride, c.nbr, c.ncount, c.flags,
+                       c.nbr_total_dev);
   });
   check_launch();
   MPS_HIP(hipMemcpyAsync(c.flags_host, c.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
