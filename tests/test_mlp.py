@@ -244,6 +244,36 @@ def test_learned_warm_start_is_judged_by_the_solver():
 
 
 @pytest.mark.gpu
+def test_learned_start_with_the_multilevel_solver():
+    """The learned start in front of the multilevel mixed solver (whose set-up
+    then runs on a second stream beside it): every solve meets the stop rule,
+    the trained model's guess is kept, and the pressures match the same
+    solver started from the previous step's pressure to the solve tolerance."""
+    from openmps_amd import learned
+    from openmps_amd import _lib as L
+    parts, env = _small_scene(12)
+    runs = []
+    for use_model in (True, False):
+        c = cases.make("gpu", env, 1e-10)
+        c.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_MULTILEVEL,
+                     warm_start=L.WARM_LEARNED if use_model else L.WARM_PREVIOUS, tol=1e-10)
+        if use_model:
+            learned.attach_model(c, learned.Model.load())
+        c.add_particles(parts)
+        used = 0
+        for _ in range(6):
+            c.forward_time(1e-4)
+            s = c.solve_stats()
+            assert s["rel_residual"] <= 1.5e-10
+            used += s["learned_used"]
+        if use_model:
+            assert used >= 1
+        runs.append(c.particles()["p"])
+    scale = np.abs(runs[1]).max()
+    assert np.abs(runs[0] - runs[1]).max() <= 1e-6 * scale
+
+
+@pytest.mark.gpu
 def test_learned_start_needs_a_particle_system():
     """A system installed with set_ppe over the particles (rows that are not
     the particles' stencils): the solve starts from the given guess with the
