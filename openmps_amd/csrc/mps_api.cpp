@@ -309,9 +309,6 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.solver.warm_start = MPS_WARM_PREVIOUS;
     c.solver.inner_tol = 1e-4;
     MPS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    MPS_HIP(hipStreamCreateWithFlags(&c.stream_aux, hipStreamNonBlocking));
-    MPS_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
-    MPS_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     // Grid.hpp:137-150
     const double NL = c.env.neighbor_length;
     c.gnx = static_cast<long long>(std::ceil((c.env.max_x - c.env.min_x) / NL)) + 2;

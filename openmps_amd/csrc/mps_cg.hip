@@ -1720,6 +1720,14 @@ void ml_setup_ahead(Context& c) {
   if (c.solver.kind != MPS_SOLVER_MIXED_IR || !c.ppe_from_particles || c.solver.precond != MPS_PRECOND_MULTILEVEL ||
       !cells_match || c.ml_setup_pending)
     return;
+  if (!c.stream_aux) {
+    // created on first use: an idle stream would still take its turn in the
+    // assignment of hardware queues to streams (contexts that never use it,
+    // such as many scenes side by side, would share fewer queues)
+    MPS_HIP(hipStreamCreateWithFlags(&c.stream_aux, hipStreamNonBlocking));
+    MPS_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    MPS_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  }
   MPS_HIP(hipEventRecord(c.ev_fork, c.stream));
   MPS_HIP(hipStreamWaitEvent(c.stream_aux, c.ev_fork, 0));
   std::swap(c.stream, c.stream_aux);  // everything below enqueues on c.stream
