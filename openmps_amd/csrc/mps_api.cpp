@@ -938,6 +938,7 @@ int mps_get_features(mps_ctx* ctx, float* out) {
   return mps::guard(ctx, [&](Context& c) {
     if (!c.have_ppe || c.ppe_n != c.n) throw Failure{MPS_ERR_STATE, "assemble the PPE first"};
     const size_t n = c.n;
+    if (n == 0) return;
     std::vector<__bf16> h(n * mps::kFeatPad);
     mps::learned_features(c);
     MPS_HIP(hipMemcpyAsync(h.data(), c.feats, h.size() * 2, hipMemcpyDeviceToHost, c.stream));

@@ -908,11 +908,12 @@ void Dataset::release() {
 
 // --------------------------- use inside a step -------------------------------------
 
-// Residual of the current x (= p_prev), the scales, and the features of every
-// particle, all on the stream without a synchronisation (the scales stay on
-// the device, in red_out).
+// The residual of p_prev, the scales, and the features of every particle,
+// all on the stream without a synchronisation (the scales stay on the
+// device, in red_out).
 void learned_features(Context& c) {
   const size_t n = c.n;
+  if (n == 0) return;
   if (!c.feats || c.feats_cap < n) {
     dfree(c.feats);
     dfree(c.ypred);
