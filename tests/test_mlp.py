@@ -158,6 +158,23 @@ def test_features_keep_the_32_nearest():
     assert (F[:, 0] > 1.0).sum() >= 500  # rows with more than 32 accepted neighbours
 
 
+@pytest.mark.gpu
+def test_features_wide_stencil():
+    """r_lap = 4.5 l0: rows of about 60 columns, list bounds past 256 (16-bit
+    column positions in the feature kernel), the 32 nearest kept."""
+    parts, env = _small_scene()
+    env = dict(env, r_lap_by_l0=4.5)
+    c = cases.make("gpu", env, 1e-10)
+    c.add_particles(parts)
+    assert c.grid()["kmax"] > 256  # list bound: 16-bit column positions
+    c.set_time(1e-4, 1e-4)
+    for st in ("search", "density", "error_correction", "explicit", "density", "save_x",
+               "ppe_assemble"):
+        c.stage(st)
+    F = _check_features(c)
+    assert (F[:, 0] > 1.5).sum() >= 500
+
+
 def _check_features(c):
     import scipy.sparse
     from openmps_amd import learned
