@@ -270,13 +270,12 @@ __device__ __forceinline__ uint32_t feat_col(const FeatArgs& a, size_t i, bool w
 // One thread per particle.  Its accepted neighbours are the off-diagonal
 // columns of its row of the assembled system (the same test, r^2 < r_lap^2
 // and not dummy, picked them), read as coalesced SELL slots; their
-// positions come from the records.  Kept neighbours are entries of LDS
-// arrays: the key float(r^2) and (for rows longer than kNbr) the column
-// position Q (8 or 16 bits), 160-192 bytes per thread.  A row of at most
-// kNbr columns makes them its entries, in order, and places entry e at its
-// rank among them (keys ascending, equal keys by ascending neighbour index),
-// counted over the keys in LDS.  A longer row goes through an insertion
-// sort (stable, in column order) that keeps the kNbr nearest.
+// positions come from the records.  A row of at most kNbr columns keeps
+// them all as (float(r^2), index) pairs in registers, sorted by a bitonic
+// network (distance, then ascending neighbour index).  A longer row goes
+// through an insertion sort in LDS (keys and column positions Q, 8 or 16
+// bits: 160-192 bytes per thread; stable in column order) that keeps the
+// kNbr nearest.
 constexpr int kFeatWalk = 8;    // columns whose loads are issued together
 constexpr int kFeatGather = 4;  // ranks whose neighbours' state is gathered together
 static_assert(kNbr % kFeatGather == 0 && kNbr % kFeatWalk == 0, "whole groups");
@@ -298,9 +297,12 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
   float f[4] = {0, 0, 0, 0};
   int m = 0;
   bool sorted = false;
-  uint32_t je[kNbr];  // the neighbour of entry e (while entries are in column order)
+  // the kept neighbours as (float bits of r^2) << 32 | index: ascending
+  // order is distance order with equal distances by index (= column order);
+  // unused entries are all ones and sort last
+  uint64_t kv[kNbr];
 #pragma unroll
-  for (int e = 0; e < kNbr; e++) je[e] = 0u;
+  for (int e = 0; e < kNbr; e++) kv[e] = ~0ull;
   if (ti != kDummy && ti != kDisabled) {
     const uint32_t cnt = a.len[i];
     auto key = [&](uint32_t j) {
@@ -308,18 +310,19 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
       const double dx = xz.x - xi, dz = xz.y - zi;
       return __float_as_uint(static_cast<float>(dx * dx + dz * dz));
     };
-    if (cnt <= static_cast<uint32_t>(kNbr)) {  // entry e is column e
+    if (cnt <= static_cast<uint32_t>(kNbr)) {  // every column is kept
       m = static_cast<int>(cnt);
 #pragma unroll
       for (int k0 = 0; k0 < kNbr; k0 += kFeatWalk) {
         if (k0 < m) {
+          uint32_t jj[kFeatWalk];
 #pragma unroll
           for (int u = 0; u < kFeatWalk; u++)
-            je[k0 + u] = (k0 + u < m) ? feat_col(a, i, wide, base, k0 + u) : static_cast<uint32_t>(i);
+            jj[u] = (k0 + u < m) ? feat_col(a, i, wide, base, k0 + u) : static_cast<uint32_t>(i);
 #pragma unroll
           for (int u = 0; u < kFeatWalk; u++) {
-            const uint32_t kf = key(je[k0 + u]);
-            if (k0 + u < m) skey[k0 + u][t] = kf;
+            const uint64_t kf = key(jj[u]);
+            if (k0 + u < m) kv[k0 + u] = (kf << 32) | jj[u];
           }
         }
       }
@@ -338,6 +341,9 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
         skey[pos][t] = kf;
         sq[pos][t] = static_cast<Q>(k);
       }
+#pragma unroll
+      for (int e = 0; e < kNbr; e++)
+        if (e < m) kv[e] = (static_cast<uint64_t>(skey[e][t]) << 32) | feat_col(a, i, wide, base, sq[e][t]);
       sorted = true;
     }
     f[0] = static_cast<float>(cnt) / 32.0f;
@@ -345,45 +351,23 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     f[2] = static_cast<float>(a.rec[i].p * inv_sp);
     f[3] = static_cast<float>(a.rec[i].r0 * inv_sr);
   }
-  // rank of every entry (keys ascending, equal keys by column), then the
-  // neighbours' indices in rank order over the keys.  (b - k) >> 31 over unsigned keys
-  // below 2^31 is 1 exactly when b < k; entry b precedes entry e if its key
-  // is smaller, or equal with b < e.  The ranks of entries 4g .. 4g + 3 are
-  // the bytes of rk[g].
-  if (!sorted) {
-    uint32_t rk[kNbr / 4];
+  // bitonic sort of the kNbr pairs in registers (ascending)
+  if (!sorted && m > 1) {
 #pragma unroll
-    for (int e0 = 0; e0 < kNbr; e0 += 4) {
-      uint32_t r = 0;
-      if (e0 < m) {
-        uint32_t ke[4];
+    for (int k = 2; k <= kNbr; k <<= 1)
 #pragma unroll
-        for (int u = 0; u < 4; u++) ke[u] = skey[e0 + u][t];  // entries past m are not used
-        for (int b = 0; b < e0; b++) {  // b < e: smaller or equal
-          const uint32_t kb = skey[b][t];
+      for (int d = k >> 1; d > 0; d >>= 1)
 #pragma unroll
-          for (int u = 0; u < 4; u++) r += ((kb - ke[u] - 1u) >> 31) << (8 * u);
+        for (int e = 0; e < kNbr; e++) {
+          const int l = e ^ d;
+          if (l > e) {
+            const uint64_t lo = kv[e] < kv[l] ? kv[e] : kv[l];
+            const uint64_t hi = kv[e] < kv[l] ? kv[l] : kv[e];
+            const bool up = (e & k) == 0;
+            kv[e] = up ? lo : hi;
+            kv[l] = up ? hi : lo;
+          }
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const uint32_t kb = (e0 + v < m) ? skey[e0 + v][t] : 0x7f800000u;
-#pragma unroll
-          for (int u = 0; u < 4; u++) r += ((kb - ke[u] - (v < u ? 1u : 0u)) >> 31) << (8 * u);
-        }
-        for (int b = e0 + 4; b < m; b++) {  // b > e: smaller
-          const uint32_t kb = skey[b][t];
-#pragma unroll
-          for (int u = 0; u < 4; u++) r += ((kb - ke[u]) >> 31) << (8 * u);
-        }
-      }
-      rk[e0 / 4] = r;
-    }
-    // every key has been read: skey[s] becomes the neighbour of rank s
-#pragma unroll
-    for (int e = 0; e < kNbr; e++)
-      if (e < m) skey[(rk[e / 4] >> (8 * (e % 4))) & 0xffu][t] = je[e];
-  } else {
-    for (int e = 0; e < m; e++) skey[e][t] = feat_col(a, i, wide, base, sq[e][t]);
   }
   // the row in registers (elements 0-3: the particle's own features;
   // 4 + 4s .. 7 + 4s: the neighbour of rank s, zero for s >= m; zeros from
@@ -398,7 +382,8 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     if (s0 < m) {
       uint32_t jj[kFeatGather];
 #pragma unroll
-      for (int u = 0; u < kFeatGather; u++) jj[u] = (s0 + u < m) ? skey[s0 + u][t] : static_cast<uint32_t>(i);
+      for (int u = 0; u < kFeatGather; u++)
+        jj[u] = (s0 + u < m) ? static_cast<uint32_t>(kv[s0 + u]) : static_cast<uint32_t>(i);
 #pragma unroll
       for (int u = 0; u < kFeatGather; u++) {
         const FeatRec rj = a.rec[jj[u]];
@@ -410,8 +395,8 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
     }
   }
   static_assert(kFeat == 4 + 4 * kNbr && (kNbr + 1) % 2 == 1 && kFeatPad % 8 == 0, "row layout");
-  // The rows leave through LDS (the key array, free once every thread has
-  // gathered), 16 rows of a wavefront at a time, as contiguous 16-byte
+  // The rows leave through LDS (the key array, free once every thread is
+  // past its sort), 16 rows of a wavefront at a time, as contiguous 16-byte
   // vectors: stored straight from the registers, each wavefront store would
   // touch 64 rows 320 bytes apart.
   constexpr int kStageRows = 16;
