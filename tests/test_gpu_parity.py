@@ -125,6 +125,17 @@ def test_stage_parity_jittered():
     run_parity("dambreak_1k", steps=4, particles=s)
 
 
+def test_stage_parity_shuffled():
+    """Particles in a random order: consecutive particles (one wavefront)
+    are scattered over the whole scene, so the neighbour walks share no
+    cells and the lists are written in index order, not spatial order."""
+    s, _ = scenes.dam_break(1.0, 0.6, 0.25, 0.5, 0.01, jitter=0.2, seed=11)
+    perm = np.random.Generator(np.random.PCG64(5)).permutation(len(s["x"]))
+    s = {k: v[perm] for k, v in s.items()}
+    say("parity shuffled jittered dambreak_1k")
+    run_parity("dambreak_1k", steps=2, particles=s)
+
+
 def test_stage_parity_125k():
     """One step at the 125k-particle configuration (l0 = 1 mm, jittered)."""
     say("parity dambreak_125k (1 step)")
