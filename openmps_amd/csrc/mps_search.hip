@@ -48,24 +48,44 @@ __device__ __forceinline__ bool cell_of(const GridParams& g, double px, double p
   return true;
 }
 
+// atomicAdd(&counter[k], 1) for every valid lane of the wavefront, one
+// atomic per run of equal keys in consecutive lanes (particles stored in
+// lattice order put three or four neighbours of a wavefront in one cell):
+// the run's first lane adds the run's length and hands the old value on.
+// Returns old + the lane's position in its run, the value of one of the
+// orders in which the single atomics could have run.  Every lane of the
+// wavefront must call it; lanes with valid == false (key kNoKey) add nothing.
+__device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t kk = valid ? k : kNoKey;
+  const uint32_t prev = __shfl_up(kk, 1, 64);
+  const bool head = lane == 0 || kk != prev || kk == kNoKey;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int start = 63 - __clzll(heads & le);  // lane 0 is a head: never empty
+  const unsigned long long above = heads & ~le;
+  const int end = above ? __ffsll(above) - 1 : 64;
+  uint32_t old = 0;
+  if (head && kk != kNoKey) old = atomicAdd(&counter[kk], static_cast<uint32_t>(end - lane));
+  return __shfl(old, start, 64) + static_cast<uint32_t>(lane - start);
+}
+
 __global__ void __launch_bounds__(kBlock) k_cell_keys(GridParams g, size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z, uint8_t* __restrict__ type,
                                                       uint32_t* __restrict__ key, uint32_t* cell_count,
                                                       int* flags) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k = kNoKey;
-  if (type[i] != kDisabled) {
+  uint32_t k = kNoKey;  // lanes past n take part in the wavefront's run count
+  if (i < n && type[i] != kDisabled) {
     long long bx, bz;
-    if (cell_of(g, x[i], z[i], bx, bz)) {
+    if (cell_of(g, x[i], z[i], bx, bz))
       k = static_cast<uint32_t>(bx * g.gnz + bz);
-      const uint32_t l = atomicAdd(&cell_count[k], 1u);
-      if (l >= g.cap) flags[0] = 1;
-    } else {
+    else
       type[i] = kDisabled;  // Computer.hpp:711-715
-    }
   }
-  key[i] = k;
+  const uint32_t l = run_atomic_add(cell_count, k, k != kNoKey);
+  if (k != kNoKey && l >= g.cap) flags[0] = 1;
+  if (i < n) key[i] = k;
 }
 
 // ---- exclusive scan of u32 counts (x mult) into u32 or u64 offsets --------------
@@ -148,11 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_scatter(size_t n, const uint32_
                                                          uint32_t* cell_fill,
                                                          uint32_t* __restrict__ sorted) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = key[i];
-  if (k == kNoKey) return;
-  const uint32_t slot = atomicAdd(&cell_fill[k], 1u);
-  sorted[cell_start[k] + slot] = static_cast<uint32_t>(i);
+  const uint32_t k = i < n ? key[i] : kNoKey;  // lanes past n take part in the run count
+  const uint32_t slot = run_atomic_add(cell_fill, k, k != kNoKey);
+  if (k != kNoKey) sorted[cell_start[k] + slot] = static_cast<uint32_t>(i);
 }
 
 // Restores ascending particle index inside each cell (the reference inserts
