@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # profiler kernel name (template arguments kept) -> kernel id of the library's
 # own per-kernel statistics (mps.h MPS_K_*)
 KERNEL_IDS = {
-    "k_cell_keys": "cell_keys", "k_cell_scatter": "cell_scatter", "k_cell_sort": "cell_sort",
+    "k_cell_keys": "cell_keys", "k_cell_scatter": "cell_scatter",
     "k_neighbors": "neighbors", "k_density<true, true, false>": "density_ecs",
     "k_density<true, false, true>": "density_save", "k_explicit": "explicit",
     "k_ppe_count": "ppe_count", "k_ppe_fill": "ppe_fill", "k_cg_init": "cg_init",
@@ -60,6 +60,9 @@ FORMULAS = {
     # record of the operator's row; 2-byte column offsets
     "k_learn_resid32": lambda N, Z: 97 * N + 2 * Z,
     "k_cg_init": lambda N, Z: 12 * Z + 32 * N,
+    # sorted ids and x, z read, the cell-ordered positions written (k_cell_sort
+    # shares the library's statistic with it and is not shown in bytes)
+    "k_cell_pos": lambda N, Z: 36 * N,
     # x, z, n, type, row length, the records and the row written; column offsets
     "k_features<unsigned char>": lambda N, Z: 381 * N + 2 * Z,
     "k_features<unsigned short>": lambda N, Z: 381 * N + 2 * Z,
