@@ -185,6 +185,29 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t ca
   const uint32_t cnt = cell_count[c];
   if (cnt < 2 || cnt > cap) return;
   uint32_t* s = sorted + cell_start[c];
+  if (cnt <= 32) {  // one load and one store per entry: a bitonic network in registers
+    uint32_t v[32];
+#pragma unroll
+    for (int e = 0; e < 32; e++) v[e] = (static_cast<uint32_t>(e) < cnt) ? s[e] : ~0u;  // padding sorts last
+#pragma unroll
+    for (int k = 2; k <= 32; k <<= 1)
+#pragma unroll
+      for (int d = k >> 1; d > 0; d >>= 1)
+#pragma unroll
+        for (int e = 0; e < 32; e++) {
+          const int l = e ^ d;
+          if (l > e) {
+            const uint32_t lo = min(v[e], v[l]), hi = max(v[e], v[l]);
+            const bool up = (e & k) == 0;
+            v[e] = up ? lo : hi;
+            v[l] = up ? hi : lo;
+          }
+        }
+#pragma unroll
+    for (int e = 0; e < 32; e++)
+      if (static_cast<uint32_t>(e) < cnt) s[e] = v[e];
+    return;
+  }
   for (uint32_t a = 1; a < cnt; a++) {
     const uint32_t v = s[a];
     uint32_t b = a;

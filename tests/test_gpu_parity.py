@@ -136,6 +136,26 @@ def test_stage_parity_shuffled():
     run_parity("dambreak_1k", steps=2, particles=s)
 
 
+@pytest.mark.parametrize("order", ["lattice", "shuffled"])
+def test_search_parity_large_cells(order):
+    """r_e = 5 l0 (the reference's HS test radius, NL = 6 l0): a cell holds
+    36 lattice sites, more entries than the cell sort keeps in registers, so
+    its index order is restored by the insertion sort."""
+    case = cases.HS
+    s = scenes.lattice(35, 35, case["env"]["l0"])
+    if order == "shuffled":
+        perm = np.random.Generator(np.random.PCG64(9)).permutation(len(s["x"]))
+        s = {k: v[perm] for k, v in s.items()}
+    g = cases.make("gpu", case["env"], case["eps"])
+    o = cases.make("oracle", case["env"], case["eps"])
+    g.add_particles(s)
+    o.add_particles(s)
+    say(f"search parity, r_e = 5 l0, {order}")
+    g.stage("search")
+    o.stage("search")
+    compare_stage(g, o, "search")
+
+
 def test_stage_parity_125k():
     """One step at the 125k-particle configuration (l0 = 1 mm, jittered)."""
     say("parity dambreak_125k (1 step)")
