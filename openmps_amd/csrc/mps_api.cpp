@@ -129,6 +129,7 @@ void free_all(Context& c) {
   dfree(c.x); dfree(c.z); dfree(c.u); dfree(c.w); dfree(c.p); dfree(c.nd); dfree(c.type);
   dfree(c.nws); dfree(c.ecs); dfree(c.ox); dfree(c.oz); dfree(c.dux); dfree(c.duz);
   dfree(c.wall_x); dfree(c.wall_z); dfree(c.hs_out); dfree(c.wall_group); dfree(c.wall_motion);
+  dfree(c.xz); dfree(c.uw);
   dfree(c.key); dfree(c.sorted); dfree(c.pos_sorted); dfree(c.in_cells); dfree(c.nbr); dfree(c.ncount);
   dfree(c.cell_count); dfree(c.cell_start); dfree(c.cell_fill); dfree(c.scan_tmp); dfree(c.flags);
   dfree(c.nbr_total_dev); dfree(c.ppe_counters);
@@ -415,6 +416,8 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
       mps::grow(c.key, 0, m, c.stream);
       mps::grow(c.sorted, 0, m, c.stream);
       mps::grow(c.pos_sorted, 0, m, c.stream);
+      mps::grow(c.xz, 0, m, c.stream);
+      mps::grow(c.uw, 0, m, c.stream);
       mps::grow(c.ncount, old, m, c.stream);
       // neighbour rows of existing particles are kept (boost::multi_array::resize)
       mps::grow(c.nbr, (mo / 64) * 64 * static_cast<size_t>(c.kstride), m * static_cast<size_t>(c.kstride),

@@ -163,6 +163,12 @@ struct Context {
   // FP32 solver may recompute off-diagonals from them (matrix-free)
   bool ppe_from_particles = false;
   long long ppe_search_serial = -1;  // search whose cells the assembled system matches
+  // (x, z) and (u, w) as 16-byte pairs for the neighbour walks' gathers (one
+  // load where two 8-byte ones would be): packed at the start of the step's
+  // first walk (pack_state) and written through by the explicit update; the
+  // PPE assembly reads them (the staged assembly packs first)
+  double2* xz = nullptr;
+  double2* uw = nullptr;
   float* ghost32 = nullptr;  // SPP ghost share of the diagonal (1 on rows without unknowns)
   uint2* qpos = nullptr;     // fixed-point positions of the matrix-free operator
   double qpos_inv_h = 0;     // their scale: q = (x - min) / h
@@ -357,6 +363,7 @@ void pressure_writeback(Context& c);
 void pressure_gradient(Context& c);
 void dynamic_stabilize(Context& c);
 void ppe_assemble(Context& c);           // mps_ppe.hip
+void pack_state(Context& c);             // mps_particles.hip: xz, uw from x, z, u, w
 // density(c, false, true, true) followed by ppe_assemble(c), the density walk
 // and the PPE count walk fused into one kernel (same results)
 void density_save_and_assemble(Context& c);

@@ -200,6 +200,16 @@ __global__ void __launch_bounds__(kBlock) k_explicit(size_t n, const double* __r
 // velocities and only the particle's own nWithoutSpp (for the SPP ghost),
 // which this walk has just summed.  Every value is computed as k_density and
 // k_explicit compute it (same expressions, same list order).
+__global__ void __launch_bounds__(kBlock) k_pack_state(size_t n, const double* __restrict__ x,
+                                                       const double* __restrict__ z, const double* __restrict__ u,
+                                                       const double* __restrict__ w, double2* __restrict__ xz,
+                                                       double2* __restrict__ uw) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  xz[i] = make_double2(x[i], z[i]);
+  uw[i] = make_double2(u[i], w[i]);
+}
+
 __global__ void __launch_bounds__(kBlock) k_density_explicit(size_t n, const double* __restrict__ x,
                                                              const double* __restrict__ z,
                                                              const double* __restrict__ u,
@@ -207,7 +217,9 @@ __global__ void __launch_bounds__(kBlock) k_density_explicit(size_t n, const dou
                                                              const uint8_t* __restrict__ type, NV nb,
                                                              Consts k, double* __restrict__ nd,
                                                              double* __restrict__ nws, double* __restrict__ ecs,
-                                                             double* __restrict__ ax, double* __restrict__ az) {
+                                                             double* __restrict__ ax, double* __restrict__ az,
+                                                             const double2* __restrict__ xz,
+                                                             const double2* __restrict__ uw) {
   const size_t i = thread_particle();
   if (i >= n) return;
   const uint8_t ti = type[i];
@@ -224,7 +236,10 @@ __global__ void __launch_bounds__(kBlock) k_density_explicit(size_t n, const dou
     uint8_t t;
   };
   walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
-                      [&](uint32_t j) { return F{x[j], z[j], u[j], w[j], type[j]}; },
+                      [&](uint32_t j) {
+                        const double2 pj = xz[j], vj = uw[j];
+                        return F{pj.x, pj.y, vj.x, vj.y, type[j]};
+                      },
                       [&](uint32_t, const F& f) {
                         const double dx = f.x - xi, dz = f.z - zi;
                         const double r2 = dx * dx + dz * dz;
@@ -281,15 +296,19 @@ __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __
                                                             const double* __restrict__ wall_z, double dt,
                                                             const uint8_t* __restrict__ group,
                                                             const mps_wall_motion* __restrict__ motion,
-                                                            unsigned n_motions, double t) {
+                                                            unsigned n_motions, double t,
+                                                            double2* __restrict__ xz, double2* __restrict__ uw) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   if (type[i] == kFluid) {
     const double un = u[i] + ax[i] * dt, wn = w[i] + az[i] * dt;
+    const double xn = x[i] + un * dt, zn = z[i] + wn * dt;
     u[i] = un;
     w[i] = wn;
-    x[i] = x[i] + un * dt;
-    z[i] = z[i] + wn * dt;
+    x[i] = xn;
+    z[i] = zn;
+    xz[i] = make_double2(xn, zn);
+    uw[i] = make_double2(un, wn);
   } else {
     double nx = wall_x[i], nz = wall_z[i];
     const unsigned g = group ? group[i] : 0u;
@@ -299,10 +318,13 @@ __global__ void __launch_bounds__(kBlock) k_explicit_update(size_t n, double* __
       nx = nx + (m.vx * t + m.ax * sn);
       nz = nz + (m.vz * t + m.az * sn);
     }
-    u[i] = (nx - x[i]) / dt;
-    w[i] = (nz - z[i]) / dt;
+    const double un = (nx - x[i]) / dt, wn = (nz - z[i]) / dt;
+    u[i] = un;
+    w[i] = wn;
     x[i] = nx;
     z[i] = nz;
+    xz[i] = make_double2(nx, nz);
+    uw[i] = make_double2(un, wn);
   }
 }
 
@@ -508,14 +530,16 @@ void explicit_update(Context& c) {
   // positionWallPre(t, dt) then positionWall(i, t, dt) for every non-fluid i
   // (Computer.hpp:993, :1015), after the accelerations and before the update.
   if (c.wall_fn || c.wall_pre) wall_targets_from_callback(c);
-  tracked(c, MPS_K_UPDATE, n * (32.0 + 1 + 16 + 16 + 32), [&] {
+  // x, z, u, w, type, accelerations and wall targets read; x, z, u, w and
+  // the packed pairs written
+  tracked(c, MPS_K_UPDATE, n * (32.0 + 1 + 16 + 16 + 32 + 32), [&] {
     hipLaunchKernelGGL(k_explicit_update, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
                        c.u, c.w, static_cast<const uint8_t*>(c.type),
                        static_cast<const double*>(c.dux), static_cast<const double*>(c.duz),
                        static_cast<const double*>(c.wall_x), static_cast<const double*>(c.wall_z),
                        c.dt, static_cast<const uint8_t*>(c.wall_motions ? c.wall_group : nullptr),
                        static_cast<const mps_wall_motion*>(c.wall_motion), static_cast<unsigned>(c.wall_motions),
-                       c.t);
+                       c.t, c.xz, c.uw);
   });
   check_launch();
 }
@@ -533,14 +557,24 @@ void explicit_forces(Context& c) {
   explicit_update(c);
 }
 
+void pack_state(Context& c) {
+  const size_t n = c.n;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_pack_state, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                     static_cast<const double*>(c.x), static_cast<const double*>(c.z),
+                     static_cast<const double*>(c.u), static_cast<const double*>(c.w), c.xz, c.uw);
+  check_launch();
+}
+
 void density_and_explicit(Context& c) {
   const size_t n = c.n;
   if (n == 0) return;
+  pack_state(c);
   // x, z, u, w, type read; n, nWithoutSpp, ECS, accelerations written
   tracked(c, MPS_K_DENSITY_ECS, n * (32.0 + 1 + 4 + 24 + 16) + 4.0 * c.nbr_total, [&] {
     hipLaunchKernelGGL(k_density_explicit, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u,
                        c.w, static_cast<const uint8_t*>(c.type), nview(c), consts(c), c.nd, c.nws, c.ecs,
-                       c.dux, c.duz);
+                       c.dux, c.duz, static_cast<const double2*>(c.xz), static_cast<const double2*>(c.uw));
   });
   check_launch();
   explicit_update(c);

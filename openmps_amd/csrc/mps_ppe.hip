@@ -121,7 +121,9 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
                                                               uint32_t* __restrict__ row_len,
                                                               uint32_t* __restrict__ slice_width,
                                                               uint2* __restrict__ qpos,
-                                                              unsigned long long* nnz_total) {
+                                                              unsigned long long* nnz_total,
+                                                              const double2* __restrict__ xz,
+                                                              const double2* __restrict__ uw) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   uint32_t cnt = 0;
@@ -158,22 +160,21 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
         uint32_t jj[kWalk];
 #pragma unroll
         for (int v = 0; v < kWalk; v++) jj[v] = nb.ids[nbr_slot(static_cast<uint32_t>(i), q + v, nb.kmax)];
-        double xj[kWalk], zj[kWalk], uj[kWalk], wj[kWalk];
+        double2 pj[kWalk], vj[kWalk];
         uint8_t tj[kWalk];
 #pragma unroll
         for (int v = 0; v < kWalk; v++) {
-          xj[v] = x[jj[v]];
-          zj[v] = z[jj[v]];
-          uj[v] = u[jj[v]];
-          wj[v] = w[jj[v]];
+          pj[v] = xz[jj[v]];
+          vj[v] = uw[jj[v]];
           tj[v] = type[jj[v]];
         }
 #pragma unroll
-        for (int v = 0; v < kWalk; v++) use(xj[v], zj[v], uj[v], wj[v], tj[v]);
+        for (int v = 0; v < kWalk; v++) use(pj[v].x, pj[v].y, vj[v].x, vj[v].y, tj[v]);
       }
       for (; q < m; q++) {
         const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
-        use(x[j], z[j], u[j], w[j], type[j]);
+        const double2 pj = xz[j], vj = uw[j];
+        use(pj.x, pj.y, vj.x, vj.y, type[j]);
       }
       nws[i] = s;
       nd[i] = (s < k.n0) ? k.n0 : s;
@@ -231,7 +232,8 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
                                          const uint32_t* __restrict__ slice_width, double* __restrict__ val,
                                          uint32_t* __restrict__ col, int16_t* __restrict__ col16,
                                          double* __restrict__ diag, float* __restrict__ ghost32, const AggSums& ag,
-                                         uint32_t (*stage)[kBlock], bool& fits, uint32_t& kk) {
+                                         uint32_t (*stage)[kBlock], bool& fits, uint32_t& kk,
+                                         const double2* __restrict__ xz) {
   (void)n;
   const int t = threadIdx.x;
   const size_t base = slice_off[i >> 6] + (i & 63);
@@ -258,7 +260,11 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
       double x, z;
       uint8_t t;
     };
-    walk_list<kWalk, F>(self, m, nb.ids, nb.kmax, [&](uint32_t j) { return F{x[j], z[j], type[j]}; },
+    walk_list<kWalk, F>(self, m, nb.ids, nb.kmax,
+                        [&](uint32_t j) {
+                          const double2 pj = xz[j];
+                          return F{pj.x, pj.y, type[j]};
+                        },
                         [&](uint32_t j, const F& f) {
       const double dx = f.x - xi, dz = f.z - zi;
       const double r2 = dx * dx + dz * dz;
@@ -300,7 +306,8 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
     if (ki != kNoCell) a0 = a1 = a2 = d;
     // value of an emitted column j in slot kk (and its share of the aggregate row sums)
     auto emit = [&](uint32_t j) {
-      const double dx = x[j] - xi, dz = z[j] - zi;
+      const double2 pj = xz[j];
+      const double dx = pj.x - xi, dz = pj.y - zi;
       const double r = sqrt(dx * dx + dz * dz);
       const size_t o = base + static_cast<size_t>(kk) * 64;
       const double a = 3 * k.rl / k.n0l / (r * r * r);
@@ -368,7 +375,8 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
     } else {
       for (uint32_t q = 0; q < m; q++) {
         const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
-        const double dx = x[j] - xi, dz = z[j] - zi;
+        const double2 pj = xz[j];
+        const double dx = pj.x - xi, dz = pj.y - zi;
         if (dx * dx + dz * dz < k.rl2 && type[j] != kDummy) emit(j);
       }
     }
@@ -399,7 +407,8 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
                                                      double* __restrict__ val, uint32_t* __restrict__ col,
                                                      int16_t* __restrict__ col16, uint8_t* __restrict__ slice_wide,
                                                      unsigned long long* nnz_wide, double* __restrict__ diag,
-                                                     float* __restrict__ ghost32, AggSums ag) {
+                                                     float* __restrict__ ghost32, AggSums ag,
+                                                     const double2* __restrict__ xz) {
   // each thread's emitted columns in list order, [slot][thread] (conflict-free)
   __shared__ uint32_t stage[kStage][kBlock];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -409,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   bool fits = true;
   uint32_t kk = 0;
   if (i < n) fill_row(n, i, x, z, type, nb, k, nws, slice_off, slice_width, val, col, col16, diag, ghost32, ag,
-                      stage, fits, kk);
+                      stage, fits, kk, xz);
   const unsigned long long far = __ballot(!fits);
   const double wsum = wave_sum(static_cast<double>(kk));
   if ((threadIdx.x & 63) == 0) {
@@ -487,7 +496,7 @@ void ppe_finish(Context& c, const PpeConsts& k) {
                        static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                        static_cast<const uint64_t*>(c.slice_off),
                        static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
-                       c.ppe_counters + 1, c.diag, c.ghost32, ag);
+                       c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
   });
   check_launch();
   // the wide nnz (byte accounting only) reaches the host with the solver's
@@ -514,6 +523,7 @@ void ppe_assemble(Context& c) {
                        c.qpos, c.ppe_counters);
   });
   check_launch();
+  pack_state(c);  // the fill gathers the packed positions
   ppe_finish(c, k);
 }
 
@@ -528,7 +538,8 @@ void density_save_and_assemble(Context& c) {
     hipLaunchKernelGGL(k_density_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u,
                        c.w, c.p, static_cast<const uint8_t*>(c.type), nb, k, c.nd, c.nws,
                        static_cast<const double*>(c.ecs), c.ox, c.oz, c.b, c.sol, c.row_len, c.slice_width,
-                       c.qpos, c.ppe_counters);
+                       c.qpos, c.ppe_counters, static_cast<const double2*>(c.xz),
+                       static_cast<const double2*>(c.uw));
   });
   check_launch();
   ppe_finish(c, k);

@@ -48,7 +48,8 @@ KERNEL_IDS = {
 # their own, from the workload's sizes (N rows, Z stored off-diagonals)
 FORMULAS = {
     "k_apply_du": lambda N, Z: 81 * N,  # u, w, x, z read and written; type
-    "k_explicit_update": lambda N, Z: 97 * N,
+    "k_explicit_update": lambda N, Z: 129 * N,  # and the packed (x, z), (u, w) pairs written
+    "k_pack_state": lambda N, Z: 64 * N,
     "k_writeback": lambda N, Z: 17 * N,  # type, sol read; p written
     "k_ir_correct": lambda N, Z: 20 * N,  # x read and written, d read
     "k_jacobi_inv": lambda N, Z: 12 * N,
