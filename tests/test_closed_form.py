@@ -176,3 +176,167 @@ def test_neighbor_distance_band():
             want = math.sqrt(q) < nl
             got = (q < lo) or (q <= hi and math.sqrt(q) < nl)
             assert got == want, (nl, q)
+
+
+# ---------------------------------------------------------------------------
+# Pins for the terms that every lattice known answer leaves at zero or on a
+# single branch: ECS with e != 0, DS with a saved position that differs from
+# the current one, DetermineDt with moving particles and the pressure
+# write-back clamp.  Expected values are evaluated here with numpy from the
+# configuration, not by the code under test.
+
+def _compressed_lattice(h_by_l0, a):
+    """5 x 5 fluid lattice at spacing h (< l0: every interior particle sees
+    more neighbours than at rest, so nWithoutSpp > n0 and e > 0), velocity
+    field u = a x, w = -a z / 2."""
+    s = scenes.lattice(5, 5, h_by_l0 * L0)
+    s["u"] = a * s["x"]
+    s["w"] = -0.5 * a * s["z"]
+    return s
+
+
+def _hand_hs_ecs(s, i, n0):
+    """nWithoutSpp, n, HS speed and ECS of particle i by their definitions
+    (Computer.hpp:780-833, :838-872, :877-910) over all pairs within r_e."""
+    re = 2.1 * L0
+    dx, dz = s["x"] - s["x"][i], s["z"] - s["z"][i]
+    du, dw = s["u"] - s["u"][i], s["w"] - s["w"][i]
+    r = np.hypot(dx, dz)
+    m = (r > 0) & (r < re)
+    nws = float(np.sum(re / r[m] - 1))
+    n = max(nws, n0)
+    hs = 0.0 if nws < n0 else float(-re * np.sum((dx[m] * du[m] + dz[m] * dw[m]) / r[m] ** 3))
+    e = (n - n0) / n0
+    return nws, n, hs, abs(e) * hs + abs(hs) * e
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("a", [-3.0, 3.0])
+def test_ecs_compressed_lattice(kind, a):
+    """ECS = |e| s + |s| e with e = (n - n0)/n0 > 0 (Computer.hpp:877-910).
+    With a < 0 the flow compresses the centre (s > 0) and ECS = 2 e s; with
+    a > 0 it expands (s < 0) and ECS = e s + |s| e = 0 exactly.  Catches a
+    sign error in either product, |e| and |s| swapped (gives 2 e s for the
+    expanding case too, or -2 e s), e taken from n0 - n, or an ECS left at 0.
+    (e from nWithoutSpp instead of n is the same function: the two differ only
+    where nWithoutSpp < n0, and there HS, hence ECS, is 0.)  Then b = -rho/(n0 dt) (s + ECS) (Computer.hpp:1207-1211): catches ECS
+    missing from the right-hand side or added with the wrong sign."""
+    s = _compressed_lattice(0.9, a)
+    c = cases.make(kind, ENV, 1e-10)
+    c.set_time(DT, DT)
+    c.add_particles(s)
+    for st in ("search", "density", "error_correction"):
+        c.stage(st)
+    n0 = c.env()["n0"]
+    centre = 12
+    nws, n, hs, ecs = _hand_hs_ecs(s, centre, n0)
+    assert nws > n0 * 1.1  # compressed: e > 0.1
+    assert c.aux("n_without_spp")[centre] == pytest.approx(nws, rel=1e-13)
+    assert c.particles()["n"][centre] == pytest.approx(n, rel=1e-13)
+    got = c.aux("ecs")[centre]
+    if a < 0:
+        e = (n - n0) / n0
+        assert hs > 0 and ecs == pytest.approx(2 * e * hs, rel=1e-12)
+        assert got == pytest.approx(ecs, rel=1e-11)
+    else:
+        assert hs < 0 and got == 0.0
+    c.stage("ppe_assemble")
+    b = c.ppe()[3]
+    assert b[centre] == pytest.approx(-998.2 / (n0 * DT) * (hs + ecs), rel=1e-11)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("theta", [0.0, 0.7])
+def test_dynamic_stabilize_rotated_pair(kind, theta):
+    """DS with X0 != X (Computer.hpp:1566-1657).  The pair is saved 0.8 l0
+    apart along direction theta, then particle 1 is moved so that the pair
+    vector is dx = (0.5, 0.4) l0 rotated by theta: e = (cos theta, sin theta)
+    comes from the saved positions, r_par = dx.e = 0.5 l0, r_perp^2 = (0.4
+    l0)^2 and Lambda_0 = -1/(2 dt n0) (sqrt(d^2 - r_perp^2) - r_par) e,
+    Lambda_1 = -Lambda_0.  Catches a direction taken from the current dx,
+    r_par replaced by |dx|, r_perp dropped (sqrt(d^2) - r_par), or the
+    components of e transposed (theta = 0.7)."""
+    ct, st = math.cos(theta), math.sin(theta)
+    s = scenes._pack([0.0, 0.8 * L0 * ct], [0.0, 0.8 * L0 * st], [0, 0])
+    c = cases.make(kind, ENV, 1e-10)
+    c.set_time(DT, DT)
+    c.add_particles(s)
+    c.stage("search")
+    c.stage("save_x")
+    # move particle 1 to (0.5, 0.4) l0 in the pair's frame
+    px, pz = 0.5 * L0, 0.4 * L0
+    x1 = px * ct - pz * st
+    z1 = px * st + pz * ct
+    c.set_particles({"x": np.array([0.0, x1]), "z": np.array([0.0, z1])})
+    c.stage("dynamic_stabilize")
+    n0 = c.env()["n0"]
+    d = L0 - 0.1 * L0
+    dx = np.array([x1, z1])
+    e = np.array([ct, st])
+    rpar = float(dx @ e)
+    rperp2 = float(np.sum((dx - rpar * e) ** 2))
+    lam0 = -1.0 / (2 * DT * n0) * (math.sqrt(d * d - rperp2) - rpar) * e
+    assert rpar == pytest.approx(0.5 * L0, rel=1e-12) and rperp2 == pytest.approx((0.4 * L0) ** 2, rel=1e-11)
+    gx, gz = c.aux("du_x"), c.aux("du_z")
+    assert gx[0] == pytest.approx(lam0[0], rel=1e-11, abs=1e-12 * abs(lam0).max())
+    assert gz[0] == pytest.approx(lam0[1], rel=1e-11, abs=1e-12 * abs(lam0).max())
+    assert gx[1] == pytest.approx(-lam0[0], rel=1e-11, abs=1e-12 * abs(lam0).max())
+    assert gz[1] == pytest.approx(-lam0[1], rel=1e-11, abs=1e-12 * abs(lam0).max())
+    P = c.particles()
+    assert P["u"][0] == pytest.approx(lam0[0], rel=1e-11, abs=1e-12 * abs(lam0).max())
+    assert P["x"][0] == pytest.approx(lam0[0] * DT, rel=1e-11, abs=1e-12 * abs(lam0).max() * DT)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_determine_dt_moving(kind):
+    """dt = min(MaxDx / max|u|, MaxDt) over ALL particles (Computer.hpp:759-777).
+    The fastest particle is a Dummy with u = (3, 4) (|u| = 5); a fluid
+    particle has u = (4.5, 0), the largest single component.  Catches the
+    maximum taken over fluid particles only (4.5), over components instead of
+    norms (4.5), or MaxDt returned regardless; then with |u| small the MaxDt
+    branch: catches a missing cap."""
+    s = scenes._pack([0.0, 0.05, 0.1, 0.15], [0.0, 0.0, 0.0, 0.0], [0, 1, 2, 0])
+    s["u"] = np.array([4.5, 0.0, 3.0, -1.0])
+    s["w"] = np.array([0.0, 2.0, 4.0, 0.5])
+    c = cases.make(kind, ENV, 1e-10)
+    c.add_particles(s)
+    env = c.env()
+    assert env["max_dx"] == pytest.approx(0.1 * L0, rel=1e-15)
+    assert c.determine_dt() == env["max_dx"] / 5.0
+    assert env["max_dx"] / 5.0 < env["max_dt"]
+    slow = dict(s, u=s["u"] * 1e-3, w=s["w"] * 1e-3)
+    c2 = cases.make(kind, ENV, 1e-10)
+    c2.add_particles(slow)
+    assert env["max_dx"] / 5e-3 > env["max_dt"]
+    assert c2.determine_dt() == env["max_dt"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_pressure_writeback_clamp(kind):
+    """p_i = x_i < 0 ? 0 : x_i for every particle that is neither Dummy nor
+    Disabled; Dummy (and Disabled) pressure is left as it was
+    (Computer.hpp:1076-1097).  x = -0.0 stays -0.0 (the comparison is false).
+    Catches a clamp applied to dummies, dummies overwritten with x, the
+    clamp missing, or a clamp at the wrong bound."""
+    cfg = scenes.config("dambreak_1k")
+    c = cases.make(kind, cfg["env"], cfg["eps"])
+    s = {k: v.copy() for k, v in cfg["particles"].items()}
+    n = len(s["x"])
+    s["p"] = np.full(n, 123.25)
+    c.add_particles(s)
+    c.set_time(cfg["dt"], cfg["dt"])
+    for st in ("search", "density", "ppe_assemble"):
+        c.stage(st)
+    t = c.particles()["type"]
+    rng = np.random.Generator(np.random.PCG64(2))
+    x = rng.uniform(-50.0, 50.0, n)
+    x[:3] = [-0.0, 0.0, -1e-300]
+    c.set_aux("ppe_x", x)
+    c.stage("pressure_writeback")
+    p = c.particles()["p"]
+    keep = (t == 2) | (t == 3)
+    want = np.where(x < 0, 0.0, x)
+    assert np.array_equal(p[~keep], want[~keep])
+    assert np.all(p[keep] == 123.25)
+    assert keep.any() and (x[~keep] < 0).any()
+    assert t[0] == 0 and math.copysign(1.0, p[0]) == -1.0 and p[2] == 0.0
