@@ -7,10 +7,13 @@
   spaces and tabs removed, columns found by header name.
 * ``read_xml``: the ``openmps`` document of the Benchmark generators
   (``condition``, ``environment`` attributes ``value``, ``particles`` CSV),
-  Main.cpp:202-274.
+  Main.cpp:185-274 (LoadParticles, LoadEnvironment, LoadCondition).
+* ``write_xml``: the same document layout, as
+  Benchmark/DamBreak/generate_koshizukaoka1996.py writes it.
 """
 from __future__ import annotations
 
+import io
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -60,19 +63,57 @@ def read_csv(text: str) -> dict:
 
 
 def read_xml(path: str) -> dict:
-    """Condition, environment and particles of an OpenMps input document."""
+    """Condition, environment and particles of an OpenMps input document.
+
+    Keys as LoadCondition (Main.cpp:258-274: startTime, endTime,
+    outputInterval, eps), LoadEnvironment (Main.cpp:202-256: l_0,
+    minStepCountPerOutput, courant, g, rho, nu, r_eByl_0, minX, minZ, maxX,
+    maxZ; the maximum step is outputInterval / minStepCountPerOutput) and
+    LoadParticles (Main.cpp:185-199: ``<particles type="csv">``, any other
+    type raises as the reference does).  Values may carry blanks (" 0.584").
+    Extra environment keys the 2D SPP method does not read (surfaceRatio, c,
+    minY, maxY) are kept in ``environment`` but not used."""
     root = ET.parse(path).getroot()
+    if root.tag != "openmps":
+        raise ValueError(f"not an OpenMps document: root element <{root.tag}>")
 
     def values(tag):
         node = root.find(tag)
-        return {child.tag: float(child.get("value")) for child in node} if node is not None else {}
+        if node is None:
+            raise KeyError(f"openmps.{tag} missing")
+        return {child.tag: float(child.get("value").strip()) for child in node
+                if isinstance(child.tag, str) and child.get("value") is not None}
 
     cond = values("condition")
     env = values("environment")
-    parts = read_csv(root.find("particles").text)
-    # LoadEnvironment, Main.cpp:234: maxDt = outputInterval / minStepCountPerOutput
-    env_args = dict(max_dt=cond["outputInterval"] / env["minStepCountPerOutput"],
+    for k in ("startTime", "endTime", "outputInterval", "eps"):
+        if k not in cond:
+            raise KeyError(f"openmps.condition.{k} missing")
+    node = root.find("particles")
+    if node is None or node.get("type") != "csv":
+        raise NotImplementedError("Not Implemented!")  # Main.cpp:196-198
+    parts = read_csv(node.text or "")
+    steps_per_output = int(env["minStepCountPerOutput"])  # std::size_t in Main.cpp:205
+    env_args = dict(max_dt=cond["outputInterval"] / steps_per_output,
                     courant=env["courant"], g=env["g"], rho=env["rho"], nu=env["nu"],
                     r_e_by_l0=env["r_eByl_0"], l0=env["l_0"], min_x=env["minX"], min_z=env["minZ"],
                     max_x=env["maxX"], max_z=env["maxZ"])
-    return {"condition": cond, "env": env_args, "particles": parts}
+    return {"condition": cond, "environment": env, "env": env_args, "particles": parts}
+
+
+def write_xml(path: str, condition: dict, environment: dict, particles: dict, digits: int = 17) -> None:
+    """An OpenMps input document: <openmps> with <condition> and
+    <environment> children carrying ``value`` attributes and the particles as
+    a CSV body of <particles type="csv"> (the layout the Benchmark generators
+    write and Main.cpp reads)."""
+    root = ET.Element("openmps")
+    for tag, d in (("condition", condition), ("environment", environment)):
+        node = ET.SubElement(root, tag)
+        for k, v in d.items():
+            ET.SubElement(node, k).set("value", repr(v) if isinstance(v, float) else str(v))
+    buf = io.StringIO()
+    write_csv(particles, buf, digits=digits)
+    p = ET.SubElement(root, "particles")
+    p.set("type", "csv")
+    p.text = "\n" + buf.getvalue()
+    ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=True)
