@@ -124,6 +124,30 @@ def static_pressure(width: int = 50, height: int = 100, l0: float = 1e-3) -> tup
     return parts, env
 
 
+def zhou1999(l0: float = 0.02) -> tuple[dict, dict]:
+    """Dam break onto a dry deck after Zhou et al. (1999), in the layout of
+    Benchmark/DamBreak/generate_zhouetal1999.py:32-151: a tank 1.2 + 2.02 m
+    wide with 1 m walls, a 1.2 m wide and 0.6 m high column against the right
+    wall (fluid particles first, x = (W - 1 - i) l0 outer, z inner), the same
+    wall, dummy, wall-top and corner layers as the other tanks.  Particle
+    counts use math.ceil of the float quotients, as the generator does
+    (3.22 / 0.02 gives 162 columns).  Domain (-4 l0, -4 l0) to ((W + 4) l0,
+    1.2 H l0)."""
+    import math
+    H = math.ceil(1000e-3 / l0)
+    W = math.ceil((1200e-3 + 2020e-3) / l0)
+    h = math.ceil(0.6 / l0)
+    w = math.ceil(1200e-3 / l0)
+    i, j = np.meshgrid(np.arange(w), np.arange(h), indexing="ij")
+    fx = ((W - i.ravel() - 1) * l0).astype(np.float64)
+    fz = (j.ravel() * l0).astype(np.float64)
+    wx, wz, wt = tank_walls(W, H, l0)
+    parts = _pack(np.concatenate([fx, wx]), np.concatenate([fz, wz]),
+                  np.concatenate([np.zeros(len(fx), dtype=np.uint8), np.asarray(wt, np.uint8)]))
+    env = {"min_x": -4 * l0, "min_z": -4 * l0, "max_x": (W + 4) * l0, "max_z": 1.2 * H * l0}
+    return parts, env
+
+
 def lattice(nx: int, nz: int, l0: float, u=None, w=None, p=None) -> dict:
     """nx * nz fluid particles on a square lattice, index i + nx * j (the
     layout of the reference unit tests, e.g. test_ComputerNumberDensity.cpp:82-99)."""
@@ -179,6 +203,8 @@ def config(name: str, seed: int = 1) -> dict:
     still_1m:    0.5 m wide tank filled 1.0 m deep at l0 = 0.7 mm, dt = 2e-5.
     static_pressure: the reference's static-pressure case, 50 x 100 fluid
                  particles at l0 = 1 mm, r_e = 2.4 l0, steps up to 5e-6 s.
+    zhou1999:    the Zhou et al. (1999) dam break of the reference's generator,
+                 l0 = 20 mm, r_e = 2.4 l0, CFL steps up to 5e-4 s.
     Each accepts a suffix ``_ref`` for the reference's single radius
     (r_lap = r_e = 2.1 l0).
     """
@@ -207,6 +233,14 @@ def config(name: str, seed: int = 1) -> dict:
         env = dict(max_dt=5e-6, courant=COURANT, g=G, rho=RHO, nu=1.004e-6, r_e_by_l0=2.4,
                    l0=1e-3, r_lap_by_l0=2.4, **dom)
         return {"name": name, "particles": parts, "env": env, "dt": 5e-6, "eps": 1e-10}
+    elif base == "zhou1999":
+        # Benchmark/DamBreak/generate_zhouetal1999.py:11-28: l0 = 20 mm, r_e = 2.4 l0
+        # (one radius), nu = 1.004e-6, maximum step outputInterval /
+        # minStepCountPerOutput = 0.005 / 10, eps = 1e-10
+        parts, dom = zhou1999(0.02)
+        env = dict(max_dt=5e-4, courant=COURANT, g=G, rho=RHO, nu=1.004e-6, r_e_by_l0=2.4,
+                   l0=0.02, r_lap_by_l0=2.4, **dom)
+        return {"name": name, "particles": parts, "env": env, "dt": None, "eps": 1e-10}
     elif base.startswith("dambreak_l0_"):
         l0 = float(base[len("dambreak_l0_"):])
         dt, eps, jit = 1e-4, 1e-10, 0.02

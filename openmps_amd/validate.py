@@ -16,6 +16,16 @@ dambreak     The collapsing water column (column width L, height 2L, tank
              against the experiments and computations shipped in
              openmps_amd/data/dambreak (front / L at their sample times).
 
+zhou         The dam break onto a dry deck of Zhou et al. (1999) (the
+             reference's Benchmark/DamBreak/zhouetal1999.py:26-39): the water
+             heights h1, h2 at the gauges x = 2020 - 1525 mm and x = 2020 -
+             1028 mm (largest z of the particles within l0 / 2 of the gauge
+             whose number density exceeds 0.1 n0) and the mean pressure p2 of
+             the left wall's Wall particles within 45 mm of z = 160 mm, every
+             output interval, against the experiment and the Fluent VOF
+             computation shipped in openmps_amd/data/zhouetal1999 (RMS
+             differences in h / H and p / (rho g H), H = 0.6 m).
+
 compare      The same scene stepped with two solver presets (e.g. mixed_ml
              against fp64) for a number of steps; the relative L2 difference of
              the pressure and the RMS position difference, in spacings.
@@ -25,6 +35,7 @@ Usage:
   python -m openmps_amd.validate hydrostatic --scene still_1m --steps 500 --solver mixed_ml
   python -m openmps_amd.validate compare --scene dambreak_1m --steps 500 --solvers mixed_ml fp64
   python -m openmps_amd.validate dambreak --scene dambreak_125k --t-star 3.0 --solver mixed_ml
+  python -m openmps_amd.validate zhou --t-star 16 --solver fp64
 Each run prints one JSON line on stdout (progress on stderr).
 """
 from __future__ import annotations
@@ -187,6 +198,87 @@ def run_dambreak(scene: str, t_star: float, preset: str, tol: float, every: floa
             "versus": cmp}
 
 
+ZHOU = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "zhouetal1999")
+
+
+def zhou_references() -> dict:
+    """name -> (t sqrt(g/H), value) of the Zhou et al. (1999) data: the
+    experiment's h1, h2 [m] and p2 [Pa] over its own clock, shifted to its
+    first sample and normalised (h / H, p / (rho g H), rho = 1000 as the
+    reference's script takes it), and the Fluent curves as shipped."""
+    H, g, rho = 0.6, 9.8, 1000.0
+    tt = np.sqrt(g / H)
+    out = {}
+    for q in ("h1", "h2", "p2"):
+        a = np.loadtxt(os.path.join(ZHOU, f"{q}.csv"), delimiter=",", skiprows=1, ndmin=2)
+        scale = H if q != "p2" else rho * g * H
+        out[f"experiment_{q}"] = ((a[:, 0] - a[0, 0]) * tt, a[:, 1] / scale)
+        f = np.loadtxt(os.path.join(ZHOU, f"abdolmalekietal2004_{q}.csv"), delimiter=",", skiprows=1, ndmin=2)
+        out[f"fluent_{q}"] = (f[:, 0], f[:, 1])
+    return out
+
+
+def zhou_gauges(P: dict, l0: float, n0: float) -> tuple[float, float, float]:
+    """h1, h2, p2 of one state (zhouetal1999.py:26-39)."""
+    x_h1, x_h2, z_p2, d = 2020e-3 - 1525e-3, 2020e-3 - 1028e-3, 160e-3, 90e-3
+    dense = P["n"] > 0.1 * n0
+    h = []
+    for xg in (x_h1, x_h2):
+        m = (np.abs(P["x"] - xg) < l0 / 2) & dense
+        h.append(float(max(0.0, P["z"][m].max())) if m.any() else 0.0)
+    m = (P["x"] < 0) & (P["type"] == 1) & (np.abs(P["z"] - z_p2) < d / 2)
+    return h[0], h[1], float(P["p"][m].mean())
+
+
+def run_zhou(t_star: float, preset: str | None, tol: float, scene: str = "zhou1999") -> dict:
+    """The Zhou et al. (1999) dam break to t sqrt(g/H) = t_star with the CFL
+    step; gauges every output interval (5 ms, the generator's)."""
+    from . import Computer, make_environment, scenes, solvers
+    cfg = scenes.config(scene)
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    if preset is not None:
+        solvers.configure(c, preset, tol)
+    c.add_particles(cfg["particles"])
+    H, g, rho = 0.6, 9.8, 1000.0
+    tt = np.sqrt(g / H)
+    l0, n0 = cfg["env"]["l0"], c.env()["n0"]
+    interval = 0.005
+    ts, series = [0.0], [zhou_gauges(c.particles(), l0, n0)]
+    next_t, steps, iters = 0.0, 0, 0
+    end = t_star / tt
+    t0 = time.perf_counter()
+    while next_t + interval <= end + 1e-12:
+        next_t += interval
+        while c.time()[0] < next_t:
+            c.forward_time()
+            steps += 1
+            iters += c.solve_stats()["iterations"]
+        ts.append(c.time()[0] * tt)
+        series.append(zhou_gauges(c.particles(), l0, n0))
+        if len(ts) % 100 == 0:
+            _log(f"  [zhou {preset or 'fp64 (reference rule)'}] t sqrt(g/H) = {ts[-1]:.2f}, {steps} steps, "
+                 f"{time.perf_counter() - t0:.1f} s")
+    ts = np.array(ts)
+    a = np.array(series)
+    ours = {"h1": a[:, 0] / H, "h2": a[:, 1] / H, "p2": a[:, 2] / (rho * g * H)}
+    versus = {}
+    for name, (rt, rv) in zhou_references().items():
+        q = name.split("_")[-1]
+        m = (rt >= ts[0]) & (rt <= ts[-1])
+        if m.sum() == 0:
+            continue
+        d = np.interp(rt[m], ts, ours[q]) - rv[m]
+        versus[name] = {"points": int(m.sum()), "rms": float(np.sqrt(np.mean(d * d))),
+                        "mean": float(d.mean()), "max_abs": float(np.abs(d).max())}
+    peak = int(np.argmax(ours["p2"]))
+    return {"check": "zhou1999", "scene": scene, "solver": preset or "fp64 (reference stop rule)",
+            "l0": l0, "steps": steps, "mean_cg_iterations": iters / max(steps, 1),
+            "t_star_end": float(ts[-1]), "p2_peak": {"t_star": float(ts[peak]), "p_over_rhogH": float(ours["p2"][peak])},
+            "series": {"t_star": [round(float(v), 4) for v in ts],
+                       **{k: [round(float(v), 4) for v in vals] for k, vals in ours.items()}},
+            "versus": versus}
+
+
 def run_compare(scene: str, steps: int, presets: list[str], tol: float) -> dict:
     from . import scenes
     cfg = scenes.config(scene)
@@ -228,8 +320,14 @@ def main(argv=None) -> int:
     d.add_argument("--t-star", type=float, default=3.0)
     d.add_argument("--solver", default="mixed_ml")
     d.add_argument("--tol", type=float, default=1e-8)
+    z = sub.add_parser("zhou")
+    z.add_argument("--t-star", type=float, default=16.0, help="end time t sqrt(g/H)")
+    z.add_argument("--solver", default=None, help="solver preset (default: the reference CG and stop rule)")
+    z.add_argument("--tol", type=float, default=1e-8)
     args = ap.parse_args(argv)
-    if args.cmd == "dambreak":
+    if args.cmd == "zhou":
+        out = run_zhou(args.t_star, args.solver, args.tol)
+    elif args.cmd == "dambreak":
         out = run_dambreak(args.scene, args.t_star, args.solver, args.tol)
     elif args.cmd == "hydrostatic":
         out = run_hydrostatic(args.scene, args.steps, args.solver, args.tol, args.end_time, args.average_from)
