@@ -1,0 +1,189 @@
+"""The HIP path at the configured scales (all tests need a GPU).
+
+* the 1M-particle dam break (2.0 x 1.2 m tank, 0.5 x 1.0 m column, l0 = 0.7 mm,
+  jittered): one step stage by stage, bit-exact against the oracle for the
+  neighbour lists, n, nWithoutSpp, ECS, the explicit update, SaveX, A, b and
+  x0, the pressure within the CG tolerance, and the stages after the solve
+  bit-exact again on the injected solution;
+* ForwardTime through the fused kernels (the path the benchmark times) at 125k
+  and 1M particles: one step against the oracle's ForwardTime, bit-exact for
+  everything computed before the solve, within the solver tolerance after it;
+* the mixed-precision solver (multilevel FP32 CG in FP64 refinement) at 1M,
+  on the dam break and on the still 1 m water column: |b - A x| <= 1e-8 |b|,
+  recomputed here in numpy on the downloaded FP64 matrix;
+* the learned warm start in front of that solver at 1M with the committed
+  weights: the model is used and the same residual is reached;
+* the 1.25k-fluid dam break run for 1000 steps on the GPU and on the oracle:
+  with the CFL step, compared through the surge front and the mean fluid
+  height; with the step fixed at 1e-3 s, both fail at the same step.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+import cases
+from openmps_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _csr(ppe):
+    rp, col, val, b, x = ppe
+    n = len(b)
+    A = scipy.sparse.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    return A, b, x
+
+
+def test_stage_parity_1m():
+    """One step of the 1M-particle dam break, stage by stage (the 125k case
+    runs the same comparison in test_gpu_parity)."""
+    import test_gpu_parity as tp
+    say("parity dambreak_1m (1 step, staged)")
+    g, o = tp.run_parity("dambreak_1m", steps=1)
+    assert g.size() > 1_000_000
+
+
+@pytest.mark.parametrize("name", ["dambreak_125k", "dambreak_1m"])
+def test_forward_time_fused_vs_oracle(name):
+    """ForwardTime(dt) through the fused kernels (first density pass fused with
+    the explicit forces, second density pass fused with SaveX and the PPE
+    count, packed-pair walks) against the oracle's ForwardTime(dt): lists, n,
+    nWithoutSpp, ECS, X0, A and b bit for bit; the pressure within the CG
+    tolerance; positions and velocities to 1e-9 of their range."""
+    cfg = scenes.config(name)
+    g = cases.make("gpu", cfg["env"], cfg["eps"])
+    o = cases.make("oracle", cfg["env"], cfg["eps"])
+    g.add_particles(cfg["particles"])
+    o.add_particles(cfg["particles"])
+    say(f"fused ForwardTime {name}: {g.size()} particles")
+    g.forward_time(cfg["dt"])
+    o.forward_time(cfg["dt"])
+    cg, ig = g.neighbors()
+    co, io = o.neighbors()
+    assert np.array_equal(cg, co) and np.array_equal(ig, io), "neighbour lists differ"
+    for k in ("n_without_spp", "ecs", "saved_x", "saved_z"):
+        assert np.array_equal(g.aux(k), o.aux(k)), k
+    P, Q = g.particles(), o.particles()
+    assert np.array_equal(P["n"], Q["n"]) and np.array_equal(P["type"], Q["type"])
+    A, B = g.ppe(), o.ppe()
+    for a, b, nm in zip(A[:4], B[:4], ("rowptr", "col", "val", "b")):
+        assert np.array_equal(a, b), f"PPE {nm} differs"
+    rel_p = np.linalg.norm(P["p"] - Q["p"]) / np.linalg.norm(Q["p"])
+    say(f"  |p_gpu - p_cpu| / |p_cpu| = {rel_p:.2e}; CG iterations gpu {g.cg_stats()['iterations']} "
+        f"cpu {o.cg_stats()['iterations']}")
+    assert rel_p < 1e-6
+    for k in ("x", "z", "u", "w"):
+        scale = max(np.abs(Q[k]).max(), 1e-12)
+        assert np.abs(P[k] - Q[k]).max() <= 1e-9 * scale, k
+    assert g.time() == o.time()
+
+
+def _mixed(name, preset, steps, weights=None):
+    from openmps_amd import Computer, make_environment, solvers
+    cfg = scenes.config(name)
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    solvers.configure(c, preset, 1e-8, weights)
+    c.add_particles(cfg["particles"])
+    out = []
+    for k in range(steps):
+        c.forward_time(cfg["dt"])
+        s = c.solve_stats()
+        A, b, x = _csr(c.ppe())
+        rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        say(f"  {name} {preset} step {k + 1}: {s['iterations']} FP32 iterations in {s['outer_iterations']} "
+            f"sweeps, reported {s['rel_residual']:.3e}, numpy {rel:.3e}, learned used {s['learned_used']}")
+        assert s["rel_residual"] <= 1e-8 and rel <= 1.0001e-8
+        assert np.all(np.isfinite(c.particles()["p"]))
+        out.append(s)
+    return c, out
+
+
+def test_mixed_ml_1m_dambreak_residual():
+    _mixed("dambreak_1m", "mixed_ml", 3)
+
+
+def test_mixed_ml_1m_still_water_residual():
+    """The still 1 m column (the static-pressure half of the 1M configuration)."""
+    c, st = _mixed("still_1m", "mixed_ml", 2)
+    P = c.particles()
+    fluid = P["type"] == 0
+    assert P["p"][fluid].max() > 0
+
+
+def test_learned_ml_1m_uses_model_and_meets_residual():
+    """The learned warm start with the committed weights in front of the
+    multilevel mixed solver at 1M: the model's guess is taken (it lowers the
+    FP64 residual of the previous-step pressure) on the steps after the first,
+    and the solve reaches the same stop rule."""
+    c, st = _mixed("dambreak_1m", "learned_ml", 4)
+    assert sum(s["learned_used"] for s in st[1:]) >= 2
+    for s in st[1:]:
+        if s["learned_used"]:
+            assert s["initial_rel_residual"] < s["prev_rel_residual"]
+
+
+def _front(P, L):
+    return P["x"][P["type"] == 0].max() / L
+
+
+def test_config1_full_run_against_oracle():
+    """The 1.25k-fluid dam break (l0 = 10 mm, eps = 1e-10) for 1000 steps on the
+    GPU and on the oracle, in the two ways the configuration can be read.
+
+    * ForwardTime() (the CFL step of Computer.hpp:759-777 capped at 1e-3 s, the
+      way the reference's Main.cpp:363-371 advances): trajectories of a
+      collapsing column separate once the rounding differences of the two CG
+      implementations grow, so the runs are compared through what they
+      predict every 100 steps: the surge front (largest fluid x over the
+      column width L) within two spacings, the mean fluid height within 1 %,
+      no particle disabled.
+    * ForwardTime(1e-3) with the step fixed: once the surge reaches the far
+      wall the step is three times the CFL limit and the reference algorithm
+      fails (the oracle raises Computer::Exception after about 345 steps, CG
+      not converged); the GPU path must fail the same way within a few steps."""
+    from openmps_amd import ComputerException
+    import oracle as om
+    cfg = scenes.config("dambreak_1k")
+    L, l0 = 0.25, cfg["env"]["l0"]
+
+    def pair():
+        g = cases.make("gpu", cfg["env"], cfg["eps"])
+        o = cases.make("oracle", cfg["env"], cfg["eps"])
+        g.add_particles(cfg["particles"])
+        o.add_particles(cfg["particles"])
+        return g, o
+
+    g, o = pair()
+    for k in range(1, 1001):
+        g.forward_time(None)
+        o.forward_time(None)
+        if k % 100 == 0:
+            P, Q = g.particles(), o.particles()
+            fg, fo = _front(P, L), _front(Q, L)
+            zg, zo = P["z"][P["type"] == 0].mean(), Q["z"][Q["type"] == 0].mean()
+            say(f"  CFL step {k}: t gpu {g.time()[0]:.4f} cpu {o.time()[0]:.4f}; front/L gpu {fg:.4f} "
+                f"cpu {fo:.4f}; mean fluid z gpu {zg:.5f} cpu {zo:.5f}")
+            assert np.array_equal(P["type"], Q["type"]) and not np.any(P["type"] == 3)
+            assert abs(fg - fo) * L <= 2 * l0
+            assert abs(zg - zo) <= 0.01 * zo
+    assert fo > 3.5  # the surge has crossed the tank (4 L wide)
+
+    def fail_step(c, exc):
+        for k in range(1, 1001):
+            try:
+                c.forward_time(cfg["dt"])
+            except exc:
+                return k
+        return None
+
+    g, o = pair()
+    kg, ko = fail_step(g, ComputerException), fail_step(o, om.CgNotConverged)
+    say(f"  fixed dt 1e-3: CG failure at step gpu {kg}, cpu {ko}")
+    assert ko is not None and kg is not None and ko > 300 and abs(kg - ko) <= 10
