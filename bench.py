@@ -8,9 +8,16 @@ dt = 1e-4 s, pressure solved to |b - A p| <= 1e-8 |b| (FP64 residual).
 
 A step is one full ForwardTime: neighbour search, two density passes, HS/ECS,
 viscosity + gravity, PPE assembly, the pressure solve, pressure gradient and
-DS.  The headline solver runs the timed steps; the plain FP64 CG (the
-reference algorithm) runs the same steps in a second context for the solve
-time comparison.  Prints ONE JSON line on stdout; progress goes to stderr.
+DS.  The headline solver (mixed_ml) steps the scene from rest: steps W+1..W+K
+are the early window; it then runs untimed to step --advance (500: t = 0.05 s,
+the collapsing column), and W warm-up + K timed steps from that developed
+state give `value`.  The plain FP64 CG (the reference algorithm), the learned
+warm start and the zero guess each start from the same developed state in
+contexts of their own (W + K steps), as does the headline solver on the state
+with the particles in a random order.  The CPU oracle runs --cpu-steps steps
+from that state too: its time is the CPU baseline, and its states give the
+error of the GPU steps against an FP64 CPU run.  Prints ONE JSON line on
+stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -20,12 +27,16 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s @1M; solve ms/step learned+mixed vs plain FP64 @res 1e-8"
 HBM_PEAK_GBS = 8000.0
 REPLAYS = 50
+# untimed steps from rest before the timed window: the flow at t = 0.05 s
+ADVANCE = 500
 # HBM bytes per launch of the dambreak_1m SpMVs measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (one run each; FETCH_SIZE scaled by 2, WRITE_SIZE
 # by 1: what the counters read for a copy kernel of known size), recorded with
@@ -45,7 +56,7 @@ def make_computer(cfg, solver: str, tol: float, weights: str | None = None):
     return c
 
 
-def run(c, steps: int, label: str):
+def run(c, steps: int, label: str, every: int = 1):
     """Runs `steps` ForwardTime calls; returns wall seconds and per-step stats.
     Every call is synchronous at return (stream synchronised inside)."""
     stats = []
@@ -54,9 +65,10 @@ def run(c, steps: int, label: str):
         c.forward_time(c._bench_dt)
         s = c.solve_stats()
         stats.append(s)
-        log(f"  [{label}] step {k + 1}/{steps}: CG {s['iterations']} it "
-            f"({s['outer_iterations']} sweeps), solve {s['solve_ms']:.1f} ms, "
-            f"|b-Ax|/|b| {s['rel_residual']:.2e} (from {s['initial_rel_residual']:.2e})")
+        if (k + 1) % every == 0 or k + 1 == steps:
+            log(f"  [{label}] step {k + 1}/{steps}: CG {s['iterations']} it "
+                f"({s['outer_iterations']} sweeps), solve {s['solve_ms']:.1f} ms, "
+                f"|b-Ax|/|b| {s['rel_residual']:.2e} (from {s['initial_rel_residual']:.2e})")
     return time.perf_counter() - t0, stats
 
 
@@ -80,24 +92,42 @@ def profiled_table(c, steps: int, label: str):
     return kernel_table(c)
 
 
-def cpu_baseline(cfg, tol: float, budget_s: float):
-    """The CPU oracle (C++/OpenMP FP64 restatement of the reference path) on
-    one step of the same scene with the same stop rule."""
+def cpu_baseline(cfg, tol: float, state: dict, t: float, dt: float, steps: int, step0: int,
+                 gpu64: dict, gpu_mixed_1: dict | None, solver: str):
+    """The CPU oracle (C++/OpenMP FP64 restatement of the reference path, FP64
+    CG with the same stop rule) for `steps` ForwardTime calls from the same
+    developed state as the GPU comparisons, timed; its first and last states
+    compared with the GPU's plain FP64 run (the reference algorithm) and its
+    first with the headline solver's first step from that state."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.Oracle(cfg["env"], cfg["eps"])
     o.set_stop_rule(1, tol)
-    o.set_verbose(True)
-    o.add_particles(cfg["particles"])
+    o.add_particles(state)
+    o.set_time(t, dt)
     n = o.size()
-    t0 = time.perf_counter()
-    o.forward_time(cfg["dt"])
-    el = time.perf_counter() - t0
-    it = o.cg_stats()["iterations"]
-    log(f"  [cpu oracle] 1 step in {el:.2f} s ({it} CG iterations, {threads} threads)")
-    return {"value": n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"1 ForwardTime step of the same {n}-particle scene, same stop rule "
-                      f"({it} CG iterations), {el:.1f} s", "ms_per_step": el * 1e3}
+    l0 = cfg["env"]["l0"]
+    times, iters, errors = [], [], {}
+    for k in range(1, steps + 1):
+        t0 = time.perf_counter()
+        o.forward_time(cfg["dt"])
+        times.append(time.perf_counter() - t0)
+        iters.append(o.cg_stats()["iterations"])
+        log(f"  [cpu oracle] step {k}/{steps} from the state: {times[-1]:.2f} s, {iters[-1]} CG iterations")
+        if k in gpu64:
+            cnt, ids = o.neighbors()
+            oc = {"counts": cnt, "ids": ids, "P": o.particles(), "b": o.aux("ppe_b")}
+            errors[f"fp64_step{k}"] = compare_to_cpu(gpu64[k], oc, l0)
+            if k == 1 and gpu_mixed_1 is not None:
+                errors[f"{solver}_step1"] = compare_to_cpu(gpu_mixed_1, oc, l0)
+    el = sum(times)
+    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} ForwardTime steps of the same {n}-particle scene from the GPU run's state "
+                      f"after {step0} steps (t = {t:.4f} s), FP64 CG to the same stop rule "
+                      f"({sum(iters) / len(iters):.0f} iterations a step), {el:.1f} s",
+            "ms_per_step": el / steps * 1e3, "cg_iterations": iters,
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads,
+            "error": errors}
 
 
 def run_batch(args) -> int:
@@ -148,23 +178,79 @@ def run_batch(args) -> int:
     return 0
 
 
+def summary(stats, elapsed=None):
+    out = {"solve_ms_per_step": sum(s["solve_ms"] for s in stats) / len(stats),
+           "cg_iterations": sum(s["iterations"] for s in stats) / len(stats),
+           "initial_rel_residual": sum(s["initial_rel_residual"] for s in stats) / len(stats),
+           "rel_residual": max(s["rel_residual"] for s in stats),
+           "learned_used": sum(s["learned_used"] for s in stats)}
+    if elapsed is not None:
+        out["step_ms"] = elapsed / len(stats) * 1e3
+    return out
+
+
+def from_state(cfg, solver: str, tol: float, state: dict, t: float, dt: float, weights=None):
+    """A context holding `state` (particles as Particles() returned them) at
+    time t: the other solvers start from the headline run's developed state."""
+    from openmps_amd import Computer, make_environment, solvers
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    solvers.configure(c, solver, tol, weights)
+    c.add_particles(state)
+    c.set_time(t, dt)
+    c._bench_dt = cfg["dt"]
+    return c
+
+
+def capture(c):
+    """Lists, particles and the assembled right-hand side after a step."""
+    cnt, ids = c.neighbors()
+    return {"counts": cnt, "ids": ids, "P": c.particles(), "b": c.aux("ppe_b")}
+
+
+def compare_to_cpu(g, o, l0):
+    """Error of a GPU step against the oracle's step from the same state."""
+    P, Q = g["P"], o["P"]
+    fl = Q["type"] == 0
+    same_lists = bool(np.array_equal(g["counts"], o["counts"]) and np.array_equal(g["ids"], o["ids"]))
+    dn = np.abs(P["n"] - Q["n"])
+    return {"neighbor_lists_equal": same_lists,
+            "n_bit_equal": bool(np.array_equal(P["n"], Q["n"])),
+            "n_max_rel_diff": float((dn / np.maximum(np.abs(Q["n"]), 1e-300)).max()),
+            "ppe_b_bit_equal": bool(np.array_equal(g["b"], o["b"])),
+            "p_rel_l2": float(np.linalg.norm(P["p"][fl] - Q["p"][fl]) / np.linalg.norm(Q["p"][fl])),
+            "position_max_diff_in_l0": float(max(np.abs(P["x"] - Q["x"]).max(), np.abs(P["z"] - Q["z"]).max()) / l0),
+            "velocity_rel_l2": float(np.sqrt(np.sum((P["u"] - Q["u"]) ** 2 + (P["w"] - Q["w"]) ** 2) /
+                                             np.sum(Q["u"] ** 2 + Q["w"] ** 2)))}
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--advance", type=int, default=ADVANCE,
+                    help="untimed steps from rest before the timed window (the developed flow)")
     ap.add_argument("--workload", default="dambreak_1m")
     from openmps_amd import solvers
-    ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
+    ap.add_argument("--solver", default="mixed_ml", choices=sorted(solvers.PRESETS),
                     help="headline solver (default: mixed_ml, the FP32 multilevel-preconditioned "
                          "CG inside FP64 refinement)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
-    ap.add_argument("--ablation", action="store_true",
-                    help="also time zero / previous / learned guesses for both solvers")
+    ap.add_argument("--ablation", action="store_true", help="also time further solver presets")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-compare", action="store_true", help="skip the other solvers")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=60.0)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps from the developed state")
     ap.add_argument("--threads", type=int, default=8, help="host threads of the batch64 workload")
     args = ap.parse_args()
 
@@ -172,69 +258,101 @@ def main() -> int:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
         return 2
-
     if args.workload == "batch64":
         return run_batch(args)
     from openmps_amd import scenes
     from openmps_amd import learned as _learned
     have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)
-    if args.solver is None:
-        args.solver = "mixed_ml"
     log(f"building scene {args.workload} ...")
     cfg = scenes.config(args.workload)
     N = len(cfg["particles"]["x"])
     nfluid = int((cfg["particles"]["type"] == 0).sum())
+    l0 = cfg["env"]["l0"]
     log(f"  {N} particles ({nfluid} fluid)")
+    W, K = args.warmup, args.steps
 
+    # 1. the early window: the headline solver's steps W+1 .. W+K from rest
     main_c = make_computer(cfg, args.solver, args.tol, args.weights)
     main_c._bench_dt = cfg["dt"]
-    if args.warmup:
-        run(main_c, args.warmup, f"{args.solver} warmup")
-    # timed steps run without per-kernel events; the kernel table comes from
-    # the same number of further steps with events on
-    elapsed, st = run(main_c, args.steps, f"{args.solver} timed")
-    kt = profiled_table(main_c, args.steps, args.solver)
-    # dominant kernel: the SpMV of the solver that ran the timed steps
+    early = {}
+    run(main_c, W, f"{args.solver} early warmup")
+    el, st = run(main_c, K, f"{args.solver} early timed")
+    early[args.solver] = summary(st, el)
+    # 2. the developed flow: untimed steps up to `advance`, then the state the
+    # other solvers start from
+    rest = max(0, args.advance - W - K)
+    run(main_c, rest, f"{args.solver} advance", every=50)
+    state = main_c.particles()
+    t_state, dt_state = main_c.time()
+    step0 = W + K + rest
+    log(f"  developed state after {step0} steps, t = {t_state:.4f} s")
+    # 3. headline window from the developed state: W warm-up + K timed steps
+    el_w, st_w = run(main_c, 1, f"{args.solver} warmup")
+    gpu_mixed_1 = capture(main_c) if not args.no_cpu_baseline else None
+    run(main_c, W - 1, f"{args.solver} warmup")
+    elapsed, st = run(main_c, K, f"{args.solver} timed")
+    kt = profiled_table(main_c, K, args.solver)
     dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32"}[
         solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
     replay = {}
     if dom in ("cg_spmv32", "cg_spmv"):
         replay[dom] = main_c.replay_spmv(dom, REPLAYS)
         log(f"  {dom}: {replay[dom] * 1e3:.1f} us per launch ({REPLAYS} back-to-back launches)")
-    solve_ms = sum(s["solve_ms"] for s in st) / len(st)
-    iters = sum(s["iterations"] for s in st) / len(st)
-
-    def summary(stats):
-        return {"ms_per_step": sum(s["solve_ms"] for s in stats) / len(stats),
-                "cg_iterations": sum(s["iterations"] for s in stats) / len(stats),
-                "initial_rel_residual": sum(s["initial_rel_residual"] for s in stats) / len(stats),
-                "rel_residual": max(s["rel_residual"] for s in stats),
-                "learned_used": sum(s["learned_used"] for s in stats)}
-
-    solve = {args.solver: summary(st)}
-    kt64 = kt if args.solver == "fp64" else {}
-    others = []
-    if not args.no_compare and args.solver != "fp64":
-        others.append("fp64")
-        if have_weights and not solvers.is_learned(args.solver):
-            others.append("learned_ml")  # the learned warm start in front of the same mixed solver
-    if args.ablation:
-        others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_ml", "mixed_ml_zero", "learned_ml",
-                               "learned_mixed")
-                   if v != args.solver and v not in others and (have_weights or "learned" not in v)]
+    solve = {args.solver: summary(st, elapsed)}
     del main_c
+
+    # 4. the other solvers from the same developed state
+    kt64 = kt if args.solver == "fp64" else {}
+    gpu64 = {}
+    others = []
+    if not args.no_compare:
+        if args.solver != "fp64":
+            others.append("fp64")
+        if have_weights and not solvers.is_learned(args.solver):
+            others.append("learned_ml")  # config 4: the learned guess ...
+        if args.solver != "mixed_ml_zero":
+            others.append("mixed_ml_zero")  # ... against a zero guess (and the previous-step one: mixed_ml)
+    if args.ablation:
+        others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_ml", "learned_mixed")
+                   if v != args.solver and v not in others and (have_weights or "learned" not in v)]
     for v in others:
-        ref_c = make_computer(cfg, v, args.tol, args.weights)
-        ref_c._bench_dt = cfg["dt"]
-        if args.warmup:
-            run(ref_c, args.warmup, f"{v} warmup")
-        _, stv = run(ref_c, args.steps, f"{v} timed")
+        c = from_state(cfg, v, args.tol, state, t_state, dt_state, args.weights)
+        if v == "fp64" and not args.no_cpu_baseline:
+            # the reference algorithm's first steps from the state, kept for the CPU comparison
+            for k in range(max(1, args.cpu_steps)):
+                run(c, 1, f"{v} from state")
+                if k == 0 or k == args.cpu_steps - 1:
+                    gpu64[k + 1] = capture(c)
+            run(c, max(0, W - max(1, args.cpu_steps)), f"{v} warmup")
+        else:
+            run(c, W, f"{v} warmup")
+        el_v, stv = run(c, K, f"{v} timed")
         if v == "fp64":
-            kt64 = profiled_table(ref_c, args.steps, v)
-            replay["cg_spmv"] = ref_c.replay_spmv("cg_spmv", REPLAYS)
+            kt64 = profiled_table(c, K, v)
+            replay["cg_spmv"] = c.replay_spmv("cg_spmv", REPLAYS)
             log(f"  cg_spmv: {replay['cg_spmv'] * 1e3:.1f} us per launch ({REPLAYS} back-to-back launches)")
-        solve[v] = summary(stv)
-        del ref_c
+        solve[v] = summary(stv, el_v)
+        del c
+    # the headline solver on the same state with the particles in a random order:
+    # the neighbour walks and the FP32 operator's 16-bit column offsets depend on index locality
+    shuffled = None
+    if not args.no_compare:
+        perm = np.random.Generator(np.random.PCG64(3)).permutation(N)
+        c = from_state(cfg, args.solver, args.tol, {k: v[perm] for k, v in state.items()}, t_state, dt_state,
+                       args.weights)
+        run(c, W, f"{args.solver} shuffled warmup")
+        el_s, sts = run(c, K, f"{args.solver} shuffled timed")
+        shuffled = summary(sts, el_s)
+        shuffled["particle_steps_per_s"] = N * K / el_s
+        del c
+    # the early window of the reference algorithm, for the speedup at rest
+    if not args.no_compare and args.solver != "fp64":
+        c = make_computer(cfg, "fp64", args.tol)
+        c._bench_dt = cfg["dt"]
+        run(c, W, "fp64 early warmup")
+        el_e, ste = run(c, K, "fp64 early timed")
+        early["fp64"] = summary(ste, el_e)
+        del c
 
     def roof(table, name):
         """Achieved GB/s of a kernel: its algorithmic bytes per launch over the
@@ -267,28 +385,38 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(cfg, args.tol, args.cpu_budget)
+            cpu = cpu_baseline(cfg, args.tol, state, t_state, dt_state, max(1, args.cpu_steps), step0,
+                               gpu64, gpu_mixed_1, args.solver)
         except Exception as e:  # the oracle is optional for the measurement itself
             log(f"  cpu baseline failed: {e}")
 
-    value = N * args.steps / elapsed
+    value = N * K / elapsed
+    sp = lambda a, b: (solve[a]["solve_ms_per_step"] / solve[b]["solve_ms_per_step"]
+                       if a in solve and b in solve else None)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "steps": K, "warmup": W, "ms_per_step": elapsed / K * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": ("f64+f32" if solvers.is_mixed(args.solver) else "f64") +
                  ("+bf16" if solvers.is_learned(args.solver) else ""),
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": N, "fluid_particles": nfluid,
-                   "l0": cfg["env"]["l0"], "dt": cfg["dt"], "r_e_by_l0": cfg["env"]["r_e_by_l0"],
+                   "l0": l0, "dt": cfg["dt"], "r_e_by_l0": cfg["env"]["r_e_by_l0"],
                    "r_lap_by_l0": cfg["env"]["r_lap_by_l0"], "solver": args.solver,
+                   "timed_steps": f"{step0 + W + 1}..{step0 + W + K} (t = {t_state:.4f} s at step {step0})",
                    "stop_rule": f"|b-Ax|/|b| <= {args.tol:g}", "parallelism": "single GPU"},
-        "fluid_particle_steps_per_s": nfluid * args.steps / elapsed,
+        "fluid_particle_steps_per_s": nfluid * K / elapsed,
+        "fp64_particle_steps_per_s": (N / (solve["fp64"]["step_ms"] * 1e-3) if "fp64" in solve else None),
         "solve": solve,
-        "solve_speedup_vs_fp64": (solve["fp64"]["ms_per_step"] / solve_ms
-                                  if "fp64" in solve and args.solver != "fp64" else None),
-        "solve_speedup_learned_ml_vs_fp64": (solve["fp64"]["ms_per_step"] / solve["learned_ml"]["ms_per_step"]
-                                             if "fp64" in solve and "learned_ml" in solve else None),
+        "solve_speedup_vs_fp64": sp("fp64", args.solver),
+        "solve_speedup_learned_ml_vs_fp64": sp("fp64", "learned_ml"),
+        "learned_ml_vs_mixed_ml_solve_ratio": (sp("mixed_ml", "learned_ml") if args.solver == "mixed_ml" else None),
+        "shuffled_order": shuffled,
+        "early_window": {"steps": f"{W + 1}..{W + K} from rest", "solve": early,
+                         "solve_speedup_vs_fp64": (early["fp64"]["solve_ms_per_step"] /
+                                                   early[args.solver]["solve_ms_per_step"]
+                                                   if "fp64" in early and args.solver in early else None),
+                         "particle_steps_per_s": N / (early[args.solver]["step_ms"] * 1e-3)},
         "roofline": roofline,
         "roofline_fp64_spmv": roofline_fp64,
         "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),
