@@ -286,6 +286,30 @@ int mps_replay_spmv(mps_ctx* ctx, int id, int reps, double* ms_per_launch);
  * traffic counters (FETCH_SIZE / WRITE_SIZE) to bytes. */
 int mps_device_copy_probe(size_t bytes, double* ms_out);
 
+/* ---- allocation-free solve (no reference counterpart) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Solves the
+ * caller's system A x = b (CSR in device memory: rowptr[n + 1], col and val
+ * [rowptr[n]], the diagonal stored as one of the row's entries) with the FP64
+ * CG of mps_set_ppe + mps_stage_ppe_solve: the same kernels, the reference's
+ * stop rule |r| < allowable_residual |r0| (Computer.hpp:1386, :1408), x the
+ * initial guess on entry and the solution on return, at most max_iter
+ * iterations (<= 0: n).  b, x, rowptr, col and val are device pointers; the
+ * work runs on `stream` (a hipStream_t; NULL: the default stream) of `device`,
+ * and the call returns once it has finished.  Allocates nothing: r, p, q, the
+ * partial sums and the status live in a fixed array of the library that each
+ * device keeps until the process ends (3 MPS_NOALLOC_MAX_ROWS + 3072 doubles,
+ * 24 MiB), so n <= MPS_NOALLOC_MAX_ROWS (MPS_ERR_BAD_ARG above, with the
+ * message in err).  Every call on a device shares that array: calls that
+ * overlap on one device race.  `err` (may be NULL) receives the message of a
+ * failed call; `iterations` (may be NULL) the CG iterations.  Returns
+ * MPS_ERR_CG_NOT_CONVERGED as the stage does. */
+#define MPS_NOALLOC_MAX_ROWS 1048576
+int mps_cg_solve_csr_noalloc(int device, size_t n, const uint64_t* rowptr, const uint32_t* col,
+                             const double* val, const double* b, double* x, double allowable_residual,
+                             int64_t max_iter, void* stream, int64_t* iterations, char* err,
+                             size_t err_len);
+
 /* Device arithmetic self-check: out_sqrt[i] = sqrt(a[i]), out_div[i] = a[i]/b[i]
  * computed by the same code the kernels use (pins rounding against the host). */
 int mps_device_math_check(size_t n, const double* a, const double* b, double* out_sqrt,

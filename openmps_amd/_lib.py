@@ -133,6 +133,8 @@ SIGNATURES = {
     "mps_attach_model": (_I, [_P, _P]),
     "mps_attach_dataset": (_I, [_P, _P, _D, ctypes.c_uint64]),
     "mps_get_features": (_I, [_P, _P]),
+    "mps_cg_solve_csr_noalloc": (_I, [_I, _SZ, _P, _P, _P, _P, _P, _D, _I64, _P,
+                                      ctypes.POINTER(_I64), ctypes.c_char_p, _SZ]),
 }
 
 _lib = None

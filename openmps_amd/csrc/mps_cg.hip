@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <mutex>
 #include <type_traits>
 
 #include "mps_device.hpp"
@@ -92,6 +94,41 @@ __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
     const size_t o = static_cast<size_t>(k) * 64;
     s = fma(__builtin_nontemporal_load(vp + o), v(__builtin_nontemporal_load(cp + o)), s);
   }
+  return s;
+}
+
+// diag_i x_i + sum_j a_ij x_j over the SELL row (the FP64 CG's product).
+template <class V>
+__device__ __forceinline__ double row_product(const Sell<double>& A, size_t i, double xi, V v) {
+  return sell_row(A, i, A.diag[i] * xi, v);
+}
+
+// A caller's system in CSR (device pointers; the diagonal is one of the row's
+// entries): the matrix of the allocation-free CG entry point.  Same block
+// partition fields as Sell.
+struct Csr {
+  const uint64_t* __restrict__ rowptr;
+  const uint32_t* __restrict__ col;
+  const double* __restrict__ val;
+  size_t n;
+  size_t rpb;
+};
+
+// sum_k val[k] x(col[k]) over the row's entries in stored order, two entries
+// in flight per iteration.
+template <class V>
+__device__ __forceinline__ double row_product(const Csr& A, size_t i, double, V v) {
+  const uint64_t lo = A.rowptr[i], hi = A.rowptr[i + 1];
+  double s = 0;
+  uint64_t k = lo;
+  for (; k + 2 <= hi; k += 2) {
+    const uint32_t j0 = A.col[k], j1 = A.col[k + 1];
+    const double a0 = A.val[k], a1 = A.val[k + 1];
+    const double x0 = v(j0), x1 = v(j1);
+    s = fma(a0, x0, s);
+    s = fma(a1, x1, s);
+  }
+  if (k < hi) s = fma(A.val[k], v(A.col[k]), s);
   return s;
 }
 
@@ -251,7 +288,8 @@ __global__ void __launch_bounds__(256) k_rec_init(size_t n, const uint2* __restr
 }
 
 // r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
-__global__ void __launch_bounds__(NT) k_cg_init(Sell<double> A, const double* __restrict__ b,
+template <class M>
+__global__ void __launch_bounds__(NT) k_cg_init(M A, const double* __restrict__ b,
                                                 const double* __restrict__ x, double* __restrict__ r,
                                                 double* __restrict__ part_rr,
                                                 double* __restrict__ part_bb) {
@@ -259,7 +297,7 @@ __global__ void __launch_bounds__(NT) k_cg_init(Sell<double> A, const double* __
   const Rows rw = block_rows(A.n, A.rpb);
   double arr = 0, abb = 0;
   for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const double s = sell_row(A, i, A.diag[i] * x[i], [&](uint32_t j) { return x[j]; });
+    const double s = row_product(A, i, x[i], [&](uint32_t j) { return x[j]; });
     const double ri = b[i] - s;
     r[i] = ri;
     arr = fma(ri, ri, arr);
@@ -284,8 +322,8 @@ struct StopRule {
 // direction's product follows by linearity:
 //   p_k = r_k + beta p_{k-1},  A p_k = A r_k + beta A p_{k-1},
 // so p and q = A p are updated in place for the block's own rows.
-template <bool FIRST>
-__global__ void __launch_bounds__(NT, 8) k_cg_spmv(Sell<double> A, const double* __restrict__ r,
+template <bool FIRST, class M>
+__global__ void __launch_bounds__(NT, 8) k_cg_spmv(M A, const double* __restrict__ r,
                                                    double* __restrict__ p, double* __restrict__ q,
                                                    const double* __restrict__ part_in,
                                                    const double* __restrict__ part_bb, int nparts,
@@ -329,7 +367,7 @@ __global__ void __launch_bounds__(NT, 8) k_cg_spmv(Sell<double> A, const double*
   double acc = 0;
   for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
     const double ri = r[i];
-    const double s = sell_row(A, i, A.diag[i] * ri, [&](uint32_t j) { return r[j]; });
+    const double s = row_product(A, i, ri, [&](uint32_t j) { return r[j]; });
     const double pi = FIRST ? ri : fma(beta, p[i], ri);
     const double qi = FIRST ? s : fma(beta, q[i], s);
     p[i] = pi;
@@ -1065,7 +1103,7 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
   MPS_HIP(hipMemsetAsync(c.st, 0, sizeof(CgStatus), c.stream));
   c.replay_ready = MPS_K_CG_SPMV;
   tracked(c, MPS_K_CG_INIT, spmv_bytes(c, 8, 8.0 + 8 + 8), [&] {
-    hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
+    hipLaunchKernelGGL(k_cg_init<Sell<double>>, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
                        static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
   });
   check_launch();
@@ -1079,12 +1117,12 @@ void solve_cg64(Context& c, const StopRule& sr, long long max_iter) {
     for (long long m = 0; m < chunk && it <= max_iter; m++, it++) {
       tracked(c, MPS_K_CG_SPMV, b_spmv, [&] {
         if (it == 1)
-          hipLaunchKernelGGL(k_cg_spmv<true>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+          hipLaunchKernelGGL((k_cg_spmv<true, Sell<double>>), dim3(L.grid), dim3(NT), 0, c.stream, A,
                              static_cast<const double*>(c.r), c.pa, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
         else
-          hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(NT), 0, c.stream, A,
+          hipLaunchKernelGGL((k_cg_spmv<false, Sell<double>>), dim3(L.grid), dim3(NT), 0, c.stream, A,
                              static_cast<const double*>(c.r), c.pa, c.q,
                              static_cast<const double*>(c.partB), static_cast<const double*>(c.partC),
                              L.grid, c.partA, c.st, sr, it);
@@ -1198,7 +1236,7 @@ void reduce2(Context& c, const double* a, const double* b, int nparts) {
 void residual64_launch(Context& c) {
   const Launch L = cg_grid(c.ppe_n);
   const Sell<double> A = sell<double>(c, c.sval, c.diag, L.rpb);
-  hipLaunchKernelGGL(k_cg_init, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
+  hipLaunchKernelGGL(k_cg_init<Sell<double>>, dim3(L.grid), dim3(NT), 0, c.stream, A, static_cast<const double*>(c.b),
                      static_cast<const double*>(c.sol), c.r, c.partB, c.partC);
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(NT), 0, c.stream, static_cast<const double*>(c.partB),
                      static_cast<const double*>(c.partC), L.grid, c.red_out);
@@ -1615,7 +1653,7 @@ double replay_spmv(Context& c, int kid, int reps) {
                          static_cast<const double*>(c.part_ml), c.part_ml_cap, static_cast<int>(pgrid),
                          c.part3b, c.st_replay, 1e-8, 2LL);
     else
-      hipLaunchKernelGGL(k_cg_spmv<false>, dim3(L.grid), dim3(NT), 0, c.stream,
+      hipLaunchKernelGGL((k_cg_spmv<false, Sell<double>>), dim3(L.grid), dim3(NT), 0, c.stream,
                          sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const double*>(c.r), c.pa, c.q,
                          static_cast<const double*>(c.partB), static_cast<const double*>(c.partC), L.grid,
                          c.partA, c.st_replay, StopRule{MPS_STOP_RHS_NORM, 1e-8}, 2LL);
@@ -1746,3 +1784,133 @@ void ml_setup_ahead(Context& c) {
 void join_aux(Context& c) { join_ml_setup(c); }
 
 }  // namespace mps
+
+// ---------------- allocation-free FP64 CG on a caller's CSR system ----------------
+//
+// mps_cg_solve_csr_noalloc runs the FP64 CG of solve_cg64 (the same kernels,
+// instantiated for a CSR matrix) without allocating: its work vectors r, p, q,
+// the partial-sum slots and the status live in `scratch`, a fixed array of
+// the code object that every device holding the library keeps until the
+// process ends.  Host code never passes `scratch` itself to a kernel (that
+// name is a host-side stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+namespace mps {
+namespace {
+
+constexpr size_t kNoAllocMaxRows = MPS_NOALLOC_MAX_ROWS;
+constexpr size_t kStatusDoubles = (sizeof(CgStatus) + 7) / 8;
+constexpr size_t kScratchDoubles = 3 * kNoAllocMaxRows + 3 * static_cast<size_t>(kRedBlocks) + kStatusDoubles;
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev` (looked up once).
+double* scratch_ptr(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Failure{MPS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+__global__ void k_status_clear(CgStatus* st) {
+  if (threadIdx.x == 0) *st = CgStatus{};
+}
+
+void csr_solve(size_t n, const uint64_t* rowptr, const uint32_t* col, const double* val, const double* b,
+               double* x, double eps, long long max_iter, hipStream_t s, double* work, long long* iters_out) {
+  double* r = work;
+  double* p = r + kNoAllocMaxRows;
+  double* q = p + kNoAllocMaxRows;
+  double* partA = q + kNoAllocMaxRows;
+  double* partB = partA + kRedBlocks;
+  double* partC = partB + kRedBlocks;
+  CgStatus* st = reinterpret_cast<CgStatus*>(partC + kRedBlocks);
+  const Launch L = cg_grid(n);
+  const Csr A{rowptr, col, val, n, L.rpb};
+  const StopRule sr{MPS_STOP_INITIAL_RESIDUAL, eps};
+  // every value the iterations read is written first: the status by the clear
+  // kernel, r and the partial slots by k_cg_init, p and q by the first
+  // iteration (FIRST: p = r, q = A r)
+  hipLaunchKernelGGL(k_status_clear, dim3(1), dim3(64), 0, s, st);
+  hipLaunchKernelGGL(k_cg_init<Csr>, dim3(L.grid), dim3(NT), 0, s, A, b, static_cast<const double*>(x), r, partB,
+                     partC);
+  check_launch();
+  CgStatus hs{};
+  long long it = 1;
+  while (true) {
+    const long long chunk = next_chunk(it - 1, 0);
+    for (long long m = 0; m < chunk && it <= max_iter; m++, it++) {
+      if (it == 1)
+        hipLaunchKernelGGL((k_cg_spmv<true, Csr>), dim3(L.grid), dim3(NT), 0, s, A, static_cast<const double*>(r), p,
+                           q, static_cast<const double*>(partB), static_cast<const double*>(partC), L.grid, partA,
+                           st, sr, it);
+      else
+        hipLaunchKernelGGL((k_cg_spmv<false, Csr>), dim3(L.grid), dim3(NT), 0, s, A, static_cast<const double*>(r),
+                           p, q, static_cast<const double*>(partB), static_cast<const double*>(partC), L.grid,
+                           partA, st, sr, it);
+      hipLaunchKernelGGL(k_cg_update, dim3(L.grid), dim3(NT), 0, s, n, L.rpb, x, r, static_cast<const double*>(p),
+                         static_cast<const double*>(q), static_cast<const double*>(partA), L.grid, partB, st, it);
+    }
+    check_launch();
+    if (it > max_iter)
+      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(NT), 0, s, static_cast<const double*>(partB), L.grid, st, sr, it);
+    MPS_HIP(hipMemcpyAsync(&hs, st, sizeof(CgStatus), hipMemcpyDeviceToHost, s));
+    MPS_HIP(hipStreamSynchronize(s));
+    if (hs.converged || it > max_iter) break;
+  }
+  if (iters_out) *iters_out = hs.converged ? hs.iterations : max_iter;
+  if (!hs.converged)
+    throw Failure{MPS_ERR_CG_NOT_CONVERGED, "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
+}
+
+void put_err(char* err, size_t len, const char* msg) {
+  if (err && len) std::snprintf(err, len, "%s", msg);
+}
+
+}  // namespace
+}  // namespace mps
+
+extern "C" int mps_cg_solve_csr_noalloc(int device, size_t n, const uint64_t* rowptr, const uint32_t* col,
+                                        const double* val, const double* b, double* x,
+                                        double allowable_residual, int64_t max_iter, void* stream,
+                                        int64_t* iterations, char* err, size_t err_len) {
+  using namespace mps;
+  put_err(err, err_len, "");
+  if (iterations) *iterations = 0;
+  if (n > kNoAllocMaxRows) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "mps_cg_solve_csr_noalloc: %zu rows exceed the fixed work area (at most %zu)", n,
+                  kNoAllocMaxRows);
+    put_err(err, err_len, msg);
+    return MPS_ERR_BAD_ARG;
+  }
+  if (device < 0 || device >= kMaxDevices || !(allowable_residual > 0) ||
+      (n > 0 && (!rowptr || !col || !val || !b || !x))) {
+    put_err(err, err_len, "mps_cg_solve_csr_noalloc: bad device, tolerance or pointer");
+    return MPS_ERR_BAD_ARG;
+  }
+  if (n == 0) return MPS_OK;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  int rc = MPS_OK;
+  try {
+    MPS_HIP(hipSetDevice(device));
+    double* work = scratch_ptr(device);
+    csr_solve(n, rowptr, col, val, b, x, allowable_residual, max_iter > 0 ? max_iter : static_cast<long long>(n),
+              static_cast<hipStream_t>(stream), work, reinterpret_cast<long long*>(iterations));
+  } catch (Failure& f) {
+    if (f.code == MPS_ERR_HIP) (void)hipGetLastError();
+    put_err(err, err_len, f.msg.c_str());
+    rc = f.code;
+  }
+  if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
+  return rc;
+}
