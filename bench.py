@@ -131,22 +131,58 @@ def cpu_baseline(cfg, tol: float, state: dict, t: float, dt: float, steps: int, 
 
 
 def run_batch(args) -> int:
-    """A batch of 64 uneven independent scenes (scenes.batch): one context
-    per scene, each with its own HIP stream; `threads` host threads each step
-    a size-balanced share of the contexts (ctypes releases the GIL inside the
-    library, so the streams run concurrently).  value = total particle-steps
-    per second over the K timed steps of every scene."""
+    """64 uneven independent scenes (scenes.batch: dam breaks and static
+    columns, 19k-207k particles, l0 0.5-2 mm): the batch API steps them
+    together (per-scene stages from `threads` host threads, the solves of all
+    scenes as grouped launches, one MLP inference over all rows), each scene on
+    its own CFL step.  value = total particle-steps per second over the K timed
+    steps.  Beside it, from the same initial state: the same scenes as 64
+    separate contexts stepped from the same host threads (each scene's solve
+    launched on its own), and every scene alone with the plain FP64 CG for the
+    W + K steps, whose final state gives the batch's per-scene error."""
     from concurrent.futures import ThreadPoolExecutor
-    from openmps_amd import scenes, solvers
+    from openmps_amd import Batch, learned, scenes, solvers
     cfgs = scenes.batch()
-    solver = args.solver or "mixed_ml"
+    solver = args.solver or "learned_ml"
+    model = learned.Model.load(args.weights or learned.DEFAULT_WEIGHTS) if solvers.is_learned(solver) else None
     log(f"building {len(cfgs)} scenes ...")
-    comps = [make_computer(cfg, solver, args.tol) for cfg in cfgs]
     sizes = [len(cfg["particles"]["x"]) for cfg in cfgs]
-    threads = args.threads
-    groups = [[] for _ in range(threads)]
-    load = [0] * threads
-    for k in sorted(range(len(comps)), key=lambda k: -sizes[k]):  # largest first onto the lightest share
+    total = sum(sizes)
+    W, K = args.warmup, args.steps
+
+    def contexts(preset):
+        out = []
+        for cfg in cfgs:
+            c = make_computer(cfg, preset, args.tol, args.weights) if model is None or not solvers.is_learned(preset) \
+                else _with_model(cfg, preset, args.tol, model)
+            out.append(c)
+        return out
+
+    # 1. the batch
+    comps = contexts(solver)
+    batch = Batch(comps, host_threads=args.threads)
+    for k in range(W):
+        batch.forward_time()
+    log(f"  batch: {len(comps)} scenes, {total} particles, {args.threads} host threads: timing {K} steps")
+    solve_ms, launches = [], []
+    t0 = time.perf_counter()
+    for k in range(K):
+        batch.forward_time()
+        info = batch.solve_info()
+        solve_ms.append(info["solve_ms"])
+        launches.append(info["launches"])
+    elapsed = time.perf_counter() - t0
+    stats = [c.solve_stats() for c in comps]
+    final = [c.particles() for c in comps]
+    log(f"  batch: {elapsed / K * 1e3:.1f} ms/step, solve {np.mean(solve_ms):.1f} ms, "
+        f"{np.mean(launches):.0f} grouped launches a solve")
+    del batch
+
+    # 2. the same scenes as separate contexts, each stepping (and solving) on its own
+    sep = contexts(solver)
+    groups = [[] for _ in range(args.threads)]
+    load = [0] * args.threads
+    for k in sorted(range(len(sep)), key=lambda k: -sizes[k]):  # largest first onto the lightest share
         w = load.index(min(load))
         groups[w].append(k)
         load[w] += sizes[k]
@@ -154,28 +190,83 @@ def run_batch(args) -> int:
     def work(idx, steps):
         for _ in range(steps):
             for k in idx:
-                comps[k].forward_time()
+                sep[k].forward_time()
         return len(idx)
 
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        if args.warmup:
-            list(ex.map(lambda g: work(g, args.warmup), groups))
-        log(f"  {len(comps)} scenes, {sum(sizes)} particles, {threads} threads: timing {args.steps} steps")
-        t0 = time.perf_counter()
-        list(ex.map(lambda g: work(g, args.steps), groups))
-        elapsed = time.perf_counter() - t0
-    iters = [c.solve_stats()["iterations"] for c in comps]
-    total = sum(sizes)
-    out = {"metric": "particle-steps/s, 64 uneven dam-break scenes batched on one GPU", "value": total * args.steps / elapsed,
-           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64+f32" if solvers.is_mixed(solver) else "f64", "data": "synthetic",
-           "config": {"workload": "batch64", "scenes": len(comps), "particles": total,
-                      "particles_min_max": [min(sizes), max(sizes)], "solver": solver, "host_threads": threads,
-                      "stop_rule": f"|b-Ax|/|b| <= {args.tol:g}", "dt": "CFL rule, at most 1e-4 s"},
-           "last_step_cg_iterations_mean": sum(iters) / len(iters)}
+    with ThreadPoolExecutor(max_workers=args.threads) as ex:
+        list(ex.map(lambda g: work(g, W), groups))
+        t1 = time.perf_counter()
+        list(ex.map(lambda g: work(g, K), groups))
+        el_sep = time.perf_counter() - t1
+    log(f"  separate contexts: {el_sep / K * 1e3:.1f} ms/step")
+    del sep
+
+    # 3. every scene alone with the FP64 CG (the reference algorithm), W + K
+    # steps; and, as a control of what the stop rule itself resolves, alone
+    # with the FP64 CG to a 100 times tighter residual
+    def err(P, Q, l0):
+        fl = Q["type"] == 0
+        return (float(np.linalg.norm(P["p"][fl] - Q["p"][fl]) / np.linalg.norm(Q["p"][fl])),
+                float(max(np.abs(P["x"] - Q["x"]).max(), np.abs(P["z"] - Q["z"]).max()) / l0))
+
+    errs = []
+    t2 = time.perf_counter()
+    for s, cfg in enumerate(cfgs):
+        runs = []
+        for tol in (args.tol, args.tol / 100):
+            o = make_computer(cfg, "fp64", tol)
+            for _ in range(W + K):
+                o.forward_time()
+            runs.append(o.particles())
+            del o
+        pe_, xe_ = err(final[s], runs[0], cfg["l0"])
+        pc_, xc_ = err(runs[1], runs[0], cfg["l0"])
+        errs.append({"scene": s, "kind": cfg["kind"], "particles": sizes[s], "l0": cfg["l0"],
+                     "p_rel_l2": pe_, "position_max_diff_in_l0": xe_,
+                     "control_fp64_tight_p_rel_l2": pc_, "control_fp64_tight_position_max_diff_in_l0": xc_,
+                     "cg_iterations": stats[s]["iterations"],
+                     "learned_used": stats[s]["learned_used"], "rel_residual": stats[s]["rel_residual"]})
+    log(f"  FP64 runs alone: {time.perf_counter() - t2:.1f} s")
+    pe = [e["p_rel_l2"] for e in errs]
+    xe = [e["position_max_diff_in_l0"] for e in errs]
+    pc = [e["control_fp64_tight_p_rel_l2"] for e in errs]
+    out = {"metric": "particle-steps/s, 64 uneven scenes batched on one GPU", "value": total * K / elapsed,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": ("f64+f32" if solvers.is_mixed(solver) else "f64") +
+                                         ("+bf16" if solvers.is_learned(solver) else ""),
+           "data": "synthetic",
+           "config": {"workload": "batch64", "scenes": len(cfgs), "particles": total,
+                      "particles_min_max": [min(sizes), max(sizes)],
+                      "kinds": {k: sum(1 for c in cfgs if c["kind"] == k) for k in ("dambreak", "static")},
+                      "solver": solver, "host_threads": args.threads,
+                      "stop_rule": f"|b-Ax|/|b| <= {args.tol:g}", "dt": "each scene's CFL step, at most 1e-4 s",
+                      "timed_steps": f"{W + 1}..{W + K} from rest"},
+           "solve_ms_per_step": float(np.mean(solve_ms)), "grouped_launches_per_solve": float(np.mean(launches)),
+           "separate_contexts": {"particle_steps_per_s": total * K / el_sep, "ms_per_step": el_sep / K * 1e3,
+                                 "host_threads": args.threads},
+           "speedup_vs_separate_contexts": el_sep / elapsed,
+           "error_vs_fp64_alone": {
+               "after_steps": W + K, "p_rel_l2_max": max(pe), "p_rel_l2_median": float(np.median(pe)),
+               "position_max_diff_in_l0_max": max(xe),
+               "by_kind": {k: {"p_rel_l2_max": max(e["p_rel_l2"] for e in errs if e["kind"] == k),
+                               "control_fp64_tight_p_rel_l2_max": max(e["control_fp64_tight_p_rel_l2"]
+                                                                      for e in errs if e["kind"] == k)}
+                           for k in ("dambreak", "static")},
+               "control_fp64_tight_p_rel_l2_median": float(np.median(pc)),
+               "per_scene": errs},
+           "max_rel_residual": max(s_["rel_residual"] for s_ in stats),
+           "learned_used_last_step": sum(s_["learned_used"] for s_ in stats)}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _with_model(cfg, preset, tol, model):
+    from openmps_amd import Computer, make_environment, solvers
+    c = Computer(cfg["eps"], make_environment(cfg["env"]))
+    solvers.configure(c, preset, tol, model=model)
+    c.add_particles(cfg["particles"])
+    return c
 
 
 def summary(stats, elapsed=None):
@@ -242,9 +333,10 @@ def main() -> int:
                     help="untimed steps from rest before the timed window (the developed flow)")
     ap.add_argument("--workload", default="dambreak_1m")
     from openmps_amd import solvers
-    ap.add_argument("--solver", default="mixed_ml", choices=sorted(solvers.PRESETS),
+    ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
                     help="headline solver (default: mixed_ml, the FP32 multilevel-preconditioned "
-                         "CG inside FP64 refinement)")
+                         "CG inside FP64 refinement; learned_ml, the same with the learned warm start, "
+                         "for batch64)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
     ap.add_argument("--ablation", action="store_true", help="also time further solver presets")
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -260,6 +352,7 @@ def main() -> int:
         return 2
     if args.workload == "batch64":
         return run_batch(args)
+    args.solver = args.solver or "mixed_ml"
     from openmps_amd import scenes
     from openmps_amd import learned as _learned
     have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)

@@ -253,6 +253,48 @@ int mps_attach_dataset(mps_ctx* ctx, mps_dataset* d, double fraction, uint64_t s
  * assembled from the particles (MPS_ERR_STATE otherwise). */
 int mps_get_features(mps_ctx* ctx, float* out);
 
+/* Storage order of the particles on the device (extension, no reference
+ * counterpart; every getter and setter works in the caller's order either
+ * way).  MPS_ORDER_CELL (default): each neighbour search stores the particles
+ * in cell order (cells in key order, ascending caller index inside a cell), so
+ * neighbours sit close together in memory whatever order the caller used.
+ * MPS_ORDER_KEEP: the current order is kept (the caller's order unless a
+ * search already permuted it).  Results are the same bit for bit up to the
+ * solver's own rounding (its sums run in storage order); a particle that
+ * leaves the grid reports an empty neighbour list in cell order (the reference
+ * keeps its last list; no stage reads it). */
+enum mps_storage_order { MPS_ORDER_CELL = 0, MPS_ORDER_KEEP = 1 };
+int mps_set_storage_order(mps_ctx* ctx, int order);
+
+/* ---- several scenes stepped together (extension, no reference counterpart) ----
+ * A batch steps independent scenes (contexts on one device) together.  Each
+ * step runs the stages before and after the pressure solve per scene on the
+ * scenes' own streams from `host_threads` host threads (0: the calling
+ * thread), and the solves of every scene on the mixed solver with the
+ * multilevel preconditioner as one set of grouped launches: each kernel of
+ * the solve is one launch over all scenes still iterating, every block
+ * working on one scene with that scene's own alpha, beta and stop test
+ * (block-diagonal CG with per-scene convergence).  Scenes on the learned warm
+ * start share one inference launch over all their feature rows (they must
+ * have the same model attached).  Scenes on other solvers solve on their own.
+ * Results per scene are those of its own ForwardTime up to the solver's
+ * rounding.  The batch borrows the contexts (destroy it first) and the
+ * contexts must not be used by other threads during a batch call. */
+typedef struct mps_batch mps_batch;
+int mps_batch_create(int device, int host_threads, mps_batch** out);
+void mps_batch_destroy(mps_batch* b);
+const char* mps_batch_last_error(const mps_batch* b);
+int mps_batch_add(mps_batch* b, mps_ctx* scene);
+/* Computer::ForwardTime for every scene (Computer.hpp:1700-1751): dt > 0 the
+ * same fixed step for all; dt <= 0 each scene's ForwardTime() with its own
+ * DetermineDt step (:1745-1751, :759-777), written to dt_out[s] if dt_out is
+ * not NULL.  An error names the scene ("scene k: ..."); the scenes without an
+ * error have completed their step. */
+int mps_batch_forward_time(mps_batch* b, double dt, double* dt_out);
+/* The last batched solve: its duration (HIP events on the batch's stream) and
+ * the number of grouped launches it made. */
+int mps_batch_solve_info(const mps_batch* b, double* ms, int64_t* launches);
+
 /* ---- measurement (no reference counterpart) ---- */
 enum mps_kernel_id {
   MPS_K_CELL_KEYS = 0, MPS_K_CELL_SCAN, MPS_K_CELL_SCATTER, MPS_K_CELL_SORT, MPS_K_NEIGHBORS,

@@ -5,12 +5,12 @@ kernels for gfx950 driven by a thin host layer.  This package is the Python
 mirror of the reference's ``Environment`` / ``Computer`` interface over that
 ABI, plus the synthetic scene generator used by the benchmarks.
 """
-from .computer import (Computer, ComputerException, CreateComputer, Environment, Grid,
+from .computer import (Batch, Computer, ComputerException, CreateComputer, Environment, Grid,
                        GridException, MpsError, ParticleType, device_copy_probe,
                        device_math_check, make_environment_from)
 from . import scenes
 
-__all__ = ["Computer", "ComputerException", "CreateComputer", "Environment", "Grid",
+__all__ = ["Batch", "Computer", "ComputerException", "CreateComputer", "Environment", "Grid",
            "GridException", "MpsError", "ParticleType", "device_copy_probe", "device_math_check", "scenes",
            "make_environment"]
 

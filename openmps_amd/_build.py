@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libmps_hip.so")
-SOURCES = ["mps_api.cpp", "mps_search.hip", "mps_particles.hip", "mps_ppe.hip", "mps_cg.hip",
+SOURCES = ["mps_api.cpp", "mps_batch.cpp", "mps_search.hip", "mps_particles.hip", "mps_ppe.hip", "mps_cg.hip",
            "mps_mlp.hip"]
 ARCH = os.environ.get("MPS_OFFLOAD_ARCH", "gfx950")
 

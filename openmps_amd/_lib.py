@@ -33,6 +33,7 @@ AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_
        "ppe_b": 6, "ppe_x": 7}
 
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL = 0, 1, 2
+ORDER_CELL, ORDER_KEEP = 0, 1
 
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
            "explicit", "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv",
@@ -133,6 +134,13 @@ SIGNATURES = {
     "mps_attach_model": (_I, [_P, _P]),
     "mps_attach_dataset": (_I, [_P, _P, _D, ctypes.c_uint64]),
     "mps_get_features": (_I, [_P, _P]),
+    "mps_set_storage_order": (_I, [_P, _I]),
+    "mps_batch_create": (_I, [_I, _I, ctypes.POINTER(_P)]),
+    "mps_batch_destroy": (None, [_P]),
+    "mps_batch_last_error": (ctypes.c_char_p, [_P]),
+    "mps_batch_add": (_I, [_P, _P]),
+    "mps_batch_forward_time": (_I, [_P, _D, _P]),
+    "mps_batch_solve_info": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     "mps_cg_solve_csr_noalloc": (_I, [_I, _SZ, _P, _P, _P, _P, _P, _D, _I64, _P,
                                       ctypes.POINTER(_I64), ctypes.c_char_p, _SZ]),
 }

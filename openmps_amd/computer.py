@@ -366,6 +366,11 @@ class Computer:
         s = self.solve_stats()
         return {"iterations": int(s["iterations"]), "rr0": s["rr0"], "rr": s["rr"]}
 
+    def set_storage_order(self, keep: bool) -> None:
+        """keep=False (default): particles stored in cell order from each
+        neighbour search on; keep=True: the current storage order is kept."""
+        _raise(L.lib().mps_set_storage_order(self._h, L.ORDER_KEEP if keep else L.ORDER_CELL), self._h)
+
     def set_profiling(self, on: bool) -> None:
         _raise(L.lib().mps_set_profiling(self._h, int(on)), self._h)
 
@@ -388,6 +393,57 @@ class Computer:
             out[s.name.decode()] = {"launches": s.launches, "ms": s.total_ms,
                                     "bytes": s.total_bytes}
         return out
+
+
+class Batch:
+    """Independent scenes stepped together (mps_batch_*): the stages before
+    and after the pressure solve per scene from `host_threads` host threads,
+    the solves of the scenes on the mixed multilevel solver as grouped
+    launches over all scenes (per-scene alpha, beta and stop test), one MLP
+    inference over all learned scenes' rows.  Holds references to its scenes."""
+
+    def __init__(self, scenes=(), host_threads: int = 8, device: int = 0):
+        self._h = None
+        h = ctypes.c_void_p()
+        rc = L.lib().mps_batch_create(device, host_threads, ctypes.byref(h))
+        if rc != L.MPS_OK:
+            raise MpsError("no HIP device available" if rc == L.MPS_ERR_NO_DEVICE else f"mps error {rc}")
+        self._h = h
+        self.scenes = []
+        for c in scenes:
+            self.add(c)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.lib().mps_batch_destroy(self._h)
+            self._h = None
+
+    def _check(self, rc: int) -> None:
+        if rc == L.MPS_OK:
+            return
+        msg = L.lib().mps_batch_last_error(self._h).decode()
+        if rc == L.MPS_ERR_CG_NOT_CONVERGED:
+            raise ComputerException(msg)
+        if rc == L.MPS_ERR_CELL_OVERFLOW:
+            raise GridException(msg)
+        raise MpsError(f"mps error {rc}: {msg}")
+
+    def add(self, c: "Computer") -> None:
+        self._check(L.lib().mps_batch_add(self._h, c._h))
+        self.scenes.append(c)
+
+    def forward_time(self, dt: Optional[float] = None) -> np.ndarray:
+        """ForwardTime of every scene: dt the fixed step, None each scene's
+        CFL step; returns the steps taken."""
+        out = np.zeros(len(self.scenes))
+        self._check(L.lib().mps_batch_forward_time(self._h, dt if dt is not None else 0.0,
+                                                   out.ctypes.data))
+        return out
+
+    def solve_info(self) -> dict:
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(L.lib().mps_batch_solve_info(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return {"solve_ms": ms.value, "launches": n.value}
 
 
 def make_environment_from(env: dict) -> Environment:

@@ -9,17 +9,6 @@
 
 #include "mps_internal.hpp"
 
-struct mps_ctx {
-  mps::Context c;
-};
-struct mps_model {
-  int device = 0;
-  mps::Model m;
-};
-struct mps_dataset {
-  int device = 0;
-  mps::Dataset d;
-};
 
 namespace mps {
 
@@ -129,7 +118,9 @@ void free_all(Context& c) {
   dfree(c.x); dfree(c.z); dfree(c.u); dfree(c.w); dfree(c.p); dfree(c.nd); dfree(c.type);
   dfree(c.nws); dfree(c.ecs); dfree(c.ox); dfree(c.oz); dfree(c.dux); dfree(c.duz);
   dfree(c.wall_x); dfree(c.wall_z); dfree(c.hs_out); dfree(c.wall_group); dfree(c.wall_motion);
-  dfree(c.xz); dfree(c.uw);
+  dfree(c.xz); dfree(c.uw); dfree(c.orig);
+  for (auto& d : c.spare_d) dfree(d);
+  dfree(c.spare_type); dfree(c.spare_group); dfree(c.spare_orig); dfree(c.spare_key);
   dfree(c.key); dfree(c.sorted); dfree(c.pos_sorted); dfree(c.in_cells); dfree(c.nbr); dfree(c.ncount);
   dfree(c.cell_count); dfree(c.cell_start); dfree(c.cell_fill); dfree(c.scan_tmp); dfree(c.flags);
   dfree(c.nbr_total_dev); dfree(c.ppe_counters);
@@ -138,7 +129,11 @@ void free_all(Context& c) {
   dfree(c.r); dfree(c.pa); dfree(c.q); dfree(c.partA); dfree(c.partB); dfree(c.partC);
   dfree(c.red_out); dfree(c.st); dfree(c.st32); dfree(c.ir);
   dfree(c.pa32); dfree(c.d32); dfree(c.dinv32);
-  dfree(c.feats); dfree(c.ypred); dfree(c.feat_rec); dfree(c.learn_part);
+  if (!c.feats_borrowed) {
+    dfree(c.feats);
+    dfree(c.ypred);
+  }
+  dfree(c.feat_rec); dfree(c.learn_part);
   dfree(c.ghost32); dfree(c.qpos); dfree(c.slice_wide); dfree(c.col16);
   dfree(c.g32a); dfree(c.g32b); dfree(c.part3a); dfree(c.part3b);
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
@@ -189,6 +184,41 @@ double* aux_ptr(Context& c, int which, size_t& count) {
   }
 }
 
+// CSR rows / columns of a particles' system between caller order and storage
+// order: row k of `out` is row map_row[k] of `in`, columns mapped through
+// map_col, ascending in each row (the diagonal is one of the row's entries).
+void permute_csr(size_t n, const uint64_t* rp, const uint32_t* col, const double* val,
+                 const std::vector<uint32_t>& map_row, const std::vector<uint32_t>& map_col,
+                 std::vector<uint64_t>& orp, std::vector<uint32_t>& ocol, std::vector<double>& oval) {
+  orp.assign(n + 1, 0);
+  ocol.clear();
+  oval.clear();
+  std::vector<std::pair<uint32_t, double>> row;
+  for (size_t k = 0; k < n; k++) {
+    const uint32_t i = map_row[k];
+    row.clear();
+    for (uint64_t e = rp[i]; e < rp[i + 1]; e++) row.emplace_back(map_col[col[e]], val[e]);
+    std::sort(row.begin(), row.end(),
+              [](const std::pair<uint32_t, double>& a, const std::pair<uint32_t, double>& b) {
+                return a.first < b.first;
+              });
+    for (auto& e : row) {
+      ocol.push_back(e.first);
+      oval.push_back(e.second);
+    }
+    orp[k + 1] = ocol.size();
+  }
+}
+
+std::vector<uint32_t> inverse(const std::vector<uint32_t>& o) {
+  std::vector<uint32_t> inv(o.size());
+  for (size_t k = 0; k < o.size(); k++) inv[o[k]] = static_cast<uint32_t>(k);
+  return inv;
+}
+
+// Rows of the installed system are the particles (in storage order).
+bool ppe_over_particles(const Context& c) { return c.have_ppe && c.n > 0 && c.ppe_n == c.n; }
+
 void forward(Context& c, double dt) {
   // Computer::ForwardTime(dt), Computer.hpp:1700-1742
   c.dt = dt;
@@ -212,6 +242,50 @@ const char* kKernelNames[MPS_K_COUNT] = {
     "ml_setup", "ml_prec", "cg_spmv32", "cg_update32"};
 
 }  // namespace
+
+const std::vector<uint32_t>& storage_order(Context& c) {
+  if (c.order_host_serial != c.order_serial || c.orig_host.size() != c.n) {
+    c.orig_host.resize(c.n);
+    if (c.n) {
+      MPS_HIP(hipMemcpyAsync(c.orig_host.data(), c.orig, c.n * 4, hipMemcpyDeviceToHost, c.stream));
+      MPS_HIP(hipStreamSynchronize(c.stream));
+    }
+    c.order_host_serial = c.order_serial;
+  }
+  return c.orig_host;
+}
+
+void download_caller_order(Context& c, const void* dev, void* host, size_t elem) {
+  const size_t n = c.n;
+  if (n == 0) return;
+  if (c.order_serial == 0) {
+    MPS_HIP(hipMemcpyAsync(host, dev, n * elem, hipMemcpyDeviceToHost, c.stream));
+    MPS_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  const std::vector<uint32_t>& o = storage_order(c);
+  std::vector<unsigned char> tmp(n * elem);
+  MPS_HIP(hipMemcpyAsync(tmp.data(), dev, n * elem, hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  unsigned char* h = static_cast<unsigned char*>(host);
+  for (size_t k = 0; k < n; k++) std::memcpy(h + o[k] * elem, tmp.data() + k * elem, elem);
+}
+
+void upload_caller_order(Context& c, const void* host, void* dev, size_t elem) {
+  const size_t n = c.n;
+  if (n == 0) return;
+  if (c.order_serial == 0) {
+    MPS_HIP(hipMemcpyAsync(dev, host, n * elem, hipMemcpyHostToDevice, c.stream));
+    MPS_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  const std::vector<uint32_t>& o = storage_order(c);
+  std::vector<unsigned char> tmp(n * elem);
+  const unsigned char* h = static_cast<const unsigned char*>(host);
+  for (size_t k = 0; k < n; k++) std::memcpy(tmp.data() + k * elem, h + o[k] * elem, elem);
+  MPS_HIP(hipMemcpyAsync(dev, tmp.data(), n * elem, hipMemcpyHostToDevice, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+}
 
 void ensure_ppe_vectors(Context& c, size_t n) {
   if (n <= c.ppe_alloc && c.r) return;
@@ -412,6 +486,12 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
       mps::grow(c.wall_x, old, m, c.stream);
       mps::grow(c.wall_z, old, m, c.stream);
       if (c.wall_group) mps::grow(c.wall_group, old, m, c.stream);
+      mps::grow(c.orig, old, m, c.stream);
+      for (auto& d : c.spare_d) mps::grow(d, 0, m, c.stream);
+      mps::grow(c.spare_type, 0, m, c.stream);
+      if (c.wall_group) mps::grow(c.spare_group, 0, m, c.stream);
+      mps::grow(c.spare_orig, 0, m, c.stream);
+      mps::grow(c.spare_key, 0, m, c.stream);
       mps::grow(c.hs_out, 0, m, c.stream);
       mps::grow(c.key, 0, m, c.stream);
       mps::grow(c.sorted, 0, m, c.stream);
@@ -437,7 +517,13 @@ int mps_add_particles(mps_ctx* ctx, size_t count, const double* x, const double*
     MPS_HIP(hipMemcpyAsync(c.wall_z + old, z, count * 8, h2d, c.stream));
     // particles added while device kinematics are on stay put (group 0)
     if (c.wall_group) MPS_HIP(hipMemsetAsync(c.wall_group + old, 0, count, c.stream));
+    // appended particles are stored after the others, in the caller's order
+    std::vector<uint32_t> ids(count);
+    for (size_t k = 0; k < count; k++) ids[k] = static_cast<uint32_t>(old + k);
+    MPS_HIP(hipMemcpyAsync(c.orig + old, ids.data(), count * 4, h2d, c.stream));
     MPS_HIP(hipStreamSynchronize(c.stream));
+    if (c.order_host_serial == c.order_serial && c.orig_host.size() == old)
+      c.orig_host.insert(c.orig_host.end(), ids.begin(), ids.end());
     c.n = nn;
     c.have_ppe = false;
   });
@@ -448,17 +534,14 @@ size_t mps_size(const mps_ctx* ctx) { return ctx ? ctx->c.n : 0; }
 int mps_get_particles(mps_ctx* ctx, double* x, double* z, double* u, double* w, double* p, double* n,
                       uint8_t* type) {
   return mps::guard(ctx, [&](Context& c) {
-    const size_t N = c.n;
-    if (N == 0) return;
-    const hipMemcpyKind d2h = hipMemcpyDeviceToHost;
-    if (x) MPS_HIP(hipMemcpyAsync(x, c.x, N * 8, d2h, c.stream));
-    if (z) MPS_HIP(hipMemcpyAsync(z, c.z, N * 8, d2h, c.stream));
-    if (u) MPS_HIP(hipMemcpyAsync(u, c.u, N * 8, d2h, c.stream));
-    if (w) MPS_HIP(hipMemcpyAsync(w, c.w, N * 8, d2h, c.stream));
-    if (p) MPS_HIP(hipMemcpyAsync(p, c.p, N * 8, d2h, c.stream));
-    if (n) MPS_HIP(hipMemcpyAsync(n, c.nd, N * 8, d2h, c.stream));
-    if (type) MPS_HIP(hipMemcpyAsync(type, c.type, N, d2h, c.stream));
-    MPS_HIP(hipStreamSynchronize(c.stream));
+    if (c.n == 0) return;
+    if (x) mps::download_caller_order(c, c.x, x, 8);
+    if (z) mps::download_caller_order(c, c.z, z, 8);
+    if (u) mps::download_caller_order(c, c.u, u, 8);
+    if (w) mps::download_caller_order(c, c.w, w, 8);
+    if (p) mps::download_caller_order(c, c.p, p, 8);
+    if (n) mps::download_caller_order(c, c.nd, n, 8);
+    if (type) mps::download_caller_order(c, c.type, type, 1);
   });
 }
 
@@ -470,15 +553,13 @@ int mps_set_particles(mps_ctx* ctx, const double* x, const double* z, const doub
     if (type)
       for (size_t k = 0; k < N; k++)
         if (type[k] > MPS_DISABLED) throw Failure{MPS_ERR_BAD_ARG, "particle type out of range"};
-    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
-    if (x) MPS_HIP(hipMemcpyAsync(c.x, x, N * 8, h2d, c.stream));
-    if (z) MPS_HIP(hipMemcpyAsync(c.z, z, N * 8, h2d, c.stream));
-    if (u) MPS_HIP(hipMemcpyAsync(c.u, u, N * 8, h2d, c.stream));
-    if (w) MPS_HIP(hipMemcpyAsync(c.w, w, N * 8, h2d, c.stream));
-    if (p) MPS_HIP(hipMemcpyAsync(c.p, p, N * 8, h2d, c.stream));
-    if (n) MPS_HIP(hipMemcpyAsync(c.nd, n, N * 8, h2d, c.stream));
-    if (type) MPS_HIP(hipMemcpyAsync(c.type, type, N, h2d, c.stream));
-    MPS_HIP(hipStreamSynchronize(c.stream));
+    if (x) mps::upload_caller_order(c, x, c.x, 8);
+    if (z) mps::upload_caller_order(c, z, c.z, 8);
+    if (u) mps::upload_caller_order(c, u, c.u, 8);
+    if (w) mps::upload_caller_order(c, w, c.w, 8);
+    if (p) mps::upload_caller_order(c, p, c.p, 8);
+    if (n) mps::upload_caller_order(c, n, c.nd, 8);
+    if (type) mps::upload_caller_order(c, type, c.type, 1);
     c.ppe_from_particles = false;  // positions no longer those the system was assembled from
   });
 }
@@ -512,7 +593,8 @@ int mps_set_wall_motion(mps_ctx* ctx, size_t n_motions, const mps_wall_motion* m
     mps::dfree(c.wall_motion);
     c.wall_motion = mps::dalloc<mps_wall_motion>(n_motions);
     MPS_HIP(hipMemcpy(c.wall_motion, motions, n_motions * sizeof(mps_wall_motion), hipMemcpyHostToDevice));
-    if (c.n) MPS_HIP(hipMemcpy(c.wall_group, group, c.n, hipMemcpyHostToDevice));
+    if (!c.spare_group) c.spare_group = mps::dalloc<uint8_t>(std::max<size_t>(c.alloc_n, 1));
+    mps::upload_caller_order(c, group, c.wall_group, 1);
     c.wall_motions = n_motions;
     c.wall_fn = nullptr;
     c.wall_pre = nullptr;
@@ -648,16 +730,16 @@ int mps_density_variation(mps_ctx* ctx, double* out) {
     if (c.n == 0) return;
     if (!out) throw Failure{MPS_ERR_BAD_ARG, "null output"};
     mps::density_variation(c, c.hs_out);
-    MPS_HIP(hipMemcpyAsync(out, c.hs_out, c.n * 8, hipMemcpyDeviceToHost, c.stream));
     mps::sync(c);
+    mps::download_caller_order(c, c.hs_out, out, 8);
   });
 }
 
 int mps_get_neighbor_counts(mps_ctx* ctx, uint32_t* counts) {
   return mps::guard(ctx, [&](Context& c) {
     if (c.n == 0) return;
-    MPS_HIP(hipMemcpyAsync(counts, c.ncount, c.n * 4, hipMemcpyDeviceToHost, c.stream));
     mps::sync(c);
+    mps::download_caller_order(c, c.ncount, counts, 4);
   });
 }
 
@@ -671,11 +753,16 @@ int mps_get_neighbors(mps_ctx* ctx, uint32_t* ids) {
     MPS_HIP(hipMemcpyAsync(cnt.data(), c.ncount, N * 4, hipMemcpyDeviceToHost, c.stream));
     MPS_HIP(hipMemcpyAsync(all.data(), c.nbr, slots * 4, hipMemcpyDeviceToHost, c.stream));
     mps::sync(c);
+    // particles in the caller's order, neighbours as the caller's indices
+    const std::vector<uint32_t>& orig = mps::storage_order(c);
+    const std::vector<uint32_t> at = mps::inverse(orig);
     size_t o = 0;
-    for (size_t i = 0; i < N; i++)
+    for (size_t e = 0; e < N; e++) {
+      const size_t i = at[e];
       for (uint32_t k = 0; k < cnt[i]; k++)
         // slot k of particle i (the device layout, mps_device.hpp nbr_slot)
-        ids[o++] = all[(((i >> 6) * (c.kstride >> 2) + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3)];
+        ids[o++] = orig[all[(((i >> 6) * (c.kstride >> 2) + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3)]];
+    }
   });
 }
 
@@ -684,8 +771,11 @@ int mps_get_aux(mps_ctx* ctx, int which, double* out) {
     size_t count = 0;
     double* p = mps::aux_ptr(c, which, count);
     if (count == 0) return;
-    MPS_HIP(hipMemcpyAsync(out, p, count * 8, hipMemcpyDeviceToHost, c.stream));
     mps::sync(c);
+    if (count == c.n)  // per particle (the PPE vectors of a particles' system too)
+      mps::download_caller_order(c, p, out, 8);
+    else
+      MPS_HIP(hipMemcpy(out, p, count * 8, hipMemcpyDeviceToHost));
   });
 }
 
@@ -694,8 +784,11 @@ int mps_set_aux(mps_ctx* ctx, int which, const double* in) {
     size_t count = 0;
     double* p = mps::aux_ptr(c, which, count);
     if (count == 0) return;
-    MPS_HIP(hipMemcpyAsync(p, in, count * 8, hipMemcpyHostToDevice, c.stream));
     mps::sync(c);
+    if (count == c.n)
+      mps::upload_caller_order(c, in, p, 8);
+    else
+      MPS_HIP(hipMemcpy(p, in, count * 8, hipMemcpyHostToDevice));
   });
 }
 
@@ -714,6 +807,17 @@ int mps_get_ppe(mps_ctx* ctx, uint64_t* rowptr, uint32_t* col, double* val) {
     std::vector<uint32_t> cl;
     std::vector<double> vl;
     mps::ppe_download_csr(c, rp, cl, vl);
+    if (mps::ppe_over_particles(c) && c.order_serial != 0) {
+      // rows and columns back to the caller's particle order
+      const std::vector<uint32_t>& orig = mps::storage_order(c);
+      std::vector<uint64_t> rp2;
+      std::vector<uint32_t> cl2;
+      std::vector<double> vl2;
+      mps::permute_csr(c.n, rp.data(), cl.data(), vl.data(), mps::inverse(orig), orig, rp2, cl2, vl2);
+      rp.swap(rp2);
+      cl.swap(cl2);
+      vl.swap(vl2);
+    }
     std::memcpy(rowptr, rp.data(), rp.size() * 8);
     std::memcpy(col, cl.data(), cl.size() * 4);
     std::memcpy(val, vl.data(), vl.size() * 8);
@@ -725,8 +829,29 @@ int mps_set_ppe(mps_ctx* ctx, size_t n, const uint64_t* rowptr, const uint32_t* 
   return mps::guard(ctx, [&](Context& c) {
     if (!rowptr || !b || !x0 || (rowptr[n] > 0 && (!col || !val)))
       throw Failure{MPS_ERR_BAD_ARG, "null system array"};
+    if (n == c.n && n > 0 && c.order_serial != 0) {
+      // a system over the particles, given in the caller's order: stored in
+      // the particles' storage order
+      const std::vector<uint32_t>& orig = mps::storage_order(c);
+      std::vector<uint64_t> rp2;
+      std::vector<uint32_t> cl2;
+      std::vector<double> vl2;
+      mps::permute_csr(n, rowptr, col, val, orig, mps::inverse(orig), rp2, cl2, vl2);
+      std::vector<double> b2(n), x2(n);
+      for (size_t k = 0; k < n; k++) {
+        b2[k] = b[orig[k]];
+        x2[k] = x0[orig[k]];
+      }
+      mps::ppe_upload_csr(c, n, rp2.data(), cl2.data(), vl2.data(), b2.data(), x2.data());
+      return;
+    }
     mps::ppe_upload_csr(c, n, rowptr, col, val, b, x0);
   });
+}
+
+int mps_set_storage_order(mps_ctx* ctx, int order) {
+  if (order != MPS_ORDER_CELL && order != MPS_ORDER_KEEP) return MPS_ERR_BAD_ARG;
+  return mps::guard(ctx, [&](Context& c) { c.keep_order = (order == MPS_ORDER_KEEP); });
 }
 
 int mps_set_profiling(mps_ctx* ctx, int on) {
@@ -946,8 +1071,10 @@ int mps_get_features(mps_ctx* ctx, float* out) {
     mps::learned_features(c);
     MPS_HIP(hipMemcpyAsync(h.data(), c.feats, h.size() * 2, hipMemcpyDeviceToHost, c.stream));
     mps::sync(c);
+    const std::vector<uint32_t>& orig = mps::storage_order(c);  // rows in the caller's order
     for (size_t r = 0; r < n; r++)
-      for (int q = 0; q < mps::kFeat; q++) out[r * mps::kFeat + q] = static_cast<float>(h[r * mps::kFeatPad + q]);
+      for (int q = 0; q < mps::kFeat; q++)
+        out[static_cast<size_t>(orig[r]) * mps::kFeat + q] = static_cast<float>(h[r * mps::kFeatPad + q]);
   });
 }
 

@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstdio>
 #include <mutex>
+#include <string>
+#include <vector>
 #include <type_traits>
 
 #include "mps_device.hpp"
@@ -46,16 +48,50 @@ struct Sell {
   size_t rpb;  // rows per block (multiple of NT)
 };
 
+// Where a block sits in its scene's grid.  A single-scene launch is the whole
+// grid, its blocks taken in XCD-aware order; a grouped launch (Group below)
+// concatenates the grids of several scenes, and each block runs as block
+// `bid` of the `nb` blocks of its scene's grid.
+struct Frame {
+  unsigned bid, nb;
+  bool remap;
+};
+__device__ __forceinline__ Frame whole_grid() { return Frame{blockIdx.x, gridDim.x, true}; }
+
 struct Rows {
   size_t lo, hi;
 };
-__device__ __forceinline__ Rows block_rows(size_t n, size_t rpb) {
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ Rows block_rows(size_t n, size_t rpb, const Frame& f) {
+  const unsigned blk = f.remap ? xcd_block(f.bid, f.nb) : f.bid;
   const size_t lo = blk * rpb;
   const size_t hi = (lo + rpb < n) ? lo + rpb : n;
   return Rows{lo, hi};
 }
+__device__ __forceinline__ Rows block_rows(size_t n, size_t rpb) { return block_rows(n, rpb, whole_grid()); }
+__device__ __forceinline__ bool lead(const Frame& f) { return f.bid == 0 && threadIdx.x == 0; }
 __device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// A grouped launch: the concatenated grids of `count` scenes, scene s owning
+// blocks first[s] .. first[s + 1] - 1; scenes with active[s] == 0 (if active
+// is given) exit at once.  Returns false for such a block; otherwise sets the
+// block's scene and its frame in that scene's grid (block-uniform).
+struct Group {
+  const uint32_t* __restrict__ first;
+  const int* __restrict__ active;
+  int count;
+};
+__device__ __forceinline__ bool group_frame(const Group& g, int& s, Frame& f) {
+  const unsigned b = blockIdx.x;
+  int lo = 0, hi = g.count;  // first[lo] <= b < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.first[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  s = lo;
+  f = Frame{b - g.first[lo], g.first[lo + 1] - g.first[lo], false};
+  return g.active == nullptr || g.active[lo] != 0;
+}
 
 // Block-uniform read of the convergence flag (thread 0 reads, LDS broadcast),
 // so that a block either runs every barrier below or none.
@@ -289,12 +325,11 @@ __global__ void __launch_bounds__(256) k_rec_init(size_t n, const uint2* __restr
 
 // r = b - A x, partial sums of r.r and b.b (Computer.hpp:1382-1386).
 template <class M>
-__global__ void __launch_bounds__(NT) k_cg_init(M A, const double* __restrict__ b,
-                                                const double* __restrict__ x, double* __restrict__ r,
-                                                double* __restrict__ part_rr,
-                                                double* __restrict__ part_bb) {
+__device__ __forceinline__ void cg_init_body(const M& A, const double* __restrict__ b, const double* __restrict__ x,
+                                             double* __restrict__ r, double* __restrict__ part_rr,
+                                             double* __restrict__ part_bb, const Frame& f) {
   __shared__ double lds[NT / 64];
-  const Rows rw = block_rows(A.n, A.rpb);
+  const Rows rw = block_rows(A.n, A.rpb, f);
   double arr = 0, abb = 0;
   for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
     const double s = row_product(A, i, x[i], [&](uint32_t j) { return x[j]; });
@@ -306,9 +341,16 @@ __global__ void __launch_bounds__(NT) k_cg_init(M A, const double* __restrict__ 
   const double srr = block_sum<NT>(arr, lds);
   const double sbb = block_sum<NT>(abb, lds);
   if (threadIdx.x == 0) {
-    part_rr[blockIdx.x] = srr;
-    part_bb[blockIdx.x] = sbb;
+    part_rr[f.bid] = srr;
+    part_bb[f.bid] = sbb;
   }
+}
+template <class M>
+__global__ void __launch_bounds__(NT) k_cg_init(M A, const double* __restrict__ b,
+                                                const double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ part_rr,
+                                                double* __restrict__ part_bb) {
+  cg_init_body(A, b, x, r, part_rr, part_bb, whole_grid());
 }
 
 struct StopRule {
@@ -796,18 +838,49 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
   }
 }
 
+// The operands of k_ml_prec.
+struct PrecArgs {
+  CellGrid G;
+  const uint32_t* list;
+  uint32_t nact;
+  const uint32_t* pid;
+  const float* dvp;
+  const float4* sc;
+  const double* r64;
+  double inv_norm;
+  float *r, *d;
+  const float *q, *dinv;
+  float* z;
+  unsigned zs;
+  const double* part_pq;
+  int npq;
+  double* part;
+  size_t part_stride;
+  CgStatus* st;
+};
+
 // One wavefront per listed 4x4 block of cells.
-__global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __restrict__ list, uint32_t nact,
-                                                 const uint32_t* __restrict__ pid, const float* __restrict__ dvp,
-                                                 const float4* __restrict__ sc,
-                                                 const double* __restrict__ r64, double inv_norm,
-                                                 float* __restrict__ r, float* __restrict__ d,
-                                                 const float* __restrict__ q, const float* __restrict__ dinv,
-                                                 float* __restrict__ z, unsigned zs,
-                                                 const double* __restrict__ part_pq,
-                                                 int npq, double* __restrict__ part, size_t part_stride,
-                                                 CgStatus* st, long long it) {
+__device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, const Frame& f) {
   __shared__ double lds[NTP / 64];
+  const CellGrid& G = a.G;
+  const uint32_t* __restrict__ list = a.list;
+  const uint32_t nact = a.nact;
+  const uint32_t* __restrict__ pid = a.pid;
+  const float* __restrict__ dvp = a.dvp;
+  const float4* __restrict__ sc = a.sc;
+  const double* __restrict__ r64 = a.r64;
+  const double inv_norm = a.inv_norm;
+  float* __restrict__ r = a.r;
+  float* __restrict__ d = a.d;
+  const float* __restrict__ q = a.q;
+  const float* __restrict__ dinv = a.dinv;
+  float* __restrict__ z = a.z;
+  const unsigned zs = a.zs;
+  const double* __restrict__ part_pq = a.part_pq;
+  const int npq = a.npq;
+  double* __restrict__ part = a.part;
+  const size_t part_stride = a.part_stride;
+  CgStatus* st = a.st;
   if (stopped_before(st, it)) return;
   // z_i is written at z[zs * i] (the value field of the SpMV's records)
   // the p.q partials of k_ml_spmv(it) are loaded first and summed once the
@@ -818,7 +891,7 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
     const int k = threadIdx.x + u * NTP;
     pq_part[u] = (it > 0 && k < npq) ? part_pq[k] : 0.0;
   }
-  const uint32_t w = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
+  const uint32_t w = f.bid * (NTP / 64) + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   const bool active = w < nact;  // wave-uniform
   float4 scl = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -851,9 +924,9 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
     double s = 0;
 #pragma unroll
     for (int u = 0; u < kPqPerThread; u++) s += pq_part[u];
-    const double a = st->gamma[(it - 1) & 1] / block_sum<NTP>(s, lds);
-    if (lead()) st->alpha_ml = a;
-    alpha = static_cast<float>(a);
+    const double al = st->gamma[(it - 1) & 1] / block_sum<NTP>(s, lds);
+    if (lead(f)) st->alpha_ml = al;
+    alpha = static_cast<float>(al);
   }
   double rho = 0, rr = 0;
   if (active) {
@@ -918,10 +991,11 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(CellGrid G, const uint32_t* __r
   const double srho = block_sum<NTP>(rho, lds);
   const double srr = block_sum<NTP>(rr, lds);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = srho;
-    part[part_stride + blockIdx.x] = srr;
+    part[f.bid] = srho;
+    part[part_stride + f.bid] = srr;
   }
 }
+__global__ void __launch_bounds__(NTP) k_ml_prec(PrecArgs a, long long it) { ml_prec_body(a, it, whole_grid()); }
 
 // See above; FIRST is it == 1 (p = z, q = A z, tolerance from r_0.r_0).  The
 // partials of k_ml_prec(it - 1) are loaded at the start and summed after the
@@ -940,13 +1014,34 @@ __device__ __forceinline__ float spmv_row(const Op& A, const float* __restrict__
 constexpr int kSpmvParts = 3;  // prec partials per thread and array in registers
 constexpr int kSpmvRows = 2;   // rows per thread whose A z waits in registers for beta
 
+// The operands of k_ml_spmv.
+template <class Op>
+struct SpmvArgs {
+  Op A;
+  const float* z;
+  float *p, *q, *d;
+  const double* part_ml;
+  size_t part_stride;
+  int nparts;
+  double* part_pq;
+  CgStatus* st;
+  double tol_rel;
+};
+
 template <bool FIRST, class Op>
-__global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict__ z, float* __restrict__ p,
-                                                   float* __restrict__ q, float* __restrict__ d,
-                                                   const double* __restrict__ part_ml, size_t part_stride,
-                                                   int nparts, double* __restrict__ part_pq, CgStatus* st,
-                                                   double tol_rel, long long it) {
+__device__ __forceinline__ void ml_spmv_body(const SpmvArgs<Op>& args, long long it, const Frame& f) {
   __shared__ double lds[2 * (NT / 64)];
+  const Op& A = args.A;
+  const float* __restrict__ z = args.z;
+  float* __restrict__ p = args.p;
+  float* __restrict__ q = args.q;
+  float* __restrict__ d = args.d;
+  const double* __restrict__ part_ml = args.part_ml;
+  const size_t part_stride = args.part_stride;
+  const int nparts = args.nparts;
+  double* __restrict__ part_pq = args.part_pq;
+  CgStatus* st = args.st;
+  const double tol_rel = args.tol_rel;
   // a stop at it - 2 or earlier was found (and finished) by an earlier launch
   if (stopped_before(st, it - 1)) return;
   double pa[kSpmvParts], pb[kSpmvParts];
@@ -956,7 +1051,7 @@ __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict
     pa[u] = (k < nparts) ? part_ml[k] : 0.0;
     pb[u] = (k < nparts) ? part_ml[part_stride + k] : 0.0;
   }
-  const Rows rw = block_rows(A.n, A.rpb);
+  const Rows rw = block_rows(A.n, A.rpb, f);
   float zc[kSpmvRows], sc[kSpmvRows];
 #pragma unroll
   for (int u = 0; u < kSpmvRows; u++) {
@@ -978,7 +1073,7 @@ __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict
   block_sum2<NT>(a, b, lds, rho, rr);
   const double tol = FIRST ? rr * tol_rel * tol_rel : st->tol;
   const bool stop = FIRST ? (tol == 0) : (rr < tol);
-  if (lead()) {
+  if (lead(f)) {
     const int par = static_cast<int>((it - 1) & 1);
     st->gamma[par] = rho;
     st->rr[par] = rr;
@@ -1023,7 +1118,11 @@ __global__ void __launch_bounds__(NT, 8) k_ml_spmv(Op A, const float* __restrict
     finish(i, zi, si);
   }
   const double sp = block_sum<NT>(acc, lds);
-  if (threadIdx.x == 0) part_pq[blockIdx.x] = sp;
+  if (threadIdx.x == 0) part_pq[f.bid] = sp;
+}
+template <bool FIRST, class Op>
+__global__ void __launch_bounds__(NT, 8) k_ml_spmv(SpmvArgs<Op> a, long long it) {
+  ml_spmv_body<FIRST, Op>(a, it, whole_grid());
 }
 
 // d += alpha p after a sweep that ended without a stop (its last k_ml_prec
@@ -1532,12 +1631,10 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
   size_t pgrid = 1;
   bool packed = false;
   auto prec = [&](double norm, long long it) {
-    hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
-                       static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const uint32_t*>(c.ml_pid),
-                       static_cast<const float*>(c.ml_dv), static_cast<const float4*>(c.ml_sc),
-                       static_cast<const double*>(c.r), 1.0 / norm, c.r32, c.d32,
-                       static_cast<const float*>(c.q32), static_cast<const float*>(c.dinv32), zval, 4u,
-                       static_cast<const double*>(p_pq), L.grid, c.part_ml, stride, c.st32, it);
+    const PrecArgs a{G,      c.ml_list, nact,   c.ml_pid, c.ml_dv, c.ml_sc,   c.r,       1.0 / norm,
+                     c.r32,  c.d32,     c.q32,  c.dinv32, zval,    4u,        p_pq,      L.grid,
+                     c.part_ml, stride, c.st32};
+    hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
   };
   double b_spmv = b_base;
   c.replay_ready = MPS_K_CG_SPMV32;
@@ -1571,16 +1668,12 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
     long long launched = 0;
     const long long inner = run_inner(c, limit, [&](long long it) {
       tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
+        const SpmvArgs<MatFreeRec> a{A,      nullptr, c.pa32,  c.q32, c.d32, c.part_ml, stride,
+                                     static_cast<int>(pgrid), p_pq, c.st32, sweep_tol};
         if (it == 1)
-          hipLaunchKernelGGL((k_ml_spmv<true, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
-                             static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
-                             c.st32, sweep_tol, it);
+          hipLaunchKernelGGL((k_ml_spmv<true, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, a, it);
         else
-          hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, A,
-                             static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
-                             static_cast<const double*>(c.part_ml), stride, static_cast<int>(pgrid), p_pq,
-                             c.st32, sweep_tol, it);
+          hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, a, it);
       }, it);
       tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, it); }, it);
       launched = it;
@@ -1630,6 +1723,337 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
     solve_mixed_cg(c, A, cols + rows + n * vec, true, sr, max_iter);
 }
 
+
+// ---------------- batched solves: several scenes per launch ----------------
+//
+// The mixed multilevel solve of several scenes at once: the scenes' systems
+// stay in their own buffers; every kernel of the solve is one grouped launch
+// whose grid concatenates the grids the scenes' own launches would use
+// (Group), so each scene's blocks compute exactly what its own launch would,
+// with its own alpha, beta and stop test (the block partial sums are per
+// scene).  A scene that stops, or is not part of the running phase, has no
+// blocks in the launch.  The refinement logic of `refine` runs per scene on
+// the host between phases.
+
+struct ResidArgs {
+  Sell<double> A;
+  const double* b;
+  const double* x;
+  double* r;
+  double* part_rr;
+  double* part_bb;
+  double* out;  // |r|^2, |b|^2
+  int nparts;
+};
+
+struct VecArgs {
+  size_t n;
+  double* x;
+  float* d;
+  const float* p;
+  CgStatus* st;
+  double norm;
+};
+
+__global__ void __launch_bounds__(NT) kg_residual(const ResidArgs* __restrict__ tab, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const ResidArgs a = tab[s];
+  cg_init_body(a.A, a.b, a.x, a.r, a.part_rr, a.part_bb, f);
+}
+
+// One block per scene of the group.
+__global__ void __launch_bounds__(NT) kg_reduce2(const ResidArgs* __restrict__ tab, Group g) {
+  __shared__ double lds[NT / 64];
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const ResidArgs a = tab[s];
+  const double sa = reduce_partials<NT>(a.part_rr, a.nparts, lds);
+  const double sb = reduce_partials<NT>(a.part_bb, a.nparts, lds);
+  if (threadIdx.x == 0) {
+    a.out[0] = sa;
+    a.out[1] = sb;
+  }
+}
+
+__global__ void __launch_bounds__(NTP) kg_ml_prec(const PrecArgs* __restrict__ tab, Group g, long long it) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const PrecArgs a = tab[s];
+  ml_prec_body(a, it, f);
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(NT, 8) kg_ml_spmv(const SpmvArgs<MatFreeRec>* __restrict__ tab, Group g,
+                                                    long long it) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const SpmvArgs<MatFreeRec> a = tab[s];
+  ml_spmv_body<FIRST, MatFreeRec>(a, it, f);
+}
+
+// d += alpha p of scenes whose sweep ended without a stop (k_ml_finish).
+__global__ void __launch_bounds__(256) kg_ml_finish(const VecArgs* __restrict__ tab, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const VecArgs a = tab[s];
+  const size_t i = f.bid * 256ull + threadIdx.x;
+  if (i < a.n) a.d[i] = fmaf(static_cast<float>(a.st->alpha_ml), a.p[i], a.d[i]);
+}
+
+// x += |r| d (k_ir_correct).
+__global__ void __launch_bounds__(256) kg_ir_correct(const VecArgs* __restrict__ tab, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const VecArgs a = tab[s];
+  const size_t i = f.bid * 256ull + threadIdx.x;
+  if (i < a.n) a.x[i] = fma(static_cast<double>(a.d[i]), a.norm, a.x[i]);
+}
+
+// The inner status of each scene of the group cleared (one block per scene).
+__global__ void __launch_bounds__(64) kg_status_clear(const VecArgs* __restrict__ tab, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  if (threadIdx.x == 0) *tab[s].st = CgStatus{};
+}
+
+enum BatchKind { kResid = 0, kRed2, kPrec, kSpmv, kFinish, kCorrect, kClear, kKindCount };
+static_assert(kKindCount <= BatchWork::kKinds, "block offset regions");
+static_assert(sizeof(ResidArgs) <= BatchWork::kArgBytes && sizeof(PrecArgs) <= BatchWork::kArgBytes &&
+                  sizeof(SpmvArgs<MatFreeRec>) <= BatchWork::kArgBytes && sizeof(VecArgs) <= BatchWork::kArgBytes,
+              "argument tables");
+
+struct SceneRun {
+  Context* c;
+  Launch L;
+  size_t pgrid = 1, vgrid = 1;
+  StopRule sr;
+  double inner_tol = 1e-4;
+  long long limit = 0, total_cap = 0, max_iter = 0;
+  long long total = 0;
+  int outer = 0;
+  double rr0 = 0, bb = 0, rr = 0, rr_prev = 0, target = 0;
+  bool refining = true, done = false, fallback = false;
+  // the running sweep
+  bool sweeping = false, swept = false;
+  double norm = 0, sweep_tol = 0;
+  long long inner = 0;
+  double rr_last = -1;
+  int flat = 0;
+};
+
+// Grouped launches of one batched solve: argument tables (4 regions: residual,
+// preconditioner, SpMV, vector updates) and block offsets per kind.
+struct Batch {
+  BatchWork& w;
+  std::vector<SceneRun>& sc;
+  size_t S;
+  template <class T>
+  T* tab_host(int region) { return reinterpret_cast<T*>(w.tab_host + region * w.cap * BatchWork::kArgBytes); }
+  template <class T>
+  const T* tab_dev(int region) const {
+    return reinterpret_cast<const T*>(w.tab + region * w.cap * BatchWork::kArgBytes);
+  }
+  void upload_tab(int region, size_t elem) {
+    const size_t off = region * w.cap * BatchWork::kArgBytes;
+    MPS_HIP(hipMemcpyAsync(w.tab + off, w.tab_host + off, S * elem, hipMemcpyHostToDevice, w.stream));
+  }
+  // offsets of `kind` over the scenes `in(s)`, each with blocks(s) blocks;
+  // returns the launch's total block count
+  template <class In, class Blocks>
+  unsigned group(int kind, In in, Blocks blocks) {
+    uint32_t* f = w.first_host + kind * (w.cap + 1);
+    f[0] = 0;
+    for (size_t s = 0; s < S; s++) f[s + 1] = f[s] + (in(s) ? static_cast<uint32_t>(blocks(s)) : 0u);
+    MPS_HIP(hipMemcpyAsync(w.first + kind * (w.cap + 1), f, (S + 1) * 4, hipMemcpyHostToDevice, w.stream));
+    return f[S];
+  }
+  Group g(int kind) const { return Group{w.first + kind * (w.cap + 1), nullptr, static_cast<int>(S)}; }
+};
+
+void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
+  const size_t S = sc.size();
+  Batch B{w, sc, S};
+  // static operands
+  ResidArgs* rt = B.tab_host<ResidArgs>(0);
+  PrecArgs* pt = B.tab_host<PrecArgs>(1);
+  SpmvArgs<MatFreeRec>* st = B.tab_host<SpmvArgs<MatFreeRec>>(2);
+  VecArgs* vt = B.tab_host<VecArgs>(3);
+  double* red_dev = reinterpret_cast<double*>(w.tab + 4 * w.cap * BatchWork::kArgBytes);
+  double* red_host = reinterpret_cast<double*>(w.tab_host + 4 * w.cap * BatchWork::kArgBytes);
+  for (size_t s = 0; s < S; s++) {
+    Context& c = *sc[s].c;
+    const Launch L = sc[s].L;
+    const MlGeom g = ml_geom(c);
+    rt[s] = ResidArgs{sell<double>(c, c.sval, c.diag, L.rpb), c.b, c.sol, c.r, c.partB, c.partC, red_dev + 2 * s,
+                      L.grid};
+    float* const zval = reinterpret_cast<float*>(c.zrec) + 2;
+    pt[s] = PrecArgs{ml_grid(c, g), c.ml_list, static_cast<uint32_t>(*c.ml_active_host), c.ml_pid, c.ml_dv,
+                     c.ml_sc, c.r, 1.0, c.r32, c.d32, c.q32, c.dinv32, zval, 4u, c.part3b, L.grid, c.part_ml,
+                     c.part_ml_cap, w.st + s};
+    st[s] = SpmvArgs<MatFreeRec>{matfree_rec(c, L.rpb), nullptr, c.pa32, c.q32, c.d32, c.part_ml, c.part_ml_cap,
+                                 static_cast<int>(sc[s].pgrid), c.part3b, w.st + s, 1e-4};
+    vt[s] = VecArgs{c.ppe_n, c.sol, c.d32, c.pa32, w.st + s, 1.0};
+  }
+  B.upload_tab(0, sizeof(ResidArgs));
+  auto launch_check = [&] {
+    check_launch();
+    w.launches++;
+  };
+  const int max_outer = 64;
+  for (;;) {
+    // FP64 residual r = b - A x of every scene still refining
+    const unsigned nr = B.group(kResid, [&](size_t s) { return sc[s].refining; },
+                                [&](size_t s) { return sc[s].L.grid; });
+    if (nr == 0) break;
+    const unsigned n2 = B.group(kRed2, [&](size_t s) { return sc[s].refining; }, [](size_t) { return 1; });
+    hipLaunchKernelGGL(kg_residual, dim3(nr), dim3(NT), 0, w.stream, B.tab_dev<ResidArgs>(0), B.g(kResid));
+    launch_check();
+    hipLaunchKernelGGL(kg_reduce2, dim3(n2), dim3(NT), 0, w.stream, B.tab_dev<ResidArgs>(0), B.g(kRed2));
+    launch_check();
+    MPS_HIP(hipMemcpyAsync(red_host, red_dev, 2 * S * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    MPS_HIP(hipStreamSynchronize(w.stream));
+    // the refinement decisions of `refine`, per scene
+    bool any = false;
+    for (size_t s = 0; s < S; s++) {
+      SceneRun& r = sc[s];
+      r.swept = false;
+      if (!r.refining) continue;
+      r.rr = red_host[2 * s];
+      r.bb = red_host[2 * s + 1];
+      if (r.outer == 0) r.rr0 = r.rr;
+      const bool stalled = r.outer > 0 && !(r.rr < 0.5 * r.rr_prev);
+      r.rr_prev = r.rr;
+      if (r.sr.rule == MPS_STOP_RHS_NORM) {
+        r.target = (r.sr.tol * r.sr.tol) * r.bb;
+        r.done = r.rr <= r.target;
+      } else {
+        r.target = r.rr0 * r.sr.tol * r.sr.tol;
+        r.done = (r.outer == 0) ? (r.target == 0) : (r.rr < r.target);
+      }
+      if (stalled && !r.done && r.rr == r.rr && r.total < r.total_cap) {
+        r.fallback = true;
+        r.refining = false;
+        continue;
+      }
+      if (r.done || stalled || r.outer >= max_outer || r.total >= r.total_cap || !(r.rr == r.rr)) {
+        r.refining = false;
+        continue;
+      }
+      r.norm = std::sqrt(r.rr);
+      r.sweep_tol = std::max(r.inner_tol, 0.5 * std::sqrt(r.target / r.rr));
+      r.sweeping = r.swept = true;
+      r.inner = 0;
+      r.rr_last = -1;
+      r.flat = 0;
+      pt[s].inv_norm = 1.0 / r.norm;
+      st[s].tol_rel = r.sweep_tol;
+      vt[s].norm = r.norm;
+      r.c->sweep_index = r.outer;
+      any = true;
+    }
+    if (!any) break;
+    // one FP32 sweep of every refining scene
+    B.upload_tab(1, sizeof(PrecArgs));
+    B.upload_tab(2, sizeof(SpmvArgs<MatFreeRec>));
+    B.upload_tab(3, sizeof(VecArgs));
+    auto sweeping = [&](size_t s) { return sc[s].sweeping; };
+    const unsigned ncl = B.group(kClear, sweeping, [](size_t) { return 1; });
+    hipLaunchKernelGGL(kg_status_clear, dim3(ncl), dim3(64), 0, w.stream, B.tab_dev<VecArgs>(3), B.g(kClear));
+    launch_check();
+    unsigned np = B.group(kPrec, sweeping, [&](size_t s) { return sc[s].pgrid; });
+    unsigned ns = B.group(kSpmv, sweeping, [&](size_t s) { return sc[s].L.grid; });
+    hipLaunchKernelGGL(kg_ml_prec, dim3(np), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), B.g(kPrec), 0LL);
+    launch_check();
+    long long it = 1, pred = 0, lim = 0;
+    for (size_t s = 0; s < S; s++)
+      if (sc[s].sweeping) {
+        Context& c = *sc[s].c;
+        pred = std::max(pred, c.sweep_pred[std::min(c.sweep_index, Context::kSweepPred - 1)]);
+        lim = std::max(lim, sc[s].limit);
+      }
+    while (true) {
+      const long long chunk = next_chunk(it - 1, pred);
+      for (long long m = 0; m < chunk && it <= lim; m++, it++) {
+        if (it == 1)
+          hipLaunchKernelGGL((kg_ml_spmv<true>), dim3(ns), dim3(NT), 0, w.stream,
+                             B.tab_dev<SpmvArgs<MatFreeRec>>(2), B.g(kSpmv), it);
+        else
+          hipLaunchKernelGGL((kg_ml_spmv<false>), dim3(ns), dim3(NT), 0, w.stream,
+                             B.tab_dev<SpmvArgs<MatFreeRec>>(2), B.g(kSpmv), it);
+        launch_check();
+        hipLaunchKernelGGL(kg_ml_prec, dim3(np), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), B.g(kPrec), it);
+        launch_check();
+      }
+      MPS_HIP(hipMemcpyAsync(w.st_host, w.st, S * sizeof(CgStatus), hipMemcpyDeviceToHost, w.stream));
+      MPS_HIP(hipStreamSynchronize(w.stream));
+      // per scene: stopped (the SpMV after the stop added the last alpha p to
+      // d), out of iterations or stagnating (d still needs it: k_ml_finish)
+      bool changed = false, finish = false;
+      std::vector<char> need_finish(S, 0);
+      for (size_t s = 0; s < S; s++) {
+        SceneRun& r = sc[s];
+        if (!r.sweeping) continue;
+        const CgStatus& h = w.st_host[s];
+        bool end = false;
+        if (h.converged) {
+          r.inner = h.iterations;
+          end = true;
+        } else if (it > r.limit) {
+          r.inner = r.limit;
+          end = need_finish[s] = 1;
+        } else {
+          const double q = std::min(h.rr[0], h.rr[1]);
+          r.flat = (r.rr_last > 0 && !(q < 0.81 * r.rr_last)) ? r.flat + 1 : 0;
+          r.rr_last = q;
+          if (r.flat >= 2) {
+            r.inner = it - 1;
+            end = need_finish[s] = 1;
+          }
+        }
+        if (end) {
+          r.sweeping = false;
+          changed = true;
+          finish = finish || need_finish[s];
+        }
+      }
+      if (finish) {
+        const unsigned nf = B.group(kFinish, [&](size_t s) { return need_finish[s] != 0; },
+                                    [&](size_t s) { return sc[s].vgrid; });
+        hipLaunchKernelGGL(kg_ml_finish, dim3(nf), dim3(256), 0, w.stream, B.tab_dev<VecArgs>(3), B.g(kFinish));
+        launch_check();
+      }
+      if (changed) {
+        np = B.group(kPrec, sweeping, [&](size_t s) { return sc[s].pgrid; });
+        ns = B.group(kSpmv, sweeping, [&](size_t s) { return sc[s].L.grid; });
+        if (np == 0) break;
+        lim = 0;
+        for (size_t s = 0; s < S; s++)
+          if (sc[s].sweeping) lim = std::max(lim, sc[s].limit);
+      }
+    }
+    // x += |r| d for the scenes of this sweep
+    const unsigned nc = B.group(kCorrect, [&](size_t s) { return sc[s].swept; }, [&](size_t s) { return sc[s].vgrid; });
+    hipLaunchKernelGGL(kg_ir_correct, dim3(nc), dim3(256), 0, w.stream, B.tab_dev<VecArgs>(3), B.g(kCorrect));
+    launch_check();
+    for (size_t s = 0; s < S; s++) {
+      SceneRun& r = sc[s];
+      if (!r.swept) continue;
+      Context& c = *r.c;
+      c.sweep_pred[std::min(r.outer, Context::kSweepPred - 1)] = r.inner;
+      r.total += r.inner;
+      r.outer++;
+    }
+  }
+}
+
 }  // namespace
 
 // Re-launches the last solve's SpMV on its operands with a status block that
@@ -1649,9 +2073,9 @@ double replay_spmv(Context& c, int kid, int reps) {
   auto launch = [&] {
     if (kid == MPS_K_CG_SPMV32)
       hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream,
-                         matfree_rec(c, L.rpb), static_cast<const float*>(nullptr), c.pa32, c.q32, c.d32,
-                         static_cast<const double*>(c.part_ml), c.part_ml_cap, static_cast<int>(pgrid),
-                         c.part3b, c.st_replay, 1e-8, 2LL);
+                         SpmvArgs<MatFreeRec>{matfree_rec(c, L.rpb), nullptr, c.pa32, c.q32, c.d32, c.part_ml,
+                                              c.part_ml_cap, static_cast<int>(pgrid), c.part3b, c.st_replay, 1e-8},
+                         2LL);
     else
       hipLaunchKernelGGL((k_cg_spmv<false, Sell<double>>), dim3(L.grid), dim3(NT), 0, c.stream,
                          sell<double>(c, c.sval, c.diag, L.rpb), static_cast<const double*>(c.r), c.pa, c.q,
@@ -1782,6 +2206,169 @@ void ml_setup_ahead(Context& c) {
 
 // Joins set-up work left on the auxiliary stream (a solve that ended early).
 void join_aux(Context& c) { join_ml_setup(c); }
+
+void BatchWork::ensure(size_t scenes) {
+  if (scenes <= cap && tab) return;
+  const int dev = device;
+  hipStream_t s = stream;
+  hipEvent_t a = ev_a, b = ev_b;
+  stream = nullptr;
+  ev_a = ev_b = nullptr;
+  release();
+  device = dev;
+  stream = s;
+  ev_a = a;
+  ev_b = b;
+  cap = std::max<size_t>(scenes, 8);
+  tab = dalloc<unsigned char>(kKinds * cap * kArgBytes);
+  first = dalloc<uint32_t>(kKinds * (cap + 1));
+  st = dalloc<CgStatus>(cap);
+  void* h = nullptr;
+  MPS_HIP(hipHostMalloc(&h, kKinds * cap * kArgBytes, hipHostMallocDefault));
+  tab_host = static_cast<unsigned char*>(h);
+  MPS_HIP(hipHostMalloc(&h, kKinds * (cap + 1) * sizeof(uint32_t), hipHostMallocDefault));
+  first_host = static_cast<uint32_t*>(h);
+  MPS_HIP(hipHostMalloc(&h, cap * sizeof(CgStatus), hipHostMallocDefault));
+  st_host = static_cast<CgStatus*>(h);
+  std::memset(tab_host, 0, kKinds * cap * kArgBytes);
+  if (!stream) MPS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (!ev_a) MPS_HIP(hipEventCreate(&ev_a));
+  if (!ev_b) MPS_HIP(hipEventCreate(&ev_b));
+}
+
+void BatchWork::release() {
+  dfree(tab);
+  dfree(first);
+  dfree(st);
+  if (tab_host) (void)hipHostFree(tab_host);
+  if (first_host) (void)hipHostFree(first_host);
+  if (st_host) (void)hipHostFree(st_host);
+  tab_host = nullptr;
+  first_host = nullptr;
+  st_host = nullptr;
+  if (ev_a) (void)hipEventDestroy(ev_a);
+  if (ev_b) (void)hipEventDestroy(ev_b);
+  if (stream) (void)hipStreamDestroy(stream);
+  ev_a = ev_b = nullptr;
+  stream = nullptr;
+  cap = 0;
+}
+
+bool batch_eligible(const Context& c) {
+  const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
+  return c.n > 0 && c.have_ppe && c.ppe_from_particles && cells_match && c.solver.kind == MPS_SOLVER_MIXED_IR &&
+         c.solver.precond == MPS_PRECOND_MULTILEVEL;
+}
+
+void ml_prepare(Context& c) {
+  const size_t n = c.ppe_n;
+  ensure_zrec(c, n);
+  if (!join_ml_setup(c)) ml_setup(c);
+  MPS_HIP(hipStreamSynchronize(c.stream));  // the count of occupied cell blocks
+  const MlGeom g = ml_geom(c);
+  const CellGrid G = ml_grid(c, g);
+  const uint32_t nact = static_cast<uint32_t>(*c.ml_active_host);
+  const size_t pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
+  const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
+  if (c.ml_pack_cap < slots) {
+    dfree(c.ml_pid);
+    dfree(c.ml_dv);
+    dfree(c.ml_sc);
+    c.ml_pid = dalloc<uint32_t>(slots * kPerLane);
+    c.ml_dv = dalloc<float>(slots * kPerLane);
+    c.ml_sc = dalloc<float4>(slots);
+    c.ml_pack_cap = slots;
+  }
+  hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+                     static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
+                     static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc);
+  hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
+                     c.zrec);
+  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+  check_launch();
+}
+
+void batch_solve(std::vector<Context*>& scenes, BatchWork& w) {
+  std::vector<SceneRun> sc;
+  sc.reserve(scenes.size());
+  for (Context* c : scenes) {
+    SceneRun r;
+    r.c = c;
+    const size_t n = c->ppe_n;
+    r.L = cg_grid(n);
+    const uint32_t nact = static_cast<uint32_t>(*c->ml_active_host);
+    r.pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
+    r.vgrid = grid_for(n);
+    r.sr = StopRule{c->solver.stop_rule, (c->solver.tol > 0) ? c->solver.tol : c->eps};
+    r.max_iter = (c->solver.max_iter > 0) ? c->solver.max_iter : static_cast<long long>(n);
+    r.limit = inner_limit(*c, r.max_iter);
+    r.total_cap = c->solver.max_iter > 0 ? c->solver.max_iter : LLONG_MAX;
+    r.inner_tol = c->solver.inner_tol > 0 ? c->solver.inner_tol : 1e-4;
+    const long long learned = c->stats.learned_used;
+    const double prev_rel = c->stats.prev_rel_residual;
+    c->stats = mps_solve_stats{};
+    c->stats.learned_used = learned;
+    c->stats.prev_rel_residual = prev_rel;
+    c->replay_ready = MPS_K_CG_SPMV32;
+    c->final_rr = -1;
+    sc.push_back(r);
+  }
+  if (sc.empty()) return;
+  w.ensure(sc.size());
+  w.launches = 0;
+  MPS_HIP(hipEventRecord(w.ev_a, w.stream));
+  batch_solve_run(sc, w);
+  MPS_HIP(hipEventRecord(w.ev_b, w.stream));
+  MPS_HIP(hipEventSynchronize(w.ev_b));
+  float ms = 0;
+  MPS_HIP(hipEventElapsedTime(&ms, w.ev_a, w.ev_b));
+  w.solve_ms = ms;
+  int code = MPS_OK;
+  std::string msg;
+  for (SceneRun& r : sc) {
+    Context& c = *r.c;
+    try {
+      if (r.fallback) {
+        // the FP32 sweeps stalled: the FP64 CG finishes from the current x,
+        // to the same target, on the scene's own stream
+        StopRule rest = r.sr;
+        if (r.sr.rule == MPS_STOP_INITIAL_RESIDUAL) rest.tol = r.sr.tol * std::sqrt(r.rr0 / r.rr);
+        const long long cap = c.solver.max_iter > 0 ? c.solver.max_iter - r.total : max_iter_fallback(c);
+        solve_cg64(c, rest, cap);
+        c.stats.iterations += r.total;
+        c.stats.outer_iterations = r.outer + 1;
+        c.stats.rr0 = r.rr0;
+        c.stats.fallback = 1;
+        double rr = 0, bb = 0;
+        residual64(c, rr, bb, false);
+        c.final_rr = rr;
+        c.final_bb = bb;
+      } else {
+        c.stats.iterations = r.total;
+        c.stats.outer_iterations = r.outer;
+        c.stats.rr0 = r.rr0;
+        c.stats.rr = r.rr;
+        c.last_iters = r.total;
+        c.final_rr = r.rr;
+        c.final_bb = r.bb;
+        if (!r.done)
+          throw Failure{MPS_ERR_CG_NOT_CONVERGED,
+                        "Conjugate Gradient method couldn't solve Pressure Poison Equation"};
+      }
+    } catch (Failure& f) {
+      if (code == MPS_OK) {
+        code = f.code;
+        msg = f.msg;
+      }
+    }
+    c.stats.solve_ms = w.solve_ms;
+    const double bb = c.final_bb;
+    c.stats.rel_residual = (bb > 0) ? std::sqrt(c.final_rr / bb) : std::sqrt(c.final_rr);
+    c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
+  }
+  if (code != MPS_OK) throw Failure{code, msg};
+}
 
 }  // namespace mps
 

@@ -136,6 +136,22 @@ struct Context {
   int* flags = nullptr;           // [0] cell overflow, [1] neighbour overflow
   int* flags_host = nullptr;      // pinned mirror
 
+  // storage order: the particles are kept in cell order, permuted at every
+  // neighbour search (cells in key order, ascending caller index inside a
+  // cell, particles outside the grid last); orig[k] is the caller's index of
+  // the particle stored at slot k.  Every per-particle array on the device is
+  // in storage order; the API maps at the boundary (storage_order).
+  uint32_t* orig = nullptr;
+  long long order_serial = 0;       // permutations applied (0: storage order = caller order)
+  bool keep_order = false;          // MPS_ORDER_KEEP: the search does not permute
+  std::vector<uint32_t> orig_host;  // host copy of orig for order_host_serial
+  long long order_host_serial = -1;
+  // arrays the permutation writes into, then swapped with the live ones
+  static constexpr int kMoved = 14;  // FP64 per-particle arrays the permutation moves
+  double* spare_d[kMoved] = {};
+  uint8_t *spare_type = nullptr, *spare_group = nullptr;
+  uint32_t *spare_orig = nullptr, *spare_key = nullptr;
+
   // neighbour lists: slot k of particle i at nbr_slot(i, k, kstride) (groups
   // of four slots per particle, a slice's 64 particles side by side)
   uint32_t *nbr = nullptr, *ncount = nullptr;
@@ -200,6 +216,9 @@ struct Context {
   // the learned warm start's residual norms are on their way to red_host
   // (learned_resolve reads them at the solver's first synchronisation)
   bool learned_pending = false;
+  // the feature rows and predictions live in a batch's buffers (mps_batch):
+  // not owned, not reallocated here
+  bool feats_borrowed = false;
   // work enqueued ahead on the auxiliary stream: the multilevel set-up, which
   // runs beside the learned warm start (fork after the assembly, join before
   // the solve)
@@ -271,6 +290,51 @@ struct Context {
 
   std::string err;
 };
+
+// ---- several scenes stepped together (mps_batch) -------------------------------
+// The pressure solves of a batch run as grouped launches: one launch per
+// kernel covers every scene still iterating, each block working in its own
+// scene's grid (mps_cg.hip: batch_solve).  Buffers for the argument tables,
+// block offsets, per-scene masks and statuses, grown to the scene count.
+struct BatchWork {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t cap = 0;                       // scenes the buffers hold
+  static constexpr int kKinds = 8;      // regions: argument tables, block offsets of each grouped launch
+  static constexpr size_t kArgBytes = 512;  // bytes reserved per scene and table
+  unsigned char* tab = nullptr;         // device: kKinds x cap x kArgBytes
+  unsigned char* tab_host = nullptr;    // pinned staging
+  uint32_t* first = nullptr;            // device: kKinds x (cap + 1) block offsets
+  uint32_t* first_host = nullptr;       // pinned
+  CgStatus* st = nullptr;               // device: inner-solve status per scene
+  CgStatus* st_host = nullptr;          // pinned
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double solve_ms = 0;                  // last batched solve, HIP events
+  long long launches = 0;               // grouped launches of the last solve
+  void ensure(size_t scenes);
+  void release();
+};
+// The multilevel mixed solver's per-scene set-up ahead of a batched solve, on
+// the scene's stream (synchronises it once for the block count).
+void ml_prepare(Context& c);
+// The scene's solve can join a batched solve (mixed precision, multilevel
+// preconditioner, a system assembled from the current particles and cells).
+bool batch_eligible(const Context& c);
+// Solves the assembled systems of the eligible scenes together on w.stream
+// (which must already wait for each scene's assembly and ml_prepare); each
+// scene's statistics, solution (c.sol) and final residual are set as its own
+// solve would.  Scenes whose FP32 sweeps stall finish with the FP64 CG alone.
+void batch_solve(std::vector<Context*>& scenes, BatchWork& w);
+
+// mps_mlp.hip.  The learned warm start of several scenes whose feature rows
+// (learned_features, run on each scene beforehand) are consecutive in
+// `feats`: one inference over all rows into `ypred`, then per scene
+// x0 = p + s_d y, its FP64 residual and the keep-or-revert decision, all on
+// stream s; the residual norms go to each scene's red_host (read them with
+// batch_learned_outcome once s has been synchronised).
+void batch_learned_start(std::vector<Context*>& scenes, Model& m, hipStream_t s, const __bf16* feats, float* ypred,
+                         size_t rows);
+void batch_learned_outcome(Context& c);
 
 // ---- error plumbing ----------------------------------------------------------
 struct Failure {
@@ -349,6 +413,15 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
   return static_cast<unsigned>((n + block - 1) / block);
 }
 
+// ---- storage order <-> caller order (mps_api.cpp) ----------------------------------
+// The caller's index of every stored particle (host copy of orig, refreshed
+// after a permutation; the stream is synchronised when it is refreshed).
+const std::vector<uint32_t>& storage_order(Context& c);
+// host[orig[k]] = dev[k] for the c.n particles (synchronous)
+void download_caller_order(Context& c, const void* dev, void* host, size_t elem);
+// dev[k] = host[orig[k]] for the c.n particles (synchronous)
+void upload_caller_order(Context& c, const void* host, void* dev, size_t elem);
+
 // ---- stage entry points (defined in the .hip units) ------------------------------
 void search(Context& c);                 // mps_search.hip
 void density(Context& c, bool ecs, bool savex, bool dens);  // mps_particles.hip
@@ -404,3 +477,16 @@ void device_math_check(size_t n, const double* a, const double* b, double* s, do
 double device_copy_probe(size_t bytes);
 
 }  // namespace mps
+
+// The opaque handles of the C ABI.
+struct mps_ctx {
+  mps::Context c;
+};
+struct mps_model {
+  int device = 0;
+  mps::Model m;
+};
+struct mps_dataset {
+  int device = 0;
+  mps::Dataset d;
+};

@@ -272,7 +272,7 @@ __device__ __forceinline__ uint32_t feat_col(const FeatArgs& a, size_t i, bool w
 // and not dummy, picked them), read as coalesced SELL slots; their
 // positions come from the records.  A row of at most kNbr columns keeps
 // them all as (float(r^2), index) pairs in registers, sorted by a bitonic
-// network (distance, then ascending neighbour index).  A longer row goes
+// network (distance, then ascending storage index of the neighbour).  A longer row goes
 // through an insertion sort in LDS (keys and column positions Q, 8 or 16
 // bits: 160-192 bytes per thread; stable in column order) that keeps the
 // kNbr nearest.
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a) {
   int m = 0;
   bool sorted = false;
   // the kept neighbours as (float bits of r^2) << 32 | index: ascending
-  // order is distance order with equal distances by index (= column order);
+  // order is distance order with equal distances by storage index (= column order);
   // unused entries are all ones and sort last
   uint64_t kv[kNbr];
 #pragma unroll
@@ -442,14 +442,21 @@ __global__ void __launch_bounds__(256) k_apply_pred(size_t n, const uint8_t* __r
   if (t != kDummy && t != kDisabled) sol[i] = p[i] + scales[9] * static_cast<double>(y[i]);
 }
 
-// sol = p_prev on alive rows unless the prediction lowered the FP64 residual
-// (red[0] = |b - A sol|^2 < red[6] = |b - A p_prev|^2).
+// The prediction is kept when its FP64 residual red[0] = |b - A sol|^2 is
+// below kKeep times red[6] = |b - A p_prev|^2, which the feature pass computed
+// with the FP32 operator (FP32 rounding moves it by far less than the margin).
+constexpr double kKeep = 0.98;
+__host__ __device__ __forceinline__ bool keep_prediction(double rr_pred, double rr_prev) {
+  return rr_pred < kKeep * rr_prev;
+}
+
+// sol = p_prev on alive rows unless the prediction is kept (keep_prediction).
 __global__ void __launch_bounds__(256) k_revert_unless_better(size_t n, const uint8_t* __restrict__ type,
                                                               const double* __restrict__ p,
                                                               const double* __restrict__ red,
                                                               double* __restrict__ sol) {
   const size_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n || red[0] < red[6]) return;
+  if (i >= n || keep_prediction(red[0], red[6])) return;
   const uint8_t t = type[i];
   if (t != kDummy && t != kDisabled) sol[i] = p[i];
 }
@@ -899,7 +906,13 @@ void Dataset::release() {
 void learned_features(Context& c) {
   const size_t n = c.n;
   if (n == 0) return;
-  if (!c.feats || c.feats_cap < n) {
+  if (c.feats_borrowed) {  // rows and predictions in a batch's buffers: the records only
+    if (!c.feat_rec || c.feats_cap < n) {
+      dfree(c.feat_rec);
+      c.feat_rec = dalloc<FeatRec>(n);
+      c.feats_cap = n;
+    }
+  } else if (!c.feats || c.feats_cap < n) {
     dfree(c.feats);
     dfree(c.ypred);
     dfree(c.feat_rec);
@@ -955,12 +968,44 @@ void learned_warm_start(Context& c) {
   }
 }
 
+void batch_learned_start(std::vector<Context*>& scenes, Model& m, hipStream_t s, const __bf16* feats, float* ypred,
+                         size_t rows) {
+  m.forward(s, rows, feats, ypred, false);
+  for (Context* cp : scenes) {
+    Context& c = *cp;
+    const size_t n = c.n;
+    hipStream_t own = c.stream;
+    c.stream = s;  // the scene's kernels below run on the batch's stream
+    try {
+      hipLaunchKernelGGL(k_apply_pred, dim3(blocks(n)), dim3(256), 0, c.stream, n,
+                         static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.p),
+                         static_cast<const float*>(c.ypred), static_cast<const double*>(c.red_out), c.sol);
+      residual64_launch(c);
+      hipLaunchKernelGGL(k_revert_unless_better, dim3(blocks(n)), dim3(256), 0, c.stream, n,
+                         static_cast<const uint8_t*>(c.type), static_cast<const double*>(c.p),
+                         static_cast<const double*>(c.red_out), c.sol);
+      check_launch();
+      MPS_HIP(hipMemcpyAsync(c.red_host, c.red_out, kRedOut * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    } catch (...) {
+      c.stream = own;
+      throw;
+    }
+    c.stream = own;
+  }
+}
+
+void batch_learned_outcome(Context& c) {
+  const double rr_new = c.red_host[0], bbv = c.red_host[1], rr_prev = c.red_host[6];
+  c.stats.prev_rel_residual = bbv > 0 ? std::sqrt(rr_prev / bbv) : 0;
+  c.stats.learned_used = keep_prediction(rr_new, rr_prev) ? 1 : 0;
+}
+
 bool learned_resolve(Context& c, double& rr, double& bb) {
   c.learned_pending = false;
   MPS_HIP(hipStreamSynchronize(c.stream));
   const double rr_new = c.red_host[0], bbv = c.red_host[1], rr_prev = c.red_host[6];
   c.stats.prev_rel_residual = bbv > 0 ? std::sqrt(rr_prev / bbv) : 0;
-  if (!(rr_new < rr_prev)) return false;  // x went back to p_prev: r is not its residual
+  if (!keep_prediction(rr_new, rr_prev)) return false;  // x went back to p_prev: r is not its residual
   c.stats.learned_used = 1;
   rr = rr_new;
   bb = bbv;

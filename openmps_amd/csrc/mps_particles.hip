@@ -9,6 +9,9 @@
 // The neighbour loop fuses what the reference does in separate passes where
 // the inputs allow it: density + HS + ECS (first density pass), density +
 // SaveX (second pass), and the SPP centroid of every stage.
+#include <algorithm>
+#include <utility>
+
 #include "mps_device.hpp"
 #include "mps_internal.hpp"
 
@@ -587,10 +590,16 @@ void wall_targets_from_callback(Context& c) {
   MPS_HIP(hipStreamSynchronize(c.stream));
   if (c.wall_pre) c.wall_pre(c.wall_user, c.t, c.dt);
   if (!c.wall_fn) return;
-  std::vector<uint32_t> idx;
-  for (size_t i = 0; i < n; i++)
-    if (t[i] != kFluid) idx.push_back(static_cast<uint32_t>(i));
-  if (idx.empty()) return;
+  // the callback sees the caller's indices, ascending (the reference's loop
+  // order); the targets go to the storage slots of those particles
+  const std::vector<uint32_t>& orig = storage_order(c);
+  std::vector<std::pair<uint32_t, uint32_t>> walls;  // (caller index, storage slot)
+  for (size_t k = 0; k < n; k++)
+    if (t[k] != kFluid) walls.emplace_back(orig[k], static_cast<uint32_t>(k));
+  if (walls.empty()) return;
+  std::sort(walls.begin(), walls.end());
+  std::vector<uint32_t> idx(walls.size());
+  for (size_t m = 0; m < walls.size(); m++) idx[m] = walls[m].first;
   std::vector<double> hx(idx.size()), hz(idx.size());
   c.wall_fn(c.wall_user, c.t, c.dt, idx.size(), idx.data(), hx.data(), hz.data());
   // scatter through full host mirrors of the target arrays
@@ -598,9 +607,9 @@ void wall_targets_from_callback(Context& c) {
   MPS_HIP(hipMemcpyAsync(wx.data(), c.wall_x, n * 8, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(wz.data(), c.wall_z, n * 8, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));
-  for (size_t m = 0; m < idx.size(); m++) {
-    wx[idx[m]] = hx[m];
-    wz[idx[m]] = hz[m];
+  for (size_t m = 0; m < walls.size(); m++) {
+    wx[walls[m].second] = hx[m];
+    wz[walls[m].second] = hz[m];
   }
   MPS_HIP(hipMemcpyAsync(c.wall_x, wx.data(), n * 8, hipMemcpyHostToDevice, c.stream));
   MPS_HIP(hipMemcpyAsync(c.wall_z, wz.data(), n * 8, hipMemcpyHostToDevice, c.stream));

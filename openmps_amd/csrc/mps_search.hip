@@ -17,6 +17,8 @@
 //                      reference order (each a contiguous range of ids and
 //                      positions) and writes its list into the sliced
 //                      neighbour layout, four slots (16 bytes) per store.
+#include <utility>
+
 #include "mps_device.hpp"
 #include "mps_internal.hpp"
 
@@ -163,32 +165,45 @@ __global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* __restrict__
   }
 }
 
+// Counting-sort scatter of the particles in cells; particles outside every
+// cell (disabled) go after them, at in_cells + (running count in flags[2]),
+// in no particular order: they take part in nothing.
 __global__ void __launch_bounds__(kBlock) k_cell_scatter(size_t n, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ cell_start,
                                                          uint32_t* cell_fill,
-                                                         uint32_t* __restrict__ sorted) {
+                                                         uint32_t* __restrict__ sorted,
+                                                         const uint64_t* __restrict__ in_cells, int* flags) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   const uint32_t k = i < n ? key[i] : kNoKey;  // lanes past n take part in the run count
   const uint32_t slot = run_atomic_add(cell_fill, k, k != kNoKey);
-  if (k != kNoKey) sorted[cell_start[k] + slot] = static_cast<uint32_t>(i);
+  if (k != kNoKey)
+    sorted[cell_start[k] + slot] = static_cast<uint32_t>(i);
+  else if (i < n)
+    sorted[in_cells[0] + static_cast<uint32_t>(atomicAdd(&flags[2], 1))] = static_cast<uint32_t>(i);
 }
 
-// Restores ascending particle index inside each cell (the reference inserts
-// serially in index order, Computer.hpp:704-717).  Overflowed cells are left
-// alone: the search reports an error for them.
+// Orders each cell's entries by the caller's index orig[] (the reference
+// inserts serially in index order, Computer.hpp:704-717; the stored order
+// is a permutation of it).  Overflowed cells are left alone: the search
+// reports an error for them.
 __global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t cap,
                                                       const uint32_t* __restrict__ cell_start,
                                                       const uint32_t* __restrict__ cell_count,
+                                                      const uint32_t* __restrict__ orig,
                                                       uint32_t* __restrict__ sorted) {
   const size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (c >= ncells) return;
   const uint32_t cnt = cell_count[c];
   if (cnt < 2 || cnt > cap) return;
   uint32_t* s = sorted + cell_start[c];
-  if (cnt <= 32) {  // one load and one store per entry: a bitonic network in registers
-    uint32_t v[32];
+  if (cnt <= 32) {  // a bitonic network in registers over (caller index, slot) pairs
+    uint64_t v[32];
 #pragma unroll
-    for (int e = 0; e < 32; e++) v[e] = (static_cast<uint32_t>(e) < cnt) ? s[e] : ~0u;  // padding sorts last
+    for (int e = 0; e < 32; e++) {
+      const uint32_t j = (static_cast<uint32_t>(e) < cnt) ? s[e] : 0u;
+      v[e] = (static_cast<uint32_t>(e) < cnt) ? ((static_cast<uint64_t>(orig[j]) << 32) | j)
+                                              : ~0ull;  // padding sorts last
+    }
 #pragma unroll
     for (int k = 2; k <= 32; k <<= 1)
 #pragma unroll
@@ -197,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t ca
         for (int e = 0; e < 32; e++) {
           const int l = e ^ d;
           if (l > e) {
-            const uint32_t lo = min(v[e], v[l]), hi = max(v[e], v[l]);
+            const uint64_t lo = v[e] < v[l] ? v[e] : v[l], hi = v[e] < v[l] ? v[l] : v[e];
             const bool up = (e & k) == 0;
             v[e] = up ? lo : hi;
             v[l] = up ? hi : lo;
@@ -205,18 +220,54 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(size_t ncells, uint32_t ca
         }
 #pragma unroll
     for (int e = 0; e < 32; e++)
-      if (static_cast<uint32_t>(e) < cnt) s[e] = v[e];
+      if (static_cast<uint32_t>(e) < cnt) s[e] = static_cast<uint32_t>(v[e]);
     return;
   }
   for (uint32_t a = 1; a < cnt; a++) {
-    const uint32_t v = s[a];
+    const uint32_t v = s[a], ov = orig[v];
     uint32_t b = a;
-    while (b > 0 && s[b - 1] > v) {
+    while (b > 0 && orig[s[b - 1]] > ov) {
       s[b] = s[b - 1];
       b--;
     }
     s[b] = v;
   }
+}
+
+// The particle state in cell order: slot k takes the particle at old slot
+// sorted[k] (then sorted[k] = k).  Every per-particle array moves, also the
+// stage results a later stage skips for some particle types and so keeps
+// (saved X, ECS, nWithoutSpp, du of dummy particles: the reference never
+// overwrites them).  Particles outside the grid (the tail) get empty
+// neighbour lists.
+struct Permute {
+  size_t n;
+  uint32_t* sorted;
+  const uint64_t* in_cells;
+  const double* src[Context::kMoved];
+  double* dst[Context::kMoved];
+  const uint8_t* type_s;
+  uint8_t* type_d;
+  const uint8_t* grp_s;  // null: no device wall groups
+  uint8_t* grp_d;
+  const uint32_t* orig_s;
+  uint32_t* orig_d;
+  const uint32_t* key_s;
+  uint32_t* key_d;
+  uint32_t* ncount;
+};
+__global__ void __launch_bounds__(kBlock) k_permute(Permute a) {
+  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (k >= a.n) return;
+  const uint32_t j = a.sorted[k];
+#pragma unroll
+  for (int m = 0; m < Context::kMoved; m++) a.dst[m][k] = a.src[m][j];
+  a.type_d[k] = a.type_s[j];
+  if (a.grp_s) a.grp_d[k] = a.grp_s[j];
+  a.orig_d[k] = a.orig_s[j];
+  a.key_d[k] = a.key_s[j];
+  a.sorted[k] = static_cast<uint32_t>(k);
+  if (k >= a.in_cells[0]) a.ncount[k] = 0;
 }
 
 // The positions in cell order (slots below the number of particles in
@@ -335,6 +386,43 @@ void scan_slices(Context& c, const uint32_t* widths, size_t nslices, uint64_t* o
   scan_u32<uint64_t>(c, widths, nslices, offsets, 64, offsets + nslices);
 }
 
+// Moves the particle state into the cell order the sort left in c.sorted.
+static void permute_to_cell_order(Context& c) {
+  const size_t n = c.n;
+  double** live[Context::kMoved] = {&c.x,   &c.z,   &c.u,      &c.w,      &c.p,  &c.nd, &c.wall_x,
+                                    &c.wall_z, &c.nws, &c.ecs, &c.ox, &c.oz, &c.dux, &c.duz};
+  Permute a{};
+  a.n = n;
+  a.sorted = c.sorted;
+  a.in_cells = c.in_cells;
+  for (int m = 0; m < Context::kMoved; m++) {
+    a.src[m] = *live[m];
+    a.dst[m] = c.spare_d[m];
+  }
+  a.type_s = c.type;
+  a.type_d = c.spare_type;
+  if (c.wall_group) {
+    a.grp_s = c.wall_group;
+    a.grp_d = c.spare_group;
+  }
+  a.orig_s = c.orig;
+  a.orig_d = c.spare_orig;
+  a.key_s = c.key;
+  a.key_d = c.spare_key;
+  a.ncount = c.ncount;
+  // each moved array read (gathered) and written once, the sorted ids read and written
+  const double bytes = 2.0 * n * (8 * Context::kMoved + 1 + 4 + 4 + (c.wall_group ? 1 : 0)) + 8.0 * n;
+  tracked(c, MPS_K_CELL_SORT, bytes, [&] {
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, a);
+  });
+  for (int m = 0; m < Context::kMoved; m++) std::swap(*live[m], c.spare_d[m]);
+  std::swap(c.type, c.spare_type);
+  if (c.wall_group) std::swap(c.wall_group, c.spare_group);
+  std::swap(c.orig, c.spare_orig);
+  std::swap(c.key, c.spare_key);
+  c.order_serial++;
+}
+
 void search(Context& c) {
   const size_t n = c.n;
   c.search_serial++;
@@ -359,13 +447,16 @@ void search(Context& c) {
   tracked(c, MPS_K_CELL_SCATTER, 16.0 * n, [&] {
     hipLaunchKernelGGL(k_cell_scatter, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
                        static_cast<const uint32_t*>(c.key), static_cast<const uint32_t*>(c.cell_start),
-                       c.cell_fill, c.sorted);
+                       c.cell_fill, c.sorted, static_cast<const uint64_t*>(c.in_cells), c.flags);
   });
-  tracked(c, MPS_K_CELL_SORT, 8.0 * n + 8.0 * c.ncells, [&] {
+  // sorted ids read and written, their caller indices gathered
+  tracked(c, MPS_K_CELL_SORT, 12.0 * n + 8.0 * c.ncells, [&] {
     hipLaunchKernelGGL(k_cell_sort, dim3(grid_for(c.ncells)), dim3(kBlock), 0, c.stream, c.ncells,
                        c.cap, static_cast<const uint32_t*>(c.cell_start),
-                       static_cast<const uint32_t*>(c.cell_count), c.sorted);
+                       static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.orig),
+                       c.sorted);
   });
+  if (!c.keep_order) permute_to_cell_order(c);
   // the sorted ids and x, z read, the positions in cell order written
   tracked(c, MPS_K_CELL_SORT, 36.0 * n, [&] {
     hipLaunchKernelGGL(k_cell_pos, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,

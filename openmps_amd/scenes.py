@@ -165,12 +165,15 @@ def lattice(nx: int, nz: int, l0: float, u=None, w=None, p=None) -> dict:
 
 
 def batch(count: int = 64, seed: int = 5) -> list[dict]:
-    """Uneven independent dam-break scenes (a batch of simulations): fluid
-    particle counts drawn from [10k, 200k], l0 from [0.5, 2] mm and the
-    column's height / width from [0.3, 2.0] (numpy PCG64, `seed`); tank 4
-    column widths wide, walls 1.2 column heights high, jitter +/-0.02 l0,
-    r_e = 2.1 l0, r_lap = 3.1 l0, steps up to 1e-4 s (CFL rule).  Returns
-    config dictionaries like config()."""
+    """Uneven independent scenes (a batch of simulations), dam breaks and
+    static-pressure columns alternately: fluid particle counts drawn from
+    [10k, 200k], l0 from [0.5, 2] mm and the column's height / width from
+    [0.3, 2.0] (numpy PCG64, `seed`).  A dam break has its column in a tank 4
+    column widths wide with walls 1.2 column heights high, jittered +/-0.02
+    l0; a static-pressure scene is the reference's layout (static_pressure:
+    the column at rest between walls as high as the column).  r_e = 2.1 l0,
+    r_lap = 3.1 l0, steps up to 1e-4 s (CFL rule).  Returns config
+    dictionaries like config(), with "kind" set."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for k in range(count):
@@ -180,11 +183,16 @@ def batch(count: int = 64, seed: int = 5) -> list[dict]:
         nx = max(4, int(round(np.sqrt(nfluid / aspect))))
         nz = max(4, int(round(nfluid / nx)))
         col_w, col_h = nx * l0, nz * l0
-        parts, dom = dam_break(4 * col_w, 1.2 * col_h, col_w, col_h, l0, 0.02, seed * 1000 + k)
+        if k % 2 == 0:
+            kind = "dambreak"
+            parts, dom = dam_break(4 * col_w, 1.2 * col_h, col_w, col_h, l0, 0.02, seed * 1000 + k)
+        else:
+            kind = "static"
+            parts, dom = static_pressure(nx, nz, l0)
         env = dict(max_dt=1e-4, courant=COURANT, g=G, rho=RHO, nu=NU, r_e_by_l0=2.1, l0=l0,
                    r_lap_by_l0=3.1, **dom)
         out.append({"name": f"batch{seed}_{k}", "particles": parts, "env": env, "dt": None, "eps": 1e-8,
-                    "l0": l0, "aspect": aspect})
+                    "l0": l0, "aspect": aspect, "kind": kind, "depth": col_h})
     return out
 
 

@@ -32,16 +32,18 @@ _WARM = {"previous": L.WARM_PREVIOUS, "zero": L.WARM_ZERO, "learned": L.WARM_LEA
 _PREC = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL}
 
 
-def configure(computer, preset: str, tol: float, weights: str | None = None) -> None:
+def configure(computer, preset: str, tol: float, weights: str | None = None, model=None) -> None:
     """Sets the solver of `computer` (stop rule |b - A x| <= tol |b|); a
-    learned preset also attaches the model stored at `weights`."""
+    learned preset also attaches `model`, or else the model stored at
+    `weights` (scenes of one batch share one model)."""
     mixed, warm, prec = PRESETS[preset]
     computer.set_solver(kind=L.SOLVER_MIXED_IR if mixed else L.SOLVER_CG_FP64,
                         stop_rule=L.STOP_RHS_NORM, warm_start=_WARM[warm], precond=_PREC[prec],
                         tol=tol, inner_tol=1e-4)
     if warm == "learned":
         from . import learned
-        learned.attach_model(computer, learned.Model.load(weights or learned.DEFAULT_WEIGHTS))
+        learned.attach_model(computer, model if model is not None
+                             else learned.Model.load(weights or learned.DEFAULT_WEIGHTS))
 
 
 def is_mixed(preset: str) -> bool:

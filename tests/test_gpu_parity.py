@@ -65,11 +65,13 @@ def compare_stage(g, o, st):
             same_aux(g, o, ["du_x", "du_z"], st)
 
 
-def run_parity(name, steps, particles=None, env_over=None):
+def run_parity(name, steps, particles=None, env_over=None, keep_order=False):
     cfg = scenes.config(name)
     env = dict(cfg["env"], **(env_over or {}))
     parts = cfg["particles"] if particles is None else particles
     g = cases.make("gpu", env, cfg["eps"])
+    if keep_order:
+        g.set_storage_order(keep=True)
     o = cases.make("oracle", env, cfg["eps"])
     g.add_particles(parts)
     o.add_particles(parts)
@@ -114,9 +116,12 @@ def test_device_sqrt_div_correctly_rounded():
 
 
 @pytest.mark.parametrize("name", ["dambreak_1k", "dambreak_1k_ref"])
-def test_stage_parity_dambreak(name):
-    say(f"parity {name}")
-    run_parity(name, steps=6)
+@pytest.mark.parametrize("storage", ["cell", "keep"])
+def test_stage_parity_dambreak(name, storage):
+    """Both storage orders: particles permuted into cell order at every search
+    (the default) and kept in the caller's order."""
+    say(f"parity {name}, storage order {storage}")
+    run_parity(name, steps=6, keep_order=(storage == "keep"))
 
 
 def test_stage_parity_jittered():

@@ -44,9 +44,14 @@ def reordered(parts, order, seed=3):
 
 
 def assembled_system(order, precond, rule=L.STOP_RHS_NORM, name="dambreak_125k"):
+    """The system after one step.  "natural": the default storage (cell order
+    from the first search on); "half" / "shuffled": the permuted caller order
+    is kept in storage, so that far columns (u32 slices) occur."""
     cfg = scenes.config(name)
     parts = reordered(cfg["particles"], order)
     g = cases.make("gpu", cfg["env"], cfg["eps"])
+    if order != "natural":
+        g.set_storage_order(keep=True)
     g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=rule, precond=precond, tol=TOL)
     g.add_particles(parts)
     dt = cfg["dt"]
