@@ -334,9 +334,9 @@ def main() -> int:
     ap.add_argument("--workload", default="dambreak_1m")
     from openmps_amd import solvers
     ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
-                    help="headline solver (default: mixed_ml, the FP32 multilevel-preconditioned "
-                         "CG inside FP64 refinement; learned_ml, the same with the learned warm start, "
-                         "for batch64)")
+                    help="headline solver (default: mixed_amg, FP32 CG with the cell-grid multigrid "
+                         "preconditioner inside FP64 refinement; learned_ml, the multilevel one with the "
+                         "learned warm start, for batch64)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
     ap.add_argument("--ablation", action="store_true", help="also time further solver presets")
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -344,6 +344,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps from the developed state")
     ap.add_argument("--threads", type=int, default=8, help="host threads of the batch64 workload")
+    ap.add_argument("--save-state", default=None, help="write the developed state (.npz) after the advance")
+    ap.add_argument("--from-state", default=None,
+                    help="start from a state written by --save-state (no early window, no advance)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -352,7 +355,7 @@ def main() -> int:
         return 2
     if args.workload == "batch64":
         return run_batch(args)
-    args.solver = args.solver or "mixed_ml"
+    args.solver = args.solver or "mixed_amg"
     from openmps_amd import scenes
     from openmps_amd import learned as _learned
     have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)
@@ -364,28 +367,38 @@ def main() -> int:
     log(f"  {N} particles ({nfluid} fluid)")
     W, K = args.warmup, args.steps
 
-    # 1. the early window: the headline solver's steps W+1 .. W+K from rest
-    main_c = make_computer(cfg, args.solver, args.tol, args.weights)
-    main_c._bench_dt = cfg["dt"]
     early = {}
-    run(main_c, W, f"{args.solver} early warmup")
-    el, st = run(main_c, K, f"{args.solver} early timed")
-    early[args.solver] = summary(st, el)
-    # 2. the developed flow: untimed steps up to `advance`, then the state the
-    # other solvers start from
-    rest = max(0, args.advance - W - K)
-    run(main_c, rest, f"{args.solver} advance", every=50)
-    state = main_c.particles()
-    t_state, dt_state = main_c.time()
-    step0 = W + K + rest
-    log(f"  developed state after {step0} steps, t = {t_state:.4f} s")
+    if args.from_state:
+        # a developed state saved by an earlier run (--save-state)
+        z = np.load(args.from_state)
+        state = {k: z[k] for k in ("x", "z", "u", "w", "p", "n", "type")}
+        t_state, dt_state, step0 = float(z["t"]), float(z["dt"]), int(z["step"])
+        main_c = from_state(cfg, args.solver, args.tol, state, t_state, dt_state, args.weights)
+        log(f"  state after {step0} steps from {args.from_state}, t = {t_state:.4f} s")
+    else:
+        # 1. the early window: the headline solver's steps W+1 .. W+K from rest
+        main_c = make_computer(cfg, args.solver, args.tol, args.weights)
+        main_c._bench_dt = cfg["dt"]
+        run(main_c, W, f"{args.solver} early warmup")
+        el, st = run(main_c, K, f"{args.solver} early timed")
+        early[args.solver] = summary(st, el)
+        # 2. the developed flow: untimed steps up to `advance`, then the state the
+        # other solvers start from
+        rest = max(0, args.advance - W - K)
+        run(main_c, rest, f"{args.solver} advance", every=50)
+        state = main_c.particles()
+        t_state, dt_state = main_c.time()
+        step0 = W + K + rest
+        log(f"  developed state after {step0} steps, t = {t_state:.4f} s")
+        if args.save_state:
+            np.savez(args.save_state, t=t_state, dt=dt_state, step=step0, **state)
     # 3. headline window from the developed state: W warm-up + K timed steps
     el_w, st_w = run(main_c, 1, f"{args.solver} warmup")
     gpu_mixed_1 = capture(main_c) if not args.no_cpu_baseline else None
     run(main_c, W - 1, f"{args.solver} warmup")
     elapsed, st = run(main_c, K, f"{args.solver} timed")
     kt = profiled_table(main_c, K, args.solver)
-    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32"}[
+    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32", "amg": "cg_spmv32"}[
         solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
     replay = {}
     if dom in ("cg_spmv32", "cg_spmv"):
@@ -398,13 +411,19 @@ def main() -> int:
     kt64 = kt if args.solver == "fp64" else {}
     gpu64 = {}
     others = []
+    prec = solvers.preconditioner(args.solver)
+    learned_v = {"amg": "learned_amg", "multilevel": "learned_ml", "jacobi": "learned_mixed"}.get(prec)
+    zero_v = {"amg": "mixed_amg_zero", "multilevel": "mixed_ml_zero"}.get(prec)
     if not args.no_compare:
         if args.solver != "fp64":
             others.append("fp64")
-        if have_weights and not solvers.is_learned(args.solver):
-            others.append("learned_ml")  # config 4: the learned guess ...
-        if args.solver != "mixed_ml_zero":
-            others.append("mixed_ml_zero")  # ... against a zero guess (and the previous-step one: mixed_ml)
+        if have_weights and learned_v and not solvers.is_learned(args.solver):
+            others.append(learned_v)  # the learned warm start ...
+        if zero_v and args.solver != zero_v:
+            others.append(zero_v)  # ... against a zero guess (and the previous-step one: the headline)
+        for v in ("mixed_ml", "mixed_amg"):  # the other preconditioner
+            if v != args.solver and v not in others:
+                others.append(v)
     if args.ablation:
         others += [v for v in ("fp64_zero", "mixed_jacobi", "mixed_ml", "learned_mixed")
                    if v != args.solver and v not in others and (have_weights or "learned" not in v)]
@@ -439,7 +458,7 @@ def main() -> int:
         shuffled["particle_steps_per_s"] = N * K / el_s
         del c
     # the early window of the reference algorithm, for the speedup at rest
-    if not args.no_compare and args.solver != "fp64":
+    if not args.no_compare and args.solver != "fp64" and not args.from_state:
         c = make_computer(cfg, "fp64", args.tol)
         c._bench_dt = cfg["dt"]
         run(c, W, "fp64 early warmup")
@@ -502,14 +521,19 @@ def main() -> int:
         "fp64_particle_steps_per_s": (N / (solve["fp64"]["step_ms"] * 1e-3) if "fp64" in solve else None),
         "solve": solve,
         "solve_speedup_vs_fp64": sp("fp64", args.solver),
-        "solve_speedup_learned_ml_vs_fp64": sp("fp64", "learned_ml"),
-        "learned_ml_vs_mixed_ml_solve_ratio": (sp("mixed_ml", "learned_ml") if args.solver == "mixed_ml" else None),
+        "learned_solver": learned_v,
+        "solve_speedup_learned_vs_fp64": sp("fp64", learned_v) if learned_v else None,
+        # > 1: the learned start is faster than the previous-step guess of the same solver
+        "learned_vs_previous_guess_solve_ratio": (sp(args.solver, learned_v)
+                                                  if learned_v and not solvers.is_learned(args.solver) else None),
+        "solve_speedup_vs_mixed_ml": sp("mixed_ml", args.solver) if args.solver != "mixed_ml" else None,
         "shuffled_order": shuffled,
         "early_window": {"steps": f"{W + 1}..{W + K} from rest", "solve": early,
                          "solve_speedup_vs_fp64": (early["fp64"]["solve_ms_per_step"] /
                                                    early[args.solver]["solve_ms_per_step"]
                                                    if "fp64" in early and args.solver in early else None),
-                         "particle_steps_per_s": N / (early[args.solver]["step_ms"] * 1e-3)},
+                         "particle_steps_per_s": N / (early[args.solver]["step_ms"] * 1e-3)}
+                        if args.solver in early else None,
         "roofline": roofline,
         "roofline_fp64_spmv": roofline_fp64,
         "kernels": {k: {"ms": round(v["ms_per_launch"], 4), "GBps": round(v["GBps"], 1),

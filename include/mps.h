@@ -148,7 +148,12 @@ enum mps_precond {
   /* Jacobi plus additive piecewise-constant corrections on the neighbour
    * search cells and on 2x2 and 4x4 blocks of cells; needs a system assembled
    * from the current particle state and cells, and uses Jacobi otherwise */
-  MPS_PRECOND_MULTILEVEL = 2
+  MPS_PRECOND_MULTILEVEL = 2,
+  /* Jacobi plus a multigrid correction on the cells: one V-cycle on the
+   * Galerkin operator of the cells (a 9-point stencil on the cell grid) and
+   * of 2x2 blocks of cells below it, added to the damped Jacobi step; same
+   * requirements as MPS_PRECOND_MULTILEVEL, and uses Jacobi otherwise */
+  MPS_PRECOND_AMG = 3
 };
 typedef struct mps_solver_opts {
   int kind;          /* mps_solver_kind */
@@ -302,7 +307,7 @@ enum mps_kernel_id {
   MPS_K_PPE_FILL, MPS_K_CG_INIT, MPS_K_CG_SPMV, MPS_K_CG_UPDATE, MPS_K_CG_ITER32,
   MPS_K_CG_INIT32, MPS_K_IR_RESIDUAL, MPS_K_GRADIENT, MPS_K_DS, MPS_K_OTHER,
   MPS_K_FEATURES, MPS_K_MLP, MPS_K_ML_SETUP, MPS_K_ML_PREC, MPS_K_CG_SPMV32, MPS_K_CG_UPDATE32,
-  MPS_K_COUNT
+  MPS_K_AMG_CYCLE, MPS_K_COUNT
 };
 typedef struct mps_kernel_stat {
   const char* name;

@@ -32,13 +32,13 @@ WARM_PREVIOUS, WARM_ZERO, WARM_LEARNED = 0, 1, 2
 AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_z": 5,
        "ppe_b": 6, "ppe_x": 7}
 
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL, PRECOND_AMG = 0, 1, 2, 3
 ORDER_CELL, ORDER_KEEP = 0, 1
 
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
            "explicit", "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv",
            "cg_update", "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other",
-           "features", "mlp", "ml_setup", "ml_prec", "cg_spmv32", "cg_update32")
+           "features", "mlp", "ml_setup", "ml_prec", "cg_spmv32", "cg_update32", "amg_cycle")
 
 
 class MpsEnv(ctypes.Structure):

@@ -139,6 +139,8 @@ void free_all(Context& c) {
   dfree(c.r32); dfree(c.z32); dfree(c.q32); dfree(c.agg_inv); dfree(c.part_ml);
   dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active); dfree(c.st_replay);
   dfree(c.ml_pid); dfree(c.ml_dv); dfree(c.ml_sc); dfree(c.zrec);
+  dfree(c.amg_S64); dfree(c.amg_S); dfree(c.amg_dinv); dfree(c.amg_r); dfree(c.amg_x); dfree(c.amg_e);
+  dfree(c.amg_inv); dfree(c.amg_rows); dfree(c.box_part);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);
@@ -239,7 +241,7 @@ const char* kKernelNames[MPS_K_COUNT] = {
     "cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs", "explicit",
     "update", "density_save", "ppe_count", "ppe_fill", "cg_init", "cg_spmv", "cg_update",
     "cg_iter32", "cg_init32", "ir_residual", "gradient", "ds", "other", "features", "mlp",
-    "ml_setup", "ml_prec", "cg_spmv32", "cg_update32"};
+    "ml_setup", "ml_prec", "cg_spmv32", "cg_update32", "amg_cycle"};
 
 }  // namespace
 
@@ -402,7 +404,7 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.flags = mps::dalloc<int>(4);
     c.nbr_total_dev = mps::dalloc<unsigned long long>(1);
     c.in_cells = mps::dalloc<uint64_t>(1);
-    c.ppe_counters = mps::dalloc<unsigned long long>(4);
+    c.ppe_counters = mps::dalloc<unsigned long long>(8);
     c.partA = mps::dalloc<double>(mps::kRedBlocks);
     c.partB = mps::dalloc<double>(mps::kRedBlocks);
     c.partC = mps::dalloc<double>(mps::kRedBlocks);
@@ -414,7 +416,7 @@ int mps_create(const mps_env* env, double allowable_residual, int device, mps_ct
     c.ir = mps::dalloc<mps::IrStatus>(1);
     mps::pinned(c.flags_host, 4);
     mps::pinned(c.nbr_total_host, 1);
-    mps::pinned(c.ppe_counters_host, 3);
+    mps::pinned(c.ppe_counters_host, 8);
     mps::pinned(c.red_host, mps::kRedOut);
     mps::pinned(c.ml_active_host, 1);
     mps::pinned(c.st_host, 1);
@@ -638,7 +640,7 @@ int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts) {
   if (opts->kind < 0 || opts->kind > MPS_SOLVER_MIXED_IR) return MPS_ERR_BAD_ARG;
   if (opts->stop_rule < 0 || opts->stop_rule > MPS_STOP_RHS_NORM) return MPS_ERR_BAD_ARG;
   if (opts->warm_start < 0 || opts->warm_start > MPS_WARM_LEARNED) return MPS_ERR_BAD_ARG;
-  if (opts->precond < 0 || opts->precond > MPS_PRECOND_MULTILEVEL) return MPS_ERR_BAD_ARG;
+  if (opts->precond < 0 || opts->precond > MPS_PRECOND_AMG) return MPS_ERR_BAD_ARG;
   ctx->c.solver = *opts;
   return MPS_OK;
 }

@@ -812,30 +812,241 @@ __global__ void __launch_bounds__(256) k_ml_compact(uint32_t nblk, const uint32_
 constexpr uint32_t kNoSlot = 0xffffffffu;
 constexpr uint32_t kOutAgg = 0x80000000u;
 
+// inv == nullptr (the cell-grid multigrid): no aggregate scales, and rows
+// without stored off-diagonals (row_len 0: the fixed surface rows) join no
+// aggregate either.
 __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __restrict__ list, uint32_t nact,
                                                  const float* __restrict__ inv, const float* __restrict__ dinv,
                                                  uint32_t* __restrict__ pid, float* __restrict__ dv,
-                                                 float4* __restrict__ sc) {
+                                                 float4* __restrict__ sc, const uint32_t* __restrict__ row_len) {
   const uint32_t w = blockIdx.x * (NTP / 64) + (threadIdx.x >> 6);
   if (w >= nact) return;  // wave-uniform
   const uint32_t lane = threadIdx.x & 63;
   const BlockLane L = block_lane(G, list[w]);
   const uint32_t cnt = L.valid ? G.count[L.cell] : 0u, st0 = L.valid ? G.start[L.cell] : 0u;
+  const bool scales = inv != nullptr && L.valid;
   sc[static_cast<size_t>(w) * 64 + lane] =
-      make_float4(L.valid ? inv[L.cell] : 0.0f, L.valid ? inv[G.off1 + L.a1] : 0.0f,
-                  L.valid ? inv[G.off2 + L.a2] : 0.0f, static_cast<float>(cnt));
+      make_float4(scales ? inv[L.cell] : 0.0f, scales ? inv[G.off1 + L.a1] : 0.0f,
+                  scales ? inv[G.off2 + L.a2] : 0.0f, static_cast<float>(cnt));
   for (int u = 0; u < kPerLane; u++) {
     const uint32_t k = L.sub + 4u * u;
     const size_t o = (static_cast<size_t>(w) * kPerLane + u) * 64 + lane;
     if (k < cnt) {
       const uint32_t i = G.sorted[st0 + k];
-      pid[o] = i | (in_agg(G.type[i]) ? 0u : kOutAgg);
+      const bool agg = in_agg(G.type[i]) && (row_len == nullptr || row_len[i] > 0);
+      pid[o] = i | (agg ? 0u : kOutAgg);
       dv[o] = dinv[i];
     } else {
       pid[o] = kNoSlot;
       dv[o] = 0.0f;
     }
   }
+}
+
+// ---------------- cell-grid aggregation multigrid (MPS_PRECOND_AMG) ----------------
+//
+// The preconditioner M r = omega D^-1 r + P V(P^T r): aggregates are the
+// neighbour-search cells (P piecewise constant over a cell's aggregated
+// rows), and V is one V-cycle on the cells' Galerkin operator P^T A P.  The
+// cell side NL exceeds the Laplacian radius, so a particle couples only to
+// its own and the 8 adjacent cells: P^T A P is a 9-point stencil on the cell
+// grid, and so is every coarser level (2x2 blocks of the level below,
+// Galerkin sums of the stencil).  Each level is a dense grid (empty points:
+// zero stencil, zero D^-1).  V at level l: x = omega D^-1 r, residual,
+// restriction (sums over 2x2 blocks), the level below, prolongation
+// (piecewise constant), post-smoothing x += omega D^-1 (r - S x); the
+// coarsest level (at most kAmgDense points) is solved with its inverse.
+// The coarse levels whose points fit kAmgLdsPts run inside one workgroup, in LDS.
+constexpr int kAmgLdsPts = 2624;  // points of the levels the single-workgroup kernel holds in LDS (133 KB + the inverse)
+constexpr int kAmgDense = 64;
+constexpr int kAmgBlockThreads = 1024;
+constexpr float kAmgOmega = 0.7f;
+
+struct AmgLevel {
+  uint32_t nx, nz;
+  uint32_t off;  // first point of the level in the per-point arrays
+};
+struct AmgGrid {
+  int levels;
+  AmgLevel L[kAmgMaxLevels];
+  const float* S;     // 9 stencil coefficients of level l at S + 9 off_l + d npts_l + p (d = 3 (dx + 1) + dz + 1)
+  const float* dinv;  // 1 / S_center (0 on empty points)
+  float* r;           // right-hand side of every level (level 0: the cells' residual sums)
+  float* x;           // pre-smoothed iterate
+  float* e;           // result (level 0: the correction per cell)
+  const float* inv;   // inverse of the coarsest level (npts x npts, row-major)
+  float omega;
+};
+__device__ __forceinline__ uint32_t amg_npts(const AmgLevel& L) { return L.nx * L.nz; }
+
+// The per-point vectors of a cycle: r, x, e and 1 / S_center of point k of
+// any level at v[k - base] (global memory: base 0; the single-workgroup
+// kernel's LDS copy: base = the first point it holds).
+struct AmgVec {
+  float *r, *x, *e;
+  const float* dinv;
+  uint32_t base;
+};
+
+// Level l -> l + 1: one thread per point of level l + 1; its 2x2 children p
+// get x_p = omega r_p / S_pp and their residuals r_p - (S x)_p, summed into
+// the parent's right-hand side (x of a neighbour is computed from its r).
+template <class SV>
+__device__ __forceinline__ void amg_down_point(const AmgGrid& g, const AmgVec& v, const SV& sten, int l, uint32_t P) {
+  const AmgLevel F = g.L[l], C = g.L[l + 1];
+  const uint32_t X = P / C.nz, Z = P % C.nz;
+  const uint32_t fo = F.off - v.base;
+  const float* __restrict__ vr = v.r;
+  const float* __restrict__ vd = v.dinv;
+  // every load first: r and 1 / S_pp on the 4x4 patch around the children,
+  // the children's stencils
+  float rr[4][4], dd[4][4], st[2][2][9];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int ix = 2 * static_cast<int>(X) - 1 + a, iz = 2 * static_cast<int>(Z) - 1 + b;
+      const bool in = ix >= 0 && iz >= 0 && ix < static_cast<int>(F.nx) && iz < static_cast<int>(F.nz);
+      const uint32_t q = fo + static_cast<uint32_t>(in ? ix : 0) * F.nz + static_cast<uint32_t>(in ? iz : 0);
+      rr[a][b] = in ? vr[q] : 0.0f;
+      dd[a][b] = in ? vd[q] : 0.0f;
+    }
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+      const uint32_t ix = 2 * X + u, iz = 2 * Z + w;
+      const bool in = ix < F.nx && iz < F.nz;
+      const uint32_t p = in ? ix * F.nz + iz : 0u;
+#pragma unroll
+      for (int d = 0; d < 9; d++) st[u][w][d] = in ? sten(l, d, p) : 0.0f;
+    }
+  float rc = 0.0f;
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+      const uint32_t ix = 2 * X + u, iz = 2 * Z + w;
+      if (ix >= F.nx || iz >= F.nz) continue;
+      float sum = 0.0f;
+#pragma unroll
+      for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++)
+          sum = fmaf(st[u][w][3 * (dx + 1) + dz + 1],
+                     g.omega * dd[1 + u + dx][1 + w + dz] * rr[1 + u + dx][1 + w + dz], sum);
+      v.x[fo + ix * F.nz + iz] = g.omega * dd[1 + u][1 + w] * rr[1 + u][1 + w];
+      rc += rr[1 + u][1 + w] - sum;
+    }
+  v.r[C.off - v.base + P] = rc;
+}
+
+// Level l + 1 -> l: one thread per point p of level l: x = x_pre + e(parent),
+// then e_p = x_p + omega (r_p - (S x)_p) / S_pp.
+template <class SV>
+__device__ __forceinline__ float amg_up_value(const AmgGrid& g, const AmgVec& v, const SV& sten, int l, uint32_t p) {
+  const AmgLevel F = g.L[l], C = g.L[l + 1];
+  const uint32_t ix = p / F.nz, iz = p % F.nz;
+  const uint32_t fo = F.off - v.base, co = C.off - v.base;
+  const float di = v.dinv[fo + p];
+  if (di == 0.0f) return 0.0f;  // empty point
+  float xq[9], sv[9];
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      const int k = 3 * (dx + 1) + dz + 1;
+      const int jx = static_cast<int>(ix) + dx, jz = static_cast<int>(iz) + dz;
+      const bool in = jx >= 0 && jz >= 0 && jx < static_cast<int>(F.nx) && jz < static_cast<int>(F.nz);
+      const uint32_t ux = static_cast<uint32_t>(in ? jx : 0), uz = static_cast<uint32_t>(in ? jz : 0);
+      xq[k] = in ? v.x[fo + ux * F.nz + uz] + v.e[co + (ux >> 1) * C.nz + (uz >> 1)] : 0.0f;
+      sv[k] = in ? sten(l, k, p) : 0.0f;
+    }
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) sum = fmaf(sv[k], xq[k], sum);
+  return fmaf(g.omega * di, v.r[fo + p] - sum, xq[4]);
+}
+
+// Level l + 1 -> l: one thread per point p of level l: x = x_pre + e(parent),
+// then e_p = x_p + omega (r_p - (S x)_p) / S_pp (0 on empty points).
+template <class SV>
+__device__ __forceinline__ void amg_up_point(const AmgGrid& g, const AmgVec& v, const SV& sten, int l, uint32_t p) {
+  v.e[g.L[l].off - v.base + p] = amg_up_value(g, v, sten, l, p);
+}
+
+// Stencil coefficient d of point p of level l, from global memory.
+struct AmgStenGlobal {
+  const AmgGrid* g;
+  __device__ __forceinline__ float operator()(int l, int d, uint32_t p) const {
+    const AmgLevel L = g->L[l];
+    return g->S[9ull * L.off + static_cast<size_t>(d) * amg_npts(L) + p];
+  }
+};
+
+__device__ __forceinline__ AmgVec amg_global(const AmgGrid& g) { return AmgVec{g.r, g.x, g.e, g.dinv, 0u}; }
+
+__global__ void __launch_bounds__(256) k_amg_down(AmgGrid g, int l) {
+  const uint32_t P = blockIdx.x * 256u + threadIdx.x;
+  if (P < amg_npts(g.L[l + 1])) amg_down_point(g, amg_global(g), AmgStenGlobal{&g}, l, P);
+}
+
+__global__ void __launch_bounds__(256) k_amg_up(AmgGrid g, int l) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p < amg_npts(g.L[l])) amg_up_point(g, amg_global(g), AmgStenGlobal{&g}, l, p);
+}
+
+// Levels l0 .. levels - 1 in one workgroup (at most kAmgLdsPts points from
+// point off_l0 on): their stencils, 1 / S_center and r of level l0 are copied
+// to LDS; the way down, the coarsest solve (e = inverse r) and the way up run
+// in LDS, ordered by barriers; e of level l0 goes back to global memory.
+constexpr int kAmgLdsFloats = 13 * kAmgLdsPts + kAmgDense * kAmgDense;  // 9 stencil + r, x, e, 1 / S_center per point; the inverse
+struct AmgStenLds {
+  const AmgGrid* g;
+  const float* s;  // the stencils of the levels from l0 on, as in global memory
+  uint32_t base;
+  __device__ __forceinline__ float operator()(int l, int d, uint32_t p) const {
+    const AmgLevel L = g->L[l];
+    return s[9u * (L.off - base) + static_cast<uint32_t>(d) * amg_npts(L) + p];
+  }
+};
+__global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) {
+  __shared__ float lds[kAmgLdsFloats];
+  const uint32_t base = g.L[l0].off;
+  const uint32_t npts = g.L[g.levels - 1].off + amg_npts(g.L[g.levels - 1]) - base;
+  float* const ls = lds;
+  float* const lv = lds + 9 * kAmgLdsPts;
+  const AmgVec v{lv, lv + kAmgLdsPts, lv + 2 * kAmgLdsPts, lv + 3 * kAmgLdsPts, base};
+  float* const vd = lv + 3 * kAmgLdsPts;
+  const uint32_t n0 = amg_npts(g.L[l0]);
+  const AmgLevel B = g.L[g.levels - 1];
+  const uint32_t nb = amg_npts(B), bo = B.off - base;
+  float* const linv = lds + 13 * kAmgLdsPts;
+  for (uint32_t k = threadIdx.x; k < 9 * npts; k += kAmgBlockThreads) ls[k] = g.S[9ull * base + k];
+  for (uint32_t k = threadIdx.x; k < nb * nb; k += kAmgBlockThreads) linv[k] = g.inv[k];
+  for (uint32_t p = threadIdx.x; p < npts; p += kAmgBlockThreads) {
+    vd[p] = g.dinv[base + p];
+    if (p < n0) v.r[p] = g.r[base + p];
+  }
+  __syncthreads();
+  const AmgStenLds sten{&g, ls, base};
+  for (int l = l0; l + 1 < g.levels; l++) {
+    const uint32_t n1 = amg_npts(g.L[l + 1]);
+    for (uint32_t P = threadIdx.x; P < n1; P += kAmgBlockThreads) amg_down_point(g, v, sten, l, P);
+    __syncthreads();
+  }
+  for (uint32_t p = threadIdx.x; p < nb; p += kAmgBlockThreads) {
+    float s = 0.0f;
+    for (uint32_t q = 0; q < nb; q++) s = fmaf(linv[p * nb + q], v.r[bo + q], s);
+    v.e[bo + p] = s;
+  }
+  __syncthreads();
+  for (int l = g.levels - 2; l >= l0; l--) {
+    const uint32_t nl = amg_npts(g.L[l]);
+    for (uint32_t p = threadIdx.x; p < nl; p += kAmgBlockThreads) amg_up_point(g, v, sten, l, p);
+    __syncthreads();
+  }
+  for (uint32_t p = threadIdx.x; p < n0; p += kAmgBlockThreads) g.e[base + p] = v.e[p];
 }
 
 // The operands of k_ml_prec.
@@ -857,10 +1068,30 @@ struct PrecArgs {
   double* part;
   size_t part_stride;
   CgStatus* st;
+  // the cell-grid multigrid (modes 1, 2): the residual's cell sums out, the
+  // coarse correction per cell in, the Jacobi damping
+  float* rc;
+  const float* ec;
+  float omega;
+  const uint32_t* row_len;
+  uint32_t bx0, bz0, bnx, bnz;  // the multigrid's level 0: cells [bx0, bx0 + bnx) x [bz0, bz0 + bnz)
 };
 
-// One wavefront per listed 4x4 block of cells.
-__device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, const Frame& f) {
+// Level-0 point of a cell of the multigrid's box, or -1 outside it.
+__device__ __forceinline__ int amg_point(const PrecArgs& a, uint32_t cell, uint32_t gnz) {
+  const uint32_t cx = cell / gnz - a.bx0, cz = cell % gnz - a.bz0;  // unsigned: below the box wraps high
+  return (cx < a.bnx && cz < a.bnz) ? static_cast<int>(cx * a.bnz + cz) : -1;
+}
+
+// One wavefront per listed 4x4 block of cells.  MODE 0: the additive
+// multilevel preconditioner (residual update, aggregate sums, z = M r).  The
+// cell-grid multigrid splits the pass in two around its coarse cycle: MODE 1
+// updates r and writes the cells' residual sums (and the r.r partials); MODE 2
+// forms z = omega r / A_ii + e(cell) on aggregated rows, r / A_ii on the
+// others (and the r.z partials).
+template <int MODE>
+__device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, const Frame& f,
+                                             const AmgGrid* amg = nullptr) {
   __shared__ double lds[NTP / 64];
   const CellGrid& G = a.G;
   const uint32_t* __restrict__ list = a.list;
@@ -889,7 +1120,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
 #pragma unroll
   for (int u = 0; u < kPqPerThread; u++) {
     const int k = threadIdx.x + u * NTP;
-    pq_part[u] = (it > 0 && k < npq) ? part_pq[k] : 0.0;
+    pq_part[u] = (MODE != 2 && it > 0 && k < npq) ? part_pq[k] : 0.0;
   }
   const uint32_t w = f.bid * (NTP / 64) + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
@@ -910,7 +1141,9 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
       rv[u] = qv[u] = 0.0f;
       if (id[u] != kNoSlot) {
         const uint32_t i = id[u] & ~kOutAgg;
-        if (it == 0) {
+        if (MODE == 2) {
+          rv[u] = r[i];
+        } else if (it == 0) {
           rv[u] = static_cast<float>(r64[i] * inv_norm);
         } else {
           rv[u] = r[i];
@@ -920,7 +1153,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
     }
   }
   float alpha = 0.0f;
-  if (it > 0) {
+  if (MODE != 2 && it > 0) {
     double s = 0;
 #pragma unroll
     for (int u = 0; u < kPqPerThread; u++) s += pq_part[u];
@@ -936,66 +1169,264 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
     BlockLane L{false, 0, 0, 0, 0};
     uint32_t st0 = 0;
     const bool over = cnt > 4u * kPerLane;
-    if (over) {
+    if (over || MODE != 0) {
       L = block_lane(G, list[w]);
-      st0 = G.start[L.cell];
+      st0 = L.valid ? G.start[L.cell] : 0u;
     }
+    // rows beyond the registers (over): aggregated as k_ml_pack decides
+    auto agg_of = [&](uint32_t i) { return in_agg(G.type[i]) && (MODE == 0 || a.row_len[i] > 0); };
     float s = 0.0f;
+    if (MODE != 2) {
 #pragma unroll
-    for (int u = 0; u < kPerLane; u++) {
-      if (id[u] != kNoSlot) {
-        const uint32_t i = id[u] & ~kOutAgg;
-        const float ri = (it == 0) ? rv[u] : fmaf(-alpha, qv[u], rv[u]);
-        rv[u] = ri;
-        r[i] = ri;
-        if (it == 0) d[i] = 0.0f;
-        if (!(id[u] & kOutAgg)) s += ri;
-        rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
-      }
-    }
-    if (over)
-      for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-        const uint32_t i = G.sorted[st0 + k];
-        float ri;
-        if (it == 0) {
-          ri = static_cast<float>(r64[i] * inv_norm);
-          d[i] = 0.0f;
-        } else {
-          ri = fmaf(-alpha, q[i], r[i]);
+      for (int u = 0; u < kPerLane; u++) {
+        if (id[u] != kNoSlot) {
+          const uint32_t i = id[u] & ~kOutAgg;
+          const float ri = (it == 0) ? rv[u] : fmaf(-alpha, qv[u], rv[u]);
+          rv[u] = ri;
+          r[i] = ri;
+          if (it == 0) d[i] = 0.0f;
+          if (!(id[u] & kOutAgg)) s += ri;
+          rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
         }
-        r[i] = ri;
-        if (in_agg(G.type[i])) s += ri;
-        rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
       }
-    float y0, y1, y2;
-    agg_sums(s, y0, y1, y2);
-    const float corr = y0 * scl.x + y1 * scl.y + y2 * scl.z;
-#pragma unroll
-    for (int u = 0; u < kPerLane; u++) {
-      if (id[u] != kNoSlot) {
-        const uint32_t i = id[u] & ~kOutAgg;
-        const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(dv[u], rv[u], corr);
-        z[static_cast<size_t>(i) * zs] = zi;
-        rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
-      }
+      if (over)
+        for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+          const uint32_t i = G.sorted[st0 + k];
+          float ri;
+          if (it == 0) {
+            ri = static_cast<float>(r64[i] * inv_norm);
+            d[i] = 0.0f;
+          } else {
+            ri = fmaf(-alpha, q[i], r[i]);
+          }
+          r[i] = ri;
+          if (agg_of(i)) s += ri;
+          rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
+        }
     }
-    if (over)
-      for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
-        const uint32_t i = G.sorted[st0 + k];
-        const float ri = r[i];
-        const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
-        z[static_cast<size_t>(i) * zs] = zi;
-        rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
+    if (MODE == 1) {
+      // the cell's residual sum: lanes 0, 16, 32, 48 hold one cell's members
+      float t = s + __shfl_xor(s, 16, 64);
+      t = t + __shfl_xor(t, 32, 64);
+      const int pt = L.valid ? amg_point(a, L.cell, G.gnz) : -1;
+      if (pt >= 0 && L.sub == 0) a.rc[pt] = t;
+    } else if (MODE == 2) {
+      const int pt = L.valid ? amg_point(a, L.cell, G.gnz) : -1;
+      float ec;
+      if (amg) {
+        // the cycle's last up-pass (level 0) for the lane's cell, on the
+        // lane holding its first members; the other three lanes take it
+        float val = 0.0f;
+        if (pt >= 0 && L.sub == 0) val = amg_up_value(*amg, amg_global(*amg), AmgStenGlobal{amg}, 0, pt);
+        ec = __shfl(val, static_cast<int>(lane & 15u), 64);
+      } else {
+        ec = (pt >= 0) ? a.ec[pt] : 0.0f;
       }
+      const float om = a.omega;
+#pragma unroll
+      for (int u = 0; u < kPerLane; u++) {
+        if (id[u] != kNoSlot) {
+          const uint32_t i = id[u] & ~kOutAgg;
+          const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(om * dv[u], rv[u], ec);
+          z[static_cast<size_t>(i) * zs] = zi;
+          rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
+        }
+      }
+      if (over)
+        for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+          const uint32_t i = G.sorted[st0 + k];
+          const float ri = r[i];
+          const float zi = agg_of(i) ? fmaf(om * dinv[i], ri, ec) : dinv[i] * ri;
+          z[static_cast<size_t>(i) * zs] = zi;
+          rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
+        }
+    } else {
+      float y0, y1, y2;
+      agg_sums(s, y0, y1, y2);
+      const float corr = y0 * scl.x + y1 * scl.y + y2 * scl.z;
+#pragma unroll
+      for (int u = 0; u < kPerLane; u++) {
+        if (id[u] != kNoSlot) {
+          const uint32_t i = id[u] & ~kOutAgg;
+          const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(dv[u], rv[u], corr);
+          z[static_cast<size_t>(i) * zs] = zi;
+          rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
+        }
+      }
+      if (over)
+        for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
+          const uint32_t i = G.sorted[st0 + k];
+          const float ri = r[i];
+          const float zi = in_agg(G.type[i]) ? fmaf(dinv[i], ri, corr) : dinv[i] * ri;
+          z[static_cast<size_t>(i) * zs] = zi;
+          rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
+        }
+    }
   }
   const double srho = block_sum<NTP>(rho, lds);
   const double srr = block_sum<NTP>(rr, lds);
   if (threadIdx.x == 0) {
-    part[f.bid] = srho;
-    part[part_stride + f.bid] = srr;
+    if (MODE != 1) part[f.bid] = srho;
+    if (MODE != 2) part[part_stride + f.bid] = srr;
   }
 }
-__global__ void __launch_bounds__(NTP) k_ml_prec(PrecArgs a, long long it) { ml_prec_body(a, it, whole_grid()); }
+template <int MODE>
+__global__ void __launch_bounds__(NTP) k_ml_prec(PrecArgs a, long long it) {
+  ml_prec_body<MODE>(a, it, whole_grid());
+}
+// MODE 2 with the cycle's level-0 up-pass folded in (it reads level 1's
+// result and level 0's pre-smoothed iterate).
+__global__ void __launch_bounds__(NTP) k_ml_prec_up(PrecArgs a, long long it, AmgGrid g) {
+  ml_prec_body<2>(a, it, whole_grid(), &g);
+}
+
+// ---- set-up (FP64 sums, FP32 levels) ----
+
+// Per aggregated row i (in an aggregate: not dummy / disabled, stored
+// off-diagonals, in a cell): its diagonal and off-diagonals to aggregated
+// columns, summed by the direction of the column's cell (9 sums, FP64).
+__global__ void __launch_bounds__(NT, 8) k_amg_rows(Sell<double> A, const uint32_t* __restrict__ row_len,
+                                                    const uint32_t* __restrict__ key,
+                                                    const uint8_t* __restrict__ type, uint32_t gnz,
+                                                    double* __restrict__ rows) {
+  const Rows rw = block_rows(A.n, A.rpb);
+  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+    double acc[9];
+#pragma unroll
+    for (int d = 0; d < 9; d++) acc[d] = 0;
+    const uint32_t ki = key[i];
+    const uint32_t m = row_len[i];
+    if (in_agg(type[i]) && m > 0 && ki != kNoCell) {
+      const int cx = static_cast<int>(ki / gnz), cz = static_cast<int>(ki % gnz);
+      acc[4] = A.diag[i];
+      const size_t base = A.off[i >> 6] + (i & 63);
+      for (uint32_t q = 0; q < m; q++) {
+        const size_t o = base + static_cast<size_t>(q) * 64;
+        const uint32_t j = A.col[o];
+        const uint32_t kj = key[j];
+        if (kj == kNoCell || !in_agg(type[j]) || row_len[j] == 0) continue;
+        const int dx = static_cast<int>(kj / gnz) - cx, dz = static_cast<int>(kj % gnz) - cz;
+        const double a = A.val[o];
+#pragma unroll
+        for (int d = 0; d < 9; d++)
+          if (d == 3 * (dx + 1) + dz + 1) acc[d] += a;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 9; d++) rows[d * A.n + i] = acc[d];
+  }
+}
+
+// Level 0 (the cells of the box x0.. x0 + nx - 1, z0 .. z0 + nz - 1): the
+// Galerkin stencil, sums of the member rows' sums.
+__global__ void __launch_bounds__(256) k_amg_cells(uint32_t x0, uint32_t z0, uint32_t nx, uint32_t nz, uint32_t gnz,
+                                                   const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ count,
+                                                   const uint32_t* __restrict__ sorted, const double* __restrict__ rows,
+                                                   size_t n, double* __restrict__ S) {
+  const size_t pt = blockIdx.x * 256ull + threadIdx.x;
+  const size_t npts = static_cast<size_t>(nx) * nz;
+  if (pt >= npts) return;
+  const size_t c = static_cast<size_t>(x0 + pt / nz) * gnz + (z0 + pt % nz);
+  const uint32_t s0 = start[c], cnt = count[c];
+  double acc[9];
+#pragma unroll
+  for (int d = 0; d < 9; d++) acc[d] = 0;
+  for (uint32_t k = 0; k < cnt; k++) {
+    const uint32_t i = sorted[s0 + k];
+#pragma unroll
+    for (int d = 0; d < 9; d++) acc[d] += rows[d * n + i];
+  }
+#pragma unroll
+  for (int d = 0; d < 9; d++) S[d * npts + pt] = acc[d];
+}
+
+// Level l + 1 from level l: the coarse stencil of each 2x2 block (FP64).
+__global__ void __launch_bounds__(256) k_amg_coarsen(const double* __restrict__ Sf, uint32_t fnx, uint32_t fnz,
+                                                     double* __restrict__ Sc, uint32_t cnx, uint32_t cnz) {
+  const uint32_t P = blockIdx.x * 256u + threadIdx.x;
+  if (P >= cnx * cnz) return;
+  const uint32_t X = P / cnz, Z = P % cnz;
+  const size_t nf = static_cast<size_t>(fnx) * fnz, nc = static_cast<size_t>(cnx) * cnz;
+  double acc[9];
+#pragma unroll
+  for (int d = 0; d < 9; d++) acc[d] = 0;
+  for (int u = 0; u < 2; u++)
+    for (int v = 0; v < 2; v++) {
+      const uint32_t ix = 2 * X + u, iz = 2 * Z + v;
+      if (ix >= fnx || iz >= fnz) continue;
+      const size_t p = static_cast<size_t>(ix) * fnz + iz;
+#pragma unroll
+      for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++) {
+          const int jx = static_cast<int>(ix) + dx, jz = static_cast<int>(iz) + dz;
+          if (jx < 0 || jz < 0 || jx >= static_cast<int>(fnx) || jz >= static_cast<int>(fnz)) continue;
+          const int DX = (jx >> 1) - static_cast<int>(X), DZ = (jz >> 1) - static_cast<int>(Z);
+          const double a = Sf[(3 * (dx + 1) + dz + 1) * nf + p];
+#pragma unroll
+          for (int d = 0; d < 9; d++)
+            if (d == 3 * (DX + 1) + DZ + 1) acc[d] += a;
+        }
+    }
+#pragma unroll
+  for (int d = 0; d < 9; d++) Sc[d * nc + P] = acc[d];
+}
+
+// FP32 stencils and 1 / S_center (0 where the point is empty) of one level.
+__global__ void __launch_bounds__(256) k_amg_f32(const double* __restrict__ S64, uint32_t off, uint32_t npts,
+                                                 float* __restrict__ S32, float* __restrict__ dinv) {
+  const size_t k = blockIdx.x * 256ull + threadIdx.x;
+  if (k >= 9ull * npts) return;
+  S32[9ull * off + k] = static_cast<float>(S64[9ull * off + k]);
+  if (k < npts) {
+    const double c = S64[9ull * off + 4ull * npts + k];
+    dinv[off + k] = (c != 0.0) ? static_cast<float>(1.0 / c) : 0.0f;
+  }
+}
+
+// The inverse of the coarsest level's operator (n <= kAmgDense points; empty
+// points become identity rows), Gauss-Jordan in FP64 in one workgroup.
+__global__ void __launch_bounds__(256) k_amg_dense(const double* __restrict__ S, uint32_t nx, uint32_t nz,
+                                                   float* __restrict__ inv) {
+  __shared__ double M[kAmgDense][2 * kAmgDense];
+  __shared__ double fcol[kAmgDense];
+  __shared__ double piv;
+  const int n = static_cast<int>(nx * nz);
+  for (int e = threadIdx.x; e < n * 2 * n; e += blockDim.x) {
+    const int p = e / (2 * n), q = e % (2 * n);
+    double v = 0;
+    if (q >= n) {
+      v = (q - n == p) ? 1.0 : 0.0;
+    } else {
+      const int px = p / static_cast<int>(nz), pz = p % static_cast<int>(nz);
+      const int qx = q / static_cast<int>(nz), qz = q % static_cast<int>(nz);
+      const double c = S[4 * n + p];
+      if (c == 0.0) {
+        v = (p == q) ? 1.0 : 0.0;
+      } else if (abs(qx - px) <= 1 && abs(qz - pz) <= 1) {
+        v = S[(3 * (qx - px + 1) + qz - pz + 1) * n + p];
+      }
+    }
+    M[p][q] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    if (threadIdx.x == 0) piv = M[k][k];
+    __syncthreads();
+    const double pk = piv;
+    for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) M[k][q] /= pk;
+    __syncthreads();
+    for (int p = threadIdx.x; p < n; p += blockDim.x) fcol[p] = M[p][k];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * 2 * n; e += blockDim.x) {
+      const int p = e / (2 * n), q = e % (2 * n);
+      if (p != k) M[p][q] -= fcol[p] * M[k][q];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) inv[e] = static_cast<float>(M[e / n][n + e % n]);
+}
 
 // See above; FIRST is it == 1 (p = z, q = A z, tolerance from r_0.r_0).  The
 // partials of k_ml_prec(it - 1) are loaded at the start and summed after the
@@ -1634,7 +2065,7 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
     const PrecArgs a{G,      c.ml_list, nact,   c.ml_pid, c.ml_dv, c.ml_sc,   c.r,       1.0 / norm,
                      c.r32,  c.d32,     c.q32,  c.dinv32, zval,    4u,        p_pq,      L.grid,
                      c.part_ml, stride, c.st32};
-    hipLaunchKernelGGL(k_ml_prec, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    hipLaunchKernelGGL(k_ml_prec<0>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
   };
   double b_spmv = b_base;
   c.replay_ready = MPS_K_CG_SPMV32;
@@ -1659,7 +2090,8 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
       tracked(c, MPS_K_ML_SETUP, n * 13.0 + slots * (8.0 * kPerLane + 16), [&] {
         hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
                            static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
-                           static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc);
+                           static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
+                           static_cast<const uint32_t*>(nullptr));
       });
       packed = true;
     }
@@ -1685,6 +2117,243 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
     // only finished d)
     uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
     uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
+    return inner;
+  }, [&](double norm) {
+    hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
+                       static_cast<const float*>(c.d32), norm);
+  });
+}
+
+// ---- cell-grid multigrid (MPS_PRECOND_AMG) ----
+
+AmgGrid amg_grid(const Context& c) {
+  AmgGrid g{};
+  g.levels = c.amg_levels;
+  for (int l = 0; l < c.amg_levels; l++) g.L[l] = AmgLevel{c.amg_nx[l], c.amg_nz[l], c.amg_off[l]};
+  g.S = c.amg_S;
+  g.dinv = c.amg_dinv;
+  g.r = c.amg_r;
+  g.x = c.amg_x;
+  g.e = c.amg_e;
+  g.inv = c.amg_inv;
+  g.omega = kAmgOmega;
+  return g;
+}
+
+// The first level the single-workgroup kernel takes: the points of that
+// level and all below fit its LDS (kAmgLdsPts).
+int amg_block_level(const Context& c) {
+  const size_t end = c.amg_pts;
+  int l0 = c.amg_levels - 1;
+  while (l0 > 0 && end - c.amg_off[l0 - 1] <= static_cast<size_t>(kAmgLdsPts)) l0--;
+  return l0;
+}
+
+// Levels and operators from the assembled FP64 system and the current cells
+// (on c.stream): Jacobi factors, the occupied 4x4 cell blocks (their count
+// reaches the host with the solver's first synchronisation), the rows'
+// per-direction sums, the cells' Galerkin stencil, the coarser stencils, their
+// FP32 copies and 1 / diagonal, and the coarsest inverse.
+void amg_setup(Context& c) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                     static_cast<const double*>(c.diag), c.dinv32);
+  // the occupied 4x4 blocks of cells, as the multilevel set-up lists them
+  const MlGeom mg = ml_geom(c);
+  if (c.ml_blk_cap < mg.nblk) {
+    dfree(c.ml_flag);
+    dfree(c.ml_off);
+    dfree(c.ml_list);
+    c.ml_flag = dalloc<uint32_t>(mg.nblk);
+    c.ml_off = dalloc<uint32_t>(mg.nblk);
+    c.ml_list = dalloc<uint32_t>(mg.nblk);
+    c.ml_blk_cap = mg.nblk;
+  }
+  if (!c.ml_active) c.ml_active = dalloc<uint64_t>(1);
+  const CellGrid G = ml_grid(c, mg);
+  // level shapes
+  amg_box(c);
+  int lv = 0;
+  uint32_t nx = c.amg_box[2], nz = c.amg_box[3];
+  size_t off = 0;
+  for (;;) {
+    if (lv == kAmgMaxLevels) throw Failure{MPS_ERR_BAD_ARG, "cell grid too large for the multigrid levels"};
+    c.amg_nx[lv] = nx;
+    c.amg_nz[lv] = nz;
+    c.amg_off[lv] = static_cast<uint32_t>(off);
+    off += static_cast<size_t>(nx) * nz;
+    lv++;
+    if (static_cast<size_t>(nx) * nz <= static_cast<size_t>(kAmgDense)) break;
+    nx = (nx + 1) / 2;
+    nz = (nz + 1) / 2;
+  }
+  if (off > 0xffffffffull) throw Failure{MPS_ERR_BAD_ARG, "cell grid too large for the multigrid levels"};
+  c.amg_levels = lv;
+  c.amg_pts = off;
+  if (c.amg_cap < off) {
+    dfree(c.amg_S64);
+    dfree(c.amg_S);
+    dfree(c.amg_dinv);
+    dfree(c.amg_r);
+    dfree(c.amg_x);
+    dfree(c.amg_e);
+    c.amg_S64 = dalloc<double>(9 * off);
+    c.amg_S = dalloc<float>(9 * off);
+    c.amg_dinv = dalloc<float>(off);
+    c.amg_r = dalloc<float>(off);
+    c.amg_x = dalloc<float>(off);
+    c.amg_e = dalloc<float>(off);
+    c.amg_cap = off;
+  }
+  if (!c.amg_inv) c.amg_inv = dalloc<float>(kAmgDense * kAmgDense);
+  if (c.amg_rows_cap < n) {
+    dfree(c.amg_rows);
+    c.amg_rows = dalloc<double>(9 * n);
+    c.amg_rows_cap = n;
+  }
+  const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
+  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 16.0 + n * (17.0 + 72) + npts0 * 8.0 + off * 9 * 12.0, [&] {
+    hipLaunchKernelGGL(k_ml_active, dim3(grid_for(mg.nblk)), dim3(256), 0, c.stream, G, c.ml_flag);
+    scan_counts(c, c.ml_flag, mg.nblk, c.ml_off, c.ml_active);
+    hipLaunchKernelGGL(k_ml_compact, dim3(grid_for(mg.nblk)), dim3(256), 0, c.stream, static_cast<uint32_t>(mg.nblk),
+                       static_cast<const uint32_t*>(c.ml_flag), static_cast<const uint32_t*>(c.ml_off), c.ml_list);
+    hipLaunchKernelGGL(k_amg_rows, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
+                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
+                       static_cast<const uint8_t*>(c.type), static_cast<uint32_t>(c.gnz), c.amg_rows);
+    hipLaunchKernelGGL(k_amg_cells, dim3(grid_for(npts0)), dim3(256), 0, c.stream, c.amg_box[0], c.amg_box[1],
+                       c.amg_box[2], c.amg_box[3], static_cast<uint32_t>(c.gnz),
+                       static_cast<const uint32_t*>(c.cell_start), static_cast<const uint32_t*>(c.cell_count),
+                       static_cast<const uint32_t*>(c.sorted), static_cast<const double*>(c.amg_rows), n, c.amg_S64);
+    for (int l = 0; l + 1 < lv; l++)
+      hipLaunchKernelGGL(k_amg_coarsen, dim3(grid_for(static_cast<size_t>(c.amg_nx[l + 1]) * c.amg_nz[l + 1])),
+                         dim3(256), 0, c.stream, static_cast<const double*>(c.amg_S64 + 9ull * c.amg_off[l]),
+                         c.amg_nx[l], c.amg_nz[l], c.amg_S64 + 9ull * c.amg_off[l + 1], c.amg_nx[l + 1],
+                         c.amg_nz[l + 1]);
+    for (int l = 0; l < lv; l++) {
+      const uint32_t np = c.amg_nx[l] * c.amg_nz[l];
+      hipLaunchKernelGGL(k_amg_f32, dim3(grid_for(9ull * np)), dim3(256), 0, c.stream,
+                         static_cast<const double*>(c.amg_S64), c.amg_off[l], np, c.amg_S, c.amg_dinv);
+    }
+    hipLaunchKernelGGL(k_amg_dense, dim3(1), dim3(256), 0, c.stream,
+                       static_cast<const double*>(c.amg_S64 + 9ull * c.amg_off[lv - 1]), c.amg_nx[lv - 1],
+                       c.amg_nz[lv - 1], c.amg_inv);
+    // cells of unlisted (empty) blocks keep a zero right-hand side
+    MPS_HIP(hipMemsetAsync(c.amg_r, 0, npts0 * sizeof(float), c.stream));
+  });
+  MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+  check_launch();
+}
+
+// One V-cycle on the cell-grid levels: level 0's right-hand side (the cells'
+// residual sums) in, its result (the correction per cell) out.
+// The level-0 up-pass is left to the caller when `fold0` (k_ml_prec_up
+// does it) and the cycle has a level below the cells' outside the
+// single-workgroup kernel; returns whether it was left.
+bool amg_cycle(Context& c, bool fold0) {
+  const AmgGrid g = amg_grid(c);
+  const int l0 = amg_block_level(c);
+  for (int l = 0; l < l0; l++)
+    hipLaunchKernelGGL(k_amg_down, dim3(grid_for(static_cast<size_t>(c.amg_nx[l + 1]) * c.amg_nz[l + 1])),
+                       dim3(256), 0, c.stream, g, l);
+  hipLaunchKernelGGL(k_amg_block, dim3(1), dim3(kAmgBlockThreads), 0, c.stream, g, l0);
+  const bool left = fold0 && l0 >= 1;
+  for (int l = l0 - 1; l >= (left ? 1 : 0); l--)
+    hipLaunchKernelGGL(k_amg_up, dim3(grid_for(static_cast<size_t>(c.amg_nx[l]) * c.amg_nz[l])), dim3(256), 0,
+                       c.stream, g, l);
+  return left;
+}
+
+void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long max_iter) {
+  const size_t n = c.ppe_n;
+  const Launch L = cg_grid(n);
+  ensure_zrec(c, n);
+  const MatFreeRec A = matfree_rec(c, L.rpb);
+  float* const zval = reinterpret_cast<float*>(c.zrec) + 2;  // z_i at zval[4 i]
+  amg_setup(c);
+  const MlGeom g = ml_geom(c);
+  if (c.part_ml_cap < g.sgrid) {
+    dfree(c.part_ml);
+    c.part_ml = dalloc<double>(2 * g.sgrid);
+    c.part_ml_cap = g.sgrid;
+  }
+  const size_t stride = c.part_ml_cap;
+  const CellGrid G = ml_grid(c, g);
+  hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
+                     c.zrec);
+  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+  check_launch();
+  double* p_pq = c.part3b;
+  const long long limit = inner_limit(c, max_iter);
+  uint32_t nact = 0;
+  size_t pgrid = 1;
+  bool packed = false;
+  double b_prec = n * 16.0;
+  const double b_cycle = static_cast<double>(c.amg_pts) * 2 * (36 + 16);
+  auto prec = [&](double norm, long long it) {
+    PrecArgs a{G,         c.ml_list, nact,  c.ml_pid, c.ml_dv,    c.ml_sc, c.r,   1.0 / norm, c.r32,
+               c.d32,     c.q32,     c.dinv32, zval, 4u,         p_pq,    L.grid, c.part_ml,  stride,
+               c.st32,    c.amg_r,   c.amg_e,   kAmgOmega, c.row_len, c.amg_box[0], c.amg_box[1],
+               c.amg_box[2], c.amg_box[3]};
+    hipLaunchKernelGGL(k_ml_prec<1>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    bool left = false;
+    tracked(c, MPS_K_AMG_CYCLE, b_cycle, [&] { left = amg_cycle(c, true); }, it);
+    if (left)
+      hipLaunchKernelGGL(k_ml_prec_up, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it,
+                         amg_grid(c));
+    else
+      hipLaunchKernelGGL(k_ml_prec<2>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+  };
+  double b_spmv = b_base;
+  c.replay_ready = MPS_K_CG_SPMV32;
+  refine(c, sr, [&](double norm, double sweep_tol) {
+    // refine() has synchronised the stream before every sweep: the block
+    // count and the wide nnz are on the host
+    nact = static_cast<uint32_t>(*c.ml_active_host);
+    b_spmv = b_base + 2.0 * mf_nnz_wide(c);
+    pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
+    b_prec = n * 24.0 + static_cast<double>(nact) * 64 * (2 * 8.0 * kPerLane + 16);
+    if (!packed) {
+      const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
+      if (c.ml_pack_cap < slots) {
+        dfree(c.ml_pid);
+        dfree(c.ml_dv);
+        dfree(c.ml_sc);
+        c.ml_pid = dalloc<uint32_t>(slots * kPerLane);
+        c.ml_dv = dalloc<float>(slots * kPerLane);
+        c.ml_sc = dalloc<float4>(slots);
+        c.ml_pack_cap = slots;
+      }
+      tracked(c, MPS_K_ML_SETUP, n * 17.0 + slots * (8.0 * kPerLane + 16), [&] {
+        hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+                           static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(nullptr),
+                           static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
+                           static_cast<const uint32_t*>(c.row_len));
+      });
+      packed = true;
+    }
+    MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));
+    tracked(c, MPS_K_CG_INIT32, n * 25.0, [&] { prec(norm, 0); });
+    long long launched = 0;
+    const long long inner = run_inner(c, limit, [&](long long it) {
+      tracked(c, MPS_K_CG_SPMV32, b_spmv, [&] {
+        const SpmvArgs<MatFreeRec> a{A,      nullptr, c.pa32,  c.q32, c.d32, c.part_ml, stride,
+                                     static_cast<int>(pgrid), p_pq, c.st32, sweep_tol};
+        if (it == 1)
+          hipLaunchKernelGGL((k_ml_spmv<true, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, a, it);
+        else
+          hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, a, it);
+      }, it);
+      tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, it); }, it);
+      launched = it;
+    });
+    if (!c.st32_host->converged && launched > 0)
+      hipLaunchKernelGGL(k_ml_finish, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
+                         static_cast<const CgStatus*>(c.st32), static_cast<const float*>(c.pa32), c.d32);
+    uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
+    uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
+    uncount(c, MPS_K_AMG_CYCLE, launched - inner, b_cycle);
     return inner;
   }, [&](double norm) {
     hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(256), 0, c.stream, n, c.sol,
@@ -1719,6 +2388,8 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
     // row length and ghost; the own record {position, z}; p, q, d read and written
     solve_mixed_ml(c, cols + n * (4 + 4 + 16 + 24.0), sr, max_iter);
+  else if (c.solver.precond == MPS_PRECOND_AMG && cells_match)
+    solve_mixed_amg(c, cols + n * (4 + 4 + 16 + 24.0), sr, max_iter);
   else
     solve_mixed_cg(c, A, cols + rows + n * vec, true, sr, max_iter);
 }
@@ -1783,7 +2454,7 @@ __global__ void __launch_bounds__(NTP) kg_ml_prec(const PrecArgs* __restrict__ t
   Frame f;
   if (!group_frame(g, s, f)) return;
   const PrecArgs a = tab[s];
-  ml_prec_body(a, it, f);
+  ml_prec_body<0>(a, it, f);
 }
 
 template <bool FIRST>
@@ -2281,7 +2952,8 @@ void ml_prepare(Context& c) {
   }
   hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
                      static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(c.agg_inv),
-                     static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc);
+                     static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
+                     static_cast<const uint32_t*>(nullptr));
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
                      c.zrec);
   MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));

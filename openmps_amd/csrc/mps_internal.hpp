@@ -48,6 +48,8 @@ struct IrStatus {
   int pad;
 };
 
+constexpr int kAmgMaxLevels = 20;  // grid levels of the cell-grid multigrid
+
 constexpr int kFeat = 132;     // warm-start feature width
 constexpr int kFeatPad = 160;  // padded to the MFMA K step (32)
 constexpr int kHid = 256;      // hidden width
@@ -195,8 +197,13 @@ struct Context {
   int16_t* col16 = nullptr;  // sell_cap entries, beside scol
   double mf_nnz_wide = 0;
   bool mf_wide_pending = false;  // mf_nnz_wide still in ppe_counters_host[2]
-  unsigned long long* ppe_counters = nullptr;       // [0] off-diagonal nnz
-  unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots, wide nnz
+  // [0] off-diagonal nnz, [1] wide nnz, [2..5] the cell box of the rows with
+  // off-diagonals (gnx - 1 - min x, max x, gnz - 1 - min z, max z: atomic maxima)
+  unsigned long long* ppe_counters = nullptr;
+  unsigned long long* ppe_counters_host = nullptr;  // pinned: nnz, slots, wide nnz, cell box [3..6]
+  uint32_t amg_box[4] = {0, 0, 0, 0};  // cells x0, z0, nx, nz of that box (the multigrid's level 0)
+  bool amg_box_ready = false;          // the box belongs to the installed system
+  uint4* box_part = nullptr;           // per-block partial boxes
   double nnz_offdiag = 0, sell_slots = 0;
 
   // CG work vectors
@@ -251,6 +258,17 @@ struct Context {
   float* dinv32 = nullptr;
   float4 *g32a = nullptr, *g32b = nullptr;  // packed (r, w, s, M) of the inner CG, by parity
   double *part3a = nullptr, *part3b = nullptr;  // 3 x kRedBlocks partial sums, by parity
+  // cell-grid multigrid (MPS_PRECOND_AMG): the level shapes (level 0 = the
+  // neighbour-search cells, then 2x2 blocks), the FP64 stencils of the set-up,
+  // the FP32 stencils, 1 / diagonal and per-point vectors of the cycle, the
+  // coarsest level's inverse, and the rows' per-direction sums (set-up)
+  int amg_levels = 0;
+  uint32_t amg_nx[kAmgMaxLevels] = {}, amg_nz[kAmgMaxLevels] = {}, amg_off[kAmgMaxLevels] = {};
+  size_t amg_pts = 0, amg_cap = 0, amg_rows_cap = 0;
+  double* amg_S64 = nullptr;
+  float *amg_S = nullptr, *amg_dinv = nullptr, *amg_r = nullptr, *amg_x = nullptr, *amg_e = nullptr;
+  float* amg_inv = nullptr;
+  double* amg_rows = nullptr;
   IrStatus* ir = nullptr;
   IrStatus* ir_host = nullptr;
   long long last_iters = 0;
@@ -442,6 +460,9 @@ void pack_state(Context& c);             // mps_particles.hip: xz, uw from x, z,
 void density_save_and_assemble(Context& c);
 void ppe_make_fp32(Context& c);
 double mf_nnz_wide(Context& c);  // mps_ppe.hip: nnz of the wide slices; valid once the stream was synchronised
+// mps_ppe.hip: the box of cells of the rows with off-diagonals (c.amg_box),
+// computed now (one synchronisation) unless the assembly left it
+void amg_box(Context& c);
 double replay_spmv(Context& c, int kid, int reps);  // mps_cg.hip: mean ms of one launch
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);

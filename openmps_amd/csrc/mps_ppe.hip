@@ -470,8 +470,108 @@ PpeConsts ppe_prepare(Context& c) {
   PpeConsts k{c.env.r_e, c.env.r_e * c.env.r_e, c.env.r_lap, c.env.r_lap * c.env.r_lap, c.env.n0,
               c.env.n0_lap, c.env.rho, c.dt, c.solver.warm_start, c.env.min_x, c.env.min_z,
               c.qpos_inv_h};
-  if (n > 0) MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 2 * sizeof(unsigned long long), c.stream));
+  if (n > 0) MPS_HIP(hipMemsetAsync(c.ppe_counters, 0, 6 * sizeof(unsigned long long), c.stream));
   return k;
+}
+
+// The box of cells holding the rows with stored off-diagonals (the unknowns
+// of the cell-grid multigrid), in two passes: per block of k_row_box the
+// extremes (minima as maxima of gn - 1 - v; 0 in the second: no row), then
+// k_row_box_final's maxima over the blocks into box[0..3].
+constexpr int kBoxBlocks = 1024;
+__global__ void __launch_bounds__(kBlock) k_row_box(size_t n, const uint32_t* __restrict__ row_len,
+                                                    const uint32_t* __restrict__ key, uint32_t gnx, uint32_t gnz,
+                                                    uint4* __restrict__ part) {
+  __shared__ uint4 lds[kBlock / 64];
+  uint4 m = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+       i += static_cast<size_t>(kBlock) * gridDim.x) {
+    if (row_len[i] == 0) continue;
+    const uint32_t k = key[i];
+    if (k == kNoCell) continue;
+    const uint32_t cx = k / gnz, cz = k % gnz;
+    m.x = max(m.x, gnx - 1 - cx);
+    m.y = max(m.y, cx + 1);
+    m.z = max(m.z, gnz - 1 - cz);
+    m.w = max(m.w, cz + 1);
+  }
+  m.x = wave_max_u32(m.x);
+  m.y = wave_max_u32(m.y);
+  m.z = wave_max_u32(m.z);
+  m.w = wave_max_u32(m.w);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 t = lds[0];
+    for (int k = 1; k < kBlock / 64; k++) {
+      t.x = max(t.x, lds[k].x);
+      t.y = max(t.y, lds[k].y);
+      t.z = max(t.z, lds[k].z);
+      t.w = max(t.w, lds[k].w);
+    }
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_row_box_final(const uint4* __restrict__ part, int nparts,
+                                                       unsigned long long* __restrict__ box) {
+  __shared__ uint4 lds[4];
+  uint4 m = make_uint4(0u, 0u, 0u, 0u);
+  for (int k = threadIdx.x; k < nparts; k += 256) {
+    const uint4 t = part[k];
+    m.x = max(m.x, t.x);
+    m.y = max(m.y, t.y);
+    m.z = max(m.z, t.z);
+    m.w = max(m.w, t.w);
+  }
+  m.x = wave_max_u32(m.x);
+  m.y = wave_max_u32(m.y);
+  m.z = wave_max_u32(m.z);
+  m.w = wave_max_u32(m.w);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 t = lds[0];
+    for (int k = 1; k < 4; k++) {
+      t.x = max(t.x, lds[k].x);
+      t.y = max(t.y, lds[k].y);
+      t.z = max(t.z, lds[k].z);
+      t.w = max(t.w, lds[k].w);
+    }
+    box[0] = t.x;
+    box[1] = t.y;
+    box[2] = t.z;
+    box[3] = t.w;
+  }
+}
+
+// Enqueues the two passes (box into ppe_counters[2..5]).
+void row_box_launch(Context& c, size_t n) {
+  if (!c.box_part) c.box_part = dalloc<uint4>(kBoxBlocks);
+  const int nb = static_cast<int>(std::min<size_t>(kBoxBlocks, std::max<size_t>(1, (n + kBlock - 1) / kBlock)));
+  hipLaunchKernelGGL(k_row_box, dim3(nb), dim3(kBlock), 0, c.stream, n, static_cast<const uint32_t*>(c.row_len),
+                     static_cast<const uint32_t*>(c.key), static_cast<uint32_t>(c.gnx),
+                     static_cast<uint32_t>(c.gnz), c.box_part);
+  hipLaunchKernelGGL(k_row_box_final, dim3(1), dim3(256), 0, c.stream, static_cast<const uint4*>(c.box_part), nb,
+                     c.ppe_counters + 2);
+}
+
+// The box of cells of the rows with off-diagonals, from the counters k_row_box
+// left in ppe_counters_host[3..6].
+void amg_box_from_counters(Context& c) {
+  const unsigned long long* h = c.ppe_counters_host + 3;
+    if (h[1] == 0) {  // no row with off-diagonals: a one-cell level
+      c.amg_box[0] = c.amg_box[1] = 0;
+      c.amg_box[2] = c.amg_box[3] = 1;
+    } else {
+      const uint32_t x0 = static_cast<uint32_t>(c.gnx - 1 - h[0]), x1 = static_cast<uint32_t>(h[1] - 1);
+      const uint32_t z0 = static_cast<uint32_t>(c.gnz - 1 - h[2]), z1 = static_cast<uint32_t>(h[3] - 1);
+      c.amg_box[0] = x0;
+      c.amg_box[1] = z0;
+      c.amg_box[2] = x1 - x0 + 1;
+      c.amg_box[3] = z1 - z0 + 1;
+    }
+  c.amg_box_ready = true;
 }
 
 // Slice offsets from the counted row lengths, then the fill.
@@ -479,6 +579,12 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   const size_t n = c.n;
   const NV nb{c.nbr, c.ncount, c.kstride};
   scan_slices(c, c.slice_width, c.nslices, c.slice_off);
+  const bool want_box = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_AMG;
+  if (want_box) {
+    row_box_launch(c, n);
+    MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 3, c.ppe_counters + 2, 4 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c.stream));
+  }
   MPS_HIP(hipMemcpyAsync(c.ppe_counters_host, c.ppe_counters, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 1, c.slice_off + c.nslices, sizeof(uint64_t),
@@ -486,6 +592,8 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.nnz_offdiag = c.ppe_counters_host[0];
   c.sell_slots = c.ppe_counters_host[1];
+  c.amg_box_ready = false;
+  if (want_box) amg_box_from_counters(c);
   // row sums of the multilevel preconditioner's aggregates, when that solver will run
   const bool want_sums = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_MULTILEVEL;
   const AggSums ag{c.key, static_cast<uint32_t>(c.gnz), want_sums ? c.z32 : nullptr, c.r32, c.d32};
@@ -543,6 +651,16 @@ void density_save_and_assemble(Context& c) {
   });
   check_launch();
   ppe_finish(c, k);
+}
+
+void amg_box(Context& c) {
+  if (c.amg_box_ready) return;
+  row_box_launch(c, c.ppe_n);
+  check_launch();
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 3, c.ppe_counters + 2, 4 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipStreamSynchronize(c.stream));
+  amg_box_from_counters(c);
 }
 
 double mf_nnz_wide(Context& c) {

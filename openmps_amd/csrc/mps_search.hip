@@ -421,6 +421,12 @@ static void permute_to_cell_order(Context& c) {
   std::swap(c.orig, c.spare_orig);
   std::swap(c.key, c.spare_key);
   c.order_serial++;
+  // a system assembled over the particles refers to the previous order
+  if (c.have_ppe && c.ppe_n == c.n) {
+    c.have_ppe = false;
+    c.have_ppe32 = false;
+    c.ppe_from_particles = false;
+  }
 }
 
 void search(Context& c) {

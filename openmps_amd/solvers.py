@@ -9,6 +9,10 @@ Each preset is (mixed precision, warm start, preconditioner of the FP32 CG):
   mixed_ml        FP32 CG with the multilevel aggregation preconditioner
                   inside FP64 iterative refinement;
   mixed_ml_zero   the same from a zero guess;
+  mixed_amg       FP32 CG with the cell-grid multigrid preconditioner (Jacobi
+                  plus one V-cycle on the cells' Galerkin operator) inside FP64
+                  iterative refinement (mixed_amg_zero: from a zero guess);
+  learned_amg     mixed_amg warm-started by the learned model;
   learned_ml      mixed_ml warm-started by the learned model;
   learned_mixed   mixed_jacobi warm-started by the learned model.
 """
@@ -26,10 +30,14 @@ PRESETS = {
     "mixed_ml_zero": (True, "zero", "multilevel"),
     "learned_ml": (True, "learned", "multilevel"),
     "learned_mixed": (True, "learned", "jacobi"),
+    "mixed_amg": (True, "previous", "amg"),
+    "mixed_amg_zero": (True, "zero", "amg"),
+    "learned_amg": (True, "learned", "amg"),
 }
 
 _WARM = {"previous": L.WARM_PREVIOUS, "zero": L.WARM_ZERO, "learned": L.WARM_LEARNED}
-_PREC = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL}
+_PREC = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL,
+         "amg": L.PRECOND_AMG}
 
 
 def configure(computer, preset: str, tol: float, weights: str | None = None, model=None) -> None:

@@ -13,7 +13,8 @@ system are equal bit for bit, and the two solutions, each meeting the rule,
 satisfy |A (p - p64)| <= 2e-8 |b| (the difference of two residuals; the
 pressure difference itself is up to the condition number times that: 1e-4
 to 2e-3 relative on the tall static columns).  Later steps report the error
-against the FP64 run and bound it loosely.
+against the FP64 run and bound it loosely (1e-6 on the dam breaks, 5e-2 on
+the static columns).
 """
 import sys
 
@@ -69,9 +70,11 @@ def compare(cfgs, preset, steps, model=None):
                 f"positions {dx:.2e} l0")
             assert st["rel_residual"] <= TOL and res <= TOL * 1.01
             # later steps: the static columns' pressure oscillates from step to
-            # step and their systems have condition numbers near 1e5; two FP64
-            # runs at 1e-8 and 1e-10 differ as much (the batch64 bench's control)
-            assert rel_p < 1e-2
+            # step and their systems have condition numbers near 1e5, so a
+            # 1e-8 residual leaves up to ~1e-3 in p per step, which the next
+            # steps carry on; two FP64 runs at 1e-8 and 1e-10 differ as much
+            # (the batch64 bench's control).  The dam breaks stay close.
+            assert rel_p < (1e-6 if cfgs[s]["kind"] == "dambreak" else 5e-2)
             assert dx < 1e-3
             if k == 0:
                 cg, ig = g.neighbors()

@@ -76,11 +76,12 @@ def fp64_solution(g, x0, rule=L.STOP_RHS_NORM):
     return g.aux("ppe_x")
 
 
-PRECONDS = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL}
+PRECONDS = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL,
+            "amg": L.PRECOND_AMG}
 
 
 @pytest.mark.parametrize("order", ["natural", "half", "shuffled"])
-@pytest.mark.parametrize("precond", ["jacobi", "none", "multilevel"])
+@pytest.mark.parametrize("precond", ["jacobi", "none", "multilevel", "amg"])
 def test_mixed_matrix_free_meets_fp64_rule(order, precond):
     g, _ = assembled_system(order, PRECONDS[precond])
     x0 = g.aux("ppe_x")
@@ -159,7 +160,18 @@ def test_multilevel_cuts_iterations():
     say(f"  FP32 iterations: Jacobi {it_jac}, multilevel {s['iterations']}")
     assert s["rel_residual"] <= TOL
     assert s["iterations"] < 0.7 * it_jac
-    g.stage("search")  # new cells: the assembled system no longer matches them
+    g.set_aux("ppe_x", x0)
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_AMG, tol=TOL)
+    g.stage("ppe_solve")
+    sa = g.solve_stats()
+    say(f"  FP32 iterations: cell-grid multigrid {sa['iterations']}")
+    assert sa["rel_residual"] <= TOL
+    assert sa["iterations"] < s["iterations"]
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_MULTILEVEL, tol=TOL)
+    # new cells in the kept order (a search in cell order would drop the
+    # assembled system): the assembled system no longer matches them
+    g.set_storage_order(keep=True)
+    g.stage("search")
     g.set_aux("ppe_x", x0)
     g.stage("ppe_solve")
     s2 = g.solve_stats()
