@@ -1001,6 +1001,7 @@ __global__ void __launch_bounds__(256) k_amg_up(AmgGrid g, int l) {
 // to LDS; the way down, the coarsest solve (e = inverse r) and the way up run
 // in LDS, ordered by barriers; e of level l0 goes back to global memory.
 constexpr int kAmgLdsFloats = 13 * kAmgLdsPts + kAmgDense * kAmgDense;  // 9 stencil + r, x, e, 1 / S_center per point; the inverse
+static_assert(kAmgBlockThreads >= 16 * kAmgDense, "16 lanes per row of the coarsest solve");
 struct AmgStenLds {
   const AmgGrid* g;
   const float* s;  // the stencils of the levels from l0 on, as in global memory
@@ -1022,11 +1023,27 @@ __global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l
   const AmgLevel B = g.L[g.levels - 1];
   const uint32_t nb = amg_npts(B), bo = B.off - base;
   float* const linv = lds + 13 * kAmgLdsPts;
-  for (uint32_t k = threadIdx.x; k < 9 * npts; k += kAmgBlockThreads) ls[k] = g.S[9ull * base + k];
-  for (uint32_t k = threadIdx.x; k < nb * nb; k += kAmgBlockThreads) linv[k] = g.inv[k];
-  for (uint32_t p = threadIdx.x; p < npts; p += kAmgBlockThreads) {
-    vd[p] = g.dinv[base + p];
-    if (p < n0) v.r[p] = g.r[base + p];
+  // one index space over the stencils, the inverse, 1 / S_center and r of
+  // level l0, copied eight loads per thread at a time (the loads of a batch
+  // are in flight together)
+  const uint32_t a0 = 9 * npts, a1 = a0 + nb * nb, a2 = a1 + npts, a3 = a2 + n0;
+  for (uint32_t k0 = threadIdx.x; k0 < a3; k0 += 8 * kAmgBlockThreads) {
+    float t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t k = k0 + u * kAmgBlockThreads;
+      const float* src = k < a0 ? g.S + 9ull * base + k
+                                : (k < a1 ? g.inv + (k - a0) : (k < a2 ? g.dinv + base + (k - a1) : g.r + base + (k - a2)));
+      t[u] = k < a3 ? *src : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint32_t k = k0 + u * kAmgBlockThreads;
+      if (k < a3)
+        lds[k < a0 ? k
+                   : (k < a1 ? 13 * kAmgLdsPts + (k - a0)
+                             : (k < a2 ? 12 * kAmgLdsPts + (k - a1) : 9 * kAmgLdsPts + (k - a2)))] = t[u];
+    }
   }
   __syncthreads();
   const AmgStenLds sten{&g, ls, base};
@@ -1035,10 +1052,15 @@ __global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l
     for (uint32_t P = threadIdx.x; P < n1; P += kAmgBlockThreads) amg_down_point(g, v, sten, l, P);
     __syncthreads();
   }
-  for (uint32_t p = threadIdx.x; p < nb; p += kAmgBlockThreads) {
+  {
+    // e = inverse r: 16 lanes per row (nb <= 64 rows), partial sums over q = part mod 16
+    const uint32_t p = threadIdx.x >> 4, part = threadIdx.x & 15;
     float s = 0.0f;
-    for (uint32_t q = 0; q < nb; q++) s = fmaf(linv[p * nb + q], v.r[bo + q], s);
-    v.e[bo + p] = s;
+    if (p < nb)
+      for (uint32_t q = part; q < nb; q += 16) s = fmaf(linv[p * nb + q], v.r[bo + q], s);
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) s += __shfl_xor(s, m, 16);
+    if (p < nb && part == 0) v.e[bo + p] = s;
   }
   __syncthreads();
   for (int l = g.levels - 2; l >= l0; l--) {
