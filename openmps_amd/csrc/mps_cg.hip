@@ -996,6 +996,193 @@ __global__ void __launch_bounds__(256) k_amg_up(AmgGrid g, int l) {
   if (p < amg_npts(g.L[l])) amg_up_point(g, amg_global(g), AmgStenGlobal{&g}, l, p);
 }
 
+// NLEV levels of the V-cycle in one launch each way (the level kernels
+// above cost a launch and a dependent round trip to memory apiece, ~5 us,
+// whatever their size).  Workgroup t owns tile t (T x T points) of the
+// coarsest of the NLEV + 1 levels l1 .. l1 + NLEV and the 2^k T x 2^k T points
+// below it k levels finer; each point is computed by its owner exactly as the
+// level kernels compute it.
+//   down (k_amg_down_chain): level l1 + k, k = 1 .. NLEV, on region D_k = the
+//   owned points and the halo that the coarser levels' residuals need (side
+//   s_k = 2 s_(k+1) + 2, origin o_k = 2 o_(k+1) - 1), r and 1 / S_pp in LDS;
+//   owned points write r and their children's x to global memory.
+//   up (k_amg_up_chain): level l1 + k, k = NLEV - 1 .. 0, on the owned points
+//   and a halo of one (their prolongation reads the parents' e), e in LDS;
+//   level l1 writes e to global memory.
+constexpr int kAmgChainT = 4;
+constexpr int kAmgChainLevels = 3;
+constexpr int kAmgChainThreads = 1024;
+
+__device__ __forceinline__ bool amg_in(const AmgLevel& L, int ix, int iz) {
+  return ix >= 0 && iz >= 0 && ix < static_cast<int>(L.nx) && iz < static_cast<int>(L.nz);
+}
+
+template <int T, int NLEV>
+struct AmgDown {
+  // s_k = 2 s_(k+1) + 2 with s_NLEV = T
+  __host__ __device__ static constexpr int side(int k) { return ((T + 2) << (NLEV - k)) - 2; }
+  // LDS floats before level k's r (levels 1 .. NLEV - 1; r then 1 / S_pp)
+  __host__ __device__ static constexpr int base(int k) {
+    int b = 0;
+    for (int m = 1; m < k; m++) b += 2 * side(m) * side(m);
+    return b;
+  }
+  static constexpr int floats = base(NLEV);
+};
+
+template <int T, int NLEV>
+__global__ void __launch_bounds__(kAmgChainThreads) k_amg_down_chain(AmgGrid g, int l1) {
+  using D = AmgDown<T, NLEV>;
+  static_assert(D::side(NLEV) == T && D::side(NLEV - 1) == 2 * T + 2 && D::floats > 0, "chain regions");
+  __shared__ float lds[D::floats > 0 ? D::floats : 1];
+  const AmgLevel top = g.L[l1 + NLEV];
+  const uint32_t tz_n = (top.nz + T - 1) / T;
+  const int tx = static_cast<int>(blockIdx.x / tz_n), tz = static_cast<int>(blockIdx.x % tz_n);
+#pragma unroll
+  for (int k = 1; k <= NLEV; k++) {
+    const AmgLevel F = g.L[l1 + k - 1], C = g.L[l1 + k];
+    const int sk = D::side(k), sf = D::side(k - 1);
+    const int own = T << (NLEV - k);  // owned side at this level
+    const int ox = tx * own - (sk - own) / 2, oz = tz * own - (sk - own) / 2;  // region origin
+    const int fox = 2 * ox - 1, foz = 2 * oz - 1;                            // the finer region's origin
+    const float* fr = lds + D::base(k - 1);
+    const float* fd = fr + sf * sf;
+    float* cr = lds + D::base(k);
+    float* cd = cr + sk * sk;
+    for (int i = threadIdx.x; i < sk * sk; i += kAmgChainThreads) {
+      const int qx = ox + i / sk, qz = oz + i % sk;
+      if (!amg_in(C, qx, qz)) {
+        if (k < NLEV) cr[i] = cd[i] = 0.0f;
+        continue;
+      }
+      float rr[4][4], dd[4][4], st[2][2][9];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int ix = 2 * qx - 1 + a, iz = 2 * qz - 1 + b;
+          if (k == 1) {
+            const bool in = amg_in(F, ix, iz);
+            const uint32_t q = F.off + static_cast<uint32_t>(in ? ix : 0) * F.nz + static_cast<uint32_t>(in ? iz : 0);
+            rr[a][b] = in ? g.r[q] : 0.0f;
+            dd[a][b] = in ? g.dinv[q] : 0.0f;
+          } else {
+            const int li = (ix - fox) * sf + (iz - foz);
+            rr[a][b] = fr[li];
+            dd[a][b] = fd[li];
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+          const int ix = 2 * qx + u, iz = 2 * qz + w;
+          const bool in = amg_in(F, ix, iz);
+          const uint32_t p = in ? static_cast<uint32_t>(ix) * F.nz + static_cast<uint32_t>(iz) : 0u;
+#pragma unroll
+          for (int d = 0; d < 9; d++)
+            st[u][w][d] = in ? g.S[9ull * F.off + static_cast<size_t>(d) * amg_npts(F) + p] : 0.0f;
+        }
+      const bool owned = qx >= tx * own && qx < (tx + 1) * own && qz >= tz * own && qz < (tz + 1) * own;
+      float rc = 0.0f;
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+          const int ix = 2 * qx + u, iz = 2 * qz + w;
+          if (!amg_in(F, ix, iz)) continue;
+          float sum = 0.0f;
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++)
+              sum = fmaf(st[u][w][3 * (dx + 1) + dz + 1],
+                         g.omega * dd[1 + u + dx][1 + w + dz] * rr[1 + u + dx][1 + w + dz], sum);
+          if (owned) g.x[F.off + static_cast<uint32_t>(ix) * F.nz + static_cast<uint32_t>(iz)] =
+              g.omega * dd[1 + u][1 + w] * rr[1 + u][1 + w];
+          rc += rr[1 + u][1 + w] - sum;
+        }
+      const uint32_t qc = C.off + static_cast<uint32_t>(qx) * C.nz + static_cast<uint32_t>(qz);
+      if (owned) g.r[qc] = rc;
+      if (k < NLEV) {
+        cr[i] = rc;
+        cd[i] = g.dinv[qc];
+      }
+    }
+    if (k < NLEV) __syncthreads();
+  }
+}
+
+template <int T, int NLEV>
+struct AmgUp {
+  // e of level l1 + k (k = 1 .. NLEV) on the owned points and a halo of one
+  __host__ __device__ static constexpr int side(int k) { return (T << (NLEV - k)) + 2; }
+  __host__ __device__ static constexpr int base(int k) {
+    int b = 0;
+    for (int m = NLEV; m > k; m--) b += side(m) * side(m);
+    return b;
+  }
+  static constexpr int floats = base(1) + side(1) * side(1);
+};
+
+template <int T, int NLEV>
+__global__ void __launch_bounds__(kAmgChainThreads) k_amg_up_chain(AmgGrid g, int l1) {
+  using U = AmgUp<T, NLEV>;
+  __shared__ float lds[U::floats];
+  const AmgLevel top = g.L[l1 + NLEV];
+  const uint32_t tz_n = (top.nz + T - 1) / T;
+  const int tx = static_cast<int>(blockIdx.x / tz_n), tz = static_cast<int>(blockIdx.x % tz_n);
+  {
+    const int sn = U::side(NLEV), ox = tx * T - 1, oz = tz * T - 1;
+    float* e = lds + U::base(NLEV);
+    for (int i = threadIdx.x; i < sn * sn; i += kAmgChainThreads) {
+      const int qx = ox + i / sn, qz = oz + i % sn;
+      e[i] = amg_in(top, qx, qz) ? g.e[top.off + static_cast<uint32_t>(qx) * top.nz + static_cast<uint32_t>(qz)] : 0.0f;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = NLEV - 1; k >= 0; k--) {
+    const AmgLevel F = g.L[l1 + k], C = g.L[l1 + k + 1];
+    const int own = T << (NLEV - k);
+    const int sk = k > 0 ? U::side(k) : own;
+    const int ox = tx * own - (k > 0 ? 1 : 0), oz = tz * own - (k > 0 ? 1 : 0);
+    const int sc = U::side(k + 1), cox = tx * (own / 2) - 1, coz = tz * (own / 2) - 1;
+    const float* ce = lds + U::base(k + 1);
+    float* fe = k > 0 ? lds + U::base(k) : nullptr;
+    for (int i = threadIdx.x; i < sk * sk; i += kAmgChainThreads) {
+      const int px = ox + i / sk, pz = oz + i % sk;
+      float val = 0.0f;
+      if (amg_in(F, px, pz)) {
+        const uint32_t p = static_cast<uint32_t>(px) * F.nz + static_cast<uint32_t>(pz);
+        const float di = g.dinv[F.off + p];
+        if (di != 0.0f) {
+          float xq[9], sv[9];
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+              const int kk = 3 * (dx + 1) + dz + 1;
+              const int jx = px + dx, jz = pz + dz;
+              const bool in = amg_in(F, jx, jz);
+              const uint32_t q = F.off + static_cast<uint32_t>(in ? jx : 0) * F.nz + static_cast<uint32_t>(in ? jz : 0);
+              const float ep = ce[((jx >> 1) - cox) * sc + ((jz >> 1) - coz)];
+              xq[kk] = in ? g.x[q] + ep : 0.0f;
+              sv[kk] = in ? g.S[9ull * F.off + static_cast<size_t>(kk) * amg_npts(F) + p] : 0.0f;
+            }
+          float sum = 0.0f;
+#pragma unroll
+          for (int kk = 0; kk < 9; kk++) sum = fmaf(sv[kk], xq[kk], sum);
+          val = fmaf(g.omega * di, g.r[F.off + p] - sum, xq[4]);
+        }
+        if (k == 0) g.e[F.off + p] = val;
+      }
+      if (k > 0) fe[i] = val;
+    }
+    if (k > 0) __syncthreads();
+  }
+}
+
 // Levels l0 .. levels - 1 in one workgroup (at most kAmgLdsPts points from
 // point off_l0 on): their stencils, 1 / S_center and r of level l0 are copied
 // to LDS; the way down, the coarsest solve (e = inverse r) and the way up run
@@ -2275,12 +2462,22 @@ void amg_setup(Context& c) {
 bool amg_cycle(Context& c, bool fold0) {
   const AmgGrid g = amg_grid(c);
   const int l0 = amg_block_level(c);
-  for (int l = 0; l < l0; l++)
+  // the kAmgChainLevels levels above the single-workgroup ones in one launch
+  // each way (level 0 keeps its own kernels)
+  const int l1 = (l0 - kAmgChainLevels >= 1) ? l0 - kAmgChainLevels : l0;
+  const unsigned tiles = ((c.amg_nx[l0] + kAmgChainT - 1) / kAmgChainT) * ((c.amg_nz[l0] + kAmgChainT - 1) / kAmgChainT);
+  for (int l = 0; l < l1; l++)
     hipLaunchKernelGGL(k_amg_down, dim3(grid_for(static_cast<size_t>(c.amg_nx[l + 1]) * c.amg_nz[l + 1])),
                        dim3(256), 0, c.stream, g, l);
+  if (l1 < l0)
+    hipLaunchKernelGGL((k_amg_down_chain<kAmgChainT, kAmgChainLevels>), dim3(tiles), dim3(kAmgChainThreads), 0,
+                       c.stream, g, l1);
   hipLaunchKernelGGL(k_amg_block, dim3(1), dim3(kAmgBlockThreads), 0, c.stream, g, l0);
+  if (l1 < l0)
+    hipLaunchKernelGGL((k_amg_up_chain<kAmgChainT, kAmgChainLevels>), dim3(tiles), dim3(kAmgChainThreads), 0,
+                       c.stream, g, l1);
   const bool left = fold0 && l0 >= 1;
-  for (int l = l0 - 1; l >= (left ? 1 : 0); l--)
+  for (int l = l1 - 1; l >= (left ? 1 : 0); l--)
     hipLaunchKernelGGL(k_amg_up, dim3(grid_for(static_cast<size_t>(c.amg_nx[l]) * c.amg_nz[l])), dim3(256), 0,
                        c.stream, g, l);
   return left;
