@@ -37,11 +37,12 @@ HBM_PEAK_GBS = 8000.0
 REPLAYS = 50
 # untimed steps from rest before the timed window: the flow at t = 0.05 s
 ADVANCE = 500
-# HBM bytes per launch of the dambreak_1m SpMVs measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (one run each; FETCH_SIZE scaled by 2, WRITE_SIZE
-# by 1: what the counters read for a copy kernel of known size), recorded with
-# their commands in profiles/dambreak_1m_step_kernels.md
-TRAFFIC = {"cg_spmv32": 116.3e6, "cg_spmv": 389.0e6}
+# HBM bytes per launch of the dambreak_1m SpMVs in the timed window (the
+# developed flow) measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one
+# run each; FETCH_SIZE scaled by 2, WRITE_SIZE by 1: what the counters read for
+# a copy kernel of known size), recorded with their commands in
+# profiles/dambreak_1m_developed_kernels.md
+TRAFFIC = {"cg_spmv32": 122.1e6, "cg_spmv": 439.0e6}
 
 
 def log(*a):

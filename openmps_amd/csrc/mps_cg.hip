@@ -1493,7 +1493,10 @@ __global__ void __launch_bounds__(NTP) k_ml_prec_up(PrecArgs a, long long it, Am
 
 // Per aggregated row i (in an aggregate: not dummy / disabled, stored
 // off-diagonals, in a cell): its diagonal and off-diagonals to aggregated
-// columns, summed by the direction of the column's cell (9 sums, FP64).
+// columns, summed by the direction of the column's cell (9 sums, FP64, at
+// rows[9 i + d]).  The fill computes the same sums while it stores the row
+// when the system is assembled for this solver; this kernel serves a system
+// assembled for another one.
 __global__ void __launch_bounds__(NT, 8) k_amg_rows(Sell<double> A, const uint32_t* __restrict__ row_len,
                                                     const uint32_t* __restrict__ key,
                                                     const uint8_t* __restrict__ type, uint32_t gnz,
@@ -1522,7 +1525,7 @@ __global__ void __launch_bounds__(NT, 8) k_amg_rows(Sell<double> A, const uint32
       }
     }
 #pragma unroll
-    for (int d = 0; d < 9; d++) rows[d * A.n + i] = acc[d];
+    for (int d = 0; d < 9; d++) rows[9 * i + d] = acc[d];
   }
 }
 
@@ -1544,7 +1547,7 @@ __global__ void __launch_bounds__(256) k_amg_cells(uint32_t x0, uint32_t z0, uin
   for (uint32_t k = 0; k < cnt; k++) {
     const uint32_t i = sorted[s0 + k];
 #pragma unroll
-    for (int d = 0; d < 9; d++) acc[d] += rows[d * n + i];
+    for (int d = 0; d < 9; d++) acc[d] += rows[9ull * i + d];
   }
 #pragma unroll
   for (int d = 0; d < 9; d++) S[d * npts + pt] = acc[d];
@@ -2416,20 +2419,19 @@ void amg_setup(Context& c) {
     c.amg_cap = off;
   }
   if (!c.amg_inv) c.amg_inv = dalloc<float>(kAmgDense * kAmgDense);
-  if (c.amg_rows_cap < n) {
-    dfree(c.amg_rows);
-    c.amg_rows = dalloc<double>(9 * n);
-    c.amg_rows_cap = n;
-  }
+  ensure_amg_rows(c, n);
+  const bool rows_ready = c.amg_rows_ready;  // left by the fill
   const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
-  tracked(c, MPS_K_ML_SETUP, c.nnz_offdiag * 16.0 + n * (17.0 + 72) + npts0 * 8.0 + off * 9 * 12.0, [&] {
+  tracked(c, MPS_K_ML_SETUP, (rows_ready ? 0.0 : c.nnz_offdiag * 16.0 + n * 17.0) + n * 72.0 + npts0 * 8.0 +
+                                 off * 9 * 12.0, [&] {
     hipLaunchKernelGGL(k_ml_active, dim3(grid_for(mg.nblk)), dim3(256), 0, c.stream, G, c.ml_flag);
     scan_counts(c, c.ml_flag, mg.nblk, c.ml_off, c.ml_active);
     hipLaunchKernelGGL(k_ml_compact, dim3(grid_for(mg.nblk)), dim3(256), 0, c.stream, static_cast<uint32_t>(mg.nblk),
                        static_cast<const uint32_t*>(c.ml_flag), static_cast<const uint32_t*>(c.ml_off), c.ml_list);
-    hipLaunchKernelGGL(k_amg_rows, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
-                       static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
-                       static_cast<const uint8_t*>(c.type), static_cast<uint32_t>(c.gnz), c.amg_rows);
+    if (!rows_ready)
+      hipLaunchKernelGGL(k_amg_rows, dim3(L.grid), dim3(NT), 0, c.stream, sell<double>(c, c.sval, c.diag, L.rpb),
+                         static_cast<const uint32_t*>(c.row_len), static_cast<const uint32_t*>(c.key),
+                         static_cast<const uint8_t*>(c.type), static_cast<uint32_t>(c.gnz), c.amg_rows);
     hipLaunchKernelGGL(k_amg_cells, dim3(grid_for(npts0)), dim3(256), 0, c.stream, c.amg_box[0], c.amg_box[1],
                        c.amg_box[2], c.amg_box[3], static_cast<uint32_t>(c.gnz),
                        static_cast<const uint32_t*>(c.cell_start), static_cast<const uint32_t*>(c.cell_count),
@@ -2508,21 +2510,35 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
   uint32_t nact = 0;
   size_t pgrid = 1;
   bool packed = false;
-  double b_prec = n * 16.0;
+  // per pass over the particles (k_ml_prec<1>, then k_ml_prec_up or
+  // k_ml_prec<2>): r, q or z per particle, the packed ids and Jacobi factors
+  // per slot; the cycle: r, x, e and the stencils of every level, twice
+  double b_pass = n * 12.0;
   const double b_cycle = static_cast<double>(c.amg_pts) * 2 * (36 + 16);
+  // it == 0 (the sweep's start) is accounted as a whole by the caller
   auto prec = [&](double norm, long long it) {
     PrecArgs a{G,         c.ml_list, nact,  c.ml_pid, c.ml_dv,    c.ml_sc, c.r,   1.0 / norm, c.r32,
                c.d32,     c.q32,     c.dinv32, zval, 4u,         p_pq,    L.grid, c.part_ml,  stride,
                c.st32,    c.amg_r,   c.amg_e,   kAmgOmega, c.row_len, c.amg_box[0], c.amg_box[1],
                c.amg_box[2], c.amg_box[3]};
-    hipLaunchKernelGGL(k_ml_prec<1>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    auto track = [&](int kid, double bytes, auto&& f) {
+      if (it == 0)
+        f();
+      else
+        tracked(c, kid, bytes, f, it);
+    };
+    track(MPS_K_ML_PREC, b_pass, [&] {
+      hipLaunchKernelGGL(k_ml_prec<1>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    });
     bool left = false;
-    tracked(c, MPS_K_AMG_CYCLE, b_cycle, [&] { left = amg_cycle(c, true); }, it);
-    if (left)
-      hipLaunchKernelGGL(k_ml_prec_up, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it,
-                         amg_grid(c));
-    else
-      hipLaunchKernelGGL(k_ml_prec<2>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    track(MPS_K_AMG_CYCLE, b_cycle, [&] { left = amg_cycle(c, true); });
+    track(MPS_K_ML_PREC, b_pass, [&] {
+      if (left)
+        hipLaunchKernelGGL(k_ml_prec_up, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it,
+                           amg_grid(c));
+      else
+        hipLaunchKernelGGL(k_ml_prec<2>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
+    });
   };
   double b_spmv = b_base;
   c.replay_ready = MPS_K_CG_SPMV32;
@@ -2532,7 +2548,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
     nact = static_cast<uint32_t>(*c.ml_active_host);
     b_spmv = b_base + 2.0 * mf_nnz_wide(c);
     pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
-    b_prec = n * 24.0 + static_cast<double>(nact) * 64 * (2 * 8.0 * kPerLane + 16);
+    b_pass = n * 12.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 8);
     if (!packed) {
       const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
       if (c.ml_pack_cap < slots) {
@@ -2564,14 +2580,14 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
         else
           hipLaunchKernelGGL((k_ml_spmv<false, MatFreeRec>), dim3(L.grid), dim3(NT), 0, c.stream, a, it);
       }, it);
-      tracked(c, MPS_K_ML_PREC, b_prec, [&] { prec(norm, it); }, it);
+      prec(norm, it);
       launched = it;
     });
     if (!c.st32_host->converged && launched > 0)
       hipLaunchKernelGGL(k_ml_finish, dim3(grid_for(n)), dim3(256), 0, c.stream, n,
                          static_cast<const CgStatus*>(c.st32), static_cast<const float*>(c.pa32), c.d32);
     uncount(c, MPS_K_CG_SPMV32, launched - inner, b_spmv);
-    uncount(c, MPS_K_ML_PREC, launched - inner, b_prec);
+    uncount(c, MPS_K_ML_PREC, 2 * (launched - inner), b_pass);
     uncount(c, MPS_K_AMG_CYCLE, launched - inner, b_cycle);
     return inner;
   }, [&](double norm) {
@@ -2945,6 +2961,15 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
 }
 
 }  // namespace
+
+void ensure_amg_rows(Context& c, size_t n) {
+  if (c.amg_rows_cap < n) {
+    dfree(c.amg_rows);
+    c.amg_rows = dalloc<double>(9 * n);
+    c.amg_rows_cap = n;
+    c.amg_rows_ready = false;
+  }
+}
 
 // Re-launches the last solve's SpMV on its operands with a status block that
 // never stops (tolerance -1, unit scalars, alpha 0: the correction d is left

@@ -241,6 +241,7 @@ struct Context {
   int replay_ready = 0;  // SpMV the last solve ran (MPS_K_CG_SPMV32 / MPS_K_CG_SPMV), 0: none
   CgStatus* st_replay = nullptr;
   bool ml_rowsums_ready = false;  // the last fill left the aggregate row sums in z32, r32, d32
+  bool amg_rows_ready = false;    // the last fill left the multigrid's row direction sums in amg_rows
   uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
   uint32_t* ml_off = nullptr;
   uint32_t* ml_list = nullptr;  // ... and their indices, ascending
@@ -463,6 +464,8 @@ double mf_nnz_wide(Context& c);  // mps_ppe.hip: nnz of the wide slices; valid o
 // mps_ppe.hip: the box of cells of the rows with off-diagonals (c.amg_box),
 // computed now (one synchronisation) unless the assembly left it
 void amg_box(Context& c);
+// c.amg_rows holds 9 doubles for each of n rows (mps_cg.hip)
+void ensure_amg_rows(Context& c, size_t n);
 double replay_spmv(Context& c, int kid, int reps);  // mps_cg.hip: mean ms of one launch
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);

@@ -220,11 +220,38 @@ struct AggSums {
   const uint32_t* __restrict__ key;
   uint32_t gnz;
   float *s0, *s1, *s2;
+  // the cell-grid multigrid's per-row sums (rows9 != nullptr; the fill's
+  // ROWS9 instance): per row i with stored off-diagonals, not dummy or
+  // disabled, in a cell: A_ii + the stored A_ij whose column is such a row
+  // too, summed by the direction of the column's cell (d = 3 (dx + 1) +
+  // dz + 1), in stored order; 0 elsewhere; FP64, rows9[9 i + d]
+  double* rows9;
+  const uint32_t* __restrict__ aggkey;  // ROWS9: the cell of each such row, kNoCell for the others
 };
+
+// The cell of each row that the multigrid aggregates (stored off-diagonals,
+// not dummy or disabled, in a cell), kNoCell for the others: one gather per
+// stored column in the fill instead of three.
+__global__ void __launch_bounds__(kBlock) k_agg_key(size_t n, const uint32_t* __restrict__ key,
+                                                    const uint8_t* __restrict__ type,
+                                                    const uint32_t* __restrict__ row_len, uint32_t* __restrict__ out) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t t = type[i];
+  out[i] = (t != kDummy && t != kDisabled && row_len[i] > 0) ? key[i] : kNoCell;
+}
 
 constexpr int kStage = 40;  // emitted columns per row staged in LDS for the merge
 
+// A stored value and the direction of its column's cell (-1: not summed).
+struct DirA {
+  int dir;
+  double a;
+};
+__device__ __forceinline__ DirA make_pair_da(int dir, double a) { return DirA{dir, a}; }
+
 // One row of k_ppe_fill; kk returns its stored non-zeros.
+template <bool ROWS9>
 __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __restrict__ x,
                                          const double* __restrict__ z, const uint8_t* __restrict__ type, NV nb,
                                          const PpeConsts& k, const double* __restrict__ nws,
@@ -243,6 +270,9 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
   kk = 0;
   double d, g = 0.0;
   double a0 = 0, a1 = 0, a2 = 0;  // aggregate row sums (AggSums)
+  double r9[9];                    // ROWS9
+#pragma unroll
+  for (int u = 0; u < 9; u++) r9[u] = 0.0;
   if (ti == kDummy || ti == kDisabled) {
     d = 1.0;
     g = 1.0;
@@ -304,6 +334,9 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
     const uint32_t ki = ag.s0 ? ag.key[i] : kNoCell;
     const uint32_t cx = ki / ag.gnz, cz = ki % ag.gnz;
     if (ki != kNoCell) a0 = a1 = a2 = d;
+    // ROWS9: the row's cell when it takes part (its row length is known from the count pass)
+    const uint32_t k9 = ROWS9 ? ag.aggkey[i] : kNoCell;
+    const int c9x = static_cast<int>(k9 / ag.gnz), c9z = static_cast<int>(k9 % ag.gnz);
     // value of an emitted column j in slot kk (and its share of the aggregate row sums)
     auto emit = [&](uint32_t j) {
       const double2 pj = xz[j];
@@ -329,6 +362,34 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
             }
           }
         }
+      }
+      int dir = -1;
+      if (ROWS9 && k9 != kNoCell) {
+        const uint32_t kj = ag.aggkey[j];
+        if (kj != kNoCell) dir = 3 * (static_cast<int>(kj / ag.gnz) - c9x + 1) + static_cast<int>(kj % ag.gnz) - c9z + 1;
+      }
+      return make_pair_da(dir, a);
+    };
+    // consecutive columns of one direction are summed in a run and the run
+    // added to its direction's sum when the direction changes (in cell order
+    // the ascending columns visit the directions in ascending order: one run
+    // each); direction 4's first run starts from the diagonal
+    int cur = -1;
+    double run = 0.0;
+    bool seen4 = false;
+    auto flush9 = [&]() {
+#pragma unroll
+      for (int u = 0; u < 9; u++) r9[u] += (u == cur) ? run : 0.0;
+    };
+    auto add9 = [&](const DirA& e) {
+      if (ROWS9 && e.dir >= 0) {
+        if (e.dir != cur) {
+          if (cur >= 0) flush9();
+          cur = e.dir;
+          run = (cur == 4 && !seen4) ? d : 0.0;
+          seen4 = seen4 || cur == 4;
+        }
+        run += e.a;
       }
     };
     if (staged) {
@@ -371,14 +432,18 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
         col[base + static_cast<size_t>(v) * 64] = best;
       }
 #pragma unroll 4
-      for (uint32_t v = 0; v < ne; v++) emit(col[base + static_cast<size_t>(v) * 64]);
+      for (uint32_t v = 0; v < ne; v++) add9(emit(col[base + static_cast<size_t>(v) * 64]));
     } else {
       for (uint32_t q = 0; q < m; q++) {
         const uint32_t j = nb.ids[nbr_slot(self, q, nb.kmax)];
         const double2 pj = xz[j];
         const double dx = pj.x - xi, dz = pj.y - zi;
-        if (dx * dx + dz * dz < k.rl2 && type[j] != kDummy) emit(j);
+        if (dx * dx + dz * dz < k.rl2 && type[j] != kDummy) add9(emit(j));
       }
+    }
+    if (ROWS9 && k9 != kNoCell) {
+      if (cur >= 0) flush9();
+      if (!seen4) r9[4] = d;
     }
   }
   const uint32_t stored = kk;
@@ -396,9 +461,16 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
     ag.s1[i] = static_cast<float>(a1);
     ag.s2[i] = static_cast<float>(a2);
   }
+  if (ROWS9) {
+#pragma unroll
+    for (int u = 0; u < 9; u++) ag.rows9[9 * i + u] = r9[u];
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __restrict__ x,
+// the stage array limits a CU to four blocks (four wavefronts per SIMD): the
+// registers of that occupancy are available
+template <bool ROWS9>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) k_ppe_fill(size_t n, const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      const uint8_t* __restrict__ type, NV nb,
                                                      PpeConsts k, const double* __restrict__ nws,
@@ -417,8 +489,9 @@ __global__ void __launch_bounds__(kBlock) k_ppe_fill(size_t n, const double* __r
   // (one wavefront of rows) is "wide" when any of its offsets does not fit
   bool fits = true;
   uint32_t kk = 0;
-  if (i < n) fill_row(n, i, x, z, type, nb, k, nws, slice_off, slice_width, val, col, col16, diag, ghost32, ag,
-                      stage, fits, kk, xz);
+  if (i < n)
+    fill_row<ROWS9>(n, i, x, z, type, nb, k, nws, slice_off, slice_width, val, col, col16, diag, ghost32, ag,
+                    stage, fits, kk, xz);
   const unsigned long long far = __ballot(!fits);
   const double wsum = wave_sum(static_cast<double>(kk));
   if ((threadIdx.x & 63) == 0) {
@@ -596,15 +669,33 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   if (want_box) amg_box_from_counters(c);
   // row sums of the multilevel preconditioner's aggregates, when that solver will run
   const bool want_sums = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_MULTILEVEL;
-  const AggSums ag{c.key, static_cast<uint32_t>(c.gnz), want_sums ? c.z32 : nullptr, c.r32, c.d32};
+  // the cell-grid multigrid's per-row direction sums, when that solver will run
+  const bool want9 = want_box;
+  if (want9) {
+    // the aggregated rows' cells into spare_key (free between permutations)
+    ensure_amg_rows(c, n);
+    hipLaunchKernelGGL(k_agg_key, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       static_cast<const uint32_t*>(c.key), static_cast<const uint8_t*>(c.type),
+                       static_cast<const uint32_t*>(c.row_len), c.spare_key);
+  }
+  const AggSums ag{c.key,  static_cast<uint32_t>(c.gnz),    want_sums ? c.z32 : nullptr, c.r32, c.d32,
+                   c.amg_rows, static_cast<const uint32_t*>(c.spare_key)};
   const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 14.0 +
-                       (want_sums ? n * 12.0 + c.nnz_offdiag * 4.0 : 0.0);
+                       (want_sums ? n * 12.0 + c.nnz_offdiag * 4.0 : 0.0) +
+                       (want9 ? n * (72.0 + 4) + c.nnz_offdiag * 4.0 : 0.0);  // k_agg_key's 13 N apart
   tracked(c, MPS_K_PPE_FILL, bytes, [&] {
-    hipLaunchKernelGGL(k_ppe_fill, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
-                       static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
-                       static_cast<const uint64_t*>(c.slice_off),
-                       static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
-                       c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
+    if (want9)
+      hipLaunchKernelGGL(k_ppe_fill<true>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
+                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                         static_cast<const uint64_t*>(c.slice_off),
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
+                         c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
+    else
+      hipLaunchKernelGGL(k_ppe_fill<false>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
+                         static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
+                         static_cast<const uint64_t*>(c.slice_off),
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
+                         c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
   });
   check_launch();
   // the wide nnz (byte accounting only) reaches the host with the solver's
@@ -613,6 +704,7 @@ void ppe_finish(Context& c, const PpeConsts& k) {
                          hipMemcpyDeviceToHost, c.stream));
   c.mf_wide_pending = true;
   c.ml_rowsums_ready = want_sums;
+  c.amg_rows_ready = want9;
   c.ppe_from_particles = true;
   c.ppe_search_serial = c.search_serial;
 }

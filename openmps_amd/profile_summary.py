@@ -32,13 +32,16 @@ KERNEL_IDS = {
     "k_density<true, false, true>": "density_save", "k_explicit": "explicit",
     "k_ppe_count": "ppe_count", "k_ppe_fill": "ppe_fill", "k_cg_init": "cg_init",
     "k_cg_spmv<false>": "cg_spmv", "k_cg_spmv<true>": "cg_spmv", "k_cg_update": "cg_update",
+    "k_cg_spmv<false, mps::(anonymous namespace)::Sell<double> >": "cg_spmv",
+    "k_cg_spmv<true, mps::(anonymous namespace)::Sell<double> >": "cg_spmv",
     "k_gradient": "gradient", "k_ds": "ds",
     "k_ml_spmv<false, mps::(anonymous namespace)::MatFree>": "cg_spmv32",
     "k_ml_spmv<true, mps::(anonymous namespace)::MatFree>": "cg_spmv32",
     "k_ml_spmv<false, mps::(anonymous namespace)::MatFreeRec>": "cg_spmv32",
     "k_ml_spmv<true, mps::(anonymous namespace)::MatFreeRec>": "cg_spmv32",
     "k_density_explicit": "density_ecs", "k_density_ppe_count": "density_save",
-    "k_ml_prec": "ml_prec",
+    "k_ml_prec": "ml_prec", "k_ml_prec<0>": "ml_prec", "k_ml_prec<1>": "ml_prec", "k_ml_prec<2>": "ml_prec",
+    "k_ml_prec_up": "ml_prec",
     "k_cgf_iter<mps::(anonymous namespace)::MatFree>": "cg_iter32",
     "k_features<unsigned char>": "features", "k_features<unsigned short>": "features",
 }
@@ -61,6 +64,7 @@ FORMULAS = {
     # record of the operator's row; 2-byte column offsets
     "k_learn_resid32": lambda N, Z: 97 * N + 2 * Z,
     "k_cg_init": lambda N, Z: 12 * Z + 32 * N,
+    "k_cg_init<mps::(anonymous namespace)::Sell<double> >": lambda N, Z: 12 * Z + 32 * N,
     # sorted ids and x, z read, the cell-ordered positions written (k_cell_sort
     # shares the library's statistic with it and is not shown in bytes)
     "k_cell_pos": lambda N, Z: 36 * N,
