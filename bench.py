@@ -68,7 +68,7 @@ def run(c, steps: int, label: str, every: int = 1):
         stats.append(s)
         if (k + 1) % every == 0 or k + 1 == steps:
             log(f"  [{label}] step {k + 1}/{steps}: CG {s['iterations']} it "
-                f"({s['outer_iterations']} sweeps), solve {s['solve_ms']:.1f} ms, "
+                f"({s['outer_iterations']} sweeps, precond {s.get('precond_used', 0)}), solve {s['solve_ms']:.1f} ms, "
                 f"|b-Ax|/|b| {s['rel_residual']:.2e} (from {s['initial_rel_residual']:.2e})")
     return time.perf_counter() - t0, stats
 
@@ -276,6 +276,10 @@ def summary(stats, elapsed=None):
            "initial_rel_residual": sum(s["initial_rel_residual"] for s in stats) / len(stats),
            "rel_residual": max(s["rel_residual"] for s in stats),
            "learned_used": sum(s["learned_used"] for s in stats)}
+    names = {0: "none", 1: "jacobi", 2: "multilevel", 3: "amg"}
+    used = [names.get(s.get("precond_used", 0), "?") for s in stats]
+    if any(u in ("multilevel", "amg") for u in used):
+        out["preconditioner_steps"] = {u: used.count(u) for u in sorted(set(used))}
     if elapsed is not None:
         out["step_ms"] = elapsed / len(stats) * 1e3
     return out
@@ -335,9 +339,9 @@ def main() -> int:
     ap.add_argument("--workload", default="dambreak_1m")
     from openmps_amd import solvers
     ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
-                    help="headline solver (default: mixed_amg, FP32 CG with the cell-grid multigrid "
-                         "preconditioner inside FP64 refinement; learned_ml, the multilevel one with the "
-                         "learned warm start, for batch64)")
+                    help="headline solver (default: mixed_auto, FP32 CG with the faster of the "
+                         "multilevel and the cell-grid multigrid preconditioners inside FP64 refinement; "
+                         "learned_ml, the multilevel one with the learned warm start, for batch64)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
     ap.add_argument("--ablation", action="store_true", help="also time further solver presets")
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -356,7 +360,7 @@ def main() -> int:
         return 2
     if args.workload == "batch64":
         return run_batch(args)
-    args.solver = args.solver or "mixed_amg"
+    args.solver = args.solver or "mixed_auto"
     from openmps_amd import scenes
     from openmps_amd import learned as _learned
     have_weights = os.path.exists(args.weights or _learned.DEFAULT_WEIGHTS)
@@ -399,7 +403,8 @@ def main() -> int:
     run(main_c, W - 1, f"{args.solver} warmup")
     elapsed, st = run(main_c, K, f"{args.solver} timed")
     kt = profiled_table(main_c, K, args.solver)
-    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32", "amg": "cg_spmv32"}[
+    dom = {"none": "cg_iter32", "jacobi": "cg_iter32", "multilevel": "cg_spmv32", "amg": "cg_spmv32",
+           "auto": "cg_spmv32"}[
         solvers.preconditioner(args.solver)] if solvers.is_mixed(args.solver) else "cg_spmv"
     replay = {}
     if dom in ("cg_spmv32", "cg_spmv"):
@@ -413,8 +418,9 @@ def main() -> int:
     gpu64 = {}
     others = []
     prec = solvers.preconditioner(args.solver)
-    learned_v = {"amg": "learned_amg", "multilevel": "learned_ml", "jacobi": "learned_mixed"}.get(prec)
-    zero_v = {"amg": "mixed_amg_zero", "multilevel": "mixed_ml_zero"}.get(prec)
+    learned_v = {"amg": "learned_amg", "multilevel": "learned_ml", "jacobi": "learned_mixed",
+                 "auto": "learned_auto"}.get(prec)
+    zero_v = {"amg": "mixed_amg_zero", "multilevel": "mixed_ml_zero", "auto": "mixed_auto_zero"}.get(prec)
     if not args.no_compare:
         if args.solver != "fp64":
             others.append("fp64")

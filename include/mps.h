@@ -153,7 +153,14 @@ enum mps_precond {
    * Galerkin operator of the cells (a 9-point stencil on the cell grid) and
    * of 2x2 blocks of cells below it, added to the damped Jacobi step; same
    * requirements as MPS_PRECOND_MULTILEVEL, and uses Jacobi otherwise */
-  MPS_PRECOND_AMG = 3
+  MPS_PRECOND_AMG = 3,
+  /* the faster of MPS_PRECOND_MULTILEVEL and MPS_PRECOND_AMG for the flow at
+   * hand (the first is cheaper per iteration, the second needs far fewer
+   * iterations once the flow has developed): the first two solves time one
+   * each (solve_ms), the faster is used and the other timed again after 16
+   * solves, then after 32, 64, ... 1024 while it keeps losing; decided when
+   * the system is assembled; mps_solve_stats.precond_used reports it */
+  MPS_PRECOND_AUTO = 4
 };
 typedef struct mps_solver_opts {
   int kind;          /* mps_solver_kind */
@@ -180,6 +187,7 @@ typedef struct mps_solve_stats {
   double prev_rel_residual; /* |b - A p_prev| / |b| when the learned guess was evaluated (A p_prev in
                                FP32 for systems assembled from particles) */
   int64_t fallback;         /* mixed: 1 when a refinement sweep stalled and the FP64 CG finished */
+  int64_t precond_used;     /* mixed: the preconditioner the FP32 CG ran with (mps_precond) */
 } mps_solve_stats;
 int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out);
 

@@ -32,7 +32,7 @@ WARM_PREVIOUS, WARM_ZERO, WARM_LEARNED = 0, 1, 2
 AUX = {"n_without_spp": 0, "ecs": 1, "saved_x": 2, "saved_z": 3, "du_x": 4, "du_z": 5,
        "ppe_b": 6, "ppe_x": 7}
 
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL, PRECOND_AMG = 0, 1, 2, 3
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MULTILEVEL, PRECOND_AMG, PRECOND_AUTO = 0, 1, 2, 3, 4
 ORDER_CELL, ORDER_KEEP = 0, 1
 
 KERNELS = ("cell_keys", "cell_scan", "cell_scatter", "cell_sort", "neighbors", "density_ecs",
@@ -57,7 +57,7 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", _I64), ("outer_iterations", _I64), ("rr0", _D), ("rr", _D),
                 ("rel_residual", _D), ("initial_rel_residual", _D), ("solve_ms", _D),
                 ("learned_used", _I64), ("prev_rel_residual", _D),
-                ("fallback", _I64)]
+                ("fallback", _I64), ("precond_used", _I64)]
 
 
 class WallMotion(ctypes.Structure):

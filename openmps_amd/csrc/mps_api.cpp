@@ -640,8 +640,16 @@ int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts) {
   if (opts->kind < 0 || opts->kind > MPS_SOLVER_MIXED_IR) return MPS_ERR_BAD_ARG;
   if (opts->stop_rule < 0 || opts->stop_rule > MPS_STOP_RHS_NORM) return MPS_ERR_BAD_ARG;
   if (opts->warm_start < 0 || opts->warm_start > MPS_WARM_LEARNED) return MPS_ERR_BAD_ARG;
-  if (opts->precond < 0 || opts->precond > MPS_PRECOND_AMG) return MPS_ERR_BAD_ARG;
-  ctx->c.solver = *opts;
+  if (opts->precond < 0 || opts->precond > MPS_PRECOND_AUTO) return MPS_ERR_BAD_ARG;
+  mps::Context& c = ctx->c;
+  if (opts->precond == MPS_PRECOND_AUTO && c.solver.precond != MPS_PRECOND_AUTO) {
+    // the automatic choice starts over: both preconditioners are timed again
+    c.auto_ms[0] = c.auto_ms[1] = -1.0;
+    c.auto_best = MPS_PRECOND_MULTILEVEL;
+    c.auto_solves = c.auto_next = 0;
+    c.auto_interval = 16;
+  }
+  c.solver = *opts;
   return MPS_OK;
 }
 

@@ -2050,7 +2050,7 @@ void refine(Context& c, const StopRule& sr, Sweep sweep, Correct correct) {
     const double sweep_tol = std::max(inner_tol, 0.5 * std::sqrt(target / rr));
     c.sweep_index = outer;
     const long long inner = sweep(norm, sweep_tol);
-    c.sweep_pred[std::min(outer, Context::kSweepPred - 1)] = inner;
+    c.pred(outer) = inner;
     total += inner;
     correct(norm);
     check_launch();
@@ -2081,7 +2081,7 @@ long long run_inner(Context& c, long long limit, Launch1 launch) {
   double rr_last = -1;
   int flat = 0;
   while (true) {
-    const long long chunk = next_chunk(it - 1, c.sweep_pred[std::min(c.sweep_index, Context::kSweepPred - 1)]);
+    const long long chunk = next_chunk(it - 1, c.pred(c.sweep_index));
     for (long long m = 0; m < chunk && it <= limit; m++, it++) launch(it);
     check_launch();
     read_status(c, c.st32, c.st32_host);
@@ -2610,6 +2610,8 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   // and written, p and d read and written
   const double vec = 16.0 * 2 + 4.0 * 4;
   if (!c.ppe_from_particles) {
+    c.stats.precond_used = c.solver.precond == MPS_PRECOND_NONE ? MPS_PRECOND_NONE : MPS_PRECOND_JACOBI;
+    c.pred_kind = static_cast<int>(c.stats.precond_used);
     ppe_make_fp32(c);
     solve_mixed_cg(c, sell<float>(c, c.sval32, c.diag32, L.rpb), spmv_bytes(c, 4, vec), false, sr, max_iter);
     return;
@@ -2620,10 +2622,15 @@ void solve_mixed(Context& c, const StopRule& sr, long long max_iter) {
   const double cols = c.nnz_offdiag * 2.0 + c.nslices * 25.0;
   const double rows = n * (4 + 4 + 8.0);
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
-  if (c.solver.precond == MPS_PRECOND_MULTILEVEL && cells_match)
+  const int pc = effective_precond(c);
+  c.stats.precond_used = cells_match && (pc == MPS_PRECOND_MULTILEVEL || pc == MPS_PRECOND_AMG)
+                             ? pc
+                             : (pc == MPS_PRECOND_NONE ? MPS_PRECOND_NONE : MPS_PRECOND_JACOBI);
+  c.pred_kind = static_cast<int>(c.stats.precond_used);
+  if (pc == MPS_PRECOND_MULTILEVEL && cells_match)
     // row length and ghost; the own record {position, z}; p, q, d read and written
     solve_mixed_ml(c, cols + n * (4 + 4 + 16 + 24.0), sr, max_iter);
-  else if (c.solver.precond == MPS_PRECOND_AMG && cells_match)
+  else if (pc == MPS_PRECOND_AMG && cells_match)
     solve_mixed_amg(c, cols + n * (4 + 4 + 16 + 24.0), sr, max_iter);
   else
     solve_mixed_cg(c, A, cols + rows + n * vec, true, sr, max_iter);
@@ -2863,6 +2870,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
       st[s].tol_rel = r.sweep_tol;
       vt[s].norm = r.norm;
       r.c->sweep_index = r.outer;
+      r.c->pred_kind = MPS_PRECOND_MULTILEVEL;
       any = true;
     }
     if (!any) break;
@@ -2882,7 +2890,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
     for (size_t s = 0; s < S; s++)
       if (sc[s].sweeping) {
         Context& c = *sc[s].c;
-        pred = std::max(pred, c.sweep_pred[std::min(c.sweep_index, Context::kSweepPred - 1)]);
+        pred = std::max(pred, c.pred(c.sweep_index));
         lim = std::max(lim, sc[s].limit);
       }
     while (true) {
@@ -2953,7 +2961,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
       SceneRun& r = sc[s];
       if (!r.swept) continue;
       Context& c = *r.c;
-      c.sweep_pred[std::min(r.outer, Context::kSweepPred - 1)] = r.inner;
+      c.pred(r.outer) = r.inner;
       r.total += r.inner;
       r.outer++;
     }
@@ -2961,6 +2969,24 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
 }
 
 }  // namespace
+
+int precond_for_system(Context& c) {
+  constexpr int ML = MPS_PRECOND_MULTILEVEL, MG = MPS_PRECOND_AMG;
+  int pick = c.solver.precond;
+  if (pick == MPS_PRECOND_AUTO) {
+    if (c.auto_ms[0] < 0) pick = ML;
+    else if (c.auto_ms[1] < 0) pick = MG;
+    else pick = (c.auto_solves >= c.auto_next) ? (c.auto_best == ML ? MG : ML) : c.auto_best;
+  }
+  c.precond_now = pick;
+  return pick;
+}
+
+int effective_precond(const Context& c) {
+  if (c.solver.precond != MPS_PRECOND_AUTO) return c.solver.precond;
+  return (c.precond_now == MPS_PRECOND_MULTILEVEL || c.precond_now == MPS_PRECOND_AMG) ? c.precond_now
+                                                                                         : MPS_PRECOND_MULTILEVEL;
+}
 
 void ensure_amg_rows(Context& c, size_t n) {
   if (c.amg_rows_cap < n) {
@@ -3006,6 +3032,38 @@ double replay_spmv(Context& c, int kid, int reps) {
   float ms = 0;
   MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
   return ms / reps;
+}
+
+// MPS_PRECOND_AUTO: the time of a solve with the multilevel or the multigrid
+// preconditioner; the first of each sets the choice, later ones of the other
+// (the re-timings) switch it when faster and otherwise double the interval to
+// the next re-timing (16 .. 1024 solves).
+void auto_record(Context& c, double ms) {
+  constexpr int ML = MPS_PRECOND_MULTILEVEL, MG = MPS_PRECOND_AMG;
+  const int used = static_cast<int>(c.stats.precond_used);
+  if (c.solver.precond != MPS_PRECOND_AUTO || (used != ML && used != MG)) return;
+  const bool first = c.auto_ms[0] < 0 || c.auto_ms[1] < 0;
+  const bool retime = !first && used != c.auto_best;
+  const double best_ms = c.auto_ms[c.auto_best == ML ? 0 : 1];
+  c.auto_ms[used == ML ? 0 : 1] = ms;
+  c.auto_solves++;
+  if (first) {
+    if (c.auto_ms[0] >= 0 && c.auto_ms[1] >= 0) {
+      c.auto_best = c.auto_ms[1] < c.auto_ms[0] ? MG : ML;
+      c.auto_interval = 16;
+      c.auto_next = c.auto_solves + c.auto_interval;
+    }
+    return;
+  }
+  if (retime) {
+    if (ms < best_ms) {
+      c.auto_best = used;
+      c.auto_interval = 16;
+    } else {
+      c.auto_interval = std::min<long long>(1024, 2 * c.auto_interval);
+    }
+    c.auto_next = c.auto_solves + c.auto_interval;
+  }
 }
 
 void ppe_solve(Context& c) {
@@ -3054,6 +3112,7 @@ void ppe_solve(Context& c) {
   c.stats.rel_residual = (bb > 0) ? std::sqrt(rr / bb) : std::sqrt(rr);
   c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
   if (rc != MPS_OK) throw Failure{rc, msg};
+  auto_record(c, ms);
   if (c.dataset && on_particles) collect_after(c);
 }
 
@@ -3094,7 +3153,7 @@ bool learned_residual32(Context& c, FeatRec* rec) {
 // kernels; solve_mixed_ml joins it.
 void ml_setup_ahead(Context& c) {
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
-  if (c.solver.kind != MPS_SOLVER_MIXED_IR || !c.ppe_from_particles || c.solver.precond != MPS_PRECOND_MULTILEVEL ||
+  if (c.solver.kind != MPS_SOLVER_MIXED_IR || !c.ppe_from_particles || effective_precond(c) != MPS_PRECOND_MULTILEVEL ||
       !cells_match || c.ml_setup_pending)
     return;
   if (!c.stream_aux) {

@@ -215,10 +215,15 @@ struct Context {
   CgStatus* st_host = nullptr;
   CgStatus* st32 = nullptr;
   CgStatus* st32_host = nullptr;
-  // iterations of the mixed solver's FP32 sweeps in the last solve, by sweep
+  // iterations of the mixed solver's FP32 sweeps in the last solve with each
+  // preconditioner (mps_precond up to MPS_PRECOND_AMG; pred_kind: the
+  // running solve's), by sweep
   static constexpr int kSweepPred = 8;
-  long long sweep_pred[kSweepPred] = {};
+  static constexpr int kPredKinds = 4;
+  long long sweep_pred[kPredKinds][kSweepPred] = {};
+  int pred_kind = 0;
   int sweep_index = 0;
+  long long& pred(int sweep) { return sweep_pred[pred_kind][sweep < kSweepPred - 1 ? sweep : kSweepPred - 1]; }
   double final_rr = -1, final_bb = 0;  // FP64 |b - A x|^2, |b|^2 of the solve's x (-1: not known)
   // the learned warm start's residual norms are on their way to red_host
   // (learned_resolve reads them at the solver's first synchronisation)
@@ -242,6 +247,13 @@ struct Context {
   CgStatus* st_replay = nullptr;
   bool ml_rowsums_ready = false;  // the last fill left the aggregate row sums in z32, r32, d32
   bool amg_rows_ready = false;    // the last fill left the multigrid's row direction sums in amg_rows
+  // MPS_PRECOND_AUTO: the preconditioner chosen for the assembled system, the
+  // latest solve time with each (multilevel, multigrid), the current choice,
+  // and the solve count at which the other one is timed again
+  int precond_now = MPS_PRECOND_MULTILEVEL;
+  double auto_ms[2] = {-1.0, -1.0};
+  int auto_best = MPS_PRECOND_MULTILEVEL;
+  long long auto_solves = 0, auto_next = 0, auto_interval = 16;
   uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
   uint32_t* ml_off = nullptr;
   uint32_t* ml_list = nullptr;  // ... and their indices, ascending
@@ -466,6 +478,10 @@ double mf_nnz_wide(Context& c);  // mps_ppe.hip: nnz of the wide slices; valid o
 void amg_box(Context& c);
 // c.amg_rows holds 9 doubles for each of n rows (mps_cg.hip)
 void ensure_amg_rows(Context& c, size_t n);
+// The mixed solver's preconditioner for a system about to be assembled from
+// the particles (MPS_PRECOND_AUTO picks it; mps_cg.hip), and the one in force.
+int precond_for_system(Context& c);
+int effective_precond(const Context& c);
 double replay_spmv(Context& c, int kid, int reps);  // mps_cg.hip: mean ms of one launch
 // exclusive scan of n u32 counts into out, grand total to *total (device); mps_search.hip
 void scan_counts(Context& c, const uint32_t* in, size_t n, uint32_t* out, uint64_t* total);

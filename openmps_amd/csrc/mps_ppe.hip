@@ -652,7 +652,8 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   const size_t n = c.n;
   const NV nb{c.nbr, c.ncount, c.kstride};
   scan_slices(c, c.slice_width, c.nslices, c.slice_off);
-  const bool want_box = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_AMG;
+  const int pc = c.solver.kind == MPS_SOLVER_MIXED_IR ? precond_for_system(c) : MPS_PRECOND_NONE;
+  const bool want_box = pc == MPS_PRECOND_AMG;
   if (want_box) {
     row_box_launch(c, n);
     MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 3, c.ppe_counters + 2, 4 * sizeof(unsigned long long),
@@ -668,7 +669,7 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   c.amg_box_ready = false;
   if (want_box) amg_box_from_counters(c);
   // row sums of the multilevel preconditioner's aggregates, when that solver will run
-  const bool want_sums = c.solver.kind == MPS_SOLVER_MIXED_IR && c.solver.precond == MPS_PRECOND_MULTILEVEL;
+  const bool want_sums = pc == MPS_PRECOND_MULTILEVEL;
   // the cell-grid multigrid's per-row direction sums, when that solver will run
   const bool want9 = want_box;
   if (want9) {

@@ -13,6 +13,11 @@ Each preset is (mixed precision, warm start, preconditioner of the FP32 CG):
                   plus one V-cycle on the cells' Galerkin operator) inside FP64
                   iterative refinement (mixed_amg_zero: from a zero guess);
   learned_amg     mixed_amg warm-started by the learned model;
+  mixed_auto      FP32 CG with the faster of the multilevel and the cell-grid
+                  multigrid preconditioner, timed as the flow goes (the first is
+                  faster while the flow is young, the second once it has
+                  developed) inside FP64 iterative refinement (mixed_auto_zero:
+                  from a zero guess; learned_auto: the learned start);
   learned_ml      mixed_ml warm-started by the learned model;
   learned_mixed   mixed_jacobi warm-started by the learned model.
 """
@@ -33,11 +38,14 @@ PRESETS = {
     "mixed_amg": (True, "previous", "amg"),
     "mixed_amg_zero": (True, "zero", "amg"),
     "learned_amg": (True, "learned", "amg"),
+    "mixed_auto": (True, "previous", "auto"),
+    "mixed_auto_zero": (True, "zero", "auto"),
+    "learned_auto": (True, "learned", "auto"),
 }
 
 _WARM = {"previous": L.WARM_PREVIOUS, "zero": L.WARM_ZERO, "learned": L.WARM_LEARNED}
 _PREC = {"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "multilevel": L.PRECOND_MULTILEVEL,
-         "amg": L.PRECOND_AMG}
+         "amg": L.PRECOND_AMG, "auto": L.PRECOND_AUTO}
 
 
 def configure(computer, preset: str, tol: float, weights: str | None = None, model=None) -> None:

@@ -222,3 +222,38 @@ def test_spmv_replay_leaves_the_step_unchanged(precond):
         del g
     for k in ("x", "z", "u", "w", "p"):
         assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+def test_auto_preconditioner_times_both_and_keeps_the_faster():
+    """MPS_PRECOND_AUTO on the 125k dam break: the first solve runs the
+    multilevel preconditioner, the second the multigrid, then the faster of
+    the two by their solve times, the other again 16 solves later; every solve
+    meets the FP64 rule; setting AUTO again starts the timing over."""
+    cfg = scenes.config("dambreak_125k")
+    g = cases.make("gpu", cfg["env"], cfg["eps"])
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_AUTO, tol=TOL)
+    g.add_particles(cfg["particles"])
+    used, ms = [], []
+    for _ in range(21):
+        g.forward_time(cfg["dt"])
+        s = g.solve_stats()
+        assert s["rel_residual"] <= TOL
+        used.append(s["precond_used"])
+        ms.append(s["solve_ms"])
+    say(f"  preconditioner per solve: {used}")
+    ML, MG = L.PRECOND_MULTILEVEL, L.PRECOND_AMG
+    assert used[0] == ML and used[1] == MG
+    best = MG if ms[1] < ms[0] else ML
+    other = ML if best == MG else MG
+    # solves 3..18 keep the faster one (16 solves after the first two), solve
+    # 19 times the other again
+    assert all(u == best for u in used[2:18])
+    assert used[18] == other
+    # AUTO set again: the multilevel one first, the multigrid second
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_JACOBI, tol=TOL)
+    g.set_solver(kind=L.SOLVER_MIXED_IR, stop_rule=L.STOP_RHS_NORM, precond=L.PRECOND_AUTO, tol=TOL)
+    again = []
+    for _ in range(2):
+        g.forward_time(cfg["dt"])
+        again.append(g.solve_stats()["precond_used"])
+    assert again == [ML, MG]
