@@ -1198,8 +1198,10 @@ struct AmgStenLds {
     return s[9u * (L.off - base) + static_cast<uint32_t>(d) * amg_npts(L) + p];
   }
 };
+static_assert(kAmgLdsPts % 4 == 0 && kAmgLdsFloats % 4 == 0, "float4 regions");
 __global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) {
-  __shared__ float lds[kAmgLdsFloats];
+  __shared__ float4 lds4[kAmgLdsFloats / 4];
+  float* const lds = reinterpret_cast<float*>(lds4);
   const uint32_t base = g.L[l0].off;
   const uint32_t npts = g.L[g.levels - 1].off + amg_npts(g.L[g.levels - 1]) - base;
   float* const ls = lds;
@@ -1211,25 +1213,30 @@ __global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l
   const uint32_t nb = amg_npts(B), bo = B.off - base;
   float* const linv = lds + 13 * kAmgLdsPts;
   // one index space over the stencils, the inverse, 1 / S_center and r of
-  // level l0, copied eight loads per thread at a time (the loads of a batch
-  // are in flight together)
-  const uint32_t a0 = 9 * npts, a1 = a0 + nb * nb, a2 = a1 + npts, a3 = a2 + n0;
+  // level l0 in float4 units (level offsets are multiples of 4; the arrays
+  // and the LDS regions have room for the rounding up), eight loads per
+  // thread in flight at a time
+  const uint32_t a0 = (9 * npts + 3) / 4, a1 = a0 + (nb * nb + 3) / 4, a2 = a1 + (npts + 3) / 4,
+                 a3 = a2 + (n0 + 3) / 4;
+  const float4* __restrict__ s4 = reinterpret_cast<const float4*>(g.S + 9ull * base);
+  const float4* __restrict__ i4 = reinterpret_cast<const float4*>(g.inv);
+  const float4* __restrict__ d4 = reinterpret_cast<const float4*>(g.dinv + base);
+  const float4* __restrict__ r4 = reinterpret_cast<const float4*>(g.r + base);
   for (uint32_t k0 = threadIdx.x; k0 < a3; k0 += 8 * kAmgBlockThreads) {
-    float t[8];
+    float4 t[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const uint32_t k = k0 + u * kAmgBlockThreads;
-      const float* src = k < a0 ? g.S + 9ull * base + k
-                                : (k < a1 ? g.inv + (k - a0) : (k < a2 ? g.dinv + base + (k - a1) : g.r + base + (k - a2)));
-      t[u] = k < a3 ? *src : 0.0f;
+      const float4* src = k < a0 ? s4 + k : (k < a1 ? i4 + (k - a0) : (k < a2 ? d4 + (k - a1) : r4 + (k - a2)));
+      t[u] = k < a3 ? *src : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const uint32_t k = k0 + u * kAmgBlockThreads;
       if (k < a3)
-        lds[k < a0 ? k
-                   : (k < a1 ? 13 * kAmgLdsPts + (k - a0)
-                             : (k < a2 ? 12 * kAmgLdsPts + (k - a1) : 9 * kAmgLdsPts + (k - a2)))] = t[u];
+        lds4[k < a0 ? k
+                    : (k < a1 ? 13 * kAmgLdsPts / 4 + (k - a0)
+                              : (k < a2 ? 12 * kAmgLdsPts / 4 + (k - a1) : 9 * kAmgLdsPts / 4 + (k - a2)))] = t[u];
     }
   }
   __syncthreads();
@@ -2357,7 +2364,7 @@ AmgGrid amg_grid(const Context& c) {
 int amg_block_level(const Context& c) {
   const size_t end = c.amg_pts;
   int l0 = c.amg_levels - 1;
-  while (l0 > 0 && end - c.amg_off[l0 - 1] <= static_cast<size_t>(kAmgLdsPts)) l0--;
+  while (l0 > 0 && end - c.amg_off[l0 - 1] <= static_cast<size_t>(kAmgLdsPts - 4)) l0--;
   return l0;
 }
 
@@ -2395,6 +2402,7 @@ void amg_setup(Context& c) {
     c.amg_nz[lv] = nz;
     c.amg_off[lv] = static_cast<uint32_t>(off);
     off += static_cast<size_t>(nx) * nz;
+    off = (off + 3) & ~static_cast<size_t>(3);  // every level 16-byte aligned (k_amg_block copies float4)
     lv++;
     if (static_cast<size_t>(nx) * nz <= static_cast<size_t>(kAmgDense)) break;
     nx = (nx + 1) / 2;
@@ -2411,9 +2419,10 @@ void amg_setup(Context& c) {
     dfree(c.amg_x);
     dfree(c.amg_e);
     c.amg_S64 = dalloc<double>(9 * off);
-    c.amg_S = dalloc<float>(9 * off);
-    c.amg_dinv = dalloc<float>(off);
-    c.amg_r = dalloc<float>(off);
+    // (room for k_amg_block's float4 copies rounding the last level up)
+    c.amg_S = dalloc<float>(9 * off + 16);
+    c.amg_dinv = dalloc<float>(off + 16);
+    c.amg_r = dalloc<float>(off + 16);
     c.amg_x = dalloc<float>(off);
     c.amg_e = dalloc<float>(off);
     c.amg_cap = off;
