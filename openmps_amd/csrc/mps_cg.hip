@@ -36,6 +36,8 @@ namespace mps {
 namespace {
 
 constexpr int NT = 512;  // threads per block of the solver kernels
+constexpr int kCgParts = 2;  // FP64 CG: partials per thread loaded ahead (1024 slots)
+constexpr int kCgRows = 2;   // FP64 CG: rows per thread whose product waits in registers for beta
 
 template <class T>
 struct Sell {
@@ -371,16 +373,47 @@ __global__ void __launch_bounds__(NT, 8) k_cg_spmv(M A, const double* __restrict
                                                    const double* __restrict__ part_bb, int nparts,
                                                    double* __restrict__ part_out, CgStatus* st,
                                                    StopRule sr, long long it) {
-  __shared__ double lds[NT / 64];
+  __shared__ double lds[2 * (NT / 64)];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double rr = reduce_partials<NT>(part_in, nparts, lds);
+  // the partials are loaded first and summed after the products of the
+  // block's first rows, which do not depend on beta (the loads of both are in
+  // flight together); sums in the order of reduce_partials
+  const bool want_bb = FIRST && sr.rule == MPS_STOP_RHS_NORM;
+  double pa[kCgParts], pb[kCgParts];
+#pragma unroll
+  for (int u = 0; u < kCgParts; u++) {
+    const int k = threadIdx.x + u * NT;
+    pa[u] = (k < nparts) ? part_in[k] : 0.0;
+    pb[u] = (want_bb && k < nparts) ? part_bb[k] : 0.0;
+  }
+  const Rows rw = block_rows(A.n, A.rpb);
+  double rv[kCgRows], sv[kCgRows];
+#pragma unroll
+  for (int u = 0; u < kCgRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    rv[u] = sv[u] = 0.0;
+    if (i < rw.hi) {
+      rv[u] = r[i];
+      sv[u] = row_product(A, i, rv[u], [&](uint32_t j) { return r[j]; });
+    }
+  }
+  double sa = 0, sb = 0;
+#pragma unroll
+  for (int u = 0; u < kCgParts; u++) {
+    sa += pa[u];
+    sb += pb[u];
+  }
+  for (int k = threadIdx.x + kCgParts * NT; k < nparts; k += NT) {
+    sa += part_in[k];
+    if (want_bb) sb += part_bb[k];
+  }
+  double rr, bb = 0;
+  block_sum2<NT>(sa, sb, lds, rr, bb);
   double beta = 0, tol;
   bool conv;
-  double bb = 0;
   if (FIRST) {
     if (sr.rule == MPS_STOP_RHS_NORM) {
-      bb = reduce_partials<NT>(part_bb, nparts, lds);
       tol = (sr.tol * sr.tol) * bb;
       conv = (rr <= tol);
     } else {
@@ -405,16 +438,22 @@ __global__ void __launch_bounds__(NT, 8) k_cg_spmv(M A, const double* __restrict
   }
   if (conv) return;
   if (!FIRST) beta = rr / st->rr[par ^ 1];
-  const Rows rw = block_rows(A.n, A.rpb);
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
-    const double ri = r[i];
-    const double s = row_product(A, i, ri, [&](uint32_t j) { return r[j]; });
+  auto finish = [&](size_t i, double ri, double s) {
     const double pi = FIRST ? ri : fma(beta, p[i], ri);
     const double qi = FIRST ? s : fma(beta, q[i], s);
     p[i] = pi;
     q[i] = qi;
     acc = fma(pi, qi, acc);
+  };
+#pragma unroll
+  for (int u = 0; u < kCgRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    if (i < rw.hi) finish(i, rv[u], sv[u]);
+  }
+  for (size_t i = rw.lo + threadIdx.x + static_cast<size_t>(kCgRows) * NT; i < rw.hi; i += NT) {
+    const double ri = r[i];
+    finish(i, ri, row_product(A, i, ri, [&](uint32_t j) { return r[j]; }));
   }
   const double sp = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part_out[blockIdx.x] = sp;
