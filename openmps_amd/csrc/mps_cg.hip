@@ -469,11 +469,43 @@ __global__ void __launch_bounds__(NT) k_cg_update(size_t n, size_t rpb, double* 
   __shared__ double lds[NT / 64];
   if (already_converged(st)) return;
   const int par = static_cast<int>(it & 1);
-  const double pq = reduce_partials<NT>(part_in, nparts, lds);
-  const double alpha = st->rr[par] / pq;
+  // the partials and the block's first rows are loaded together; sums in the
+  // order of reduce_partials
+  double pa[kCgParts];
+#pragma unroll
+  for (int u = 0; u < kCgParts; u++) {
+    const int k = threadIdx.x + u * NT;
+    pa[u] = (k < nparts) ? part_in[k] : 0.0;
+  }
   const Rows rw = block_rows(n, rpb);
+  double xv[kCgRows], pv[kCgRows], qv[kCgRows], rv[kCgRows];
+#pragma unroll
+  for (int u = 0; u < kCgRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    const bool in = i < rw.hi;
+    xv[u] = in ? x[i] : 0.0;
+    pv[u] = in ? p[i] : 0.0;
+    qv[u] = in ? q[i] : 0.0;
+    rv[u] = in ? r[i] : 0.0;
+  }
+  double sa = 0;
+#pragma unroll
+  for (int u = 0; u < kCgParts; u++) sa += pa[u];
+  for (int k = threadIdx.x + kCgParts * NT; k < nparts; k += NT) sa += part_in[k];
+  const double pq = block_sum<NT>(sa, lds);
+  const double alpha = st->rr[par] / pq;
   double acc = 0;
-  for (size_t i = rw.lo + threadIdx.x; i < rw.hi; i += NT) {
+#pragma unroll
+  for (int u = 0; u < kCgRows; u++) {
+    const size_t i = rw.lo + threadIdx.x + static_cast<size_t>(u) * NT;
+    if (i < rw.hi) {
+      x[i] = fma(alpha, pv[u], xv[u]);
+      const double ri = fma(-alpha, qv[u], rv[u]);
+      r[i] = ri;
+      acc = fma(ri, ri, acc);
+    }
+  }
+  for (size_t i = rw.lo + threadIdx.x + static_cast<size_t>(kCgRows) * NT; i < rw.hi; i += NT) {
     x[i] = fma(alpha, p[i], x[i]);
     const double ri = fma(-alpha, q[i], r[i]);
     r[i] = ri;
