@@ -31,6 +31,7 @@ KERNEL_IDS = {
     "k_neighbors": "neighbors", "k_density<true, true, false>": "density_ecs",
     "k_density<true, false, true>": "density_save", "k_explicit": "explicit",
     "k_ppe_count": "ppe_count", "k_ppe_fill": "ppe_fill", "k_cg_init": "cg_init",
+    "k_ppe_fill<true>": "ppe_fill", "k_ppe_fill<false>": "ppe_fill",
     "k_cg_spmv<false>": "cg_spmv", "k_cg_spmv<true>": "cg_spmv", "k_cg_update": "cg_update",
     "k_cg_spmv<false, mps::(anonymous namespace)::Sell<double> >": "cg_spmv",
     "k_cg_spmv<true, mps::(anonymous namespace)::Sell<double> >": "cg_spmv",
