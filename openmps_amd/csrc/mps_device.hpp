@@ -89,6 +89,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned t = __shfl_xor(v, o, 64);
+    v = (t < v) ? t : v;
+  }
+  return v;
+}
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {

@@ -286,6 +286,17 @@ __global__ void __launch_bounds__(kBlock) k_cell_pos(size_t n, const uint64_t* _
 // Computer::SearchNeighbor's per-particle walk, Computer.hpp:720-755: the 9
 // cells in Grid.hpp:393-401 order, j != i, R(x_i, x_j) < NeighborLength.
 // Disabled particles keep their previous list, as in the reference.
+//
+// A lane's found neighbours form groups of four slots (16 bytes, the list
+// layout's unit).  Stored by each lane as it completes them, the groups of a
+// wavefront reach a cache line at different times and the lines are written
+// back part-filled (2.2x the list bytes).  Instead each lane keeps its last
+// kRing completed groups in a ring in LDS, and after each of the 9 cells
+// (uniform for the wavefront) the groups every lane has completed are
+// stored together: one 1-KB coalesced store per group index.  A lane more
+// than kRing groups ahead of the slowest stores its oldest ring group
+// directly to make room; the end of the walk stores the rest.
+constexpr int kRing = 4;
 __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z,
                                                       const uint8_t* __restrict__ type,
@@ -296,65 +307,96 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
                                                       uint32_t kstride, uint32_t* __restrict__ nbr,
                                                       uint32_t* __restrict__ ncount, int* flags,
                                                       unsigned long long* total) {
+  __shared__ uint4 ring[kRing][kBlock];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  const bool active = i < n && type[i] != kDisabled;
   uint32_t cnt = 0;
-  if (i < n && type[i] != kDisabled) {
-    const double xi = x[i], zi = z[i];
-    // found neighbours are stored a group of four (16 bytes) at a time
-    uint4 grp = make_uint4(0u, 0u, 0u, 0u);
-    uint4* const out = reinterpret_cast<uint4*>(nbr);
-    long long bx = 0, bz = 0;
+  // complete groups of this lane (those whose last slot is below kmax);
+  // groups below `flushed` (a wavefront value) are stored, and of the others
+  // the last kRing are in ring[group % kRing], the older ones stored
+  uint32_t ngrp = 0, flushed = 0;
+  uint4 grp = make_uint4(0u, 0u, 0u, 0u);
+  uint4* const out = reinterpret_cast<uint4*>(nbr);
+  const uint32_t self = static_cast<uint32_t>(i);
+  auto store = [&](uint32_t gi, const uint4& v) { out[nbr_slot(self, 4 * gi, kstride) >> 2] = v; };
+  double xi = 0, zi = 0;
+  long long bx = 0, bz = 0;
+  if (active) {
+    xi = x[i];
+    zi = z[i];
     cell_of(g, xi, zi, bx, bz);
-    const double nl = g.nl;
-    for (int ox = -1; ox <= 1; ox++) {
-      const long long cx = bx + ox;
-      if (cx < 0 || cx >= g.gnx) continue;
-      for (int oz = -1; oz <= 1; oz++) {
-        const long long cz = bz + oz;
-        if (cz < 0 || cz >= g.gnz) continue;
+  }
+  const double nl = g.nl;
+  for (int ox = -1; ox <= 1; ox++) {
+    for (int oz = -1; oz <= 1; oz++) {
+      const long long cx = bx + ox, cz = bz + oz;
+      uint32_t s = 0, m = 0;
+      if (active && cx >= 0 && cx < g.gnx && cz >= 0 && cz < g.gnz) {
         const size_t c = static_cast<size_t>(cx * g.gnz + cz);
-        const uint32_t s = cell_start[c], m = cell_count[c];
-        auto test = [&](uint32_t j, double xj, double zj) {
-          if (j == i) return;
-          const double rx = xi - xj, rz = zi - zj;
-          // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
-          // rounded, so the test only depends on r.r near NL^2: outside the
-          // narrow band [lo, hi] the comparison of r.r decides it exactly.
-          const double q = rx * rx + rz * rz;
-          const bool within = (q < g.lo) || (q <= g.hi && sqrt(q) < nl);
-          if (within) {
-            const uint32_t lane = cnt & 3u;
-            grp.x = (lane == 0) ? j : grp.x;
-            grp.y = (lane == 1) ? j : grp.y;
-            grp.z = (lane == 2) ? j : grp.z;
-            grp.w = (lane == 3) ? j : grp.w;
-            if (lane == 3 && cnt < kmax) out[nbr_slot(static_cast<uint32_t>(i), cnt - 3, kstride) >> 2] = grp;
-            cnt++;
+        s = cell_start[c];
+        m = cell_count[c];
+      }
+      auto test = [&](uint32_t j, double xj, double zj) {
+        if (j == i) return;
+        const double rx = xi - xj, rz = zi - zj;
+        // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
+        // rounded, so the test only depends on r.r near NL^2: outside the
+        // narrow band [lo, hi] the comparison of r.r decides it exactly.
+        const double q = rx * rx + rz * rz;
+        const bool within = (q < g.lo) || (q <= g.hi && sqrt(q) < nl);
+        if (within) {
+          const uint32_t lane = cnt & 3u;
+          grp.x = (lane == 0) ? j : grp.x;
+          grp.y = (lane == 1) ? j : grp.y;
+          grp.z = (lane == 2) ? j : grp.z;
+          grp.w = (lane == 3) ? j : grp.w;
+          if (lane == 3 && cnt < kmax) {
+            // group ngrp is complete; a full ring gives up its oldest group
+            if (ngrp >= flushed + kRing) store(ngrp - kRing, ring[ngrp % kRing][threadIdx.x]);
+            ring[ngrp % kRing][threadIdx.x] = grp;
+            ngrp++;
           }
-        };
-        // four candidates per round trip to memory: ids and positions of the
-        // cell's contiguous range, loaded together
-        uint32_t l = 0;
-        for (; l + 4 <= m; l += 4) {
-          uint32_t jj[4];
-          double2 pj[4];
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            jj[v] = sorted[s + l + v];
-            pj[v] = pos[s + l + v];
-          }
-#pragma unroll
-          for (int v = 0; v < 4; v++) test(jj[v], pj[v].x, pj[v].y);
+          cnt++;
         }
-        for (; l < m; l++) {
-          const double2 pj = pos[s + l];
-          test(sorted[s + l], pj.x, pj.y);
+      };
+      // four candidates per round trip to memory: ids and positions of the
+      // cell's contiguous range, loaded together
+      uint32_t l = 0;
+      for (; l + 4 <= m; l += 4) {
+        uint32_t jj[4];
+        double2 pj[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          jj[v] = sorted[s + l + v];
+          pj[v] = pos[s + l + v];
         }
+#pragma unroll
+        for (int v = 0; v < 4; v++) test(jj[v], pj[v].x, pj[v].y);
+      }
+      for (; l < m; l++) {
+        const double2 pj = pos[s + l];
+        test(sorted[s + l], pj.x, pj.y);
+      }
+      // the groups every active lane has completed
+      const uint32_t upto = wave_min_u32(active ? ngrp : 0xffffffffu);
+      if (upto != 0xffffffffu) {
+        for (uint32_t gi = flushed; gi < upto; gi++)
+          if (active && gi + kRing >= ngrp) store(gi, ring[gi % kRing][threadIdx.x]);
+        flushed = upto > flushed ? upto : flushed;
       }
     }
-    // the last, partial group (slots past the count are never read)
-    if ((cnt & 3u) && cnt <= kmax) out[nbr_slot(static_cast<uint32_t>(i), cnt & ~3u, kstride) >> 2] = grp;
+  }
+  // the rest: the complete groups still in the ring and the last, partial
+  // one (its slots past the count are never read)
+  const bool partial = active && (cnt & 3u) && cnt <= kmax;
+  const uint32_t mine = active ? ngrp + (partial ? 1u : 0u) : 0u;
+  const uint32_t last = wave_max_u32(mine);
+  for (uint32_t gi = flushed; gi < last; gi++) {
+    if (gi < ngrp && gi + kRing >= ngrp) store(gi, ring[gi % kRing][threadIdx.x]);
+    if (partial && gi == ngrp) store(gi, grp);
+  }
+  if (active) {
     if (cnt > kmax) {
       flags[1] = 1;
       cnt = kmax;

@@ -109,6 +109,19 @@ def test_mixed_ml_1m_dambreak_residual():
     _mixed("dambreak_1m", "mixed_ml", 3)
 
 
+def test_mixed_auto_1m_dambreak_residual():
+    """The bench's headline solver: both preconditioners in the first two
+    steps, then the faster one; every step meets the rule."""
+    c, st = _mixed("dambreak_1m", "mixed_auto", 4)
+    used = [s["precond_used"] for s in st]
+    assert used[:2] == [2, 3] and used[2] in (2, 3) and used[3] == used[2]
+
+
+def test_mixed_amg_1m_still_water_residual():
+    """The cell-grid multigrid on the still 1 m column."""
+    _mixed("still_1m", "mixed_amg", 2)
+
+
 def test_mixed_ml_1m_still_water_residual():
     """The still 1 m column (the static-pressure half of the 1M configuration)."""
     c, st = _mixed("still_1m", "mixed_ml", 2)
