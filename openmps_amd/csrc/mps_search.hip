@@ -14,9 +14,10 @@
 //                      restores exactly the serial insertion order; k_cell_pos
 //                      stores the positions in that order;
 //   5. k_neighbors     one thread per particle walks the 9 cells in the
-//                      reference order (each a contiguous range of ids and
-//                      positions) and writes its list into the sliced
-//                      neighbour layout, four slots (16 bytes) per store.
+//                      reference order (three contiguous ranges of ids and
+//                      positions, one per column of cells) and writes its
+//                      list into the sliced neighbour layout, four slots
+//                      (16 bytes) per store.
 #include <utility>
 
 #include "mps_device.hpp"
@@ -287,16 +288,26 @@ __global__ void __launch_bounds__(kBlock) k_cell_pos(size_t n, const uint64_t* _
 // cells in Grid.hpp:393-401 order, j != i, R(x_i, x_j) < NeighborLength.
 // Disabled particles keep their previous list, as in the reference.
 //
+// The cells (cx, cz - 1), (cx, cz), (cx, cz + 1) are consecutive cell
+// indices, so their members are one contiguous range of the cell-ordered
+// slots, in the reference's visiting order (cells in turn, each cell's
+// members by index): the walk is three ranges, one per column of cells.
+// IDENT: the particles are stored in cell order (slot k holds particle k),
+// so the candidate ids are the slots themselves and are not loaded.
+//
 // A lane's found neighbours form groups of four slots (16 bytes, the list
 // layout's unit).  Stored by each lane as it completes them, the groups of a
 // wavefront reach a cache line at different times and the lines are written
 // back part-filled (2.2x the list bytes).  Instead each lane keeps its last
-// kRing completed groups in a ring in LDS, and after each of the 9 cells
-// (uniform for the wavefront) the groups every lane has completed are
-// stored together: one 1-KB coalesced store per group index.  A lane more
-// than kRing groups ahead of the slowest stores its oldest ring group
-// directly to make room; the end of the walk stores the rest.
+// kRing completed groups in a ring in LDS, and after each batch of
+// candidates (the batch loop is uniform for the wavefront) the groups every
+// lane has completed are stored together: one 1-KB coalesced store per group
+// index.  A lane more than kRing groups ahead of the slowest stores its
+// oldest ring group directly to make room; the end of the walk stores the
+// rest.
 constexpr int kRing = 4;
+constexpr int kNbrBatch = 4;  // candidates per lane per round trip to memory
+template <bool IDENT>
 __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z,
                                                       const uint8_t* __restrict__ type,
@@ -325,59 +336,62 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(GridParams g, size_t n, co
   if (active) {
     xi = x[i];
     zi = z[i];
-    cell_of(g, xi, zi, bx, bz);
+    cell_of(g, xi, zi, bx, bz);  // in the grid: the others are disabled
   }
   const double nl = g.nl;
+  auto test = [&](bool valid, uint32_t j, double xj, double zj) {
+    if (!valid || j == i) return;
+    const double rx = xi - xj, rz = zi - zj;
+    // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
+    // rounded, so the test only depends on r.r near NL^2: outside the
+    // narrow band [lo, hi] the comparison of r.r decides it exactly.
+    const double q = rx * rx + rz * rz;
+    bool within = q < g.lo;
+    // the band is a few ulps of NL^2 wide: its square root is taken on its
+    // own branch, which a wavefront almost never enters (evaluated beside
+    // the comparisons it costs every candidate beyond lo twenty FP64
+    // instructions)
+    if (q >= g.lo && q <= g.hi) within = sqrt(q) < nl;
+    if (within) {
+      const uint32_t lane = cnt & 3u;
+      grp.x = (lane == 0) ? j : grp.x;
+      grp.y = (lane == 1) ? j : grp.y;
+      grp.z = (lane == 2) ? j : grp.z;
+      grp.w = (lane == 3) ? j : grp.w;
+      if (lane == 3 && cnt < kmax) {
+        // group ngrp is complete; a full ring gives up its oldest group
+        if (ngrp >= flushed + kRing) store(ngrp - kRing, ring[ngrp % kRing][threadIdx.x]);
+        ring[ngrp % kRing][threadIdx.x] = grp;
+        ngrp++;
+      }
+      cnt++;
+    }
+  };
   for (int ox = -1; ox <= 1; ox++) {
-    for (int oz = -1; oz <= 1; oz++) {
-      const long long cx = bx + ox, cz = bz + oz;
-      uint32_t s = 0, m = 0;
-      if (active && cx >= 0 && cx < g.gnx && cz >= 0 && cz < g.gnz) {
-        const size_t c = static_cast<size_t>(cx * g.gnz + cz);
-        s = cell_start[c];
-        m = cell_count[c];
-      }
-      auto test = [&](uint32_t j, double xj, double zj) {
-        if (j == i) return;
-        const double rx = xi - xj, rz = zi - zj;
-        // Computer::R, Computer.hpp:568-572: sqrt(r.r) < NL.  sqrt is correctly
-        // rounded, so the test only depends on r.r near NL^2: outside the
-        // narrow band [lo, hi] the comparison of r.r decides it exactly.
-        const double q = rx * rx + rz * rz;
-        const bool within = (q < g.lo) || (q <= g.hi && sqrt(q) < nl);
-        if (within) {
-          const uint32_t lane = cnt & 3u;
-          grp.x = (lane == 0) ? j : grp.x;
-          grp.y = (lane == 1) ? j : grp.y;
-          grp.z = (lane == 2) ? j : grp.z;
-          grp.w = (lane == 3) ? j : grp.w;
-          if (lane == 3 && cnt < kmax) {
-            // group ngrp is complete; a full ring gives up its oldest group
-            if (ngrp >= flushed + kRing) store(ngrp - kRing, ring[ngrp % kRing][threadIdx.x]);
-            ring[ngrp % kRing][threadIdx.x] = grp;
-            ngrp++;
-          }
-          cnt++;
-        }
-      };
-      // four candidates per round trip to memory: ids and positions of the
-      // cell's contiguous range, loaded together
-      uint32_t l = 0;
-      for (; l + 4 <= m; l += 4) {
-        uint32_t jj[4];
-        double2 pj[4];
+    const long long cx = bx + ox;
+    uint32_t s = 0, m = 0;
+    if (active && cx >= 0 && cx < g.gnx) {
+      const size_t c0 = static_cast<size_t>(cx * g.gnz + (bz > 0 ? bz - 1 : 0));
+      const size_t c1 = static_cast<size_t>(cx * g.gnz + (bz + 1 < g.gnz ? bz + 1 : bz));
+      s = cell_start[c0];
+      m = cell_start[c1] + cell_count[c1] - s;
+    }
+    // a batch loop uniform for the wavefront (lanes past their range skip
+    // the tests), so that the completed groups can be stored after a batch
+    const uint32_t mw = wave_max_u32(m);
+    for (uint32_t l = 0; l < mw; l += kNbrBatch) {
+      uint32_t jj[kNbrBatch];
+      double2 pj[kNbrBatch];
+      bool ok[kNbrBatch];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          jj[v] = sorted[s + l + v];
-          pj[v] = pos[s + l + v];
-        }
+      for (int v = 0; v < kNbrBatch; v++) {
+        ok[v] = l + v < m;
+        const uint32_t k = s + (ok[v] ? l + v : 0u);  // a valid slot (unused when !ok)
+        jj[v] = IDENT ? k : (ok[v] ? sorted[k] : 0u);
+        pj[v] = ok[v] ? pos[k] : make_double2(0.0, 0.0);
+      }
 #pragma unroll
-        for (int v = 0; v < 4; v++) test(jj[v], pj[v].x, pj[v].y);
-      }
-      for (; l < m; l++) {
-        const double2 pj = pos[s + l];
-        test(sorted[s + l], pj.x, pj.y);
-      }
+      for (int v = 0; v < kNbrBatch; v++) test(ok[v], jj[v], pj[v].x, pj[v].y);
       // the groups every active lane has completed
       const uint32_t upto = wave_min_u32(active ? ngrp : 0xffffffffu);
       if (upto != 0xffffffffu) {
@@ -515,7 +529,9 @@ void search(Context& c) {
   // positions and the list lengths; the list itself (4 bytes a neighbour) is
   // added below
   tracked(c, MPS_K_NEIGHBORS, 41.0 * n + 8.0 * c.ncells, [&] {
-    hipLaunchKernelGGL(k_neighbors, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
+    // in cell order (after the permutation) slot k holds particle k
+    auto walk = c.keep_order ? k_neighbors<false> : k_neighbors<true>;
+    hipLaunchKernelGGL(walk, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, g, n, c.x, c.z,
                        static_cast<const uint8_t*>(c.type), static_cast<const uint32_t*>(c.cell_start),
                        static_cast<const uint32_t*>(c.cell_count), static_cast<const uint32_t*>(c.sorted),
                        static_cast<const double2*>(c.pos_sorted), c.kmax, c.kstride, c.nbr, c.ncount, c.flags,
