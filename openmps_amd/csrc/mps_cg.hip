@@ -916,7 +916,7 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
 
 // ---------------- cell-grid aggregation multigrid (MPS_PRECOND_AMG) ----------------
 //
-// The preconditioner M r = omega D^-1 r + P V(P^T r): aggregates are the
+// The preconditioner M r = omega_r D^-1 r + P V(P^T r): aggregates are the
 // neighbour-search cells (P piecewise constant over a cell's aggregated
 // rows), and V is one V-cycle on the cells' Galerkin operator P^T A P.  The
 // cell side NL exceeds the Laplacian radius, so a particle couples only to
@@ -928,10 +928,21 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
 // (piecewise constant), post-smoothing x += omega D^-1 (r - S x); the
 // coarsest level (at most kAmgDense points) is solved with its inverse.
 // The coarse levels whose points fit kAmgLdsPts run inside one workgroup, in LDS.
+//
+// The weights: the cell levels' stencils are weakly diagonally dominant with
+// non-positive off-diagonals (Galerkin sums over aggregates of the HL
+// Laplacian's rows, whose off-diagonals are negative and whose row sums are
+// not), so lambda_max(D^-1 S) <= 2 (Gershgorin; about 1.8-1.97 on the 1M
+// dam break) and Jacobi with omega <= 1 is a convergent smoother: the
+// symmetric V-cycle, and M, stay positive definite.  omega = 1 on the cells
+// and omega_r = 0.5 on the particles take the PCG on the developed 1M dam
+// break from 63 to 54 iterations (FP64, 0.7 on both: 63; 0.7 / 0.9 / 1.0 on
+// the cells with 0.5 on the particles: 56 / 55 / 54; early flow 37 -> 35).
 constexpr int kAmgLdsPts = 2624;  // points of the levels the single-workgroup kernel holds in LDS (133 KB + the inverse)
 constexpr int kAmgDense = 64;
 constexpr int kAmgBlockThreads = 1024;
-constexpr float kAmgOmega = 0.7f;
+constexpr float kAmgOmega = 1.0f;      // the cell levels' Jacobi smoother
+constexpr float kAmgOmegaRows = 0.5f;  // the particle-level (additive) Jacobi term
 
 struct AmgLevel {
   uint32_t nx, nz;
@@ -2599,7 +2610,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
   auto prec = [&](double norm, long long it) {
     PrecArgs a{G,         c.ml_list, nact,  c.ml_pid, c.ml_dv,    c.ml_sc, c.r,   1.0 / norm, c.r32,
                c.d32,     c.q32,     c.dinv32, zval, 4u,         p_pq,    L.grid, c.part_ml,  stride,
-               c.st32,    c.amg_r,   c.amg_e,   kAmgOmega, c.row_len, c.amg_box[0], c.amg_box[1],
+               c.st32,    c.amg_r,   c.amg_e,   kAmgOmegaRows, c.row_len, c.amg_box[0], c.amg_box[1],
                c.amg_box[2], c.amg_box[3]};
     auto track = [&](int kid, double bytes, auto&& f) {
       if (it == 0)

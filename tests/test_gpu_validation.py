@@ -13,9 +13,16 @@ validation/*.json (python -m openmps_amd.validate ...).
   front by far more.
 * Zhou et al. (1999) dam break onto a dry deck (Benchmark/DamBreak/
   zhouetal1999.py:26-39) at the reference generator's l0 = 20 mm to
-  t sqrt(g/H) = 8: the wall-pressure peak p2 (measured 1.21 rho g H at
-  t* = 4.65; experiment about 1.1 at t* = 4.5-5) and the RMS differences of
-  h1, h2 from the experiment (0.17 H and 0.20 H at this coarse spacing).
+  t sqrt(g/H) = 8: the wall pressure p2's impact (arrival: the first sample
+  above 0.3 rho g H, experiment t* = 2.85, measured 2.87 with the mixed
+  cell-grid multigrid and 3.21 with the FP64 CG; plateau: the mean over
+  3.2 <= t* <= 5.5, experiment 0.55, measured 0.35-0.37: this coarse spacing
+  underpredicts it), its largest sample (no pressure blow-up) and the RMS
+  differences of h1, h2, p2 from the experiment (about 0.16 H, 0.17-0.21 H
+  and 0.20-0.26).  The time of the largest sample is not compared: the
+  experiment's is the returning wave (0.86 at t* = 6.14), a run's is decided
+  by the spikes of a point-sampled MPS pressure (4.65, 5.09 or 6.18 in runs
+  that differ only in the solver's rounding).
   The mixed solvers stop at 1e-10 |b| on this scene: at 1e-8 |b| the mixed
   Jacobi and cell-grid multigrid runs let a particle near the impact
   accelerate to 100-300 m/s at t = 1.08-1.11 s (the CFL step collapses), the
@@ -44,11 +51,12 @@ def test_surge_front_against_the_published_computations():
 
 def test_zhou_wall_pressure_and_gauge_heights():
     out = validate.run_zhou(8.0, "mixed_amg", 1e-10)
-    v, peak = out["versus"], out["p2_peak"]
-    print(out["steps"], "steps; p2 peak", peak, {k: round(d["rms"], 3) for k, d in v.items()}, flush=True)
+    v, peak, imp = out["versus"], out["p2_peak"], out["p2_impact"]
+    print(out["steps"], "steps; p2 peak", peak, imp, {k: round(d["rms"], 3) for k, d in v.items()}, flush=True)
     assert out["t_star_end"] > 7.9
-    assert 4.0 < peak["t_star"] < 5.5
-    assert 0.95 < peak["p_over_rhogH"] < 1.45
+    assert abs(imp["ours"]["arrival_t_star"] - imp["experiment"]["arrival_t_star"]) < 0.5
+    assert 0.25 < imp["ours"]["plateau_mean"] < 0.75
+    assert 0.5 < peak["p_over_rhogH"] < 1.6
     assert v["experiment_h1"]["rms"] < 0.25
     assert v["experiment_h2"]["rms"] < 0.25
     assert v["experiment_p2"]["rms"] < 0.32
