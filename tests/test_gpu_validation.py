@@ -14,12 +14,12 @@ validation/*.json (python -m openmps_amd.validate ...).
 * Zhou et al. (1999) dam break onto a dry deck (Benchmark/DamBreak/
   zhouetal1999.py:26-39) at the reference generator's l0 = 20 mm to
   t sqrt(g/H) = 8: the wall pressure p2's impact (arrival: the first sample
-  above 0.3 rho g H, experiment t* = 2.85, measured 2.87 with the mixed
+  above 0.3 rho g H, experiment t* = 2.85, measured 2.87-3.09 with the mixed
   cell-grid multigrid and 3.21 with the FP64 CG; plateau: the mean over
-  3.2 <= t* <= 5.5, experiment 0.55, measured 0.35-0.37: this coarse spacing
+  3.2 <= t* <= 5.5, experiment 0.55, measured 0.34-0.37: this coarse spacing
   underpredicts it), its largest sample (no pressure blow-up) and the RMS
-  differences of h1, h2, p2 from the experiment (about 0.16 H, 0.17-0.21 H
-  and 0.20-0.26).  The time of the largest sample is not compared: the
+  differences of h1, h2, p2 from the experiment (about 0.15-0.18 H,
+  0.17-0.21 H and 0.20-0.26).  The time of the largest sample is not compared: the
   experiment's is the returning wave (0.86 at t* = 6.14), a run's is decided
   by the spikes of a point-sampled MPS pressure (4.65, 5.09 or 6.18 in runs
   that differ only in the solver's rounding).
