@@ -938,11 +938,16 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
 // and omega_r = 0.5 on the particles take the PCG on the developed 1M dam
 // break from 63 to 54 iterations (FP64, 0.7 on both: 63; 0.7 / 0.9 / 1.0 on
 // the cells with 0.5 on the particles: 56 / 55 / 54; early flow 37 -> 35).
+// The piecewise-constant prolongation of aggregation underestimates the
+// coarse correction; weighting it by 1.1 (below 2, so the V-cycle stays
+// positive definite) takes the developed flow to 49 and the early flow to 34
+// iterations (1.2: 46 and 37; 1.4: 51 and 48).
 constexpr int kAmgLdsPts = 2624;  // points of the levels the single-workgroup kernel holds in LDS (133 KB + the inverse)
 constexpr int kAmgDense = 64;
 constexpr int kAmgBlockThreads = 1024;
 constexpr float kAmgOmega = 1.0f;      // the cell levels' Jacobi smoother
 constexpr float kAmgOmegaRows = 0.5f;  // the particle-level (additive) Jacobi term
+constexpr float kAmgCorr = 1.1f;       // the cell levels' coarse-correction weight
 
 struct AmgLevel {
   uint32_t nx, nz;
@@ -958,6 +963,7 @@ struct AmgGrid {
   float* e;           // result (level 0: the correction per cell)
   const float* inv;   // inverse of the coarsest level (npts x npts, row-major)
   float omega;
+  float corr;  // weight of the coarse correction: x = x_pre + corr e(parent)
 };
 __device__ __forceinline__ uint32_t amg_npts(const AmgLevel& L) { return L.nx * L.nz; }
 
@@ -1023,7 +1029,7 @@ __device__ __forceinline__ void amg_down_point(const AmgGrid& g, const AmgVec& v
   v.r[C.off - v.base + P] = rc;
 }
 
-// Level l + 1 -> l: one thread per point p of level l: x = x_pre + e(parent),
+// Level l + 1 -> l: one thread per point p of level l: x = x_pre + corr e(parent),
 // then e_p = x_p + omega (r_p - (S x)_p) / S_pp.
 template <class SV>
 __device__ __forceinline__ float amg_up_value(const AmgGrid& g, const AmgVec& v, const SV& sten, int l, uint32_t p) {
@@ -1041,7 +1047,7 @@ __device__ __forceinline__ float amg_up_value(const AmgGrid& g, const AmgVec& v,
       const int jx = static_cast<int>(ix) + dx, jz = static_cast<int>(iz) + dz;
       const bool in = jx >= 0 && jz >= 0 && jx < static_cast<int>(F.nx) && jz < static_cast<int>(F.nz);
       const uint32_t ux = static_cast<uint32_t>(in ? jx : 0), uz = static_cast<uint32_t>(in ? jz : 0);
-      xq[k] = in ? v.x[fo + ux * F.nz + uz] + v.e[co + (ux >> 1) * C.nz + (uz >> 1)] : 0.0f;
+      xq[k] = in ? fmaf(g.corr, v.e[co + (ux >> 1) * C.nz + (uz >> 1)], v.x[fo + ux * F.nz + uz]) : 0.0f;
       sv[k] = in ? sten(l, k, p) : 0.0f;
     }
   float sum = 0.0f;
@@ -1249,7 +1255,7 @@ __global__ void __launch_bounds__(kAmgChainThreads) k_amg_up_chain(AmgGrid g, in
               const bool in = amg_in(F, jx, jz);
               const uint32_t q = F.off + static_cast<uint32_t>(in ? jx : 0) * F.nz + static_cast<uint32_t>(in ? jz : 0);
               const float ep = ce[((jx >> 1) - cox) * sc + ((jz >> 1) - coz)];
-              xq[kk] = in ? g.x[q] + ep : 0.0f;
+              xq[kk] = in ? fmaf(g.corr, ep, g.x[q]) : 0.0f;
               sv[kk] = in ? g.S[9ull * F.off + static_cast<size_t>(kk) * amg_npts(F) + p] : 0.0f;
             }
           float sum = 0.0f;
@@ -2438,6 +2444,7 @@ AmgGrid amg_grid(const Context& c) {
   g.e = c.amg_e;
   g.inv = c.amg_inv;
   g.omega = kAmgOmega;
+  g.corr = kAmgCorr;
   return g;
 }
 

@@ -18,11 +18,17 @@ validation/*.json (python -m openmps_amd.validate ...).
   cell-grid multigrid and 3.21 with the FP64 CG; plateau: the mean over
   3.2 <= t* <= 5.5, experiment 0.55, measured 0.34-0.37: this coarse spacing
   underpredicts it), its largest sample (no pressure blow-up) and the RMS
-  differences of h1, h2, p2 from the experiment (about 0.15-0.18 H,
-  0.17-0.21 H and 0.20-0.26).  The time of the largest sample is not compared: the
+  differences of h1, h2, p2 from the experiment.  The gauges are the
+  reference's (zhouetal1999.py:26-39: the highest particle with n > 0.1 n0
+  within l0 / 2 of the gauge line), so splashes count: before t* = 4 (the
+  collapse, the run-up, the impact) runs agree, h1 0.05-0.15 H, h2 0.06 H;
+  from t* = 4 on the reflected wave's splashes pass the gauges and runs that
+  differ only in the solver's rounding differ widely (h1 over 4 <= t* < 6:
+  0.14, 0.29 with the FP64 CG, 0.61 H; over t* <= 8: 0.15-0.34 H).  The
+  tight bounds are on t* < 4, the whole-range ones only catch gross
+  failures.  The time of the largest p2 sample is not compared either: the
   experiment's is the returning wave (0.86 at t* = 6.14), a run's is decided
-  by the spikes of a point-sampled MPS pressure (4.65, 5.09 or 6.18 in runs
-  that differ only in the solver's rounding).
+  by the spikes of a point-sampled MPS pressure (4.65, 5.09, 5.68 or 6.18).
   The mixed solvers stop at 1e-10 |b| on this scene: at 1e-8 |b| the mixed
   Jacobi and cell-grid multigrid runs let a particle near the impact
   accelerate to 100-300 m/s at t = 1.08-1.11 s (the CFL step collapses), the
@@ -57,6 +63,9 @@ def test_zhou_wall_pressure_and_gauge_heights():
     assert abs(imp["ours"]["arrival_t_star"] - imp["experiment"]["arrival_t_star"]) < 0.5
     assert 0.25 < imp["ours"]["plateau_mean"] < 0.75
     assert 0.5 < peak["p_over_rhogH"] < 1.6
-    assert v["experiment_h1"]["rms"] < 0.25
-    assert v["experiment_h2"]["rms"] < 0.25
+    assert v["experiment_h1"]["rms_before_t4"] < 0.2
+    assert v["experiment_h2"]["rms_before_t4"] < 0.12
+    assert v["experiment_p2"]["rms_before_t4"] < 0.4
+    assert v["experiment_h1"]["rms"] < 0.5
+    assert v["experiment_h2"]["rms"] < 0.35
     assert v["experiment_p2"]["rms"] < 0.32
