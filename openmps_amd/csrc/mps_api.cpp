@@ -140,7 +140,7 @@ void free_all(Context& c) {
   dfree(c.ml_flag); dfree(c.ml_off); dfree(c.ml_list); dfree(c.ml_active); dfree(c.st_replay);
   dfree(c.ml_pid); dfree(c.ml_dv); dfree(c.ml_sc); dfree(c.zrec);
   dfree(c.amg_S64); dfree(c.amg_S); dfree(c.amg_dinv); dfree(c.amg_r); dfree(c.amg_x); dfree(c.amg_e);
-  dfree(c.amg_inv); dfree(c.amg_rows); dfree(c.box_part);
+  dfree(c.amg_inv); dfree(c.amg_rows); dfree(c.box_part); dfree(c.amg_q9); dfree(c.amg_tq);
   if (c.flags_host) (void)hipHostFree(c.flags_host);
   if (c.nbr_total_host) (void)hipHostFree(c.nbr_total_host);
   if (c.ppe_counters_host) (void)hipHostFree(c.ppe_counters_host);

@@ -916,9 +916,15 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
 
 // ---------------- cell-grid aggregation multigrid (MPS_PRECOND_AMG) ----------------
 //
-// The preconditioner M r = omega_r D^-1 r + P V(P^T r): aggregates are the
-// neighbour-search cells (P piecewise constant over a cell's aggregated
-// rows), and V is one V-cycle on the cells' Galerkin operator P^T A P.  The
+// The preconditioner M r = omega_r D^-1 r + P_s V(P_s^T r): aggregates are
+// the neighbour-search cells (P piecewise constant over a cell's aggregated
+// rows), V is one V-cycle on the cells' Galerkin operator P^T A P, and P_s =
+// (I - omega_s D^-1 A) P is P smoothed by one Jacobi step.  Row i of P_s has
+// 9 entries, one per cell around i's own (A P's row is the row's stored
+// values summed by the direction of the column's cell, which the fill
+// computes), so P_s^T r and P_s e are 9-term sums per particle; the coarse
+// operator stays P^T A P (a non-Galerkin pair: the Galerkin P_s^T A P would be
+// a 25-point stencil).  The
 // cell side NL exceeds the Laplacian radius, so a particle couples only to
 // its own and the 8 adjacent cells: P^T A P is a 9-point stencil on the cell
 // grid, and so is every coarser level (2x2 blocks of the level below,
@@ -939,15 +945,21 @@ __global__ void __launch_bounds__(NTP) k_ml_pack(CellGrid G, const uint32_t* __r
 // break from 63 to 54 iterations (FP64, 0.7 on both: 63; 0.7 / 0.9 / 1.0 on
 // the cells with 0.5 on the particles: 56 / 55 / 54; early flow 37 -> 35).
 // The piecewise-constant prolongation of aggregation underestimates the
-// coarse correction; weighting it by 1.1 (below 2, so the V-cycle stays
+// coarse correction: weighting it by 1.1 (below 2, so the V-cycle stays
 // positive definite) takes the developed flow to 49 and the early flow to 34
-// iterations (1.2: 46 and 37; 1.4: 51 and 48).
+// iterations.  The smoothed P_s (omega_s = 1, corrections unweighted)
+// takes them to 36 and 22 (omega_s = 0.67: 37 and 23; P_s with the exact
+// coarse solve instead of V: 22 on the developed flow, P: 33); its FP16
+// coefficients cost no iterations, and since restriction and prolongation
+// use the same rounded values M stays symmetric.
 constexpr int kAmgLdsPts = 2624;  // points of the levels the single-workgroup kernel holds in LDS (133 KB + the inverse)
 constexpr int kAmgDense = 64;
 constexpr int kAmgBlockThreads = 1024;
 constexpr float kAmgOmega = 1.0f;      // the cell levels' Jacobi smoother
 constexpr float kAmgOmegaRows = 0.5f;  // the particle-level (additive) Jacobi term
-constexpr float kAmgCorr = 1.1f;       // the cell levels' coarse-correction weight
+constexpr float kAmgCorr = 1.0f;       // the cell levels' coarse-correction weight
+constexpr double kAmgOmegaP = 1.0;     // the smoothing of the prolongation
+constexpr int kAmgQWords = 3;          // P_s row: 9 FP16 coefficients in 3 x 8 bytes per packed slot
 
 struct AmgLevel {
   uint32_t nx, nz;
@@ -1379,7 +1391,29 @@ struct PrecArgs {
   float omega;
   const uint32_t* row_len;
   uint32_t bx0, bz0, bnx, bnz;  // the multigrid's level 0: cells [bx0, bx0 + bnx) x [bz0, bz0 + bnz)
+  // the smoothed prolongation (modes 1, 2): P_s's rows per packed slot, and
+  // mode 1's 9 direction sums per level-0 point, at tq[d * tq_stride + p]
+  const uint2* q9;
+  float* tq;
+  size_t tq_stride;
 };
+
+// P_s's row of packed slot (w, u, lane): 9 FP16 coefficients.
+__device__ __forceinline__ void load_q9(const uint2* __restrict__ q9, uint32_t w, int u, uint32_t lane, float q[9]) {
+  const size_t row = static_cast<size_t>(w) * kPerLane + u;
+  uint32_t h[6];
+#pragma unroll
+  for (int k = 0; k < kAmgQWords; k++) {
+    const uint2 v = q9[(row * kAmgQWords + k) * 64 + lane];
+    h[2 * k] = v.x;
+    h[2 * k + 1] = v.y;
+  }
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    const uint32_t bits = (h[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+    q[e] = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(bits)));
+  }
+}
 
 // Level-0 point of a cell of the multigrid's box, or -1 outside it.
 __device__ __forceinline__ int amg_point(const PrecArgs& a, uint32_t cell, uint32_t gnz) {
@@ -1479,7 +1513,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
     }
     // rows beyond the registers (over): aggregated as k_ml_pack decides
     auto agg_of = [&](uint32_t i) { return in_agg(G.type[i]) && (MODE == 0 || a.row_len[i] > 0); };
-    float s = 0.0f;
+    float s = 0.0f, s_over = 0.0f;
     if (MODE != 2) {
 #pragma unroll
       for (int u = 0; u < kPerLane; u++) {
@@ -1504,34 +1538,79 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
             ri = fmaf(-alpha, q[i], r[i]);
           }
           r[i] = ri;
-          if (agg_of(i)) s += ri;
+          if (agg_of(i)) s_over += ri;
           rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
         }
+      s += s_over;
     }
     if (MODE == 1) {
-      // the cell's residual sum: lanes 0, 16, 32, 48 hold one cell's members
-      float t = s + __shfl_xor(s, 16, 64);
-      t = t + __shfl_xor(t, 32, 64);
+      // the cell's 9 direction sums of P_s^T r: T_d = sum_i q_i[d] r_i over its
+      // aggregated rows (rows beyond the registers keep P's row: direction
+      // 4 only); lanes 0, 16, 32, 48 hold one cell's members
+      float t9[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) t9[e] = 0.0f;
+#pragma unroll
+      for (int u = 0; u < kPerLane; u++) {
+        if (id[u] != kNoSlot && !(id[u] & kOutAgg)) {
+          float qv9[9];
+          load_q9(a.q9, w, u, lane, qv9);
+#pragma unroll
+          for (int e = 0; e < 9; e++) t9[e] = fmaf(qv9[e], rv[u], t9[e]);
+        }
+      }
+      t9[4] += s_over;
       const int pt = L.valid ? amg_point(a, L.cell, G.gnz) : -1;
-      if (pt >= 0 && L.sub == 0) a.rc[pt] = t;
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        float t = t9[e] + __shfl_xor(t9[e], 16, 64);
+        t = t + __shfl_xor(t, 32, 64);
+        if (pt >= 0 && L.sub == 0) a.tq[static_cast<size_t>(e) * a.tq_stride + static_cast<uint32_t>(pt)] = t;
+      }
     } else if (MODE == 2) {
+      // z = omega_r r / A_ii + P_s e on aggregated rows: e of the 9 cells
+      // around the lane's cell (level 0 of the cycle; 0 outside the box)
       const int pt = L.valid ? amg_point(a, L.cell, G.gnz) : -1;
-      float ec;
+      float e9[9];
       if (amg) {
-        // the cycle's last up-pass (level 0) for the lane's cell, on the
-        // lane holding its first members; the other three lanes take it
+        // the cycle's last up-pass (level 0) folded in: lane l < 36 computes
+        // e of cell (l / 6, l % 6) of the 6x6 cells around the wavefront's
+        // 4x4 block, and each lane takes its cell's 9 from those lanes
+        const uint32_t b = list[w];
+        const int cx = static_cast<int>(4 * (b / G.bnz)) - 1 + static_cast<int>(lane / 6) - static_cast<int>(a.bx0);
+        const int cz = static_cast<int>(4 * (b % G.bnz)) - 1 + static_cast<int>(lane % 6) - static_cast<int>(a.bz0);
         float val = 0.0f;
-        if (pt >= 0 && L.sub == 0) val = amg_up_value(*amg, amg_global(*amg), AmgStenGlobal{amg}, 0, pt);
-        ec = __shfl(val, static_cast<int>(lane & 15u), 64);
+        if (lane < 36 && cx >= 0 && cz >= 0 && cx < static_cast<int>(a.bnx) && cz < static_cast<int>(a.bnz))
+          val = amg_up_value(*amg, amg_global(*amg), AmgStenGlobal{amg}, 0,
+                             static_cast<uint32_t>(cx) * a.bnz + static_cast<uint32_t>(cz));
+        const int ux = static_cast<int>((lane & 15u) >> 2), uz = static_cast<int>(lane & 3u);
+#pragma unroll
+        for (int e = 0; e < 9; e++) e9[e] = __shfl(val, (1 + ux + e / 3 - 1) * 6 + (1 + uz + e % 3 - 1), 64);
       } else {
-        ec = (pt >= 0) ? a.ec[pt] : 0.0f;
+        const int bx = (pt >= 0) ? pt / static_cast<int>(a.bnz) : 0, bz = (pt >= 0) ? pt % static_cast<int>(a.bnz) : 0;
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          const int x = bx + e / 3 - 1, zc = bz + e % 3 - 1;
+          const bool in = pt >= 0 && x >= 0 && zc >= 0 && x < static_cast<int>(a.bnx) && zc < static_cast<int>(a.bnz);
+          e9[e] = in ? a.ec[static_cast<uint32_t>(x) * a.bnz + static_cast<uint32_t>(zc)] : 0.0f;
+        }
       }
       const float om = a.omega;
 #pragma unroll
       for (int u = 0; u < kPerLane; u++) {
         if (id[u] != kNoSlot) {
           const uint32_t i = id[u] & ~kOutAgg;
-          const float zi = (id[u] & kOutAgg) ? dv[u] * rv[u] : fmaf(om * dv[u], rv[u], ec);
+          float zi;
+          if (id[u] & kOutAgg) {
+            zi = dv[u] * rv[u];
+          } else {
+            float qv9[9];
+            load_q9(a.q9, w, u, lane, qv9);
+            float ez = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 9; e++) ez = fmaf(qv9[e], e9[e], ez);
+            zi = fmaf(om * dv[u], rv[u], ez);
+          }
           z[static_cast<size_t>(i) * zs] = zi;
           rho = fma(static_cast<double>(rv[u]), static_cast<double>(zi), rho);
         }
@@ -1540,7 +1619,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
         for (uint32_t k = L.sub + 4u * kPerLane; k < cnt; k += 4) {
           const uint32_t i = G.sorted[st0 + k];
           const float ri = r[i];
-          const float zi = agg_of(i) ? fmaf(om * dinv[i], ri, ec) : dinv[i] * ri;
+          const float zi = agg_of(i) ? fmaf(om * dinv[i], ri, e9[4]) : dinv[i] * ri;
           z[static_cast<size_t>(i) * zs] = zi;
           rho = fma(static_cast<double>(ri), static_cast<double>(zi), rho);
         }
@@ -1582,6 +1661,48 @@ __global__ void __launch_bounds__(NTP) k_ml_prec(PrecArgs a, long long it) {
 // result and level 0's pre-smoothed iterate).
 __global__ void __launch_bounds__(NTP) k_ml_prec_up(PrecArgs a, long long it, AmgGrid g) {
   ml_prec_body<2>(a, it, whole_grid(), &g);
+}
+
+// The set-up of P_s's rows in packed-slot order (after k_ml_pack): slot o
+// of particle i (aggregated) gets q[d] = [d == 4] - omega_s (A P)_i,d / A_ii,
+// (A P)_i,d = rows9[9 i + d]; other slots 0.
+__global__ void __launch_bounds__(256) k_amg_q(size_t nslots, const uint32_t* __restrict__ pid,
+                                               const double* __restrict__ rows9, const double* __restrict__ diag,
+                                               double omega_s, uint2* __restrict__ q9) {
+  const size_t o = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (o >= nslots) return;
+  const uint32_t id = pid[o];
+  uint32_t h[2 * kAmgQWords] = {};
+  if (id != kNoSlot && !(id & kOutAgg)) {
+    const double sc = omega_s / diag[id];
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      const double v = (e == 4 ? 1.0 : 0.0) - sc * rows9[9ull * id + e];
+      const uint32_t bits = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+      h[e >> 1] |= bits << (16 * (e & 1));
+    }
+  }
+  const size_t row = o / 64, lane = o % 64;
+#pragma unroll
+  for (int k = 0; k < kAmgQWords; k++) q9[(row * kAmgQWords + k) * 64 + lane] = make_uint2(h[2 * k], h[2 * k + 1]);
+}
+
+// Level 0's right-hand side P_s^T r from pass 1's direction sums:
+// r0(p) = sum_d T_d(p - off(d)), T_d(c) the share of cell c's rows for the
+// cell c + off(d) (off(d) = (d / 3 - 1, d % 3 - 1)).
+__global__ void __launch_bounds__(256) k_amg_rfix(const float* __restrict__ tq, size_t stride, uint32_t nx,
+                                                  uint32_t nz, float* __restrict__ r0) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= nx * nz) return;
+  const int x = static_cast<int>(p / nz), zc = static_cast<int>(p % nz);
+  float s = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    const int sx = x - (e / 3 - 1), sz = zc - (e % 3 - 1);
+    if (sx >= 0 && sz >= 0 && sx < static_cast<int>(nx) && sz < static_cast<int>(nz))
+      s += tq[static_cast<size_t>(e) * stride + static_cast<uint32_t>(sx) * nz + static_cast<uint32_t>(sz)];
+  }
+  r0[p] = s;
 }
 
 // ---- set-up (FP64 sums, FP32 levels) ----
@@ -2554,11 +2675,10 @@ void amg_setup(Context& c) {
   check_launch();
 }
 
-// One V-cycle on the cell-grid levels: level 0's right-hand side (the cells'
-// residual sums) in, its result (the correction per cell) out.
-// The level-0 up-pass is left to the caller when `fold0` (k_ml_prec_up
-// does it) and the cycle has a level below the cells' outside the
-// single-workgroup kernel; returns whether it was left.
+// One V-cycle on the cell-grid levels: level 0's right-hand side in, its
+// result (the correction per cell) out.  The level-0 up-pass is left to the
+// caller when `fold0` (k_ml_prec_up does it) and the cycle has a level below
+// the cells' outside the single-workgroup kernel; returns whether it was left.
 bool amg_cycle(Context& c, bool fold0) {
   const AmgGrid g = amg_grid(c);
   const int l0 = amg_block_level(c);
@@ -2608,17 +2728,31 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
   uint32_t nact = 0;
   size_t pgrid = 1;
   bool packed = false;
-  // per pass over the particles (k_ml_prec<1>, then k_ml_prec_up or
-  // k_ml_prec<2>): r, q or z per particle, the packed ids and Jacobi factors
-  // per slot; the cycle: r, x, e and the stencils of every level, twice
-  double b_pass = n * 12.0;
-  const double b_cycle = static_cast<double>(c.amg_pts) * 2 * (36 + 16);
+  // per pass over the particles (k_ml_prec<1>, then k_ml_prec<2>): r, q or z
+  // per particle, the packed ids and Jacobi factors per slot, P_s's rows (24
+  // bytes per particle); the cycle: level 0's direction sums written and
+  // gathered (9 + 9 + 1 floats per point), then r, x, e and the stencils of
+  // every level, twice
+  double b_pass = n * 36.0;
+  const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
+  const double b_cycle = static_cast<double>(c.amg_pts) * 2 * (36 + 16) + npts0 * 19.0 * 4;
+  const size_t tq_stride = (npts0 + 63) & ~static_cast<size_t>(63);
+  if (c.amg_tq_cap < 9 * tq_stride) {
+    dfree(c.amg_tq);
+    c.amg_tq = dalloc<float>(9 * tq_stride);
+    c.amg_tq_cap = 9 * tq_stride;
+  }
+  // the direction sums of cells in unlisted (empty) blocks stay zero
+  MPS_HIP(hipMemsetAsync(c.amg_tq, 0, 9 * tq_stride * sizeof(float), c.stream));
   // it == 0 (the sweep's start) is accounted as a whole by the caller
   auto prec = [&](double norm, long long it) {
     PrecArgs a{G,         c.ml_list, nact,  c.ml_pid, c.ml_dv,    c.ml_sc, c.r,   1.0 / norm, c.r32,
                c.d32,     c.q32,     c.dinv32, zval, 4u,         p_pq,    L.grid, c.part_ml,  stride,
                c.st32,    c.amg_r,   c.amg_e,   kAmgOmegaRows, c.row_len, c.amg_box[0], c.amg_box[1],
                c.amg_box[2], c.amg_box[3]};
+    a.q9 = c.amg_q9;
+    a.tq = c.amg_tq;
+    a.tq_stride = tq_stride;
     auto track = [&](int kid, double bytes, auto&& f) {
       if (it == 0)
         f();
@@ -2629,7 +2763,11 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
       hipLaunchKernelGGL(k_ml_prec<1>, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it);
     });
     bool left = false;
-    track(MPS_K_AMG_CYCLE, b_cycle, [&] { left = amg_cycle(c, true); });
+    track(MPS_K_AMG_CYCLE, b_cycle, [&] {
+      hipLaunchKernelGGL(k_amg_rfix, dim3(grid_for(npts0)), dim3(256), 0, c.stream,
+                         static_cast<const float*>(c.amg_tq), tq_stride, c.amg_box[2], c.amg_box[3], c.amg_r);
+      left = amg_cycle(c, true);
+    });
     track(MPS_K_ML_PREC, b_pass, [&] {
       if (left)
         hipLaunchKernelGGL(k_ml_prec_up, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, a, it,
@@ -2646,7 +2784,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
     nact = static_cast<uint32_t>(*c.ml_active_host);
     b_spmv = b_base + 2.0 * mf_nnz_wide(c);
     pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
-    b_pass = n * 12.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 8);
+    b_pass = n * 36.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 8);
     if (!packed) {
       const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
       if (c.ml_pack_cap < slots) {
@@ -2664,6 +2802,19 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
                            static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
                            static_cast<const uint32_t*>(c.row_len));
       });
+      // P_s's rows in the same slot order
+      const size_t nslots = static_cast<size_t>(nact) * kPerLane * 64;
+      if (c.amg_q9_cap < nslots * kAmgQWords) {
+        dfree(c.amg_q9);
+        c.amg_q9 = dalloc<uint2>(std::max<size_t>(nslots, 64) * kAmgQWords);
+        c.amg_q9_cap = std::max<size_t>(nslots, 64) * kAmgQWords;
+      }
+      if (nslots > 0)
+        tracked(c, MPS_K_ML_SETUP, nslots * (4.0 + 8.0 * kAmgQWords) + n * 80.0, [&] {
+          hipLaunchKernelGGL(k_amg_q, dim3(grid_for(nslots)), dim3(256), 0, c.stream, nslots,
+                             static_cast<const uint32_t*>(c.ml_pid), static_cast<const double*>(c.amg_rows),
+                             static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9);
+        });
       packed = true;
     }
     MPS_HIP(hipMemsetAsync(c.st32, 0, sizeof(CgStatus), c.stream));

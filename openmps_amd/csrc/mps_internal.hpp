@@ -282,6 +282,11 @@ struct Context {
   float *amg_S = nullptr, *amg_dinv = nullptr, *amg_r = nullptr, *amg_x = nullptr, *amg_e = nullptr;
   float* amg_inv = nullptr;
   double* amg_rows = nullptr;
+  // the smoothed prolongation: its rows per packed slot (FP16) and pass 1's
+  // direction sums per level-0 point (9 planes)
+  uint2* amg_q9 = nullptr;
+  float* amg_tq = nullptr;
+  size_t amg_q9_cap = 0, amg_tq_cap = 0;
   IrStatus* ir = nullptr;
   IrStatus* ir_host = nullptr;
   long long last_iters = 0;
