@@ -1801,25 +1801,34 @@ __global__ void __launch_bounds__(256) k_amg_coarsen(const double* __restrict__ 
   for (int d = 0; d < 9; d++) Sc[d * nc + P] = acc[d];
 }
 
-// FP32 stencils and 1 / S_center (0 where the point is empty) of one level.
-__global__ void __launch_bounds__(256) k_amg_f32(const double* __restrict__ S64, uint32_t off, uint32_t npts,
-                                                 float* __restrict__ S32, float* __restrict__ dinv) {
-  const size_t k = blockIdx.x * 256ull + threadIdx.x;
-  if (k >= 9ull * npts) return;
-  S32[9ull * off + k] = static_cast<float>(S64[9ull * off + k]);
-  if (k < npts) {
-    const double c = S64[9ull * off + 4ull * npts + k];
-    dinv[off + k] = (c != 0.0) ? static_cast<float>(1.0 / c) : 0.0f;
+// FP32 copies of every level's stencils and 1 / S_pp (one thread per point
+// of any level; the padding between levels is skipped).
+__global__ void __launch_bounds__(256) k_amg_f32_all(AmgGrid g, const double* __restrict__ S64, size_t pts,
+                                                     float* __restrict__ S32, float* __restrict__ dinv) {
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= pts) return;
+  int l = 0;
+  for (int m = 1; m < g.levels; m++)
+    if (g.L[m].off <= k) l = m;
+  const AmgLevel L = g.L[l];
+  const uint32_t np = L.nx * L.nz, p = k - L.off;
+  if (p >= np) return;
+#pragma unroll
+  for (int d = 0; d < 9; d++) {
+    const size_t o = 9ull * L.off + static_cast<size_t>(d) * np + p;
+    S32[o] = static_cast<float>(S64[o]);
   }
+  const double c = S64[9ull * L.off + 4ull * np + p];
+  dinv[L.off + p] = (c != 0.0) ? static_cast<float>(1.0 / c) : 0.0f;
 }
 
 // The inverse of the coarsest level's operator (n <= kAmgDense points; empty
-// points become identity rows), Gauss-Jordan in FP64 in one workgroup.
-__global__ void __launch_bounds__(256) k_amg_dense(const double* __restrict__ S, uint32_t nx, uint32_t nz,
-                                                   float* __restrict__ inv) {
-  __shared__ double M[kAmgDense][2 * kAmgDense];
-  __shared__ double fcol[kAmgDense];
-  __shared__ double piv;
+// points become identity rows), Gauss-Jordan in FP64 in one workgroup's LDS:
+// pivot k turns buffer b into buffer 1 - b (row k divided by the pivot, the
+// other rows less their multiple of it), one barrier per pivot.
+__global__ void __launch_bounds__(1024) k_amg_dense(const double* __restrict__ S, uint32_t nx, uint32_t nz,
+                                                    float* __restrict__ inv) {
+  __shared__ double M[2][kAmgDense][2 * kAmgDense];
   const int n = static_cast<int>(nx * nz);
   for (int e = threadIdx.x; e < n * 2 * n; e += blockDim.x) {
     const int p = e / (2 * n), q = e % (2 * n);
@@ -1836,24 +1845,21 @@ __global__ void __launch_bounds__(256) k_amg_dense(const double* __restrict__ S,
         v = S[(3 * (qx - px + 1) + qz - pz + 1) * n + p];
       }
     }
-    M[p][q] = v;
+    M[0][p][q] = v;
   }
   __syncthreads();
+  int b = 0;
   for (int k = 0; k < n; k++) {
-    if (threadIdx.x == 0) piv = M[k][k];
-    __syncthreads();
-    const double pk = piv;
-    for (int q = threadIdx.x; q < 2 * n; q += blockDim.x) M[k][q] /= pk;
-    __syncthreads();
-    for (int p = threadIdx.x; p < n; p += blockDim.x) fcol[p] = M[p][k];
-    __syncthreads();
+    const double pk = M[b][k][k];
     for (int e = threadIdx.x; e < n * 2 * n; e += blockDim.x) {
       const int p = e / (2 * n), q = e % (2 * n);
-      if (p != k) M[p][q] -= fcol[p] * M[k][q];
+      const double rk = M[b][k][q] / pk;
+      M[1 - b][p][q] = (p == k) ? rk : M[b][p][q] - M[b][p][k] * rk;
     }
+    b = 1 - b;
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) inv[e] = static_cast<float>(M[e / n][n + e % n]);
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) inv[e] = static_cast<float>(M[b][e / n][n + e % n]);
 }
 
 // See above; FIRST is it == 1 (p = z, q = A z, tolerance from r_0.r_0).  The
@@ -2660,12 +2666,9 @@ void amg_setup(Context& c) {
                          dim3(256), 0, c.stream, static_cast<const double*>(c.amg_S64 + 9ull * c.amg_off[l]),
                          c.amg_nx[l], c.amg_nz[l], c.amg_S64 + 9ull * c.amg_off[l + 1], c.amg_nx[l + 1],
                          c.amg_nz[l + 1]);
-    for (int l = 0; l < lv; l++) {
-      const uint32_t np = c.amg_nx[l] * c.amg_nz[l];
-      hipLaunchKernelGGL(k_amg_f32, dim3(grid_for(9ull * np)), dim3(256), 0, c.stream,
-                         static_cast<const double*>(c.amg_S64), c.amg_off[l], np, c.amg_S, c.amg_dinv);
-    }
-    hipLaunchKernelGGL(k_amg_dense, dim3(1), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_amg_f32_all, dim3(grid_for(off)), dim3(256), 0, c.stream, amg_grid(c),
+                       static_cast<const double*>(c.amg_S64), off, c.amg_S, c.amg_dinv);
+    hipLaunchKernelGGL(k_amg_dense, dim3(1), dim3(1024), 0, c.stream,
                        static_cast<const double*>(c.amg_S64 + 9ull * c.amg_off[lv - 1]), c.amg_nx[lv - 1],
                        c.amg_nz[lv - 1], c.amg_inv);
     // cells of unlisted (empty) blocks keep a zero right-hand side
