@@ -48,6 +48,10 @@ struct Sell {
   const T* __restrict__ diag;
   size_t n;
   size_t rpb;  // rows per block (multiple of NT)
+  // optional: the columns as int16 offsets j - i (slices not flagged wide),
+  // 2 bytes a slot instead of col's 4
+  const int16_t* __restrict__ col16 = nullptr;
+  const uint8_t* __restrict__ wide = nullptr;
 };
 
 // Where a block sits in its scene's grid.  A single-scene launch is the whole
@@ -112,9 +116,34 @@ template <class T, class V>
 __device__ __forceinline__ T sell_row(const Sell<T>& A, size_t i, T s, V v) {
   const size_t base = A.off[i >> 6] + (i & 63);
   const uint32_t wdt = A.width[i >> 6];
-  const uint32_t* __restrict__ cp = A.col + base;
   const T* __restrict__ vp = A.val + base;
   uint32_t k = 0;
+  if (A.col16 && !A.wide[i >> 6]) {
+    // the same slots in the same order, columns from the 16-bit offsets
+    const int16_t* __restrict__ cp = A.col16 + base;
+    const int ii = static_cast<int>(i);
+    for (; k + 4 <= wdt; k += 4) {
+      const size_t o = static_cast<size_t>(k) * 64;
+      const uint32_t j0 = static_cast<uint32_t>(ii + __builtin_nontemporal_load(cp + o)),
+                     j1 = static_cast<uint32_t>(ii + __builtin_nontemporal_load(cp + o + 64)),
+                     j2 = static_cast<uint32_t>(ii + __builtin_nontemporal_load(cp + o + 128)),
+                     j3 = static_cast<uint32_t>(ii + __builtin_nontemporal_load(cp + o + 192));
+      const T a0 = __builtin_nontemporal_load(vp + o), a1 = __builtin_nontemporal_load(vp + o + 64),
+              a2 = __builtin_nontemporal_load(vp + o + 128), a3 = __builtin_nontemporal_load(vp + o + 192);
+      const T x0 = v(j0), x1 = v(j1), x2 = v(j2), x3 = v(j3);
+      s = fma(a0, x0, s);
+      s = fma(a1, x1, s);
+      s = fma(a2, x2, s);
+      s = fma(a3, x3, s);
+    }
+    for (; k < wdt; k++) {
+      const size_t o = static_cast<size_t>(k) * 64;
+      s = fma(__builtin_nontemporal_load(vp + o),
+              v(static_cast<uint32_t>(ii + __builtin_nontemporal_load(cp + o))), s);
+    }
+    return s;
+  }
+  const uint32_t* __restrict__ cp = A.col + base;
   for (; k + 4 <= wdt; k += 4) {
     const size_t o = static_cast<size_t>(k) * 64;
     const uint32_t j0 = __builtin_nontemporal_load(cp + o), j1 = __builtin_nontemporal_load(cp + o + 64),
@@ -2031,7 +2060,8 @@ Launch cg_grid(size_t n) {
 
 template <class T>
 Sell<T> sell(const Context& c, const T* val, const T* diag, size_t rpb) {
-  return Sell<T>{c.slice_off, c.slice_width, val, c.scol, diag, c.ppe_n, rpb};
+  return Sell<T>{c.slice_off,  c.slice_width, val, c.scol, diag, c.ppe_n, rpb,
+                 c.ppe_col16 ? c.col16 : nullptr, c.ppe_col16 ? c.slice_wide : nullptr};
 }
 
 // Reads the device status; folds the timed launches, dropping those that

@@ -180,6 +180,7 @@ struct Context {
   // the system was assembled from the current particle positions, so the
   // FP32 solver may recompute off-diagonals from them (matrix-free)
   bool ppe_from_particles = false;
+  bool ppe_col16 = false;  // the system's 16-bit column offsets (col16, slice_wide) are set: assembled by the fill
   long long ppe_search_serial = -1;  // search whose cells the assembled system matches
   // (x, z) and (u, w) as 16-byte pairs for the neighbour walks' gathers (one
   // load where two 8-byte ones would be): packed at the start of the step's

@@ -707,6 +707,7 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   c.ml_rowsums_ready = want_sums;
   c.amg_rows_ready = want9;
   c.ppe_from_particles = true;
+  c.ppe_col16 = true;
   c.ppe_search_serial = c.search_serial;
 }
 
@@ -828,6 +829,7 @@ void ppe_upload_csr(Context& c, size_t n, const uint64_t* rowptr, const uint32_t
   c.nnz_offdiag = nnz;
   c.have_ppe = true;
   c.have_ppe32 = false;
+  c.ppe_col16 = false;
   c.ppe_from_particles = false;
 }
 
