@@ -392,7 +392,13 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
         run += e.a;
       }
     };
-    if (staged) {
+    if (staged && nr <= 1) {
+      // one ascending run (the particles in cell order: the list visits the
+      // cells, and each cell's members, in ascending index order): the
+      // staged columns in their order
+#pragma unroll 4
+      for (uint32_t v = 0; v < ne; v++) add9(emit(stage[v][t]));
+    } else if (staged) {
       // 9-way merge of the staged runs (LDS only on its dependency chain),
       // the ascending columns written first, the values computed after
       uint32_t cur[kRuns], end[kRuns], hd[kRuns];
