@@ -42,7 +42,7 @@ ADVANCE = 500
 # run each; FETCH_SIZE scaled by 2, WRITE_SIZE by 1: what the counters read for
 # a copy kernel of known size), recorded with their commands in
 # profiles/dambreak_1m_developed_kernels.md
-TRAFFIC = {"cg_spmv32": 122.1e6, "cg_spmv": 439.0e6}
+TRAFFIC = {"cg_spmv32": 122.0e6, "cg_spmv": 372.3e6}
 
 
 def log(*a):

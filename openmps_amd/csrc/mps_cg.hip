@@ -2082,11 +2082,13 @@ long long next_chunk(long long done, long long predicted) {
 }
 
 // Algorithmic HBM bytes of one SpMV launch: every stored non-zero (value +
-// column index) once, the diagonal once, `extra_per_row` bytes of streamed
-// per-row vectors, and the slice table.
+// column index: 2 bytes, 4 in wide slices or without 16-bit offsets) once,
+// the diagonal once, `extra_per_row` bytes of streamed per-row vectors, and
+// the slice table.  The wide slices' count is the last one synchronised.
 double spmv_bytes(const Context& c, int vbytes, double extra_per_row) {
-  return static_cast<double>(c.nnz_offdiag) * (vbytes + 4) + c.ppe_n * (vbytes + extra_per_row) +
-         c.nslices * 12.0;
+  const double nnz = static_cast<double>(c.nnz_offdiag);
+  const double cols = c.ppe_col16 ? 2.0 * nnz + 2.0 * std::min(c.mf_nnz_wide, nnz) : 4.0 * nnz;
+  return nnz * vbytes + cols + c.ppe_n * (vbytes + extra_per_row) + c.nslices * 13.0;
 }
 
 // Plain FP64 CG (the reference algorithm); x = c.sol holds the warm start.

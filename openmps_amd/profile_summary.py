@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # own per-kernel statistics (mps.h MPS_K_*)
 KERNEL_IDS = {
     "k_cell_keys": "cell_keys", "k_cell_scatter": "cell_scatter",
-    "k_neighbors": "neighbors", "k_density<true, true, false>": "density_ecs",
+    "k_neighbors": "neighbors", "k_neighbors<true>": "neighbors", "k_neighbors<false>": "neighbors", "k_density<true, true, false>": "density_ecs",
     "k_density<true, false, true>": "density_save", "k_explicit": "explicit",
     "k_ppe_count": "ppe_count", "k_ppe_fill": "ppe_fill", "k_cg_init": "cg_init",
     "k_ppe_fill<true>": "ppe_fill", "k_ppe_fill<false>": "ppe_fill",
