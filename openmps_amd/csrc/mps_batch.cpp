@@ -187,7 +187,8 @@ void batch_forward(mps_batch& b, double dt_fixed, double* dt_out) {
           learned_features(c);
           learned[s] = 1;
         }
-        ml_prepare(c);
+        if (batch_amg(c)) amg_prepare(c);
+        else ml_prepare(c);
         MPS_HIP(hipEventRecord(b.ev[s], c.stream));
         joined[s] = 1;
       } else {

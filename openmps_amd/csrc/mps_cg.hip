@@ -1328,7 +1328,7 @@ struct AmgStenLds {
   }
 };
 static_assert(kAmgLdsPts % 4 == 0 && kAmgLdsFloats % 4 == 0, "float4 regions");
-__global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) {
+__device__ __forceinline__ void amg_block_body(const AmgGrid& g, int l0) {
   __shared__ float4 lds4[kAmgLdsFloats / 4];
   float* const lds = reinterpret_cast<float*>(lds4);
   const uint32_t base = g.L[l0].off;
@@ -1393,6 +1393,7 @@ __global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l
   }
   for (uint32_t p = threadIdx.x; p < n0; p += kAmgBlockThreads) g.e[base + p] = v.e[p];
 }
+__global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) { amg_block_body(g, l0); }
 
 // The operands of k_ml_prec.
 struct PrecArgs {
@@ -1719,10 +1720,8 @@ __global__ void __launch_bounds__(256) k_amg_q(size_t nslots, const uint32_t* __
 // Level 0's right-hand side P_s^T r from pass 1's direction sums:
 // r0(p) = sum_d T_d(p - off(d)), T_d(c) the share of cell c's rows for the
 // cell c + off(d) (off(d) = (d / 3 - 1, d % 3 - 1)).
-__global__ void __launch_bounds__(256) k_amg_rfix(const float* __restrict__ tq, size_t stride, uint32_t nx,
-                                                  uint32_t nz, float* __restrict__ r0) {
-  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-  if (p >= nx * nz) return;
+__device__ __forceinline__ void amg_rfix_point(const float* __restrict__ tq, size_t stride, uint32_t nx,
+                                               uint32_t nz, float* __restrict__ r0, uint32_t p) {
   const int x = static_cast<int>(p / nz), zc = static_cast<int>(p % nz);
   float s = 0.0f;
 #pragma unroll
@@ -1732,6 +1731,11 @@ __global__ void __launch_bounds__(256) k_amg_rfix(const float* __restrict__ tq, 
       s += tq[static_cast<size_t>(e) * stride + static_cast<uint32_t>(sx) * nz + static_cast<uint32_t>(sz)];
   }
   r0[p] = s;
+}
+__global__ void __launch_bounds__(256) k_amg_rfix(const float* __restrict__ tq, size_t stride, uint32_t nx,
+                                                  uint32_t nz, float* __restrict__ r0) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p < nx * nz) amg_rfix_point(tq, stride, nx, nz, r0, p);
 }
 
 // ---- set-up (FP64 sums, FP32 levels) ----
@@ -2983,6 +2987,63 @@ __global__ void __launch_bounds__(NTP) kg_ml_prec(const PrecArgs* __restrict__ t
   ml_prec_body<0>(a, it, f);
 }
 
+// The cell-grid multigrid of one scene in grouped launches: its levels, its
+// single-workgroup level and pass 1's direction sums (k_amg_rfix's input).
+// The grouped cycle runs every level above l0 with the per-level kernels
+// (the chains compute the same values, one launch per scene per chain).
+struct AmgArgs {
+  AmgGrid g;
+  int l0;
+  const float* tq;
+  size_t tq_stride;
+};
+template <int MODE>
+__global__ void __launch_bounds__(NTP) kg_ml_prec_m(const PrecArgs* __restrict__ tab, Group g, long long it) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const PrecArgs a = tab[s];
+  ml_prec_body<MODE>(a, it, f);
+}
+__global__ void __launch_bounds__(NTP) kg_ml_prec_up(const PrecArgs* __restrict__ tab,
+                                                     const AmgArgs* __restrict__ am, Group g, long long it) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const PrecArgs a = tab[s];
+  ml_prec_body<2>(a, it, f, &am[s].g);
+}
+__global__ void __launch_bounds__(256) kg_amg_rfix(const AmgArgs* __restrict__ am, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const AmgArgs& A = am[s];
+  const uint32_t nx = A.g.L[0].nx, nz = A.g.L[0].nz, p = f.bid * 256u + threadIdx.x;
+  if (p < nx * nz) amg_rfix_point(A.tq, A.tq_stride, nx, nz, A.g.r + A.g.L[0].off, p);
+}
+__global__ void __launch_bounds__(256) kg_amg_down(const AmgArgs* __restrict__ am, Group g, int l) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const AmgGrid& G = am[s].g;
+  const uint32_t P = f.bid * 256u + threadIdx.x;
+  if (P < amg_npts(G.L[l + 1])) amg_down_point(G, amg_global(G), AmgStenGlobal{&G}, l, P);
+}
+__global__ void __launch_bounds__(256) kg_amg_up(const AmgArgs* __restrict__ am, Group g, int l) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  const AmgGrid& G = am[s].g;
+  const uint32_t p = f.bid * 256u + threadIdx.x;
+  if (p < amg_npts(G.L[l])) amg_up_point(G, amg_global(G), AmgStenGlobal{&G}, l, p);
+}
+__global__ void __launch_bounds__(kAmgBlockThreads) kg_amg_block(const AmgArgs* __restrict__ am, Group g) {
+  int s;
+  Frame f;
+  if (!group_frame(g, s, f)) return;
+  amg_block_body(am[s].g, am[s].l0);
+}
+
 template <bool FIRST>
 __global__ void __launch_bounds__(NT, 8) kg_ml_spmv(const SpmvArgs<MatFreeRec>* __restrict__ tab, Group g,
                                                     long long it) {
@@ -3021,8 +3082,16 @@ __global__ void __launch_bounds__(64) kg_status_clear(const VecArgs* __restrict_
   if (threadIdx.x == 0) *tab[s].st = CgStatus{};
 }
 
-enum BatchKind { kResid = 0, kRed2, kPrec, kSpmv, kFinish, kCorrect, kClear, kKindCount };
+enum BatchKind {
+  kResid = 0, kRed2, kPrec, kSpmv, kFinish, kCorrect, kClear,
+  // the multigrid scenes: pass 1, pass 2 (with or without level 0's up
+  // step), level 0's right-hand side, the single-workgroup levels, and the
+  // down and up steps of each level
+  kPrec1, kPrec2, kPrecUp, kRfix, kBlock, kDown0, kUp0 = kDown0 + kAmgMaxLevels,
+  kKindCount = kUp0 + kAmgMaxLevels
+};
 static_assert(kKindCount <= BatchWork::kKinds, "block offset regions");
+static_assert(sizeof(AmgArgs) <= BatchWork::kArgBytes, "multigrid table entries");
 static_assert(sizeof(ResidArgs) <= BatchWork::kArgBytes && sizeof(PrecArgs) <= BatchWork::kArgBytes &&
                   sizeof(SpmvArgs<MatFreeRec>) <= BatchWork::kArgBytes && sizeof(VecArgs) <= BatchWork::kArgBytes,
               "argument tables");
@@ -3044,6 +3113,13 @@ struct SceneRun {
   long long inner = 0;
   double rr_last = -1;
   int flat = 0;
+  // the cell-grid multigrid: its single-workgroup level, the blocks of level
+  // 0's right-hand side and of each level's down (level l -> l + 1) and up
+  // (level l) step
+  bool amg = false;
+  int l0 = 0;
+  size_t rgrid = 1;
+  size_t dgrid[kAmgMaxLevels] = {}, ugrid[kAmgMaxLevels] = {};
 };
 
 // Grouped launches of one batched solve: argument tables (4 regions: residual,
@@ -3085,6 +3161,8 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
   VecArgs* vt = B.tab_host<VecArgs>(3);
   double* red_dev = reinterpret_cast<double*>(w.tab + 4 * w.cap * BatchWork::kArgBytes);
   double* red_host = reinterpret_cast<double*>(w.tab_host + 4 * w.cap * BatchWork::kArgBytes);
+  AmgArgs* at = B.tab_host<AmgArgs>(5);
+  bool any_amg = false;
   for (size_t s = 0; s < S; s++) {
     Context& c = *sc[s].c;
     const Launch L = sc[s].L;
@@ -3098,8 +3176,26 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
     st[s] = SpmvArgs<MatFreeRec>{matfree_rec(c, L.rpb), nullptr, c.pa32, c.q32, c.d32, c.part_ml, c.part_ml_cap,
                                  static_cast<int>(sc[s].pgrid), c.part3b, w.st + s, 1e-4};
     vt[s] = VecArgs{c.ppe_n, c.sol, c.d32, c.pa32, w.st + s, 1.0};
+    if (sc[s].amg) {
+      // pass 1 and 2 of the multigrid: no aggregate scales (k_ml_pack ran
+      // with the multigrid's rules), level 0's box, P_s's rows
+      pt[s].rc = c.amg_r;
+      pt[s].ec = c.amg_e;
+      pt[s].omega = kAmgOmegaRows;
+      pt[s].row_len = c.row_len;
+      pt[s].bx0 = c.amg_box[0];
+      pt[s].bz0 = c.amg_box[1];
+      pt[s].bnx = c.amg_box[2];
+      pt[s].bnz = c.amg_box[3];
+      pt[s].q9 = c.amg_q9;
+      pt[s].tq = c.amg_tq;
+      pt[s].tq_stride = c.amg_tq_stride;
+      at[s] = AmgArgs{amg_grid(c), sc[s].l0, c.amg_tq, c.amg_tq_stride};
+      any_amg = true;
+    }
   }
   B.upload_tab(0, sizeof(ResidArgs));
+  if (any_amg) B.upload_tab(5, sizeof(AmgArgs));
   auto launch_check = [&] {
     check_launch();
     w.launches++;
@@ -3154,7 +3250,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
       st[s].tol_rel = r.sweep_tol;
       vt[s].norm = r.norm;
       r.c->sweep_index = r.outer;
-      r.c->pred_kind = MPS_PRECOND_MULTILEVEL;
+      r.c->pred_kind = r.amg ? MPS_PRECOND_AMG : MPS_PRECOND_MULTILEVEL;
       any = true;
     }
     if (!any) break;
@@ -3166,10 +3262,73 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
     const unsigned ncl = B.group(kClear, sweeping, [](size_t) { return 1; });
     hipLaunchKernelGGL(kg_status_clear, dim3(ncl), dim3(64), 0, w.stream, B.tab_dev<VecArgs>(3), B.g(kClear));
     launch_check();
-    unsigned np = B.group(kPrec, sweeping, [&](size_t s) { return sc[s].pgrid; });
-    unsigned ns = B.group(kSpmv, sweeping, [&](size_t s) { return sc[s].L.grid; });
-    hipLaunchKernelGGL(kg_ml_prec, dim3(np), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), B.g(kPrec), 0LL);
-    launch_check();
+    // the preconditioner of every sweeping scene: the multilevel pass, or the
+    // multigrid's passes around its cycle (grouped per step; a scene joins
+    // the level steps it has)
+    unsigned np = 0, ns = 0;
+    unsigned na[kKindCount] = {};
+    int lmax = 0;
+    auto regroup = [&] {
+      auto ml = [&](size_t s) { return sc[s].sweeping && !sc[s].amg; };
+      auto mg = [&](size_t s) { return sc[s].sweeping && sc[s].amg; };
+      np = B.group(kPrec, ml, [&](size_t s) { return sc[s].pgrid; });
+      ns = B.group(kSpmv, sweeping, [&](size_t s) { return sc[s].L.grid; });
+      if (!any_amg) return;
+      lmax = 0;
+      for (size_t s = 0; s < S; s++)
+        if (mg(s)) lmax = std::max(lmax, sc[s].l0);
+      na[kPrec1] = B.group(kPrec1, mg, [&](size_t s) { return sc[s].pgrid; });
+      na[kRfix] = B.group(kRfix, mg, [&](size_t s) { return sc[s].rgrid; });
+      na[kBlock] = B.group(kBlock, mg, [](size_t) { return 1; });
+      na[kPrecUp] = B.group(kPrecUp, [&](size_t s) { return mg(s) && sc[s].l0 >= 1; },
+                            [&](size_t s) { return sc[s].pgrid; });
+      na[kPrec2] = B.group(kPrec2, [&](size_t s) { return mg(s) && sc[s].l0 == 0; },
+                           [&](size_t s) { return sc[s].pgrid; });
+      for (int l = 0; l < lmax; l++) {
+        na[kDown0 + l] = B.group(kDown0 + l, [&](size_t s) { return mg(s) && l < sc[s].l0; },
+                                 [&](size_t s) { return sc[s].dgrid[l]; });
+        if (l >= 1)
+          na[kUp0 + l] = B.group(kUp0 + l, [&](size_t s) { return mg(s) && l < sc[s].l0; },
+                                 [&](size_t s) { return sc[s].ugrid[l]; });
+      }
+    };
+    auto precondition = [&](long long it) {
+      if (np) {
+        hipLaunchKernelGGL(kg_ml_prec, dim3(np), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), B.g(kPrec), it);
+        launch_check();
+      }
+      if (!any_amg || na[kPrec1] == 0) return;
+      const AmgArgs* am = B.tab_dev<AmgArgs>(5);
+      hipLaunchKernelGGL(kg_ml_prec_m<1>, dim3(na[kPrec1]), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1),
+                         B.g(kPrec1), it);
+      launch_check();
+      hipLaunchKernelGGL(kg_amg_rfix, dim3(na[kRfix]), dim3(256), 0, w.stream, am, B.g(kRfix));
+      launch_check();
+      for (int l = 0; l < lmax; l++)
+        if (na[kDown0 + l]) {
+          hipLaunchKernelGGL(kg_amg_down, dim3(na[kDown0 + l]), dim3(256), 0, w.stream, am, B.g(kDown0 + l), l);
+          launch_check();
+        }
+      hipLaunchKernelGGL(kg_amg_block, dim3(na[kBlock]), dim3(kAmgBlockThreads), 0, w.stream, am, B.g(kBlock));
+      launch_check();
+      for (int l = lmax - 1; l >= 1; l--)
+        if (na[kUp0 + l]) {
+          hipLaunchKernelGGL(kg_amg_up, dim3(na[kUp0 + l]), dim3(256), 0, w.stream, am, B.g(kUp0 + l), l);
+          launch_check();
+        }
+      if (na[kPrecUp]) {
+        hipLaunchKernelGGL(kg_ml_prec_up, dim3(na[kPrecUp]), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), am,
+                           B.g(kPrecUp), it);
+        launch_check();
+      }
+      if (na[kPrec2]) {
+        hipLaunchKernelGGL(kg_ml_prec_m<2>, dim3(na[kPrec2]), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1),
+                           B.g(kPrec2), it);
+        launch_check();
+      }
+    };
+    regroup();
+    precondition(0);
     long long it = 1, pred = 0, lim = 0;
     for (size_t s = 0; s < S; s++)
       if (sc[s].sweeping) {
@@ -3187,8 +3346,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
           hipLaunchKernelGGL((kg_ml_spmv<false>), dim3(ns), dim3(NT), 0, w.stream,
                              B.tab_dev<SpmvArgs<MatFreeRec>>(2), B.g(kSpmv), it);
         launch_check();
-        hipLaunchKernelGGL(kg_ml_prec, dim3(np), dim3(NTP), 0, w.stream, B.tab_dev<PrecArgs>(1), B.g(kPrec), it);
-        launch_check();
+        precondition(it);
       }
       MPS_HIP(hipMemcpyAsync(w.st_host, w.st, S * sizeof(CgStatus), hipMemcpyDeviceToHost, w.stream));
       MPS_HIP(hipStreamSynchronize(w.stream));
@@ -3229,9 +3387,8 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
         launch_check();
       }
       if (changed) {
-        np = B.group(kPrec, sweeping, [&](size_t s) { return sc[s].pgrid; });
-        ns = B.group(kSpmv, sweeping, [&](size_t s) { return sc[s].L.grid; });
-        if (np == 0) break;
+        regroup();
+        if (ns == 0) break;
         lim = 0;
         for (size_t s = 0; s < S; s++)
           if (sc[s].sweeping) lim = std::max(lim, sc[s].limit);
@@ -3515,7 +3672,65 @@ void BatchWork::release() {
 bool batch_eligible(const Context& c) {
   const bool cells_match = c.have_neighbors && c.ppe_search_serial == c.search_serial && c.ppe_n == c.n;
   return c.n > 0 && c.have_ppe && c.ppe_from_particles && cells_match && c.solver.kind == MPS_SOLVER_MIXED_IR &&
-         c.solver.precond == MPS_PRECOND_MULTILEVEL;
+         (c.solver.precond == MPS_PRECOND_MULTILEVEL || c.solver.precond == MPS_PRECOND_AMG);
+}
+
+bool batch_amg(const Context& c) { return c.solver.precond == MPS_PRECOND_AMG; }
+
+void amg_prepare(Context& c) {
+  const size_t n = c.ppe_n;
+  ensure_zrec(c, n);
+  amg_setup(c);
+  const MlGeom g = ml_geom(c);
+  if (c.part_ml_cap < g.sgrid) {
+    dfree(c.part_ml);
+    c.part_ml = dalloc<double>(2 * g.sgrid);
+    c.part_ml_cap = g.sgrid;
+  }
+  MPS_HIP(hipStreamSynchronize(c.stream));  // the count of occupied cell blocks
+  const CellGrid G = ml_grid(c, g);
+  const uint32_t nact = static_cast<uint32_t>(*c.ml_active_host);
+  const size_t pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
+  const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
+  if (c.ml_pack_cap < slots) {
+    dfree(c.ml_pid);
+    dfree(c.ml_dv);
+    dfree(c.ml_sc);
+    c.ml_pid = dalloc<uint32_t>(slots * kPerLane);
+    c.ml_dv = dalloc<float>(slots * kPerLane);
+    c.ml_sc = dalloc<float4>(slots);
+    c.ml_pack_cap = slots;
+  }
+  // the multigrid's rules: no aggregate scales, rows without stored
+  // off-diagonals outside every aggregate (as solve_mixed_amg packs)
+  hipLaunchKernelGGL(k_ml_pack, dim3(static_cast<unsigned>(pgrid)), dim3(NTP), 0, c.stream, G,
+                     static_cast<const uint32_t*>(c.ml_list), nact, static_cast<const float*>(nullptr),
+                     static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
+                     static_cast<const uint32_t*>(c.row_len));
+  const size_t nslots = static_cast<size_t>(nact) * kPerLane * 64;
+  if (c.amg_q9_cap < std::max<size_t>(nslots, 64) * kAmgQWords) {
+    dfree(c.amg_q9);
+    c.amg_q9 = dalloc<uint2>(std::max<size_t>(nslots, 64) * kAmgQWords);
+    c.amg_q9_cap = std::max<size_t>(nslots, 64) * kAmgQWords;
+  }
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_amg_q, dim3(grid_for(nslots)), dim3(256), 0, c.stream, nslots,
+                       static_cast<const uint32_t*>(c.ml_pid), static_cast<const double*>(c.amg_rows),
+                       static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9);
+  const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
+  const size_t tq_stride = (npts0 + 63) & ~static_cast<size_t>(63);
+  if (c.amg_tq_cap < 9 * tq_stride) {
+    dfree(c.amg_tq);
+    c.amg_tq = dalloc<float>(9 * tq_stride);
+    c.amg_tq_cap = 9 * tq_stride;
+  }
+  c.amg_tq_stride = tq_stride;
+  MPS_HIP(hipMemsetAsync(c.amg_tq, 0, 9 * tq_stride * sizeof(float), c.stream));
+  hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
+                     c.zrec);
+  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
+  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+  check_launch();
 }
 
 void ml_prepare(Context& c) {
@@ -3559,6 +3774,15 @@ void batch_solve(std::vector<Context*>& scenes, BatchWork& w) {
     const uint32_t nact = static_cast<uint32_t>(*c->ml_active_host);
     r.pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
     r.vgrid = grid_for(n);
+    r.amg = batch_amg(*c);
+    if (r.amg) {
+      r.l0 = amg_block_level(*c);
+      r.rgrid = grid_for(static_cast<size_t>(c->amg_nx[0]) * c->amg_nz[0]);
+      for (int l = 0; l < r.l0; l++) {
+        r.dgrid[l] = grid_for(static_cast<size_t>(c->amg_nx[l + 1]) * c->amg_nz[l + 1]);
+        r.ugrid[l] = grid_for(static_cast<size_t>(c->amg_nx[l]) * c->amg_nz[l]);
+      }
+    }
     r.sr = StopRule{c->solver.stop_rule, (c->solver.tol > 0) ? c->solver.tol : c->eps};
     r.max_iter = (c->solver.max_iter > 0) ? c->solver.max_iter : static_cast<long long>(n);
     r.limit = inner_limit(*c, r.max_iter);

@@ -287,7 +287,7 @@ struct Context {
   // direction sums per level-0 point (9 planes)
   uint2* amg_q9 = nullptr;
   float* amg_tq = nullptr;
-  size_t amg_q9_cap = 0, amg_tq_cap = 0;
+  size_t amg_q9_cap = 0, amg_tq_cap = 0, amg_tq_stride = 0;
   IrStatus* ir = nullptr;
   IrStatus* ir_host = nullptr;
   long long last_iters = 0;
@@ -337,7 +337,7 @@ struct BatchWork {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t cap = 0;                       // scenes the buffers hold
-  static constexpr int kKinds = 8;      // regions: argument tables, block offsets of each grouped launch
+  static constexpr int kKinds = 56;     // regions: argument tables, block offsets of each grouped launch
   static constexpr size_t kArgBytes = 512;  // bytes reserved per scene and table
   unsigned char* tab = nullptr;         // device: kKinds x cap x kArgBytes
   unsigned char* tab_host = nullptr;    // pinned staging
@@ -354,9 +354,16 @@ struct BatchWork {
 // The multilevel mixed solver's per-scene set-up ahead of a batched solve, on
 // the scene's stream (synchronises it once for the block count).
 void ml_prepare(Context& c);
-// The scene's solve can join a batched solve (mixed precision, multilevel
-// preconditioner, a system assembled from the current particles and cells).
+// The cell-grid multigrid's per-scene set-up ahead of a batched solve (levels,
+// operators, packed slots, the smoothed prolongation's rows), on the scene's
+// stream (synchronises it once for the block count).
+void amg_prepare(Context& c);
+// The scene's solve can join a batched solve (mixed precision, the multilevel
+// or the cell-grid multigrid preconditioner, a system assembled from the
+// current particles and cells).
 bool batch_eligible(const Context& c);
+// ... with the multigrid (amg_prepare; otherwise ml_prepare)
+bool batch_amg(const Context& c);
 // Solves the assembled systems of the eligible scenes together on w.stream
 // (which must already wait for each scene's assembly and ml_prepare); each
 // scene's statistics, solution (c.sol) and final residual are set as its own

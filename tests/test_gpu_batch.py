@@ -4,7 +4,8 @@ run (all tests need a GPU).
 
 The batch (4 scenes of scenes.batch(count=4, seed=11): 40k and 95k-particle
 dam breaks, 37k and 192k-particle static columns, l0 from 0.54 to 1.92 mm)
-runs the mixed multilevel solver as grouped launches; every scene is also
+runs the mixed multilevel solver, or the cell-grid multigrid, as grouped
+launches; every scene is also
 run alone with the plain FP64 CG (the reference algorithm).  Per scene and
 step the pressure of the batched solve meets |b - A p| <= 1e-8 |b|,
 recomputed here in numpy on the scene's FP64 matrix.  In the first step both
@@ -97,6 +98,43 @@ def compare(cfgs, preset, steps, model=None):
 def test_batched_mixed_solve_against_fp64_alone():
     cfgs = scenes.batch(count=4, seed=11)
     compare(cfgs, "mixed_ml", 3)
+
+
+def test_batched_multigrid_solve_against_fp64_alone():
+    """The cell-grid multigrid (smoothed prolongation) in the grouped solve:
+    per scene its levels, its single-workgroup level and the per-level steps
+    as grouped launches."""
+    cfgs = scenes.batch(count=4, seed=11)
+    compare(cfgs, "mixed_amg", 3)
+
+
+def test_batched_multigrid_matches_each_scenes_own_solve():
+    """Multigrid and multilevel scenes in one batch: a multigrid scene's
+    batched step takes the iterations of its own ForwardTime with the same
+    solver and ends within rounding of it (the grouped per-level kernels
+    compute what the scene's own kernels compute; the grouped FP64 residual
+    norms are summed in another order, which moves the refinement's scale in
+    the last bit: measured 1.6e-20 relative in p after one step)."""
+    cfgs = scenes.batch(count=3, seed=11)
+    presets = ["mixed_amg", "mixed_ml", "mixed_amg"]
+    g = [make(cfg, pr) for cfg, pr in zip(cfgs, presets)]
+    o = [make(cfg, pr) for cfg, pr in zip(cfgs, presets)]
+    batch = Batch(g, host_threads=3)
+    for k in range(2):
+        batch.forward_time(1e-4)
+        for s in range(3):
+            o[s].forward_time(1e-4)
+            st, so = g[s].solve_stats(), o[s].solve_stats()
+            P, Q = g[s].particles(), o[s].particles()
+            rel = np.linalg.norm(P["p"] - Q["p"]) / np.linalg.norm(Q["p"])
+            say(f"  step {k + 1} scene {s} ({presets[s]}, {g[s].size()} particles): batched "
+                f"{st['iterations']} iterations, alone {so['iterations']}; p difference {rel:.2e}")
+            assert st["rel_residual"] <= TOL
+            if presets[s] == "mixed_amg":
+                assert st["iterations"] == so["iterations"]
+                assert rel < 1e-12
+                for key in ("x", "z"):
+                    assert np.abs(P[key] - Q[key]).max() < 1e-12 * cfgs[s]["l0"], key
 
 
 def test_batched_learned_solve_against_fp64_alone():
