@@ -144,7 +144,7 @@ def run_batch(args) -> int:
     from concurrent.futures import ThreadPoolExecutor
     from openmps_amd import Batch, learned, scenes, solvers
     cfgs = scenes.batch()
-    solver = args.solver or "learned_ml"
+    solver = args.solver or "learned_amg"
     model = learned.Model.load(args.weights or learned.DEFAULT_WEIGHTS) if solvers.is_learned(solver) else None
     log(f"building {len(cfgs)} scenes ...")
     sizes = [len(cfg["particles"]["x"]) for cfg in cfgs]
@@ -341,7 +341,7 @@ def main() -> int:
     ap.add_argument("--solver", default=None, choices=sorted(solvers.PRESETS),
                     help="headline solver (default: mixed_auto, FP32 CG with the faster of the "
                          "multilevel and the cell-grid multigrid preconditioners inside FP64 refinement; "
-                         "learned_ml, the multilevel one with the learned warm start, for batch64)")
+                         "learned_amg, the multigrid one with the learned warm start, for batch64)")
     ap.add_argument("--weights", default=None, help="warm-start MLP weights (.npz)")
     ap.add_argument("--ablation", action="store_true", help="also time further solver presets")
     ap.add_argument("--tol", type=float, default=1e-8)
