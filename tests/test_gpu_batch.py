@@ -137,6 +137,31 @@ def test_batched_multigrid_matches_each_scenes_own_solve():
                     assert np.abs(P[key] - Q[key]).max() < 1e-12 * cfgs[s]["l0"], key
 
 
+def test_batched_multigrid_small_scenes():
+    """Scenes whose multigrid levels all fit the single workgroup (level 0
+    included: level 0's right-hand side on its own, pass 2 without level 0's
+    up step) beside a multilevel scene: each multigrid scene takes the
+    iterations of its own solve and ends within rounding of it."""
+    names = [("dambreak_l0_0.004", "mixed_amg"), ("static_pressure", "mixed_amg"), ("dambreak_l0_0.004", "mixed_ml")]
+    cfgs = [scenes.config(nm, seed=2 + k) for k, (nm, _) in enumerate(names)]
+    g = [make(cfg, pr) for cfg, (_, pr) in zip(cfgs, names)]
+    o = [make(cfg, pr) for cfg, (_, pr) in zip(cfgs, names)]
+    batch = Batch(g, host_threads=3)
+    for k in range(3):
+        batch.forward_time(5e-6)
+        for s in range(3):
+            o[s].forward_time(5e-6)
+            st, so = g[s].solve_stats(), o[s].solve_stats()
+            P, Q = g[s].particles(), o[s].particles()
+            rel = np.linalg.norm(P["p"] - Q["p"]) / max(np.linalg.norm(Q["p"]), 1e-300)
+            say(f"  step {k + 1} scene {s} ({names[s][0]}, {names[s][1]}, {g[s].size()} particles): batched "
+                f"{st['iterations']} iterations, alone {so['iterations']}; p difference {rel:.2e}")
+            assert st["rel_residual"] <= TOL
+            if names[s][1] == "mixed_amg":
+                assert st["iterations"] == so["iterations"]
+                assert rel < 1e-12
+
+
 def test_batched_learned_solve_against_fp64_alone():
     """Learned warm start: one inference over the rows of all scenes."""
     from openmps_amd import learned
