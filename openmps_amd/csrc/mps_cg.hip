@@ -54,6 +54,14 @@ struct Sell {
   const uint8_t* __restrict__ wide = nullptr;
 };
 
+// Column of slot o of row i: with the 16-bit offsets the 32-bit columns are
+// stored for the wide slices only.
+template <class T>
+__device__ __forceinline__ uint32_t sell_col(const Sell<T>& A, size_t i, size_t o) {
+  if (A.col16 && !A.wide[i >> 6]) return static_cast<uint32_t>(static_cast<int>(i) + A.col16[o]);
+  return A.col[o];
+}
+
 // Where a block sits in its scene's grid.  A single-scene launch is the whole
 // grid, its blocks taken in XCD-aware order; a grouped launch (Group below)
 // concatenates the grids of several scenes, and each block runs as block
@@ -800,7 +808,7 @@ __global__ void __launch_bounds__(NT, 8) k_ml_rowsum(Sell<double> A, const uint3
       const uint32_t m = row_len[i];
       for (uint32_t q = 0; q < m; q++) {
         const size_t o = base + static_cast<size_t>(q) * 64;
-        const uint32_t kj = key[A.col[o]];
+        const uint32_t kj = key[sell_col(A, i, o)];
         if (kj == kNoCell) continue;
         const uint32_t jx = kj / gnz, jz = kj % gnz;
         const double a = A.val[o];
@@ -1763,7 +1771,7 @@ __global__ void __launch_bounds__(NT, 8) k_amg_rows(Sell<double> A, const uint32
       const size_t base = A.off[i >> 6] + (i & 63);
       for (uint32_t q = 0; q < m; q++) {
         const size_t o = base + static_cast<size_t>(q) * 64;
-        const uint32_t j = A.col[o];
+        const uint32_t j = sell_col(A, i, o);
         const uint32_t kj = key[j];
         if (kj == kNoCell || !in_agg(type[j]) || row_len[j] == 0) continue;
         const int dx = static_cast<int>(kj / gnz) - cx, dz = static_cast<int>(kj % gnz) - cz;

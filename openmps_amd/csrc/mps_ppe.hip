@@ -5,12 +5,13 @@
 //
 //   k_ppe_count  RHS b_i (HS + ECS, FP64), warm start x0_i = p_i, the
 //                off-diagonal count of row i and, per slice, the wavefront
-//                maximum of those counts;
+//                maximum of those counts and whether the slice is wide (a
+//                column j with j - i outside int16);
 //   scan         slice widths -> slice offsets;
 //   k_ppe_fill   HL off-diagonals 3 r_e / (n0 r^3) with ascending columns
-//                (u32, and int16 offsets j - i for the FP32 operator), zero
-//                padding, and the diagonal -sum(a_ij) including the SPP ghost
-//                (Computer.hpp:1290-1292).
+//                (int16 offsets j - i, and u32 columns in the wide slices
+//                only), zero padding, and the diagonal -sum(a_ij) including
+//                the SPP ghost (Computer.hpp:1290-1292).
 //
 // The reference builds per-row vectors in parallel and scatters them serially
 // into a uBLAS compressed matrix before copying to ViennaCL CSR
@@ -40,6 +41,29 @@ struct PpeConsts {
 
 constexpr int kWalk = 4;  // neighbours whose loads are issued together (walk_list)
 
+// Column j of row i fits the 16-bit offset j - i.
+__device__ __forceinline__ bool offset16(uint32_t j, size_t i) {
+  const long long off = static_cast<long long>(j) - static_cast<long long>(i);
+  return off >= -32768 && off <= 32767;
+}
+
+// Per slice (one wavefront of rows, all lanes converged): the longest row,
+// the stored count, and whether any row has a column beyond a 16-bit offset
+// (then the fill stores the slice's 32-bit columns too).
+__device__ __forceinline__ void count_slice(size_t n, size_t sl, uint32_t cnt, bool fits,
+                                            uint32_t* __restrict__ slice_width, uint8_t* __restrict__ slice_wide,
+                                            unsigned long long* nnz_total, unsigned long long* nnz_wide) {
+  const unsigned wmax = wave_max_u32(cnt);
+  const unsigned long long wsum = static_cast<unsigned long long>(wave_sum(static_cast<double>(cnt)));
+  const unsigned long long far = __ballot(!fits);
+  if ((threadIdx.x & 63) == 0 && sl * 64 < n) {
+    slice_width[sl] = wmax;
+    slice_wide[sl] = static_cast<uint8_t>(far != 0);
+    atomicAdd(nnz_total, wsum);
+    if (far) atomicAdd(nnz_wide, wsum);
+  }
+}
+
 // b and x0 (Computer.hpp:1185-1221), row lengths and slice widths.
 __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z,
@@ -52,11 +76,14 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
                                                       double* __restrict__ b, double* __restrict__ sol,
                                                       uint32_t* __restrict__ row_len,
                                                       uint32_t* __restrict__ slice_width,
+                                                      uint8_t* __restrict__ slice_wide,
                                                       uint2* __restrict__ qpos,
-                                                      unsigned long long* nnz_total) {
+                                                      unsigned long long* nnz_total,
+                                                      unsigned long long* nnz_wide) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   uint32_t cnt = 0;
+  bool fits = true;
   if (i < n) {
     // integer differences of q are exact, so r_ij in the FP32 operator
     // carries only the quantisation step (|domain| / 2^30)
@@ -75,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
       };
       walk_list<kWalk, F>(static_cast<uint32_t>(i), nb.count[i], nb.ids, nb.kmax,
                           [&](uint32_t j) { return F{x[j], z[j], u[j], w[j], type[j]}; },
-                          [&](uint32_t, const F& f) {
+                          [&](uint32_t j, const F& f) {
                             const double dx = f.x - xi, dz = f.z - zi;
                             const double r2 = dx * dx + dz * dz;
                             if (r2 < k.re2) {
@@ -83,7 +110,10 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
                               const double du = f.u - ui, dw = f.w - wi;
                               h += (dx * du + dz * dw) / (r * r * r);
                             }
-                            if (r2 < k.rl2 && f.t != kDummy) cnt++;
+                            if (r2 < k.rl2 && f.t != kDummy) {
+                              cnt++;
+                              fits = fits && offset16(j, i);
+                            }
                           });
       const double speed = (nws[i] < k.n0) ? 0.0 : -k.re * h;  // HS, :838-872
       b[i] = -k.rho / (k.n0 * k.dt) * (speed + ecs[i]);
@@ -91,15 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_ppe_count(size_t n, const double* __
     }
     row_len[i] = cnt;
   }
-  const unsigned wmax = wave_max_u32(cnt);
-  const unsigned long long wsum = static_cast<unsigned long long>(wave_sum(static_cast<double>(cnt)));
-  if ((threadIdx.x & 63) == 0) {
-    const size_t s = (blk * static_cast<size_t>(blockDim.x) + threadIdx.x) >> 6;
-    if (s * 64 < n) {
-      slice_width[s] = wmax;
-      atomicAdd(nnz_total, wsum);
-    }
-  }
+  count_slice(n, i >> 6, cnt, fits, slice_width, slice_wide, nnz_total, nnz_wide);
 }
 
 // The second ComputeNeighborDensities pass with SaveX (Computer.hpp:780-833,
@@ -120,13 +142,16 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
                                                               double* __restrict__ b, double* __restrict__ sol,
                                                               uint32_t* __restrict__ row_len,
                                                               uint32_t* __restrict__ slice_width,
+                                                              uint8_t* __restrict__ slice_wide,
                                                               uint2* __restrict__ qpos,
                                                               unsigned long long* nnz_total,
+                                                              unsigned long long* nnz_wide,
                                                               const double2* __restrict__ xz,
                                                               const double2* __restrict__ uw) {
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
   uint32_t cnt = 0;
+  bool fits = true;
   if (i < n) {
     const double xi = x[i], zi = z[i];
     qpos[i] = make_uint2(static_cast<uint32_t>(static_cast<long long>(rint((xi - k.qx0) * k.qinv))),
@@ -142,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
       const double ui = u[i], wi = w[i];
       double s = 0.0, h = 0.0;
       const uint32_t m = nb.count[i];
-      auto use = [&](double xj, double zj, double uj, double wj, uint8_t tj) {
+      auto use = [&](uint32_t j, double xj, double zj, double uj, double wj, uint8_t tj) {
         const double dx = xj - xi, dz = zj - zi;
         const double r2 = dx * dx + dz * dz;
         if (r2 < k.re2) {
@@ -151,7 +176,10 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
           const double du = uj - ui, dw = wj - wi;
           h += (dx * du + dz * dw) / (r * r * r);
         }
-        if (r2 < k.rl2 && tj != kDummy) cnt++;
+        if (r2 < k.rl2 && tj != kDummy) {
+          cnt++;
+          fits = fits && offset16(j, i);
+        }
       };
       // kWalk neighbours per round trip: their ids, then all their fields,
       // loaded before any is used; the sums still run in list order
@@ -169,12 +197,12 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
           tj[v] = type[jj[v]];
         }
 #pragma unroll
-        for (int v = 0; v < kWalk; v++) use(pj[v].x, pj[v].y, vj[v].x, vj[v].y, tj[v]);
+        for (int v = 0; v < kWalk; v++) use(jj[v], pj[v].x, pj[v].y, vj[v].x, vj[v].y, tj[v]);
       }
       for (; q < m; q++) {
         const uint32_t j = nb.ids[nbr_slot(static_cast<uint32_t>(i), q, nb.kmax)];
         const double2 pj = xz[j], vj = uw[j];
-        use(pj.x, pj.y, vj.x, vj.y, type[j]);
+        use(j, pj.x, pj.y, vj.x, vj.y, type[j]);
       }
       nws[i] = s;
       nd[i] = (s < k.n0) ? k.n0 : s;
@@ -184,15 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_density_ppe_count(size_t n, const do
     }
     row_len[i] = cnt;
   }
-  const unsigned wmax = wave_max_u32(cnt);
-  const unsigned long long wsum = static_cast<unsigned long long>(wave_sum(static_cast<double>(cnt)));
-  if ((threadIdx.x & 63) == 0) {
-    const size_t sl = (blk * static_cast<size_t>(blockDim.x) + threadIdx.x) >> 6;
-    if (sl * 64 < n) {
-      slice_width[sl] = wmax;
-      atomicAdd(nnz_total, wsum);
-    }
-  }
+  count_slice(n, i >> 6, cnt, fits, slice_width, slice_wide, nnz_total, nnz_wide);
 }
 
 // HL off-diagonals and diagonal (Computer.hpp:1232-1328).  ghost32 keeps the
@@ -259,7 +279,7 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
                                          const uint32_t* __restrict__ slice_width, double* __restrict__ val,
                                          uint32_t* __restrict__ col, int16_t* __restrict__ col16,
                                          double* __restrict__ diag, float* __restrict__ ghost32, const AggSums& ag,
-                                         uint32_t (*stage)[kBlock], bool& fits, uint32_t& kk,
+                                         uint32_t (*stage)[kBlock], bool wide, uint32_t& kk,
                                          const double2* __restrict__ xz) {
   (void)n;
   const int t = threadIdx.x;
@@ -345,10 +365,8 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
       const size_t o = base + static_cast<size_t>(kk) * 64;
       const double a = 3 * k.rl / k.n0l / (r * r * r);
       val[o] = a;
-      col[o] = j;
-      const long long off = static_cast<long long>(j) - static_cast<long long>(self);
-      fits = fits && off >= -32768 && off <= 32767;
-      col16[o] = static_cast<int16_t>(off);
+      if (wide) col[o] = j;
+      col16[o] = static_cast<int16_t>(static_cast<long long>(j) - static_cast<long long>(self));
       kk++;
       if (ki != kNoCell) {
         const uint32_t kj = ag.key[j];
@@ -456,7 +474,7 @@ __device__ __forceinline__ void fill_row(size_t n, size_t i, const double* __res
   for (uint32_t pad = kk; pad < width; pad++) {
     const size_t o = base + static_cast<size_t>(pad) * 64;
     val[o] = 0.0;
-    col[o] = self;
+    if (wide) col[o] = self;
     col16[o] = 0;
   }
   kk = stored;
@@ -483,30 +501,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
                                                      const uint64_t* __restrict__ slice_off,
                                                      const uint32_t* __restrict__ slice_width,
                                                      double* __restrict__ val, uint32_t* __restrict__ col,
-                                                     int16_t* __restrict__ col16, uint8_t* __restrict__ slice_wide,
-                                                     unsigned long long* nnz_wide, double* __restrict__ diag,
+                                                     int16_t* __restrict__ col16,
+                                                     const uint8_t* __restrict__ slice_wide, double* __restrict__ diag,
                                                      float* __restrict__ ghost32, AggSums ag,
                                                      const double2* __restrict__ xz) {
   // each thread's emitted columns in list order, [slot][thread] (conflict-free)
   __shared__ uint32_t stage[kStage][kBlock];
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const size_t i = blk * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  // int16 column offsets j - i beside the u32 columns (same slots): a slice
-  // (one wavefront of rows) is "wide" when any of its offsets does not fit
-  bool fits = true;
+  // int16 column offsets j - i in every slot; the u32 columns (same slots)
+  // only in the slices the count pass flagged wide (an offset beyond int16)
   uint32_t kk = 0;
   if (i < n)
     fill_row<ROWS9>(n, i, x, z, type, nb, k, nws, slice_off, slice_width, val, col, col16, diag, ghost32, ag,
-                    stage, fits, kk, xz);
-  const unsigned long long far = __ballot(!fits);
-  const double wsum = wave_sum(static_cast<double>(kk));
-  if ((threadIdx.x & 63) == 0) {
-    const size_t sl = i >> 6;
-    if (sl * 64 < n) {
-      slice_wide[sl] = static_cast<uint8_t>(far != 0);
-      if (far) atomicAdd(nnz_wide, static_cast<unsigned long long>(wsum));
-    }
-  }
+                    stage, slice_wide[i >> 6] != 0, kk, xz);
 }
 
 __global__ void __launch_bounds__(kBlock) k_to_f32(size_t n, const double* __restrict__ a,
@@ -669,9 +677,14 @@ void ppe_finish(Context& c, const PpeConsts& k) {
                          hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 1, c.slice_off + c.nslices, sizeof(uint64_t),
                          hipMemcpyDeviceToHost, c.stream));
+  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 2, c.ppe_counters + 1, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipStreamSynchronize(c.stream));
   c.nnz_offdiag = c.ppe_counters_host[0];
   c.sell_slots = c.ppe_counters_host[1];
+  // the wide slices' stored count (byte accounting only), from the count pass
+  c.mf_nnz_wide = static_cast<double>(c.ppe_counters_host[2]);
+  c.mf_wide_pending = false;
   c.amg_box_ready = false;
   if (want_box) amg_box_from_counters(c);
   // row sums of the multilevel preconditioner's aggregates, when that solver will run
@@ -687,7 +700,9 @@ void ppe_finish(Context& c, const PpeConsts& k) {
   }
   const AggSums ag{c.key,  static_cast<uint32_t>(c.gnz),    want_sums ? c.z32 : nullptr, c.r32, c.d32,
                    c.amg_rows, static_cast<const uint32_t*>(c.spare_key)};
-  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 14.0 +
+  // slots: value and 16-bit offset; the wide slices' stored entries also a 32-bit column
+  const double bytes = n * (16.0 + 1 + 8 + 4 + 8 + 4) + 4.0 * c.nbr_total + c.sell_slots * 10.0 +
+                       4.0 * c.mf_nnz_wide +
                        (want_sums ? n * 12.0 + c.nnz_offdiag * 4.0 : 0.0) +
                        (want9 ? n * (72.0 + 4) + c.nnz_offdiag * 4.0 : 0.0);  // k_agg_key's 13 N apart
   tracked(c, MPS_K_PPE_FILL, bytes, [&] {
@@ -695,21 +710,18 @@ void ppe_finish(Context& c, const PpeConsts& k) {
       hipLaunchKernelGGL(k_ppe_fill<true>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
                          static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                          static_cast<const uint64_t*>(c.slice_off),
-                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
-                         c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16,
+                         static_cast<const uint8_t*>(c.slice_wide), c.diag, c.ghost32, ag,
+                         static_cast<const double2*>(c.xz));
     else
       hipLaunchKernelGGL(k_ppe_fill<false>, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z,
                          static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                          static_cast<const uint64_t*>(c.slice_off),
-                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16, c.slice_wide,
-                         c.ppe_counters + 1, c.diag, c.ghost32, ag, static_cast<const double2*>(c.xz));
+                         static_cast<const uint32_t*>(c.slice_width), c.sval, c.scol, c.col16,
+                         static_cast<const uint8_t*>(c.slice_wide), c.diag, c.ghost32, ag,
+                         static_cast<const double2*>(c.xz));
   });
   check_launch();
-  // the wide nnz (byte accounting only) reaches the host with the solver's
-  // next synchronisation (mf_nnz_wide)
-  MPS_HIP(hipMemcpyAsync(c.ppe_counters_host + 2, c.ppe_counters + 1, sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, c.stream));
-  c.mf_wide_pending = true;
   c.ml_rowsums_ready = want_sums;
   c.amg_rows_ready = want9;
   c.ppe_from_particles = true;
@@ -728,7 +740,7 @@ void ppe_assemble(Context& c) {
     hipLaunchKernelGGL(k_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u, c.w,
                        c.p, static_cast<const uint8_t*>(c.type), nb, k, static_cast<const double*>(c.nws),
                        static_cast<const double*>(c.ecs), c.b, c.sol, c.row_len, c.slice_width,
-                       c.qpos, c.ppe_counters);
+                       c.slice_wide, c.qpos, c.ppe_counters, c.ppe_counters + 1);
   });
   check_launch();
   pack_state(c);  // the fill gathers the packed positions
@@ -746,7 +758,8 @@ void density_save_and_assemble(Context& c) {
     hipLaunchKernelGGL(k_density_ppe_count, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n, c.x, c.z, c.u,
                        c.w, c.p, static_cast<const uint8_t*>(c.type), nb, k, c.nd, c.nws,
                        static_cast<const double*>(c.ecs), c.ox, c.oz, c.b, c.sol, c.row_len, c.slice_width,
-                       c.qpos, c.ppe_counters, static_cast<const double2*>(c.xz),
+                       c.slice_wide, c.qpos, c.ppe_counters, c.ppe_counters + 1,
+                       static_cast<const double2*>(c.xz),
                        static_cast<const double2*>(c.uw));
   });
   check_launch();
@@ -845,12 +858,18 @@ void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uin
   const size_t n = c.ppe_n, ns = c.nslices;
   std::vector<double> dg(n), sv(c.sell_slots);
   std::vector<uint32_t> len(n), sc(c.sell_slots);
+  std::vector<int16_t> s16(c.ppe_col16 ? c.sell_slots : 0);
+  std::vector<uint8_t> wide(c.ppe_col16 ? ns : 0);
   std::vector<uint64_t> off(ns + 1);
   MPS_HIP(hipMemcpyAsync(dg.data(), c.diag, n * 8, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(len.data(), c.row_len, n * 4, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(off.data(), c.slice_off, (ns + 1) * 8, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(sv.data(), c.sval, c.sell_slots * 8, hipMemcpyDeviceToHost, c.stream));
   MPS_HIP(hipMemcpyAsync(sc.data(), c.scol, c.sell_slots * 4, hipMemcpyDeviceToHost, c.stream));
+  if (c.ppe_col16) {  // assembled by the fill: u32 columns in the wide slices only
+    MPS_HIP(hipMemcpyAsync(s16.data(), c.col16, c.sell_slots * 2, hipMemcpyDeviceToHost, c.stream));
+    MPS_HIP(hipMemcpyAsync(wide.data(), c.slice_wide, ns, hipMemcpyDeviceToHost, c.stream));
+  }
   MPS_HIP(hipStreamSynchronize(c.stream));
   rowptr.assign(n + 1, 0);
   col.clear();
@@ -859,7 +878,11 @@ void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uin
   for (size_t i = 0; i < n; i++) {
     row.clear();
     const size_t base = off[i / 64] + (i % 64);
-    for (uint32_t k = 0; k < len[i]; k++) row.emplace_back(sc[base + 64ull * k], sv[base + 64ull * k]);
+    const bool narrow = c.ppe_col16 && !wide[i / 64];
+    for (uint32_t k = 0; k < len[i]; k++) {
+      const size_t o = base + 64ull * k;
+      row.emplace_back(narrow ? static_cast<uint32_t>(static_cast<long long>(i) + s16[o]) : sc[o], sv[o]);
+    }
     row.emplace_back(static_cast<uint32_t>(i), dg[i]);
     std::stable_sort(row.begin(), row.end(),
                      [](const std::pair<uint32_t, double>& a, const std::pair<uint32_t, double>& b) {
