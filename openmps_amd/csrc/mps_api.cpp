@@ -645,6 +645,7 @@ int mps_set_solver(mps_ctx* ctx, const mps_solver_opts* opts) {
   if (opts->precond == MPS_PRECOND_AUTO && c.solver.precond != MPS_PRECOND_AUTO) {
     // the automatic choice starts over: both preconditioners are timed again
     c.auto_ms[0] = c.auto_ms[1] = -1.0;
+    c.auto_at[0] = c.auto_at[1] = 0;
     c.auto_best = MPS_PRECOND_MULTILEVEL;
     c.auto_solves = c.auto_next = 0;
     c.auto_interval = 16;

@@ -3487,6 +3487,17 @@ double replay_spmv(Context& c, int kid, int reps) {
 // preconditioner; the first of each sets the choice, later ones of the other
 // (the re-timings) switch it when faster and otherwise double the interval to
 // the next re-timing (16 .. 1024 solves).
+// A solve on the current choice this much slower than the other's last
+// timing, when that timing is at most kAutoFresh solves old, has the other
+// timed again on the next solve (right after a switch the flow can change the
+// two solvers' costs at different rates; an older timing says nothing about
+// the other's cost now, and timing it again is left to the interval).
+constexpr double kAutoSlack = 1.05;
+constexpr long long kAutoFresh = 16;
+// A timing of the other that loses grows the interval 2x, or 8x when it took
+// kAutoFar times the current choice's (far behind: not worth timing soon).
+constexpr double kAutoFar = 1.5;
+
 void auto_record(Context& c, double ms) {
   constexpr int ML = MPS_PRECOND_MULTILEVEL, MG = MPS_PRECOND_AMG;
   const int used = static_cast<int>(c.stats.precond_used);
@@ -3495,6 +3506,7 @@ void auto_record(Context& c, double ms) {
   const bool retime = !first && used != c.auto_best;
   const double best_ms = c.auto_ms[c.auto_best == ML ? 0 : 1];
   c.auto_ms[used == ML ? 0 : 1] = ms;
+  c.auto_at[used == ML ? 0 : 1] = c.auto_solves;
   c.auto_solves++;
   if (first) {
     if (c.auto_ms[0] >= 0 && c.auto_ms[1] >= 0) {
@@ -3504,15 +3516,19 @@ void auto_record(Context& c, double ms) {
     }
     return;
   }
-  if (retime) {
-    if (ms < best_ms) {
-      c.auto_best = used;
-      c.auto_interval = 16;
-    } else {
-      c.auto_interval = std::min<long long>(1024, 2 * c.auto_interval);
-    }
-    c.auto_next = c.auto_solves + c.auto_interval;
+  if (!retime) {
+    const int other = used == ML ? 1 : 0;
+    if (c.auto_solves - c.auto_at[other] <= kAutoFresh && ms > kAutoSlack * c.auto_ms[other])
+      c.auto_next = c.auto_solves;
+    return;
   }
+  if (ms < best_ms) {
+    c.auto_best = used;
+    c.auto_interval = 16;
+  } else {
+    c.auto_interval = std::min<long long>(1024, (ms >= kAutoFar * best_ms ? 8 : 2) * c.auto_interval);
+  }
+  c.auto_next = c.auto_solves + c.auto_interval;
 }
 
 void ppe_solve(Context& c) {

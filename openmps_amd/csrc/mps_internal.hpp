@@ -253,6 +253,7 @@ struct Context {
   // and the solve count at which the other one is timed again
   int precond_now = MPS_PRECOND_MULTILEVEL;
   double auto_ms[2] = {-1.0, -1.0};
+  long long auto_at[2] = {0, 0};  // solve count when auto_ms[k] was recorded
   int auto_best = MPS_PRECOND_MULTILEVEL;
   long long auto_solves = 0, auto_next = 0, auto_interval = 16;
   uint32_t* ml_flag = nullptr;  // 4x4 cell blocks holding particles: flags, then their scan
