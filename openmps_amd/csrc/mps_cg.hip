@@ -996,7 +996,8 @@ constexpr float kAmgOmega = 1.0f;      // the cell levels' Jacobi smoother
 constexpr float kAmgOmegaRows = 0.5f;  // the particle-level (additive) Jacobi term
 constexpr float kAmgCorr = 1.0f;       // the cell levels' coarse-correction weight
 constexpr double kAmgOmegaP = 1.0;     // the smoothing of the prolongation
-constexpr int kAmgQWords = 3;          // P_s row: 9 FP16 coefficients in 3 x 8 bytes per packed slot
+// P_s's rows per packed slot o: coefficients 0..7 as 8 FP16 in q9[o] (16
+// bytes), the ninth as one FP16 in a plane after them, q9c[o] (18 bytes a slot)
 
 struct AmgLevel {
   uint32_t nx, nz;
@@ -1431,26 +1432,23 @@ struct PrecArgs {
   uint32_t bx0, bz0, bnx, bnz;  // the multigrid's level 0: cells [bx0, bx0 + bnx) x [bz0, bz0 + bnz)
   // the smoothed prolongation (modes 1, 2): P_s's rows per packed slot, and
   // mode 1's 9 direction sums per level-0 point, at tq[d * tq_stride + p]
-  const uint2* q9;
+  const uint4* q9;
+  const uint16_t* q9c;
   float* tq;
   size_t tq_stride;
 };
 
 // P_s's row of packed slot (w, u, lane): 9 FP16 coefficients.
-__device__ __forceinline__ void load_q9(const uint2* __restrict__ q9, uint32_t w, int u, uint32_t lane, float q[9]) {
-  const size_t row = static_cast<size_t>(w) * kPerLane + u;
-  uint32_t h[6];
+__device__ __forceinline__ void load_q9(const PrecArgs& a, uint32_t w, int u, uint32_t lane, float q[9]) {
+  const size_t o = (static_cast<size_t>(w) * kPerLane + u) * 64 + lane;
+  const uint4 v = a.q9[o];
+  const uint32_t h[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int k = 0; k < kAmgQWords; k++) {
-    const uint2 v = q9[(row * kAmgQWords + k) * 64 + lane];
-    h[2 * k] = v.x;
-    h[2 * k + 1] = v.y;
-  }
-#pragma unroll
-  for (int e = 0; e < 9; e++) {
+  for (int e = 0; e < 8; e++) {
     const uint32_t bits = (h[e >> 1] >> (16 * (e & 1))) & 0xffffu;
     q[e] = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(bits)));
   }
+  q[8] = static_cast<float>(__builtin_bit_cast(_Float16, a.q9c[o]));
 }
 
 // Level-0 point of a cell of the multigrid's box, or -1 outside it.
@@ -1592,7 +1590,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
       for (int u = 0; u < kPerLane; u++) {
         if (id[u] != kNoSlot && !(id[u] & kOutAgg)) {
           float qv9[9];
-          load_q9(a.q9, w, u, lane, qv9);
+          load_q9(a, w, u, lane, qv9);
 #pragma unroll
           for (int e = 0; e < 9; e++) t9[e] = fmaf(qv9[e], rv[u], t9[e]);
         }
@@ -1643,7 +1641,7 @@ __device__ __forceinline__ void ml_prec_body(const PrecArgs& a, long long it, co
             zi = dv[u] * rv[u];
           } else {
             float qv9[9];
-            load_q9(a.q9, w, u, lane, qv9);
+            load_q9(a, w, u, lane, qv9);
             float ez = 0.0f;
 #pragma unroll
             for (int e = 0; e < 9; e++) ez = fmaf(qv9[e], e9[e], ez);
@@ -1706,11 +1704,12 @@ __global__ void __launch_bounds__(NTP) k_ml_prec_up(PrecArgs a, long long it, Am
 // (A P)_i,d = rows9[9 i + d]; other slots 0.
 __global__ void __launch_bounds__(256) k_amg_q(size_t nslots, const uint32_t* __restrict__ pid,
                                                const double* __restrict__ rows9, const double* __restrict__ diag,
-                                               double omega_s, uint2* __restrict__ q9) {
+                                               double omega_s, uint4* __restrict__ q9,
+                                               uint16_t* __restrict__ q9c) {
   const size_t o = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (o >= nslots) return;
   const uint32_t id = pid[o];
-  uint32_t h[2 * kAmgQWords] = {};
+  uint32_t h[5] = {};
   if (id != kNoSlot && !(id & kOutAgg)) {
     const double sc = omega_s / diag[id];
 #pragma unroll
@@ -1720,9 +1719,8 @@ __global__ void __launch_bounds__(256) k_amg_q(size_t nslots, const uint32_t* __
       h[e >> 1] |= bits << (16 * (e & 1));
     }
   }
-  const size_t row = o / 64, lane = o % 64;
-#pragma unroll
-  for (int k = 0; k < kAmgQWords; k++) q9[(row * kAmgQWords + k) * 64 + lane] = make_uint2(h[2 * k], h[2 * k + 1]);
+  q9[o] = make_uint4(h[0], h[1], h[2], h[3]);
+  q9c[o] = static_cast<uint16_t>(h[4]);
 }
 
 // Level 0's right-hand side P_s^T r from pass 1's direction sums:
@@ -2604,6 +2602,18 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
 
 // ---- cell-grid multigrid (MPS_PRECOND_AMG) ----
 
+// Capacity for P_s's rows of nslots packed slots (q9, then q9c) in c.amg_q9.
+void ensure_q9(Context& c, size_t nslots) {
+  const size_t m = std::max<size_t>(nslots, 64);
+  const size_t words = m + (m + 7) / 8;  // uint4 words: the rows, then the ninth coefficients
+  if (c.amg_q9_cap < words) {
+    dfree(c.amg_q9);
+    c.amg_q9 = dalloc<uint4>(words);
+    c.amg_q9_cap = words;
+  }
+  c.amg_q9c = reinterpret_cast<uint16_t*>(c.amg_q9 + m);
+}
+
 AmgGrid amg_grid(const Context& c) {
   AmgGrid g{};
   g.levels = c.amg_levels;
@@ -2776,11 +2786,11 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
   size_t pgrid = 1;
   bool packed = false;
   // per pass over the particles (k_ml_prec<1>, then k_ml_prec<2>): r, q or z
-  // per particle, the packed ids and Jacobi factors per slot, P_s's rows (24
+  // per particle, the packed ids and Jacobi factors per slot, P_s's rows (18
   // bytes per particle); the cycle: level 0's direction sums written and
   // gathered (9 + 9 + 1 floats per point), then r, x, e and the stencils of
   // every level, twice
-  double b_pass = n * 36.0;
+  double b_pass = n * 30.0;
   const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
   const double b_cycle = static_cast<double>(c.amg_pts) * 2 * (36 + 16) + npts0 * 19.0 * 4;
   const size_t tq_stride = (npts0 + 63) & ~static_cast<size_t>(63);
@@ -2798,6 +2808,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
                c.st32,    c.amg_r,   c.amg_e,   kAmgOmegaRows, c.row_len, c.amg_box[0], c.amg_box[1],
                c.amg_box[2], c.amg_box[3]};
     a.q9 = c.amg_q9;
+    a.q9c = c.amg_q9c;
     a.tq = c.amg_tq;
     a.tq_stride = tq_stride;
     auto track = [&](int kid, double bytes, auto&& f) {
@@ -2831,7 +2842,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
     nact = static_cast<uint32_t>(*c.ml_active_host);
     b_spmv = b_base + 2.0 * mf_nnz_wide(c);
     pgrid = std::max<size_t>(1, (nact + NTP / 64 - 1) / (NTP / 64));
-    b_pass = n * 36.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 8);
+    b_pass = n * 30.0 + static_cast<double>(nact) * 64 * (8.0 * kPerLane + 8);
     if (!packed) {
       const size_t slots = static_cast<size_t>(std::max<uint32_t>(nact, 1)) * 64;
       if (c.ml_pack_cap < slots) {
@@ -2851,16 +2862,12 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
       });
       // P_s's rows in the same slot order
       const size_t nslots = static_cast<size_t>(nact) * kPerLane * 64;
-      if (c.amg_q9_cap < nslots * kAmgQWords) {
-        dfree(c.amg_q9);
-        c.amg_q9 = dalloc<uint2>(std::max<size_t>(nslots, 64) * kAmgQWords);
-        c.amg_q9_cap = std::max<size_t>(nslots, 64) * kAmgQWords;
-      }
+      ensure_q9(c, nslots);
       if (nslots > 0)
-        tracked(c, MPS_K_ML_SETUP, nslots * (4.0 + 8.0 * kAmgQWords) + n * 80.0, [&] {
+        tracked(c, MPS_K_ML_SETUP, nslots * (4.0 + 18.0) + n * 80.0, [&] {
           hipLaunchKernelGGL(k_amg_q, dim3(grid_for(nslots)), dim3(256), 0, c.stream, nslots,
                              static_cast<const uint32_t*>(c.ml_pid), static_cast<const double*>(c.amg_rows),
-                             static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9);
+                             static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9, c.amg_q9c);
         });
       packed = true;
     }
@@ -3196,6 +3203,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
       pt[s].bnx = c.amg_box[2];
       pt[s].bnz = c.amg_box[3];
       pt[s].q9 = c.amg_q9;
+      pt[s].q9c = c.amg_q9c;
       pt[s].tq = c.amg_tq;
       pt[s].tq_stride = c.amg_tq_stride;
       at[s] = AmgArgs{amg_grid(c), sc[s].l0, c.amg_tq, c.amg_tq_stride};
@@ -3732,15 +3740,11 @@ void amg_prepare(Context& c) {
                      static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
                      static_cast<const uint32_t*>(c.row_len));
   const size_t nslots = static_cast<size_t>(nact) * kPerLane * 64;
-  if (c.amg_q9_cap < std::max<size_t>(nslots, 64) * kAmgQWords) {
-    dfree(c.amg_q9);
-    c.amg_q9 = dalloc<uint2>(std::max<size_t>(nslots, 64) * kAmgQWords);
-    c.amg_q9_cap = std::max<size_t>(nslots, 64) * kAmgQWords;
-  }
+  ensure_q9(c, nslots);
   if (nslots > 0)
     hipLaunchKernelGGL(k_amg_q, dim3(grid_for(nslots)), dim3(256), 0, c.stream, nslots,
                        static_cast<const uint32_t*>(c.ml_pid), static_cast<const double*>(c.amg_rows),
-                       static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9);
+                       static_cast<const double*>(c.diag), kAmgOmegaP, c.amg_q9, c.amg_q9c);
   const size_t npts0 = static_cast<size_t>(c.amg_box[2]) * c.amg_box[3];
   const size_t tq_stride = (npts0 + 63) & ~static_cast<size_t>(63);
   if (c.amg_tq_cap < 9 * tq_stride) {

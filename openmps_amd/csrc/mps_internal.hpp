@@ -286,7 +286,8 @@ struct Context {
   double* amg_rows = nullptr;
   // the smoothed prolongation: its rows per packed slot (FP16) and pass 1's
   // direction sums per level-0 point (9 planes)
-  uint2* amg_q9 = nullptr;
+  uint4* amg_q9 = nullptr;      // P_s's rows per packed slot (16 bytes), then their ninth coefficients
+  uint16_t* amg_q9c = nullptr;  // inside amg_q9
   float* amg_tq = nullptr;
   size_t amg_q9_cap = 0, amg_tq_cap = 0, amg_tq_stride = 0;
   IrStatus* ir = nullptr;
