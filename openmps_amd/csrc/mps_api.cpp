@@ -662,7 +662,17 @@ int mps_get_solver(const mps_ctx* ctx, mps_solver_opts* out) {
 
 int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out) {
   if (!ctx || !out) return MPS_ERR_BAD_ARG;
-  *out = ctx->c.stats;
+  // the solve time is read from the solve's events on first use (the
+  // statistics read changes no observable state; a last error stays set)
+  Context& c = const_cast<mps_ctx*>(ctx)->c;
+  try {
+    mps::settle_solve(c);
+  } catch (Failure& e) {
+    c.err = e.msg;
+    if (e.code == MPS_ERR_HIP) (void)hipGetLastError();
+    return e.code;
+  }
+  *out = c.stats;
   return MPS_OK;
 }
 

@@ -3429,6 +3429,7 @@ void batch_solve_run(std::vector<SceneRun>& sc, BatchWork& w) {
 
 int precond_for_system(Context& c) {
   constexpr int ML = MPS_PRECOND_MULTILEVEL, MG = MPS_PRECOND_AMG;
+  settle_solve(c);  // the last solve's time, for the automatic choice
   int pick = c.solver.precond;
   if (pick == MPS_PRECOND_AUTO) {
     if (c.auto_ms[0] < 0) pick = ML;
@@ -3539,8 +3540,22 @@ void auto_record(Context& c, double ms) {
   c.auto_next = c.auto_solves + c.auto_interval;
 }
 
+// The last solve's time from its events, read on first use: the solve itself
+// does not wait for its end event (its last FP64 residual has synchronised
+// the stream already); the automatic choice records it here.
+void settle_solve(Context& c) {
+  if (!c.solve_pending) return;
+  c.solve_pending = false;
+  MPS_HIP(hipEventSynchronize(c.solve_b));
+  float ms = 0;
+  MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
+  c.stats.solve_ms = ms;
+  if (c.solve_pending_auto) auto_record(c, ms);
+}
+
 void ppe_solve(Context& c) {
   if (!c.have_ppe) throw Failure{MPS_ERR_STATE, "no pressure Poisson system assembled"};
+  settle_solve(c);
   const size_t n = c.ppe_n;
   c.stats = mps_solve_stats{};
   c.replay_ready = 0;
@@ -3574,18 +3589,19 @@ void ppe_solve(Context& c) {
     msg = f.msg;
   }
   MPS_HIP(hipEventRecord(c.solve_b, c.stream));
-  MPS_HIP(hipEventSynchronize(c.solve_b));
-  float ms = 0;
-  MPS_HIP(hipEventElapsedTime(&ms, c.solve_a, c.solve_b));
-  c.stats.solve_ms = ms;
+  c.solve_pending = true;
+  c.solve_pending_auto = false;
   // true FP64 residual of the returned x (reported, not part of the solve;
   // the refinement's last residual is that one already)
   double rr = c.final_rr, bb = c.final_bb;
   if (rr < 0) residual64(c, rr, bb, false);
   c.stats.rel_residual = (bb > 0) ? std::sqrt(rr / bb) : std::sqrt(rr);
   c.stats.initial_rel_residual = (bb > 0) ? std::sqrt(c.stats.rr0 / bb) : std::sqrt(c.stats.rr0);
-  if (rc != MPS_OK) throw Failure{rc, msg};
-  auto_record(c, ms);
+  if (rc != MPS_OK) {
+    settle_solve(c);
+    throw Failure{rc, msg};
+  }
+  c.solve_pending_auto = true;
   if (c.dataset && on_particles) collect_after(c);
 }
 
@@ -3818,6 +3834,7 @@ void batch_solve(std::vector<Context*>& scenes, BatchWork& w) {
     r.inner_tol = c->solver.inner_tol > 0 ? c->solver.inner_tol : 1e-4;
     const long long learned = c->stats.learned_used;
     const double prev_rel = c->stats.prev_rel_residual;
+    settle_solve(*c);
     c->stats = mps_solve_stats{};
     c->stats.learned_used = learned;
     c->stats.prev_rel_residual = prev_rel;

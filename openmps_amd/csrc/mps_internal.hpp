@@ -326,6 +326,8 @@ struct Context {
   std::vector<EventPair> events;
   size_t events_used = 0;
   hipEvent_t solve_a = nullptr, solve_b = nullptr;
+  bool solve_pending = false;       // solve_b recorded, stats.solve_ms not read yet (settle_solve)
+  bool solve_pending_auto = false;  // ... and the automatic choice records that time
 
   std::string err;
 };
@@ -507,6 +509,7 @@ void ppe_download_csr(Context& c, std::vector<uint64_t>& rowptr, std::vector<uin
 void ensure_sell_capacity(Context& c, size_t nslices, size_t slots);
 void ensure_ppe_vectors(Context& c, size_t n);  // mps_api.cpp
 void ppe_solve(Context& c);              // mps_cg.hip
+void settle_solve(Context& c);           // mps_cg.hip: the last solve's time into stats (first use)
 void residual64(Context& c, double& rr, double& bb, bool track);
 void reduce2(Context& c, const double* a, const double* b, int nparts);  // -> c.red_host[0..1]
 void residual64_launch(Context& c);  // r = b - A x; |r|^2, |b|^2 -> c.red_out[0..1] (no sync)
