@@ -1014,8 +1014,16 @@ struct AmgGrid {
   const float* inv;   // inverse of the coarsest level (npts x npts, row-major)
   float omega;
   float corr;  // weight of the coarse correction: x = x_pre + corr e(parent)
+  // the inner solve whose iteration `it` the cycle belongs to (nullptr: none);
+  // a cycle of an iteration past the solve's stop does nothing
+  const CgStatus* st = nullptr;
+  long long it = 0;
 };
 __device__ __forceinline__ uint32_t amg_npts(const AmgLevel& L) { return L.nx * L.nz; }
+
+// Block-uniform: the cycle belongs to an iteration launched past the stop
+// (every thread of the block calls it, before any divergent exit).
+__device__ __forceinline__ bool amg_skip(const AmgGrid& g) { return g.st && stopped_before(g.st, g.it); }
 
 // The per-point vectors of a cycle: r, x, e and 1 / S_center of point k of
 // any level at v[k - base] (global memory: base 0; the single-workgroup
@@ -1125,11 +1133,13 @@ struct AmgStenGlobal {
 __device__ __forceinline__ AmgVec amg_global(const AmgGrid& g) { return AmgVec{g.r, g.x, g.e, g.dinv, 0u}; }
 
 __global__ void __launch_bounds__(256) k_amg_down(AmgGrid g, int l) {
+  if (amg_skip(g)) return;
   const uint32_t P = blockIdx.x * 256u + threadIdx.x;
   if (P < amg_npts(g.L[l + 1])) amg_down_point(g, amg_global(g), AmgStenGlobal{&g}, l, P);
 }
 
 __global__ void __launch_bounds__(256) k_amg_up(AmgGrid g, int l) {
+  if (amg_skip(g)) return;
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p < amg_npts(g.L[l])) amg_up_point(g, amg_global(g), AmgStenGlobal{&g}, l, p);
 }
@@ -1170,6 +1180,7 @@ struct AmgDown {
 
 template <int T, int NLEV>
 __global__ void __launch_bounds__(kAmgChainThreads) k_amg_down_chain(AmgGrid g, int l1) {
+  if (amg_skip(g)) return;
   using D = AmgDown<T, NLEV>;
   static_assert(D::side(NLEV) == T && D::side(NLEV - 1) == 2 * T + 2 && D::floats > 0, "chain regions");
   __shared__ float lds[D::floats > 0 ? D::floats : 1];
@@ -1265,6 +1276,7 @@ struct AmgUp {
 
 template <int T, int NLEV>
 __global__ void __launch_bounds__(kAmgChainThreads) k_amg_up_chain(AmgGrid g, int l1) {
+  if (amg_skip(g)) return;
   using U = AmgUp<T, NLEV>;
   __shared__ float lds[U::floats];
   const AmgLevel top = g.L[l1 + NLEV];
@@ -1402,7 +1414,10 @@ __device__ __forceinline__ void amg_block_body(const AmgGrid& g, int l0) {
   }
   for (uint32_t p = threadIdx.x; p < n0; p += kAmgBlockThreads) g.e[base + p] = v.e[p];
 }
-__global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) { amg_block_body(g, l0); }
+__global__ void __launch_bounds__(kAmgBlockThreads) k_amg_block(AmgGrid g, int l0) {
+  if (amg_skip(g)) return;
+  amg_block_body(g, l0);
+}
 
 // The operands of k_ml_prec.
 struct PrecArgs {
@@ -1739,7 +1754,9 @@ __device__ __forceinline__ void amg_rfix_point(const float* __restrict__ tq, siz
   r0[p] = s;
 }
 __global__ void __launch_bounds__(256) k_amg_rfix(const float* __restrict__ tq, size_t stride, uint32_t nx,
-                                                  uint32_t nz, float* __restrict__ r0) {
+                                                  uint32_t nz, float* __restrict__ r0, const CgStatus* st,
+                                                  long long it) {
+  if (stopped_before(st, it)) return;  // an iteration past the stop
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p < nx * nz) amg_rfix_point(tq, stride, nx, nz, r0, p);
 }
@@ -2736,8 +2753,12 @@ void amg_setup(Context& c) {
 // result (the correction per cell) out.  The level-0 up-pass is left to the
 // caller when `fold0` (k_ml_prec_up does it) and the cycle has a level below
 // the cells' outside the single-workgroup kernel; returns whether it was left.
-bool amg_cycle(Context& c, bool fold0) {
-  const AmgGrid g = amg_grid(c);
+// The cycle of inner iteration `it` of the solve with status `st` does
+// nothing once that solve has stopped before it (launches past the stop).
+bool amg_cycle(Context& c, bool fold0, const CgStatus* st, long long it) {
+  AmgGrid g = amg_grid(c);
+  g.st = st;
+  g.it = it;
   const int l0 = amg_block_level(c);
   // the kAmgChainLevels levels above the single-workgroup ones in one launch
   // each way (level 0 keeps its own kernels)
@@ -2823,8 +2844,9 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
     bool left = false;
     track(MPS_K_AMG_CYCLE, b_cycle, [&] {
       hipLaunchKernelGGL(k_amg_rfix, dim3(grid_for(npts0)), dim3(256), 0, c.stream,
-                         static_cast<const float*>(c.amg_tq), tq_stride, c.amg_box[2], c.amg_box[3], c.amg_r);
-      left = amg_cycle(c, true);
+                         static_cast<const float*>(c.amg_tq), tq_stride, c.amg_box[2], c.amg_box[3], c.amg_r,
+                         static_cast<const CgStatus*>(c.st32), it);
+      left = amg_cycle(c, true, c.st32, it);
     });
     track(MPS_K_ML_PREC, b_pass, [&] {
       if (left)
