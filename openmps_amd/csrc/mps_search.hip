@@ -73,6 +73,20 @@ __device__ __forceinline__ uint32_t run_atomic_add(uint32_t* counter, uint32_t k
   return __shfl(old, start, 64) + static_cast<uint32_t>(lane - start);
 }
 
+// The search's counters cleared in one launch: cell counts and fill
+// positions, the flags and the list total.
+__global__ void __launch_bounds__(kBlock) k_search_clear(size_t ncells, uint32_t* __restrict__ count,
+                                                         uint32_t* __restrict__ fill, int* __restrict__ flags,
+                                                         unsigned long long* __restrict__ total) {
+  const size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+  if (i < ncells) {
+    count[i] = 0;
+    fill[i] = 0;
+  }
+  if (i < 4) flags[i] = 0;
+  if (i == 0) *total = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_cell_keys(GridParams g, size_t n, const double* __restrict__ x,
                                                       const double* __restrict__ z, uint8_t* __restrict__ type,
                                                       uint32_t* __restrict__ key, uint32_t* cell_count,
@@ -491,10 +505,9 @@ void search(Context& c) {
   const double nl2 = c.env.neighbor_length * c.env.neighbor_length;
   GridParams g{c.env.min_x, c.env.min_z, c.env.neighbor_length, c.gnx, c.gnz, c.cap,
                nl2 * (1.0 - 0x1p-48), nl2 * (1.0 + 0x1p-48)};
-  MPS_HIP(hipMemsetAsync(c.cell_count, 0, c.ncells * sizeof(uint32_t), c.stream));
-  MPS_HIP(hipMemsetAsync(c.cell_fill, 0, c.ncells * sizeof(uint32_t), c.stream));
-  MPS_HIP(hipMemsetAsync(c.flags, 0, 4 * sizeof(int), c.stream));
-  MPS_HIP(hipMemsetAsync(c.nbr_total_dev, 0, sizeof(unsigned long long), c.stream));
+  hipLaunchKernelGGL(k_search_clear, dim3(grid_for(std::max<size_t>(c.ncells, 4))), dim3(kBlock), 0, c.stream,
+                     c.ncells, c.cell_count, c.cell_fill, c.flags, c.nbr_total_dev);
+  check_launch();
   if (n == 0) {
     c.have_neighbors = true;
     return;
