@@ -354,11 +354,16 @@ __device__ __forceinline__ float rec_apply(const MatFreeRec& A, size_t i, float&
 }
 
 // {qx, qz, 0, 0} for every row (rows outside every cell keep z = 0).
-__global__ void __launch_bounds__(256) k_rec_init(size_t n, const uint2* __restrict__ qpos, uint4* __restrict__ rec) {
+// The records' positions (z = 0), and the FP32 r and d cleared: rows that
+// no preconditioner pass visits keep r = d = 0.
+__global__ void __launch_bounds__(256) k_rec_init(size_t n, const uint2* __restrict__ qpos, uint4* __restrict__ rec,
+                                                  float* __restrict__ r, float* __restrict__ d) {
   const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (i < n) {
     const uint2 q = qpos[i];
     rec[i] = make_uint4(q.x, q.y, 0u, 0u);
+    r[i] = 0.0f;
+    d[i] = 0.0f;
   }
 }
 
@@ -2542,9 +2547,7 @@ void solve_mixed_ml(Context& c, double b_base, const StopRule& sr, long long max
   const CellGrid G = ml_grid(c, g);
   // rows outside every cell keep r = d = z = 0; the records take the positions
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
-                     c.zrec);
-  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+                     c.zrec, c.r32, c.d32);
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
@@ -2797,9 +2800,7 @@ void solve_mixed_amg(Context& c, double b_base, const StopRule& sr, long long ma
   const size_t stride = c.part_ml_cap;
   const CellGrid G = ml_grid(c, g);
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
-                     c.zrec);
-  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+                     c.zrec, c.r32, c.d32);
   check_launch();
   double* p_pq = c.part3b;
   const long long limit = inner_limit(c, max_iter);
@@ -3793,9 +3794,7 @@ void amg_prepare(Context& c) {
   c.amg_tq_stride = tq_stride;
   MPS_HIP(hipMemsetAsync(c.amg_tq, 0, 9 * tq_stride * sizeof(float), c.stream));
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
-                     c.zrec);
-  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+                     c.zrec, c.r32, c.d32);
   check_launch();
 }
 
@@ -3823,9 +3822,7 @@ void ml_prepare(Context& c) {
                      static_cast<const float*>(c.dinv32), c.ml_pid, c.ml_dv, c.ml_sc,
                      static_cast<const uint32_t*>(nullptr));
   hipLaunchKernelGGL(k_rec_init, dim3(grid_for(n)), dim3(256), 0, c.stream, n, static_cast<const uint2*>(c.qpos),
-                     c.zrec);
-  MPS_HIP(hipMemsetAsync(c.r32, 0, n * sizeof(float), c.stream));
-  MPS_HIP(hipMemsetAsync(c.d32, 0, n * sizeof(float), c.stream));
+                     c.zrec, c.r32, c.d32);
   check_launch();
 }
 
