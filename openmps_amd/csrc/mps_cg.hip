@@ -2745,8 +2745,8 @@ void amg_setup(Context& c) {
     hipLaunchKernelGGL(k_amg_dense, dim3(1), dim3(1024), 0, c.stream,
                        static_cast<const double*>(c.amg_S64 + 9ull * c.amg_off[lv - 1]), c.amg_nx[lv - 1],
                        c.amg_nz[lv - 1], c.amg_inv);
-    // cells of unlisted (empty) blocks keep a zero right-hand side
-    MPS_HIP(hipMemsetAsync(c.amg_r, 0, npts0 * sizeof(float), c.stream));
+    // (level 0's right-hand side needs no clearing: k_amg_rfix writes every
+    // level-0 point each cycle, from direction sums cleared per solve)
   });
   MPS_HIP(hipMemcpyAsync(c.ml_active_host, c.ml_active, sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
   check_launch();
