@@ -189,6 +189,10 @@ typedef struct mps_solve_stats {
   int64_t fallback;         /* mixed: 1 when a refinement sweep stalled and the FP64 CG finished */
   int64_t precond_used;     /* mixed: the preconditioner the FP32 CG ran with (mps_precond) */
 } mps_solve_stats;
+/* The last solve's statistics.  solve_ms is read from the solve's HIP events
+   on the first call after the solve (the solve does not wait for its end
+   event), so this call can return MPS_ERR_HIP; the last error message is kept
+   otherwise. */
 int mps_get_solve_stats(const mps_ctx* ctx, mps_solve_stats* out);
 
 /* ---- stages (the friend-class test hooks of Computer.hpp:437-460) ---- */
